@@ -1,0 +1,254 @@
+/*
+ * mercurydpm_dem.h -- C ABI of the MI355X (gfx950) DEM time-step engine.
+ *
+ * The engine executes MercuryDPM's per-timestep hot path on one GPU:
+ *   integrateBeforeForceComputation -> periodic wrap -> periodic images ->
+ *   broad phase (multilevel cell grid + Verlet lists) -> pair forces
+ *   (LinearViscoelastic / HertzianViscoelastic normal laws, sliding /
+ *   rolling / torsion / Mindlin friction) -> wall forces (InfiniteWall,
+ *   CylindricalWall) -> external forces -> integrateAfterForceComputation ->
+ *   eraseOldInteractions.
+ *
+ * Every entry point below replaces a piece of the reference's CPU path; the
+ * comment of each names the reference function(s) (paths relative to the
+ * MercuryDPM source tree).  A C++ host (mercurydpm_fork_amd/host, class
+ * GpuMercury3D) keeps the DPMBase/Mercury3D, SpeciesHandler, WallHandler and
+ * BoundaryHandler API and calls these functions from its computeOneTimeStep()
+ * override, so a driver's computeAllForces()/hGridFindContacts path becomes a
+ * drop-in on one GPU.
+ *
+ * Conventions
+ *  - Every function returns 0 on success and a negative DEM_ERR_* code on
+ *    failure; dem_last_error() returns the message.  Nothing throws or exits
+ *    across the ABI (the reference's logger(ERROR) calls std::exit,
+ *    Kernel/Logger.cc:194; the host shim maps codes back to logger calls).
+ *  - Host buffers are borrowed for the duration of a call only.  Vectors are
+ *    interleaved per particle (x0,y0,z0,x1,...), i.e. the memory layout of an
+ *    array of Vec3D; quaternions are (q0,q1,q2,q3) per particle.
+ *  - Particles are identified by their 32-bit id (BaseObject::getId).  Their
+ *    position in the upload arrays is the ParticleHandler index.  Ids must
+ *    increase strictly with the index (true whenever no particle has been
+ *    removed since insertion); the periodic-history rule of
+ *    InteractionHandler::removeObjectKeepingPeriodics (InteractionHandler.cc:
+ *    316-371) compares indices, the id order of DPMBase::computeInternalForce
+ *    (DPMBase.cc:3130) compares ids, and the engine relies on both agreeing.
+ *  - All arithmetic is IEEE binary64 evaluated in the reference's operation
+ *    order without contraction (no FMA), so contact sets, springs and
+ *    per-contact forces are bit-identical to the CPU path on identical input.
+ */
+#ifndef MERCURYDPM_DEM_H
+#define MERCURYDPM_DEM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DEM_ABI_VERSION 1
+
+/* error codes */
+#define DEM_OK 0
+#define DEM_ERR_ARG (-1)        /* invalid argument / precondition violated */
+#define DEM_ERR_STATE (-2)      /* call out of order (e.g. step before upload) */
+#define DEM_ERR_DEVICE (-3)     /* HIP runtime error */
+#define DEM_ERR_CAPACITY (-4)   /* neighbour list / hash table overflow */
+#define DEM_ERR_UNSUPPORTED (-5)/* configuration outside the engine's scope */
+
+/* Interaction partner key for wall contacts: DEM_WALL_KEY | wall index. */
+#define DEM_WALL_KEY 0x80000000u
+
+typedef struct dem_ctx dem_ctx;
+
+/* Normal force laws (Kernel/Interactions/NormalForceInteractions). */
+enum dem_normal_law {
+    DEM_NORMAL_LINEAR_VISCOELASTIC = 0,  /* LinearViscoelasticInteraction.cc:105-132 */
+    DEM_NORMAL_HERTZIAN_VISCOELASTIC = 1 /* HertzianViscoelasticInteraction.cc:106-141 */
+};
+
+/* Friction force laws (Kernel/Interactions/FrictionForceInteractions). */
+enum dem_friction_law {
+    DEM_FRICTION_NONE = 0,     /* EmptyFrictionInteraction */
+    DEM_FRICTION_SLIDING = 1,  /* SlidingFrictionInteraction.cc:106-196 */
+    DEM_FRICTION_FULL = 2,     /* FrictionInteraction.cc:108-222 (sliding+rolling+torsion) */
+    DEM_FRICTION_MINDLIN = 3   /* MindlinInteraction.cc:279-344 */
+};
+
+/*
+ * Resolved parameters of one (possibly mixed) species pair.  The host derives
+ * them with the reference's species setters (e.g.
+ * LinearViscoelasticNormalSpecies::setCollisionTimeAndRestitutionCoefficient,
+ * LinearViscoelasticNormalSpecies.cc:225-245) and mixing rule
+ * (BaseSpecies::average, BaseSpecies.cc:110-115); the engine only reads them.
+ * Replaces the per-contact SpeciesHandler::getMixedObject lookup
+ * (SpeciesHandler.cc:922-945).
+ */
+typedef struct dem_pair_params {
+    int32_t normal_law;            /* enum dem_normal_law */
+    int32_t friction_law;          /* enum dem_friction_law */
+    int32_t constant_restitution;  /* BaseNormalForce::getConstantRestitution */
+    int32_t reserved;
+    double stiffness;              /* linear k */
+    double dissipation;            /* linear gamma; Hertz: dimensionless eta */
+    double elastic_modulus;        /* Hertz effective E* */
+    double sliding_stiffness;      /* k_t (sliding / full) */
+    double sliding_dissipation;    /* gamma_t; Mindlin: dimensionless */
+    double sliding_mu;             /* getSlidingFrictionCoefficient */
+    double sliding_mu_static;      /* getSlidingFrictionCoefficientStatic */
+    double shear_modulus;          /* Mindlin effective G* */
+    double rolling_stiffness, rolling_dissipation, rolling_mu, rolling_mu_static;
+    double torsion_stiffness, torsion_dissipation, torsion_mu, torsion_mu_static;
+    double interaction_distance;   /* adhesive range; must be 0 for these laws */
+} dem_pair_params;
+
+enum dem_wall_type {
+    DEM_WALL_INFINITE = 0, /* InfiniteWall.cc:151-175 */
+    DEM_WALL_CYLINDER = 1  /* CylindricalWall.cc:72-83 (axis = z through origin) */
+};
+
+/*
+ * One wall (WallHandler object).  For an InfiniteWall the host passes the
+ * wall position p0 (BaseInteractable::getPosition), the three coefficients of
+ * Quaternion::getDistance (Quaternion.cc:658-669: q00+q11-q22-q33, q12+q03,
+ * q13-q02, with q12=2q1q2 etc.) and the normal Quaternion::getAxis
+ * (Quaternion.cc:501-512) of the wall orientation, so the engine evaluates the
+ * distance and normal with the reference's own rounding.
+ */
+typedef struct dem_wall {
+    int32_t type;          /* enum dem_wall_type */
+    int32_t species;       /* index into the species table */
+    uint32_t id;           /* BaseObject id, reported in contact keys */
+    int32_t reserved;
+    double position[3];    /* InfiniteWall p0 */
+    double dist_coef[3];   /* InfiniteWall distance coefficients */
+    double axis[3];        /* InfiniteWall normal (getAxis) */
+    double radius;         /* CylindricalWall radius */
+} dem_wall;
+
+/*
+ * One PeriodicBoundary (PeriodicBoundary.cc:84-94), axis-aligned: normal_
+ * = unit vector along `axis`, distanceLeft_/distanceRight_ as stored by
+ * PeriodicBoundary::set, shift = (distanceRight_-distanceLeft_) (the non-zero
+ * component of shift_).  Boundaries are applied in the given order, which is
+ * the BoundaryHandler order (it decides how corner images are chained,
+ * PeriodicBoundary.cc:349-364).
+ */
+typedef struct dem_periodic {
+    int32_t axis;      /* 0 = x, 1 = y, 2 = z */
+    int32_t reserved;
+    double left;       /* distanceLeft_ */
+    double right;      /* distanceRight_ */
+    double shift;      /* distanceRight_ - distanceLeft_ */
+} dem_periodic;
+
+typedef struct dem_config {
+    int32_t device;          /* HIP device ordinal */
+    int32_t max_neighbors;   /* Verlet-list capacity per particle; 0 = 64 */
+    double skin;             /* Verlet skin distance; <= 0 = 0.05 * min diameter */
+    int32_t grid_levels;     /* cell-grid levels; 0 = automatic (<= 3) */
+    int32_t reserved;
+} dem_config;
+
+typedef struct dem_stats {
+    int64_t n_particles;
+    int64_t n_steps;              /* getNumberOfTimeSteps */
+    int64_t n_rebuilds;           /* grid re-sorts + Verlet-list rebuilds */
+    int64_t n_contacts;           /* stored interactions after the last step */
+    int64_t n_wall_contacts;      /* of which particle-wall */
+    int64_t n_list_entries;       /* Verlet-list entries (both directions) */
+    int32_t max_list_length;      /* longest list at the last rebuild */
+    int32_t grid_levels;
+    double time;                  /* DPMBase::getTime */
+    double skin;
+    double cell_size[4];          /* per level */
+} dem_stats;
+
+/* Create/destroy a context bound to one GPU.  No counterpart object in the
+ * reference: it owns the device state that Mercury3D's handlers hold on the
+ * CPU (ParticleHandler, InteractionHandler, HGrid). */
+int dem_create(dem_ctx** out, const dem_config* cfg);
+int dem_destroy(dem_ctx* ctx);
+const char* dem_last_error(const dem_ctx* ctx);
+int dem_abi_version(void);
+
+/* SpeciesHandler objects + mixed objects as an n x n table (row-major,
+ * symmetric; entry [a*n+b] is getMixedObject(a,b)).  Replaces
+ * SpeciesHandler::getMixedObject (SpeciesHandler.cc:922) and
+ * InteractionHandler::getInteraction's species lookup (InteractionHandler.cc:149). */
+int dem_set_species(dem_ctx* ctx, int32_t n_species, const dem_pair_params* table);
+
+/* WallHandler contents (static walls).  Replaces the wall loop of
+ * DPMBase::computeAllForces (DPMBase.cc:3434-3439) / computeWallForces
+ * (DPMBase.cc:5451-5462). */
+int dem_set_walls(dem_ctx* ctx, int32_t n_walls, const dem_wall* walls);
+
+/* BoundaryHandler periodic boundaries.  Replaces checkInteractionWithBoundaries
+ * (DPMBase.cc:3272), checkAndDuplicatePeriodicParticles (DPMBase.cc:5043) and
+ * removeDuplicatePeriodicParticles (DPMBase.cc:5000). */
+int dem_set_periodic(dem_ctx* ctx, int32_t n_boundaries, const dem_periodic* b);
+
+/* DPMBase::setGravity / setBackgroundDrag / setTimeStep / setRotation and the
+ * time counters (time_, numberOfTimeSteps_).  Used by computeExternalForces
+ * (DPMBase.cc:3168-3179) and the integrators. */
+int dem_set_gravity(dem_ctx* ctx, const double g[3]);
+int dem_set_background_drag(dem_ctx* ctx, double drag);
+int dem_set_timestep(dem_ctx* ctx, double dt);
+int dem_set_rotation(dem_ctx* ctx, int32_t rotation);
+int dem_set_time(dem_ctx* ctx, double time, int64_t n_time_steps);
+
+/*
+ * Upload the ParticleHandler (real particles only; no periodic ghosts) in
+ * handler order.  inv_inertia holds MatrixSymmetric3D (XX,XY,XZ,YY,YZ,ZZ) per
+ * particle; spheres must be isotropic (BaseParticle::computeMass,
+ * BaseParticle.cc:874-883).  f and t (force/torque of the previous force pass)
+ * may be NULL (zero).  Replaces ParticleHandler::copyAndAddObject for the
+ * device copy; invalidates all stored contacts (upload them afterwards).
+ */
+int dem_upload_particles(dem_ctx* ctx, int64_t n, const uint32_t* id, const int32_t* species,
+                         const double* radius, const double* inv_mass, const double* inv_inertia,
+                         const double* x, const double* v, const double* w, const double* q,
+                         const double* f, const double* t);
+
+/*
+ * Upload the InteractionHandler contents (history of particle-particle and
+ * particle-wall interactions).  p_id/i_id are BaseInteraction::getP()/getI()
+ * ids (i_id = DEM_WALL_KEY|wall index for a wall); for particle pairs p_id
+ * must be the lower id.  spring holds 9 doubles per contact: sliding spring,
+ * rolling spring, torsion spring (zeros where unused); time_stamp is
+ * BaseInteraction::getTimeStamp.
+ */
+int dem_upload_contacts(dem_ctx* ctx, int64_t n, const uint32_t* p_id, const uint32_t* i_id,
+                        const double* spring, const uint32_t* time_stamp);
+
+/* The force pass of DPMBase::initialiseSolve (DPMBase.cc:4260-4275):
+ * images, broad phase, computeAllForces, remove images.  No integration, no
+ * erase of old interactions. */
+int dem_compute_forces(dem_ctx* ctx);
+
+/* n_steps x DPMBase::computeOneTimeStep (DPMBase.cc:4315-4389) without
+ * output: integrateBeforeForceComputation, checkInteractionWithBoundaries,
+ * checkAndDuplicatePeriodicParticles, hGridActionsBeforeTimeStep,
+ * computeAllForces, removeDuplicatePeriodicParticles,
+ * integrateAfterForceComputation, eraseOldInteractions, time_ += dt. */
+int dem_step(dem_ctx* ctx, int64_t n_steps);
+
+/* Download state in handler order (any pointer may be NULL). */
+int dem_download_particles(dem_ctx* ctx, double* x, double* v, double* w, double* q,
+                           double* f, double* t);
+
+/* Number of stored interactions (InteractionHandler::getNumberOfObjects). */
+int dem_contact_count(dem_ctx* ctx, int64_t* n);
+
+/* Download the stored interactions, sorted by (p_id, i_id).  capacity is the
+ * length of the arrays (spring: 9*capacity doubles).  n_out receives the
+ * number written. */
+int dem_download_contacts(dem_ctx* ctx, int64_t capacity, int64_t* n_out, uint32_t* p_id,
+                          uint32_t* i_id, double* spring, uint32_t* time_stamp);
+
+int dem_get_stats(dem_ctx* ctx, dem_stats* stats);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MERCURYDPM_DEM_H */

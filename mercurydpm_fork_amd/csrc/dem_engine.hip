@@ -1,0 +1,1653 @@
+// dem_engine.hip -- MI355X (gfx950) implementation of MercuryDPM's per-timestep
+// DEM hot path behind the C ABI of include/mercurydpm_dem.h.
+//
+// Data layout in HBM (all particle arrays in grid-sorted order, SoA of
+// 32-byte records so one particle's hot fields are one aligned load):
+//   posr  double4 (x, y, z, radius)        vel   double4 (vx, vy, vz, invMass)
+//   omg   double4 (wx, wy, wz, invInertia) quat  double4 (q0..q3)
+//   frc   double4 (Fx, Fy, Fz, -)          trq   double4 (Tx, Ty, Tz, -)
+//   disp  double4 (displacement since the last list build)
+//   id, orig (handler index), spc (species | level << 16)
+// Broad phase: a multilevel cell grid (the stand-in for HGrid,
+// Kernel/HGrid.cc, MercuryBase.cc:204-306): each particle is binned on its
+// level, particles are counting-sorted by (level, hashed cell) and by id inside
+// a bucket, and a Verlet list (radius sum + skin, minimum image across periodic
+// boundaries) is built from the sorted cells.  The list is rebuilt when any
+// particle has moved more than half the skin; the decision is taken on the
+// device (the rebuild kernels exit at once when no rebuild is flagged), so a
+// dem_step() call never synchronises with the host until it returns.
+// Contact history: one record per stored interaction, in the list slot of its
+// writer (the lower id), double-buffered across steps.  On every rebuild the
+// active records go into a device hash table keyed by (id_lo, id_hi) or
+// (id, WALL|wall) and are re-matched to the new list slots.
+// Pair forces are gathered per particle (each side of a pair evaluates it, no
+// atomics), which makes every run bit-reproducible.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/mercurydpm_dem.h"
+#include "dem_laws.h"
+#include "dem_vec.h"
+
+using namespace dem;
+
+namespace {
+
+constexpr int kMaxWalls = 32;
+constexpr int kMaxLevels = 4;
+constexpr int kBlock = 256;
+constexpr uint64_t kEmptyKey = ~0ull;
+
+// device flag words
+enum Flag {
+    F_REBUILD = 0,
+    F_LIST_OVERFLOW = 1,
+    F_HASH_OVERFLOW = 2,
+    F_LIST_ASYM = 3,
+    F_MAX_LIST = 4,
+    F_REBUILDS = 5,
+    F_COUNT_A = 6,
+    F_COUNT_B = 7,
+    F_NUM = 8
+};
+
+template <int NS>
+struct Rec {
+    double s[NS];
+    uint32_t ts;
+    uint32_t pad;
+};
+
+struct Params {
+    int n, maxn, nwalls, ns, levels, rotation, nper;
+    int erase;
+    int paxis[3];
+    double pleft[3], pright[3], pshift[3];
+    double dt, drag, g[3];
+    double rmax, skin, thr2;
+    uint32_t nt;
+    // grid
+    double h[kMaxLevels];       // cell size of a non-periodic axis
+    double rmaxl[kMaxLevels];   // largest radius on each level
+    double hc[kMaxLevels][3];   // cell size per axis (periodic axes: L / ncell)
+    int ncell[kMaxLevels][3];   // cells per periodic axis (0 = not periodic)
+    uint32_t nb[kMaxLevels];    // buckets per level (power of two)
+    uint32_t nboff[kMaxLevels]; // bucket offset of each level
+};
+
+template <int NS>
+struct Dev {
+    double4 *posr, *vel, *omg, *quat, *frc, *trq, *disp;
+    uint32_t *id, *orig, *spc;
+    int4* cellc;
+    // gather scratch
+    double4 *t_posr, *t_vel, *t_omg, *t_quat, *t_frc, *t_trq;
+    uint32_t *t_id, *t_orig, *t_spc;
+    int4* t_cellc;
+    // Verlet lists
+    uint32_t* nbr;
+    int32_t* ncount;
+    uint8_t* rev;
+    // contact records
+    Rec<NS>* rec[2];
+    unsigned long long* mask[2];
+    Rec<NS>* wrec[2];
+    uint32_t* wmask[2];
+    // grid
+    uint32_t *key, *count, *start, *cursor, *perm;
+    // hash
+    unsigned long long* hkey;
+    Rec<NS>* hval;
+    uint64_t hcap;
+    // tables
+    const dem_pair_params* sp;
+    const dem_wall* walls;
+    int* flags;
+};
+
+__device__ inline uint64_t mix64(uint64_t k)
+{
+    // MurmurHash3 fmix64 finaliser
+    k ^= k >> 33;
+    k *= 0xff51afd7ed558ccdull;
+    k ^= k >> 33;
+    k *= 0xc4ceb9fe1a85ec53ull;
+    k ^= k >> 33;
+    return k;
+}
+
+__device__ inline uint32_t cell_hash(int cx, int cy, int cz)
+{
+    uint64_t k = ((uint64_t)(uint32_t)cx * 0x9E3779B97F4A7C15ull) ^ ((uint64_t)(uint32_t)cy * 0xC2B2AE3D27D4EB4Full) ^
+                 ((uint64_t)(uint32_t)cz * 0x165667B19E3779F9ull);
+    return (uint32_t)mix64(k);
+}
+
+__device__ inline double comp(V3 v, int a) { return a == 0 ? v.x : (a == 1 ? v.y : v.z); }
+__device__ inline void set_comp(V3& v, int a, double x)
+{
+    if (a == 0)
+        v.x = x;
+    else if (a == 1)
+        v.y = x;
+    else
+        v.z = x;
+}
+__device__ inline V3 xyz(double4 a) { return mk(a.x, a.y, a.z); }
+
+// PeriodicBoundary::getDistance (PeriodicBoundary.cc:207-212); an
+// axis-aligned normal makes dot(position, normal) the axis component.
+__device__ inline double periodic_distance(const Params& p, int b, V3 pos)
+{
+    const int a = p.paxis[b];
+    V3 n = mk(a == 0 ? 1.0 : 0.0, a == 1 ? 1.0 : 0.0, a == 2 ? 1.0 : 0.0);
+    double d = vdot(pos, n);
+    double l = d - p.pleft[b], r = p.pright[b] - d;
+    return l < r ? l : r;
+}
+
+// PeriodicBoundary::shiftPosition (PeriodicBoundary.cc:219-229) with
+// isClosestToLeftBoundary (286-290).
+__device__ inline V3 periodic_shift(const Params& p, int b, V3 pos)
+{
+    const int a = p.paxis[b];
+    V3 n = mk(a == 0 ? 1.0 : 0.0, a == 1 ? 1.0 : 0.0, a == 2 ? 1.0 : 0.0);
+    V3 s = vmul(n, p.pshift[b]);
+    double d = vdot(pos, n);
+    if (p.pright[b] - d > d - p.pleft[b]) return vadd(pos, s);
+    return vadd(pos, vneg(s));
+}
+
+__device__ inline bool rebuild_pending(const int* flags) { return __atomic_load_n(&flags[F_REBUILD], __ATOMIC_RELAXED) != 0; }
+
+// ------------------------------------------------------------------------
+// Velocity Verlet.  One launch applies, in this order,
+//  (after)  BaseParticle::integrateAfterForceComputation of the previous step
+//           (BaseParticle.cc:760-776),
+//  (before) BaseParticle::integrateBeforeForceComputation (BaseParticle.cc:
+//           718-752; fixed particles: BaseInteractable.cc:538-609), the
+//           periodic wrap of checkInteractionWithBoundaries (DPMBase.cc:3272,
+//           PeriodicBoundary.cc:374-391) and the Verlet-list displacement test.
+// ------------------------------------------------------------------------
+template <int NS>
+__global__ void __launch_bounds__(kBlock) k_kick_drift(Dev<NS> d, Params p, int after, int before)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= p.n) return;
+    double4 vl = d.vel[i];
+    double4 om = d.omg[i];
+    const double inv_mass = vl.w;
+    V3 v = xyz(vl);
+    V3 w = xyz(om);
+    double4 qq = d.quat[i];
+    double q[4] = {qq.x, qq.y, qq.z, qq.w};
+    const double dt = p.dt;
+    if (after && inv_mass != 0.0) {
+        const V3 F = xyz(d.frc[i]);
+        v = vadd(v, vmul(vmul(vmul(F, inv_mass), 0.5), dt));
+        if (p.rotation) {
+            const V3 T = xyz(d.trq[i]);
+            double R[6];
+            rotate_inv_inertia(q, om.w, R);
+            w = vadd(w, vmul(vmul(sym_mul(R, T), 0.5), dt));
+        }
+    }
+    if (before) {
+        double4 pr = d.posr[i];
+        V3 x = xyz(pr);
+        V3 step;
+        if (inv_mass == 0.0) {
+            step = vmul(v, dt);
+            x = vadd(x, step);
+            rotate_quaternion(q, vmul(w, dt));
+        } else {
+            const V3 F = xyz(d.frc[i]);
+            v = vadd(v, vmul(vmul(vmul(F, inv_mass), 0.5), dt));
+            step = vmul(v, dt);
+            x = vadd(x, step);
+            if (p.rotation) {
+                const V3 T = xyz(d.trq[i]);
+                double R[6];
+                rotate_inv_inertia(q, om.w, R);
+                w = vadd(w, vmul(vmul(sym_mul(R, T), 0.5), dt));
+                rotate_quaternion(q, vmul(w, dt));
+            }
+        }
+        for (int b = 0; b < p.nper; ++b)
+            if (periodic_distance(p, b, x) < 0) x = periodic_shift(p, b, x);
+        d.posr[i] = make_double4(x.x, x.y, x.z, pr.w);
+        d.quat[i] = make_double4(q[0], q[1], q[2], q[3]);
+        double4 ds = d.disp[i];
+        ds.x += step.x;
+        ds.y += step.y;
+        ds.z += step.z;
+        d.disp[i] = ds;
+        if (ds.x * ds.x + ds.y * ds.y + ds.z * ds.z > p.thr2) atomicOr(&d.flags[F_REBUILD], 1);
+    }
+    d.vel[i] = make_double4(v.x, v.y, v.z, inv_mass);
+    d.omg[i] = make_double4(w.x, w.y, w.z, om.w);
+}
+
+// ------------------------------------------------------------------------
+// Rebuild: contact records -> hash table (keyed by particle ids).
+// ------------------------------------------------------------------------
+template <int NS>
+__device__ inline void hash_insert(Dev<NS>& d, uint64_t key, const Rec<NS>& r)
+{
+    uint64_t h = mix64(key) & (d.hcap - 1);
+    for (uint64_t probe = 0; probe < d.hcap; ++probe) {
+        unsigned long long prev = atomicCAS(&d.hkey[h], (unsigned long long)kEmptyKey, (unsigned long long)key);
+        if (prev == kEmptyKey || prev == key) {
+            d.hval[h] = r;
+            return;
+        }
+        h = (h + 1) & (d.hcap - 1);
+    }
+    atomicOr(&d.flags[F_HASH_OVERFLOW], 1);
+}
+
+template <int NS>
+__device__ inline bool hash_find(const Dev<NS>& d, uint64_t key, Rec<NS>& r)
+{
+    uint64_t h = mix64(key) & (d.hcap - 1);
+    for (uint64_t probe = 0; probe < d.hcap; ++probe) {
+        unsigned long long k = d.hkey[h];
+        if (k == key) {
+            r = d.hval[h];
+            return true;
+        }
+        if (k == kEmptyKey) return false;
+        h = (h + 1) & (d.hcap - 1);
+    }
+    return false;
+}
+
+template <int NS>
+__global__ void __launch_bounds__(kBlock) k_hash_store(Dev<NS> d, Params p, int cur)
+{
+    if (!rebuild_pending(d.flags)) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= p.n) return;
+    const uint32_t idi = d.id[i];
+    unsigned long long m = d.mask[cur][i];
+    while (m) {
+        const int k = __ffsll((long long)m) - 1;
+        m &= m - 1;
+        const uint32_t j = d.nbr[(size_t)i * p.maxn + k];
+        const uint64_t key = ((uint64_t)idi << 32) | d.id[j];
+        hash_insert(d, key, d.rec[cur][(size_t)i * p.maxn + k]);
+    }
+    uint32_t wm = d.wmask[cur][i];
+    while (wm) {
+        const int w = __ffs(wm) - 1;
+        wm &= wm - 1;
+        const uint64_t key = ((uint64_t)idi << 32) | (DEM_WALL_KEY | (uint32_t)w);
+        hash_insert(d, key, d.wrec[cur][(size_t)i * p.nwalls + w]);
+    }
+}
+
+// ------------------------------------------------------------------------
+// Rebuild: counting sort by (level, hashed cell), then by id in a bucket.
+// ------------------------------------------------------------------------
+template <int NS>
+__global__ void k_zero_u32(Dev<NS> d, uint32_t* a, uint32_t n)
+{
+    if (!rebuild_pending(d.flags)) return;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = 0;
+}
+
+__device__ inline int4 cell_of(const Params& p, int lvl, V3 x)
+{
+    int c[3];
+    for (int a = 0; a < 3; ++a) {
+        const double xa = comp(x, a);
+        int pb = -1;
+        for (int b = 0; b < p.nper; ++b)
+            if (p.paxis[b] == a) pb = b;
+        if (pb >= 0) {
+            const int nc = p.ncell[lvl][a];
+            int t = (int)floor((xa - p.pleft[pb]) / p.hc[lvl][a]);
+            t %= nc;
+            if (t < 0) t += nc;
+            c[a] = t;
+        } else {
+            c[a] = (int)floor(xa / p.h[lvl]);
+        }
+    }
+    return make_int4(lvl, c[0], c[1], c[2]);
+}
+
+template <int NS>
+__global__ void __launch_bounds__(kBlock) k_keys(Dev<NS> d, Params p)
+{
+    if (!rebuild_pending(d.flags)) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= p.n) return;
+    const int lvl = (int)(d.spc[i] >> 16);
+    const int4 c = cell_of(p, lvl, xyz(d.posr[i]));
+    const uint32_t key = p.nboff[lvl] + (cell_hash(c.y, c.z, c.w) & (p.nb[lvl] - 1));
+    d.key[i] = key;
+    d.cellc[i] = c;
+    atomicAdd(&d.count[key], 1u);
+}
+
+// exclusive scan, 256 threads x 8 elements per block
+constexpr int kScanPer = 8;
+constexpr int kScanTile = kBlock * kScanPer;
+
+template <int NS>
+__global__ void __launch_bounds__(kBlock) k_scan_block(Dev<NS> d, const uint32_t* in, uint32_t* out, uint32_t* sums,
+                                                       uint32_t n)
+{
+    if (!rebuild_pending(d.flags)) return;
+    __shared__ uint32_t s[kBlock];
+    const uint32_t base = blockIdx.x * kScanTile + threadIdx.x * kScanPer;
+    uint32_t v[kScanPer];
+    uint32_t t = 0;
+    for (int k = 0; k < kScanPer; ++k) {
+        const uint32_t idx = base + k;
+        v[k] = idx < n ? in[idx] : 0u;
+        t += v[k];
+    }
+    s[threadIdx.x] = t;
+    __syncthreads();
+    for (int off = 1; off < kBlock; off <<= 1) {
+        uint32_t x = threadIdx.x >= (unsigned)off ? s[threadIdx.x - off] : 0u;
+        __syncthreads();
+        s[threadIdx.x] += x;
+        __syncthreads();
+    }
+    uint32_t run = s[threadIdx.x] - t;
+    for (int k = 0; k < kScanPer; ++k) {
+        const uint32_t idx = base + k;
+        if (idx < n) out[idx] = run;
+        run += v[k];
+    }
+    if (threadIdx.x == kBlock - 1 && sums) sums[blockIdx.x] = s[kBlock - 1];
+}
+
+template <int NS>
+__global__ void __launch_bounds__(kBlock) k_scan_add(Dev<NS> d, uint32_t* out, const uint32_t* add, uint32_t n)
+{
+    if (!rebuild_pending(d.flags)) return;
+    const uint32_t idx = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t tile = idx / kScanTile;
+    if (idx < n) out[idx] += add[tile];
+}
+
+template <int NS>
+__global__ void __launch_bounds__(kBlock) k_scatter(Dev<NS> d, Params p)
+{
+    if (!rebuild_pending(d.flags)) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= p.n) return;
+    const uint32_t key = d.key[i];
+    const uint32_t pos = d.start[key] + atomicAdd(&d.cursor[key], 1u);
+    d.perm[pos] = (uint32_t)i;
+}
+
+template <int NS>
+__global__ void __launch_bounds__(kBlock) k_sort_buckets(Dev<NS> d, uint32_t nbuckets, uint32_t n)
+{
+    if (!rebuild_pending(d.flags)) return;
+    const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nbuckets) return;
+    const uint32_t s = d.start[b];
+    const uint32_t e = (b + 1 < nbuckets) ? d.start[b + 1] : n;
+    for (uint32_t a = s + 1; a < e; ++a) {
+        const uint32_t v = d.perm[a];
+        const uint32_t vid = d.id[v];
+        uint32_t c = a;
+        while (c > s && d.id[d.perm[c - 1]] > vid) {
+            d.perm[c] = d.perm[c - 1];
+            --c;
+        }
+        d.perm[c] = v;
+    }
+}
+
+template <int NS>
+__global__ void __launch_bounds__(kBlock) k_gather(Dev<NS> d, Params p)
+{
+    if (!rebuild_pending(d.flags)) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= p.n) return;
+    const uint32_t o = d.perm[i];
+    d.t_posr[i] = d.posr[o];
+    d.t_vel[i] = d.vel[o];
+    d.t_omg[i] = d.omg[o];
+    d.t_quat[i] = d.quat[o];
+    d.t_frc[i] = d.frc[o];
+    d.t_trq[i] = d.trq[o];
+    d.t_id[i] = d.id[o];
+    d.t_orig[i] = d.orig[o];
+    d.t_spc[i] = d.spc[o];
+    d.t_cellc[i] = d.cellc[o];
+}
+
+template <int NS>
+__global__ void __launch_bounds__(kBlock) k_copy_back(Dev<NS> d, Params p)
+{
+    if (!rebuild_pending(d.flags)) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= p.n) return;
+    d.posr[i] = d.t_posr[i];
+    d.vel[i] = d.t_vel[i];
+    d.omg[i] = d.t_omg[i];
+    d.quat[i] = d.t_quat[i];
+    d.frc[i] = d.t_frc[i];
+    d.trq[i] = d.t_trq[i];
+    d.id[i] = d.t_id[i];
+    d.orig[i] = d.t_orig[i];
+    d.spc[i] = d.t_spc[i];
+    d.cellc[i] = d.t_cellc[i];
+    d.disp[i] = make_double4(0.0, 0.0, 0.0, 0.0);
+}
+
+// ------------------------------------------------------------------------
+// Rebuild: Verlet lists from the sorted multilevel grid.
+// ------------------------------------------------------------------------
+__device__ inline V3 min_image_delta(const Params& p, V3 xi, V3 xj)
+{
+    V3 dlt = vsub(xj, xi);
+    for (int b = 0; b < p.nper; ++b) {
+        const int a = p.paxis[b];
+        const double L = p.pshift[b];
+        double c = comp(dlt, a);
+        if (c > 0.5 * L)
+            c = c - L;
+        else if (c < -0.5 * L)
+            c = c + L;
+        set_comp(dlt, a, c);
+    }
+    return dlt;
+}
+
+template <int NS>
+__global__ void __launch_bounds__(kBlock) k_build_lists(Dev<NS> d, Params p)
+{
+    if (!rebuild_pending(d.flags)) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= p.n) return;
+    const double4 pr = d.posr[i];
+    const V3 xi = xyz(pr);
+    const bool fixed_i = d.vel[i].w == 0.0;
+    const size_t base = (size_t)i * p.maxn;
+    int cnt = 0;
+    bool overflow = false;
+    for (int m = 0; m < p.levels; ++m) {
+        const double R = pr.w + p.rmaxl[m] + p.skin;
+        int lo[3], hi[3], nc[3];
+        for (int a = 0; a < 3; ++a) {
+            const double xa = comp(xi, a);
+            int pb = -1;
+            for (int b = 0; b < p.nper; ++b)
+                if (p.paxis[b] == a) pb = b;
+            if (pb >= 0) {
+                nc[a] = p.ncell[m][a];
+                lo[a] = (int)floor((xa - R - p.pleft[pb]) / p.hc[m][a]);
+                hi[a] = (int)floor((xa + R - p.pleft[pb]) / p.hc[m][a]);
+                if (hi[a] - lo[a] + 1 >= nc[a]) {
+                    lo[a] = 0;
+                    hi[a] = nc[a] - 1;
+                }
+            } else {
+                nc[a] = 0;
+                lo[a] = (int)floor((xa - R) / p.h[m]);
+                hi[a] = (int)floor((xa + R) / p.h[m]);
+            }
+        }
+        for (int tz = lo[2]; tz <= hi[2]; ++tz) {
+            int cz = tz;
+            if (nc[2]) cz = ((tz % nc[2]) + nc[2]) % nc[2];
+            for (int ty = lo[1]; ty <= hi[1]; ++ty) {
+                int cy = ty;
+                if (nc[1]) cy = ((ty % nc[1]) + nc[1]) % nc[1];
+                for (int tx = lo[0]; tx <= hi[0]; ++tx) {
+                    int cx = tx;
+                    if (nc[0]) cx = ((tx % nc[0]) + nc[0]) % nc[0];
+                    const uint32_t b = p.nboff[m] + (cell_hash(cx, cy, cz) & (p.nb[m] - 1));
+                    const uint32_t s = d.start[b];
+                    const uint32_t e = d.start[b + 1];
+                    for (uint32_t q = s; q < e; ++q) {
+                        if (q == (uint32_t)i) continue;
+                        const int4 cq = d.cellc[q];
+                        if (cq.x != m || cq.y != cx || cq.z != cy || cq.w != cz) continue;
+                        if (fixed_i && d.vel[q].w == 0.0) continue;
+                        const double4 pq = d.posr[q];
+                        const V3 dl = min_image_delta(p, xi, xyz(pq));
+                        const double cut = pr.w + pq.w + p.skin;
+                        if (vlen2(dl) < cut * cut) {
+                            if (cnt < p.maxn)
+                                d.nbr[base + cnt] = q;
+                            else
+                                overflow = true;
+                            ++cnt;
+                        }
+                    }
+                }
+            }
+        }
+    }
+    if (overflow) atomicOr(&d.flags[F_LIST_OVERFLOW], 1);
+    atomicMax(&d.flags[F_MAX_LIST], cnt);
+    d.ncount[i] = cnt < p.maxn ? cnt : p.maxn;
+}
+
+// Reader slots find the writer's slot; writer slots and wall slots take their
+// history back from the hash table.
+template <int NS>
+__global__ void __launch_bounds__(kBlock) k_rematch(Dev<NS> d, Params p, int cur)
+{
+    if (!rebuild_pending(d.flags)) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= p.n) return;
+    const uint32_t idi = d.id[i];
+    const int cnt = d.ncount[i];
+    const size_t base = (size_t)i * p.maxn;
+    unsigned long long m = 0;
+    for (int k = 0; k < cnt; ++k) {
+        const uint32_t j = d.nbr[base + k];
+        const uint32_t idj = d.id[j];
+        if (idi < idj) {
+            Rec<NS> r;
+            if (hash_find(d, ((uint64_t)idi << 32) | idj, r)) {
+                d.rec[cur][base + k] = r;
+                m |= 1ull << k;
+            }
+            d.rev[base + k] = 0xFF;
+        } else {
+            const int cj = d.ncount[j];
+            const size_t bj = (size_t)j * p.maxn;
+            int kk = -1;
+            for (int a = 0; a < cj; ++a)
+                if (d.nbr[bj + a] == (uint32_t)i) {
+                    kk = a;
+                    break;
+                }
+            if (kk < 0) atomicOr(&d.flags[F_LIST_ASYM], 1);
+            d.rev[base + k] = (uint8_t)(kk < 0 ? 0xFF : kk);
+        }
+    }
+    d.mask[cur][i] = m;
+    uint32_t wm = 0;
+    for (int w = 0; w < p.nwalls; ++w) {
+        Rec<NS> r;
+        if (hash_find(d, ((uint64_t)idi << 32) | (DEM_WALL_KEY | (uint32_t)w), r)) {
+            d.wrec[cur][(size_t)i * p.nwalls + w] = r;
+            wm |= 1u << w;
+        }
+    }
+    d.wmask[cur][i] = wm;
+}
+
+template <int NS>
+__global__ void __launch_bounds__(kBlock) k_hash_clear(Dev<NS> d)
+{
+    if (!rebuild_pending(d.flags)) return;
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < d.hcap) d.hkey[i] = kEmptyKey;
+}
+
+__global__ void k_rebuild_done(int* flags)
+{
+    if (flags[F_REBUILD]) {
+        flags[F_REBUILD] = 0;
+        flags[F_REBUILDS] += 1;
+    }
+}
+
+// ------------------------------------------------------------------------
+// Forces: DPMBase::computeAllForces (DPMBase.cc:3397-3526) for one particle:
+// pair contacts from its Verlet list (computeInternalForce, DPMBase.cc:
+// 3113-3162; getInteractionWith, BaseParticle.cc:688-710; periodic images,
+// PeriodicBoundary.cc:304-364, with the history rule of
+// InteractionHandler::removeObjectKeepingPeriodics, InteractionHandler.cc:
+// 316-371), wall contacts (computeForcesDueToWalls, DPMBase.cc:3188-3215;
+// BaseWall::getInteractionWith, BaseWall.cc:367-430) and external forces
+// (computeExternalForces, DPMBase.cc:3168-3179).  Contact records untouched in
+// this pass are dropped when their time stamp is not newer than the step
+// counter (eraseOldInteractions, InteractionHandler.cc:377-393).
+// ------------------------------------------------------------------------
+template <int NS>
+__global__ void __launch_bounds__(kBlock) k_force(Dev<NS> d, Params p, int cur)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= p.n) return;
+    const double4 pr = d.posr[i];
+    const double4 vl = d.vel[i];
+    const double4 om = d.omg[i];
+    const uint32_t idi = d.id[i];
+    const int si = (int)(d.spc[i] & 0xFFFF);
+    const bool fixed_i = vl.w == 0.0;
+    Body bi;
+    bi.pos = xyz(pr);
+    bi.vel = xyz(vl);
+    bi.omega = xyz(om);
+    bi.radius = pr.w;
+    bi.inv_mass = vl.w;
+    V3 F = zero3(), T = zero3();
+    const uint32_t ts_new = p.nt + 1;
+    const Rec<NS>* rin = d.rec[cur];
+    Rec<NS>* rout = d.rec[cur ^ 1];
+    const unsigned long long m_in = d.mask[cur][i];
+    unsigned long long m_out = 0, touched = 0;
+    const int cnt = d.ncount[i];
+    const size_t base = (size_t)i * p.maxn;
+
+    for (int k = 0; k < cnt; ++k) {
+        const uint32_t j = d.nbr[base + k];
+        const double4 pq = d.posr[j];
+        V3 xj = xyz(pq);
+        bool img = false, exists = true;
+        for (int b = 0; b < p.nper; ++b) {
+            const int a = p.paxis[b];
+            const double c = comp(xj, a) - comp(bi.pos, a);
+            if (c > 0.5 * p.pshift[b] || c < -0.5 * p.pshift[b]) {
+                img = true;
+                if (!(periodic_distance(p, b, xj) < pq.w + p.rmax)) exists = false;
+                xj = periodic_shift(p, b, xj);
+            }
+        }
+        if (img && !exists) continue;
+        const uint32_t idj = d.id[j];
+        const bool i_is_P = img || idi < idj;
+        const V3 posP = i_is_P ? bi.pos : xj;
+        const V3 posI = i_is_P ? xj : bi.pos;
+        const double rP = i_is_P ? pr.w : pq.w;
+        const double rI = i_is_P ? pq.w : pr.w;
+        const V3 branch = vsub(posP, posI);
+        const double d2 = vlen2(branch);
+        const double sum_r = rI + rP;
+        if (d2 >= sum_r * sum_r) continue;
+
+        const double4 vq = d.vel[j];
+        const double4 oq = d.omg[j];
+        Body bj;
+        bj.pos = xj;
+        bj.vel = xyz(vq);
+        bj.omega = xyz(oq);
+        bj.radius = pq.w;
+        bj.inv_mass = vq.w;
+        const Body& P = i_is_P ? bi : bj;
+        const Body& I = i_is_P ? bj : bi;
+        ContactGeom g;
+        const double distance = sqrt(d2);
+        g.normal = vdiv(branch, distance);
+        g.overlap = rP + rI - distance;
+        g.distance = distance;
+        g.contact = vsub(posP, vmul(g.normal, rP - 0.5 * g.overlap));
+        g.i_is_wall = false;
+
+        const bool writer = idi < idj;
+        double spring[NS];
+        for (int a = 0; a < NS; ++a) spring[a] = 0.0;
+        if (writer) {
+            if ((m_in >> k) & 1ull)
+                for (int a = 0; a < NS; ++a) spring[a] = rin[base + k].s[a];
+        } else {
+            const uint32_t kk = d.rev[base + k];
+            if (kk != 0xFF && ((d.mask[cur][j] >> kk) & 1ull)) {
+                const Rec<NS>& r = rin[(size_t)j * p.maxn + kk];
+                for (int a = 0; a < NS; ++a) spring[a] = r.s[a];
+            }
+            if (img) {
+                // the stored frame has the lower id as P; this image evaluation
+                // has the higher id as P (BaseInteraction::reverseHistory)
+                spring[0] = -spring[0];
+                spring[1] = -spring[1];
+                spring[2] = -spring[2];
+                if constexpr (NS == 9) {
+                    spring[6] = -spring[6];
+                    spring[7] = -spring[7];
+                    spring[8] = -spring[8];
+                }
+            }
+        }
+        const int sj = (int)(d.spc[j] & 0xFFFF);
+        const dem_pair_params sp = d.sp[si * p.ns + sj];
+        V3 f, tc;
+        contact_force<NS>(sp, P, I, g, spring, p.dt, f, tc);
+        if (i_is_P) {
+            F = vadd(F, f);
+            if (p.rotation) T = vadd(T, vsub(tc, vcross(vsub(posP, g.contact), f)));
+        } else {
+            F = vadd(F, vneg(f));
+            if (p.rotation) T = vadd(T, vadd(vneg(tc), vcross(vsub(posI, g.contact), f)));
+        }
+        if (writer) {
+            Rec<NS> r;
+            for (int a = 0; a < NS; ++a) r.s[a] = spring[a];
+            r.ts = ts_new;
+            r.pad = 0;
+            // An image contact is stored through the ghost's copy of the
+            // interaction, which replaces the real-real record only when its
+            // time stamp is newer (InteractionHandler.cc:339-350); with equal
+            // stamps (the initial force pass and step 0 both stamp 1) the
+            // previous record survives unchanged.
+            if (img && ((m_in >> k) & 1ull) && rin[base + k].ts >= ts_new) r = rin[base + k];
+            rout[base + k] = r;
+            m_out |= 1ull << k;
+            touched |= 1ull << k;
+        }
+    }
+    unsigned long long carry = m_in & ~touched;
+    while (carry) {
+        const int k = __ffsll((long long)carry) - 1;
+        carry &= carry - 1;
+        const Rec<NS> r = rin[base + k];
+        if (!p.erase || r.ts > p.nt) {
+            rout[base + k] = r;
+            m_out |= 1ull << k;
+        }
+    }
+    d.mask[cur ^ 1][i] = m_out;
+
+    // walls
+    if (p.nwalls > 0) {
+        const uint32_t wm_in = d.wmask[cur][i];
+        uint32_t wm_out = 0;
+        const size_t wbase = (size_t)i * p.nwalls;
+        for (int w = 0; w < p.nwalls; ++w) {
+            const dem_wall& wl = d.walls[w];
+            bool contact = false;
+            double distance = 0.0;
+            V3 n = zero3();
+            if (!fixed_i) {
+                const double wall_r = pr.w + d.sp[si * p.ns + wl.species].interaction_distance;
+                if (wl.type == DEM_WALL_INFINITE) {
+                    distance = wl.dist_coef[0] * (wl.position[0] - bi.pos.x) +
+                               (wl.dist_coef[1] * (wl.position[1] - bi.pos.y) +
+                                wl.dist_coef[2] * (wl.position[2] - bi.pos.z));
+                    if (distance < wall_r) {
+                        contact = true;
+                        n = mk(wl.axis[0], wl.axis[1], wl.axis[2]);
+                    }
+                } else {
+                    const double R = sqrt(bi.pos.x * bi.pos.x + bi.pos.y * bi.pos.y);
+                    distance = wl.radius - R;
+                    if (distance < pr.w) {
+                        contact = true;
+                        const V3 nr = mk(bi.pos.x, bi.pos.y, 0.0);
+                        n = vdiv(nr, sqrt(vlen2(nr)));
+                    }
+                }
+            }
+            if (contact) {
+                ContactGeom g;
+                g.normal = vneg(n);
+                g.distance = distance;
+                g.overlap = pr.w - distance;
+                g.contact = vsub(bi.pos, vmul(g.normal, pr.w - 0.5 * g.overlap));
+                g.i_is_wall = true;
+                Body bw;
+                bw.pos = wl.type == DEM_WALL_INFINITE ? mk(wl.position[0], wl.position[1], wl.position[2]) : zero3();
+                bw.vel = zero3();
+                bw.omega = zero3();
+                bw.radius = 0.0;
+                bw.inv_mass = 0.0;
+                double spring[NS];
+                for (int a = 0; a < NS; ++a) spring[a] = 0.0;
+                if ((wm_in >> w) & 1u)
+                    for (int a = 0; a < NS; ++a) spring[a] = d.wrec[cur][wbase + w].s[a];
+                const dem_pair_params sp = d.sp[si * p.ns + wl.species];
+                V3 f, tc;
+                contact_force<NS>(sp, bi, bw, g, spring, p.dt, f, tc);
+                F = vadd(F, f);
+                if (p.rotation) T = vadd(T, vsub(tc, vcross(vsub(bi.pos, g.contact), f)));
+                Rec<NS> r;
+                for (int a = 0; a < NS; ++a) r.s[a] = spring[a];
+                r.ts = ts_new;
+                r.pad = 0;
+                d.wrec[cur ^ 1][wbase + w] = r;
+                wm_out |= 1u << w;
+            } else if ((wm_in >> w) & 1u) {
+                const Rec<NS> r = d.wrec[cur][wbase + w];
+                if (!p.erase || r.ts > p.nt) {
+                    d.wrec[cur ^ 1][wbase + w] = r;
+                    wm_out |= 1u << w;
+                }
+            }
+        }
+        d.wmask[cur ^ 1][i] = wm_out;
+    }
+
+    if (!fixed_i) {
+        const double mass = 1.0 / vl.w;
+        F = vadd(F, vsub(vmul(mk(p.g[0], p.g[1], p.g[2]), mass), vmul(bi.vel, p.drag)));
+    }
+    d.frc[i] = make_double4(F.x, F.y, F.z, 0.0);
+    d.trq[i] = make_double4(T.x, T.y, T.z, 0.0);
+}
+
+// ------------------------------------------------------------------------
+// Diagnostics: stored interactions.
+// ------------------------------------------------------------------------
+template <int NS>
+__global__ void __launch_bounds__(kBlock) k_count_contacts(Dev<NS> d, Params p, int cur)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= p.n) return;
+    const int c = __popcll(d.mask[cur][i]);
+    const int w = p.nwalls > 0 ? __popc(d.wmask[cur][i]) : 0;
+    if (c) atomicAdd(&d.flags[F_COUNT_A], c);
+    if (w) atomicAdd(&d.flags[F_COUNT_B], w);
+}
+
+template <int NS>
+__global__ void __launch_bounds__(kBlock) k_collect_contacts(Dev<NS> d, Params p, int cur, uint32_t* pid,
+                                                             uint32_t* iid, double* spring, uint32_t* ts,
+                                                             int* counter, int cap)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= p.n) return;
+    const uint32_t idi = d.id[i];
+    unsigned long long m = d.mask[cur][i];
+    while (m) {
+        const int k = __ffsll((long long)m) - 1;
+        m &= m - 1;
+        const int o = atomicAdd(counter, 1);
+        if (o >= cap) continue;
+        const size_t s = (size_t)i * p.maxn + k;
+        pid[o] = idi;
+        iid[o] = d.id[d.nbr[s]];
+        for (int a = 0; a < 9; ++a) spring[9 * (size_t)o + a] = a < NS ? d.rec[cur][s].s[a] : 0.0;
+        ts[o] = d.rec[cur][s].ts;
+    }
+    if (p.nwalls > 0) {
+        uint32_t wm = d.wmask[cur][i];
+        while (wm) {
+            const int w = __ffs(wm) - 1;
+            wm &= wm - 1;
+            const int o = atomicAdd(counter, 1);
+            if (o >= cap) continue;
+            const size_t s = (size_t)i * p.nwalls + w;
+            pid[o] = idi;
+            iid[o] = DEM_WALL_KEY | (uint32_t)w;
+            for (int a = 0; a < 9; ++a) spring[9 * (size_t)o + a] = a < NS ? d.wrec[cur][s].s[a] : 0.0;
+            ts[o] = d.wrec[cur][s].ts;
+        }
+    }
+}
+
+// Upload of InteractionHandler records: straight into the hash table; the
+// rebuild that follows every upload matches them to list slots.
+template <int NS>
+__global__ void __launch_bounds__(kBlock) k_hash_upload(Dev<NS> d, int n, const uint32_t* pid, const uint32_t* iid,
+                                                        const double* spring, const uint32_t* ts)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    Rec<NS> r;
+    for (int a = 0; a < NS; ++a) r.s[a] = spring[9 * (size_t)k + a];
+    r.ts = ts[k];
+    r.pad = 0;
+    hash_insert(d, ((uint64_t)pid[k] << 32) | iid[k], r);
+}
+
+template <int NS>
+__global__ void k_hash_clear_all(Dev<NS> d)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < d.hcap) d.hkey[i] = kEmptyKey;
+}
+
+template <int NS>
+__global__ void k_zero_masks(Dev<NS> d, Params p)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= p.n) return;
+    d.mask[0][i] = 0;
+    d.mask[1][i] = 0;
+    d.wmask[0][i] = 0;
+    d.wmask[1][i] = 0;
+    d.ncount[i] = 0;
+    d.disp[i] = make_double4(0.0, 0.0, 0.0, 0.0);
+}
+
+inline unsigned grid_for(uint64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+
+uint32_t next_pow2(uint64_t x)
+{
+    uint64_t p = 1;
+    while (p < x) p <<= 1;
+    return (uint32_t)p;
+}
+
+}  // namespace
+
+// ==========================================================================
+// Host side: the context and the C ABI.
+// ==========================================================================
+struct dem_ctx {
+    std::string err;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int maxn_cfg = 64;
+    double skin_cfg = 0.0;
+    int levels_cfg = 0;
+    // host-side settings
+    std::vector<dem_pair_params> species;
+    int ns = 0;
+    std::vector<dem_wall> walls;
+    std::vector<dem_periodic> periodic;
+    double g[3] = {0, 0, 0};
+    double drag = 0.0, dt = 0.0, time = 0.0;
+    int rotation = 1;
+    int64_t nt = 0;
+    int ns_springs = 3; // 3 or 9
+    // particles
+    int64_t n = 0;
+    bool uploaded = false;
+    int cur = 0;
+    Params prm{};
+    // device buffers
+    std::vector<void*> allocs;
+    Dev<3> d3{};
+    Dev<9> d9{};
+    dem_pair_params* d_sp = nullptr;
+    dem_wall* d_walls = nullptr;
+    int* d_flags = nullptr;
+    std::vector<uint32_t*> scan_levels; // scan sums per recursion level
+    std::vector<uint32_t> scan_sizes;
+    uint32_t nbuckets = 0;
+    int64_t last_rebuilds = 0;
+    int max_list = 0;
+};
+
+namespace {
+
+int set_err(dem_ctx* c, int code, const std::string& msg)
+{
+    if (c) c->err = msg;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                             \
+    do {                                                                                         \
+        hipError_t e_ = (expr);                                                                  \
+        if (e_ != hipSuccess)                                                                    \
+            return set_err(ctx, DEM_ERR_DEVICE, std::string(#expr ": ") + hipGetErrorString(e_)); \
+    } while (0)
+
+template <typename T>
+int dalloc(dem_ctx* ctx, T** p, size_t count)
+{
+    void* v = nullptr;
+    const size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+    hipError_t e = hipMalloc(&v, bytes);
+    if (e != hipSuccess)
+        return set_err(ctx, DEM_ERR_DEVICE, std::string("hipMalloc failed: ") + hipGetErrorString(e));
+    ctx->allocs.push_back(v);
+    *p = (T*)v;
+    return DEM_OK;
+}
+
+void free_all(dem_ctx* ctx)
+{
+    for (void* v : ctx->allocs) (void)hipFree(v);
+    ctx->allocs.clear();
+    ctx->scan_levels.clear();
+    ctx->scan_sizes.clear();
+    ctx->d_sp = nullptr;
+    ctx->d_walls = nullptr;
+    ctx->d_flags = nullptr;
+    ctx->uploaded = false;
+}
+
+template <int NS>
+Dev<NS>& dev(dem_ctx* c);
+template <>
+Dev<3>& dev<3>(dem_ctx* c) { return c->d3; }
+template <>
+Dev<9>& dev<9>(dem_ctx* c) { return c->d9; }
+
+template <int NS>
+int alloc_particles(dem_ctx* ctx, int64_t n)
+{
+    Dev<NS>& d = dev<NS>(ctx);
+    d = Dev<NS>{};
+    int r = 0;
+#define A(ptr, cnt)                          \
+    if ((r = dalloc(ctx, &(ptr), (cnt))) != 0) \
+        return r;
+    A(d.posr, n) A(d.vel, n) A(d.omg, n) A(d.quat, n) A(d.frc, n) A(d.trq, n) A(d.disp, n);
+    A(d.id, n) A(d.orig, n) A(d.spc, n) A(d.cellc, n);
+    A(d.t_posr, n) A(d.t_vel, n) A(d.t_omg, n) A(d.t_quat, n) A(d.t_frc, n) A(d.t_trq, n);
+    A(d.t_id, n) A(d.t_orig, n) A(d.t_spc, n) A(d.t_cellc, n);
+    const size_t slots = (size_t)n * ctx->maxn_cfg;
+    A(d.nbr, slots) A(d.ncount, n) A(d.rev, slots);
+    A(d.rec[0], slots) A(d.rec[1], slots) A(d.mask[0], n) A(d.mask[1], n);
+    const size_t wslots = (size_t)n * std::max<size_t>(ctx->walls.size(), 1);
+    A(d.wrec[0], wslots) A(d.wrec[1], wslots) A(d.wmask[0], n) A(d.wmask[1], n);
+    A(d.key, n) A(d.perm, n);
+    A(d.count, ctx->nbuckets) A(d.cursor, ctx->nbuckets) A(d.start, (size_t)ctx->nbuckets + 1);
+    d.hcap = next_pow2(std::max<int64_t>(4096, 8 * n + 4 * n * (int64_t)ctx->walls.size()));
+    A(d.hkey, d.hcap) A(d.hval, d.hcap);
+#undef A
+    // scan hierarchy: level 0 scans count into start; every further level
+    // scans the block sums of the level below in place
+    ctx->scan_levels.clear();
+    ctx->scan_sizes.clear();
+    ctx->scan_levels.push_back(d.start);
+    ctx->scan_sizes.push_back(ctx->nbuckets);
+    uint32_t sz = ctx->nbuckets;
+    while (sz > (uint32_t)kScanTile) {
+        const uint32_t nb = (sz + kScanTile - 1) / kScanTile;
+        uint32_t* sums = nullptr;
+        if ((r = dalloc(ctx, &sums, nb)) != 0) return r;
+        ctx->scan_levels.push_back(sums);
+        ctx->scan_sizes.push_back(nb);
+        sz = nb;
+    }
+    d.sp = ctx->d_sp;
+    d.walls = ctx->d_walls;
+    d.flags = ctx->d_flags;
+    return DEM_OK;
+}
+
+// exclusive scan of count -> start over nbuckets entries (start[nbuckets]
+// holds n and is written once at upload)
+template <int NS>
+void launch_scan(dem_ctx* ctx)
+{
+    Dev<NS>& d = dev<NS>(ctx);
+    hipStream_t s = ctx->stream;
+    const size_t L = ctx->scan_levels.size();
+    for (size_t k = 0; k < L; ++k) {
+        const uint32_t n = ctx->scan_sizes[k];
+        const uint32_t nblk = (n + kScanTile - 1) / kScanTile;
+        uint32_t* in = k == 0 ? d.count : ctx->scan_levels[k];
+        uint32_t* out = ctx->scan_levels[k];
+        uint32_t* sums = k + 1 < L ? ctx->scan_levels[k + 1] : nullptr;
+        hipLaunchKernelGGL(k_scan_block<NS>, dim3(nblk), dim3(kBlock), 0, s, d, in, out, sums, n);
+    }
+    for (size_t k = L - 1; k-- > 0;) {
+        const uint32_t n = ctx->scan_sizes[k];
+        hipLaunchKernelGGL(k_scan_add<NS>, dim3(grid_for(n)), dim3(kBlock), 0, s, d, ctx->scan_levels[k],
+                           ctx->scan_levels[k + 1], n);
+    }
+}
+
+template <int NS>
+void launch_rebuild(dem_ctx* ctx)
+{
+    Dev<NS>& d = dev<NS>(ctx);
+    hipStream_t s = ctx->stream;
+    const Params& p = ctx->prm;
+    const unsigned gn = grid_for(ctx->n);
+    hipLaunchKernelGGL(k_hash_store<NS>, dim3(gn), dim3(kBlock), 0, s, d, p, ctx->cur);
+    hipLaunchKernelGGL(k_zero_u32<NS>, dim3(grid_for(ctx->nbuckets)), dim3(kBlock), 0, s, d, d.count, ctx->nbuckets);
+    hipLaunchKernelGGL(k_zero_u32<NS>, dim3(grid_for(ctx->nbuckets)), dim3(kBlock), 0, s, d, d.cursor, ctx->nbuckets);
+    hipLaunchKernelGGL(k_keys<NS>, dim3(gn), dim3(kBlock), 0, s, d, p);
+    launch_scan<NS>(ctx);
+    hipLaunchKernelGGL(k_scatter<NS>, dim3(gn), dim3(kBlock), 0, s, d, p);
+    hipLaunchKernelGGL(k_sort_buckets<NS>, dim3(grid_for(ctx->nbuckets)), dim3(kBlock), 0, s, d, ctx->nbuckets,
+                       (uint32_t)ctx->n);
+    hipLaunchKernelGGL(k_gather<NS>, dim3(gn), dim3(kBlock), 0, s, d, p);
+    hipLaunchKernelGGL(k_copy_back<NS>, dim3(gn), dim3(kBlock), 0, s, d, p);
+    hipLaunchKernelGGL(k_build_lists<NS>, dim3(gn), dim3(kBlock), 0, s, d, p);
+    hipLaunchKernelGGL(k_rematch<NS>, dim3(gn), dim3(kBlock), 0, s, d, p, ctx->cur);
+    hipLaunchKernelGGL(k_hash_clear<NS>, dim3(grid_for(d.hcap)), dim3(kBlock), 0, s, d);
+    hipLaunchKernelGGL(k_rebuild_done, dim3(1), dim3(1), 0, s, d.flags);
+}
+
+template <int NS>
+void launch_force(dem_ctx* ctx, int erase)
+{
+    Dev<NS>& d = dev<NS>(ctx);
+    ctx->prm.erase = erase;
+    ctx->prm.nt = (uint32_t)ctx->nt;
+    hipLaunchKernelGGL(k_force<NS>, dim3(grid_for(ctx->n)), dim3(kBlock), 0, ctx->stream, d, ctx->prm, ctx->cur);
+    ctx->cur ^= 1;
+}
+
+template <int NS>
+void launch_kick(dem_ctx* ctx, int after, int before)
+{
+    Dev<NS>& d = dev<NS>(ctx);
+    hipLaunchKernelGGL(k_kick_drift<NS>, dim3(grid_for(ctx->n)), dim3(kBlock), 0, ctx->stream, d, ctx->prm, after,
+                       before);
+}
+
+int check_flags(dem_ctx* ctx)
+{
+    int f[F_NUM];
+    HIPCHK(hipMemcpyAsync(f, ctx->d_flags, sizeof f, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    ctx->last_rebuilds = f[F_REBUILDS];
+    ctx->max_list = f[F_MAX_LIST];
+    if (f[F_LIST_OVERFLOW])
+        return set_err(ctx, DEM_ERR_CAPACITY,
+                       "Verlet list overflow: a particle has " + std::to_string(f[F_MAX_LIST]) +
+                           " neighbours, more than max_neighbors=" + std::to_string(ctx->maxn_cfg));
+    if (f[F_HASH_OVERFLOW]) return set_err(ctx, DEM_ERR_CAPACITY, "contact hash table overflow");
+    if (f[F_LIST_ASYM]) return set_err(ctx, DEM_ERR_STATE, "internal error: asymmetric Verlet list");
+    return DEM_OK;
+}
+
+template <int NS>
+int run_steps(dem_ctx* ctx, int64_t nsteps)
+{
+    for (int64_t s = 0; s < nsteps; ++s) {
+        launch_kick<NS>(ctx, s > 0 ? 1 : 0, 1);
+        launch_rebuild<NS>(ctx);
+        launch_force<NS>(ctx, 1);
+        ctx->time += ctx->dt;
+        ctx->nt += 1;
+    }
+    if (nsteps > 0) launch_kick<NS>(ctx, 1, 0);
+    HIPCHK(hipGetLastError());
+    return check_flags(ctx);
+}
+
+template <int NS>
+int run_forces(dem_ctx* ctx)
+{
+    launch_rebuild<NS>(ctx);
+    launch_force<NS>(ctx, 0);
+    HIPCHK(hipGetLastError());
+    return check_flags(ctx);
+}
+
+int ready(dem_ctx* ctx)
+{
+    if (!ctx) return DEM_ERR_ARG;
+    if (!ctx->uploaded) return set_err(ctx, DEM_ERR_STATE, "no particles uploaded");
+    if (ctx->dt <= 0.0) return set_err(ctx, DEM_ERR_STATE, "time step not set");
+    return DEM_OK;
+}
+
+void refresh_params(dem_ctx* ctx)
+{
+    Params& p = ctx->prm;
+    p.dt = ctx->dt;
+    p.drag = ctx->drag;
+    for (int a = 0; a < 3; ++a) p.g[a] = ctx->g[a];
+    p.rotation = ctx->rotation;
+}
+
+}  // namespace
+
+extern "C" {
+
+int dem_abi_version(void) { return DEM_ABI_VERSION; }
+
+int dem_create(dem_ctx** out, const dem_config* cfg)
+{
+    if (!out) return DEM_ERR_ARG;
+    dem_ctx* ctx = new dem_ctx();
+    *out = ctx;
+    if (cfg) {
+        ctx->device = cfg->device;
+        if (cfg->max_neighbors > 0) ctx->maxn_cfg = cfg->max_neighbors;
+        ctx->skin_cfg = cfg->skin;
+        ctx->levels_cfg = cfg->grid_levels;
+    }
+    if (ctx->maxn_cfg > 64) return set_err(ctx, DEM_ERR_ARG, "max_neighbors must be <= 64");
+    if (ctx->levels_cfg > kMaxLevels) return set_err(ctx, DEM_ERR_ARG, "grid_levels must be <= 4");
+    int ndev = 0;
+    HIPCHK(hipGetDeviceCount(&ndev));
+    if (ctx->device < 0 || ctx->device >= ndev) return set_err(ctx, DEM_ERR_ARG, "no such HIP device");
+    HIPCHK(hipSetDevice(ctx->device));
+    HIPCHK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    return DEM_OK;
+}
+
+int dem_destroy(dem_ctx* ctx)
+{
+    if (!ctx) return DEM_OK;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    free_all(ctx);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return DEM_OK;
+}
+
+const char* dem_last_error(const dem_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int dem_set_species(dem_ctx* ctx, int32_t n, const dem_pair_params* table)
+{
+    if (!ctx || n <= 0 || n > 0xFFFF || !table) return set_err(ctx, DEM_ERR_ARG, "set_species: bad arguments");
+    if (ctx->uploaded) return set_err(ctx, DEM_ERR_STATE, "set_species must precede upload_particles");
+    for (int k = 0; k < n * n; ++k) {
+        const dem_pair_params& s = table[k];
+        if (s.interaction_distance != 0.0)
+            return set_err(ctx, DEM_ERR_UNSUPPORTED, "only non-adhesive laws (interaction_distance 0) are supported");
+        if (s.normal_law != DEM_NORMAL_LINEAR_VISCOELASTIC && s.normal_law != DEM_NORMAL_HERTZIAN_VISCOELASTIC)
+            return set_err(ctx, DEM_ERR_ARG, "unknown normal law");
+        if (s.friction_law < DEM_FRICTION_NONE || s.friction_law > DEM_FRICTION_MINDLIN)
+            return set_err(ctx, DEM_ERR_ARG, "unknown friction law");
+        if (s.friction_law == DEM_FRICTION_FULL && ((s.rolling_mu != 0.0 && s.rolling_stiffness == 0.0) ||
+                                                    (s.torsion_mu != 0.0 && s.torsion_stiffness == 0.0)))
+            return set_err(ctx, DEM_ERR_ARG, "rolling/torsion friction needs a non-zero stiffness");
+    }
+    ctx->species.assign(table, table + (size_t)n * n);
+    ctx->ns = n;
+    ctx->ns_springs = 3;
+    for (const auto& s : ctx->species)
+        if (s.friction_law == DEM_FRICTION_FULL && (s.rolling_mu != 0.0 || s.torsion_mu != 0.0)) ctx->ns_springs = 9;
+    return DEM_OK;
+}
+
+int dem_set_walls(dem_ctx* ctx, int32_t n, const dem_wall* walls)
+{
+    if (!ctx || n < 0 || n > kMaxWalls || (n > 0 && !walls)) return set_err(ctx, DEM_ERR_ARG, "set_walls: bad arguments (at most 32 walls)");
+    if (ctx->uploaded) return set_err(ctx, DEM_ERR_STATE, "set_walls must precede upload_particles");
+    for (int k = 0; k < n; ++k)
+        if (walls[k].type != DEM_WALL_INFINITE && walls[k].type != DEM_WALL_CYLINDER)
+            return set_err(ctx, DEM_ERR_ARG, "unknown wall type");
+    ctx->walls.assign(walls, walls + n);
+    return DEM_OK;
+}
+
+int dem_set_periodic(dem_ctx* ctx, int32_t n, const dem_periodic* b)
+{
+    if (!ctx || n < 0 || n > 3 || (n > 0 && !b)) return set_err(ctx, DEM_ERR_ARG, "set_periodic: bad arguments");
+    if (ctx->uploaded) return set_err(ctx, DEM_ERR_STATE, "set_periodic must precede upload_particles");
+    for (int k = 0; k < n; ++k) {
+        if (b[k].axis < 0 || b[k].axis > 2) return set_err(ctx, DEM_ERR_ARG, "periodic axis must be 0, 1 or 2");
+        if (!(b[k].right > b[k].left)) return set_err(ctx, DEM_ERR_ARG, "periodic: right must exceed left");
+        for (int j = 0; j < k; ++j)
+            if (b[j].axis == b[k].axis) return set_err(ctx, DEM_ERR_ARG, "two periodic boundaries on one axis");
+    }
+    ctx->periodic.assign(b, b + n);
+    return DEM_OK;
+}
+
+int dem_set_gravity(dem_ctx* ctx, const double g[3])
+{
+    if (!ctx || !g) return DEM_ERR_ARG;
+    for (int a = 0; a < 3; ++a) ctx->g[a] = g[a];
+    return DEM_OK;
+}
+
+int dem_set_background_drag(dem_ctx* ctx, double drag)
+{
+    if (!ctx) return DEM_ERR_ARG;
+    ctx->drag = drag;
+    return DEM_OK;
+}
+
+int dem_set_timestep(dem_ctx* ctx, double dt)
+{
+    if (!ctx || !(dt > 0)) return set_err(ctx, DEM_ERR_ARG, "time step must be positive");
+    ctx->dt = dt;
+    return DEM_OK;
+}
+
+int dem_set_rotation(dem_ctx* ctx, int32_t rotation)
+{
+    if (!ctx) return DEM_ERR_ARG;
+    ctx->rotation = rotation ? 1 : 0;
+    return DEM_OK;
+}
+
+int dem_set_time(dem_ctx* ctx, double time, int64_t nt)
+{
+    if (!ctx || nt < 0) return DEM_ERR_ARG;
+    ctx->time = time;
+    ctx->nt = nt;
+    return DEM_OK;
+}
+
+int dem_upload_particles(dem_ctx* ctx, int64_t n, const uint32_t* id, const int32_t* species, const double* radius,
+                         const double* inv_mass, const double* inv_inertia, const double* x, const double* v,
+                         const double* w, const double* q, const double* f, const double* t)
+{
+    if (!ctx || n <= 0 || !id || !species || !radius || !inv_mass || !inv_inertia || !x || !v)
+        return set_err(ctx, DEM_ERR_ARG, "upload_particles: bad arguments");
+    if (n > (int64_t)0x7FFFFFFF) return set_err(ctx, DEM_ERR_ARG, "too many particles");
+    if (ctx->ns == 0) return set_err(ctx, DEM_ERR_STATE, "set_species before upload_particles");
+    HIPCHK(hipSetDevice(ctx->device));
+    // --- validate and derive the grid on the host
+    double rmin = 1e300, rmax = 0.0;
+    for (int64_t i = 0; i < n; ++i) {
+        if (i > 0 && id[i] <= id[i - 1]) return set_err(ctx, DEM_ERR_ARG, "ids must increase with the handler index");
+        if (id[i] >= DEM_WALL_KEY) return set_err(ctx, DEM_ERR_ARG, "particle ids must be < 2^31");
+        if (species[i] < 0 || species[i] >= ctx->ns) return set_err(ctx, DEM_ERR_ARG, "species index out of range");
+        if (!(radius[i] > 0)) return set_err(ctx, DEM_ERR_ARG, "radius must be positive");
+        const double* I = inv_inertia + 6 * i;
+        if (I[1] != 0.0 || I[2] != 0.0 || I[4] != 0.0 || I[0] != I[3] || I[0] != I[5])
+            return set_err(ctx, DEM_ERR_UNSUPPORTED, "inverse inertia must be isotropic (spheres)");
+        if ((inv_mass[i] == 0.0) != (I[0] == 0.0))
+            return set_err(ctx, DEM_ERR_ARG, "fixed particles need zero inverse mass and inertia");
+        rmin = std::min(rmin, radius[i]);
+        rmax = std::max(rmax, radius[i]);
+    }
+    const double skin = ctx->skin_cfg > 0 ? ctx->skin_cfg : 0.1 * rmin;
+    // levels (EXPONENTIAL cell-size distribution of MercuryBase::hGridRebuild,
+    // MercuryBase.cc:204-306, between the smallest and largest diameter)
+    int L = ctx->levels_cfg;
+    if (L <= 0) L = rmax / rmin < 1.5 ? 1 : (rmax / rmin < 2.5 ? 2 : 3);
+    double hl[kMaxLevels];
+    for (int l = 0; l < L; ++l)
+        hl[l] = (l == L - 1) ? 2.0 * rmax : 2.0 * rmin * std::pow(rmax / rmin, (double)(l + 1) / L);
+    std::vector<uint32_t> spc(n);
+    std::vector<int64_t> per_level(L, 0);
+    double rmaxl[kMaxLevels] = {0, 0, 0, 0};
+    for (int64_t i = 0; i < n; ++i) {
+        int l = 0;
+        while (l < L - 1 && 2.0 * radius[i] > hl[l]) ++l;
+        spc[i] = (uint32_t)species[i] | ((uint32_t)l << 16);
+        per_level[l]++;
+        rmaxl[l] = std::max(rmaxl[l], radius[i]);
+    }
+    Params& p = ctx->prm;
+    p = Params{};
+    p.n = (int)n;
+    p.maxn = ctx->maxn_cfg;
+    p.nwalls = (int)ctx->walls.size();
+    p.ns = ctx->ns;
+    p.levels = L;
+    p.nper = (int)ctx->periodic.size();
+    p.rmax = rmax;
+    p.skin = skin;
+    p.thr2 = (0.5 * skin) * (0.5 * skin) * 0.999;
+    for (int b = 0; b < p.nper; ++b) {
+        p.paxis[b] = ctx->periodic[b].axis;
+        p.pleft[b] = ctx->periodic[b].left;
+        p.pright[b] = ctx->periodic[b].right;
+        p.pshift[b] = ctx->periodic[b].shift;
+        if (!(p.pshift[b] > 2.0 * (2.0 * rmax + skin)))
+            return set_err(ctx, DEM_ERR_UNSUPPORTED, "periodic box must be wider than 2 * (largest diameter + skin)");
+    }
+    uint32_t off = 0;
+    for (int l = 0; l < L; ++l) {
+        if (rmaxl[l] == 0.0) rmaxl[l] = 0.5 * hl[l];
+        p.rmaxl[l] = rmaxl[l];
+        p.h[l] = 2.0 * rmaxl[l] + skin;
+        for (int a = 0; a < 3; ++a) {
+            p.ncell[l][a] = 0;
+            p.hc[l][a] = p.h[l];
+        }
+        for (int b = 0; b < p.nper; ++b) {
+            const int a = p.paxis[b];
+            int nc = (int)std::floor(p.pshift[b] / p.h[l]);
+            if (nc < 1) nc = 1;
+            p.ncell[l][a] = nc;
+            p.hc[l][a] = p.pshift[b] / nc;
+        }
+        p.nb[l] = next_pow2(std::max<int64_t>(64, 2 * per_level[l]));
+        p.nboff[l] = off;
+        off += p.nb[l];
+    }
+    ctx->nbuckets = off;
+    refresh_params(ctx);
+
+    // --- (re)allocate
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    free_all(ctx);
+    int r;
+    if ((r = dalloc(ctx, &ctx->d_flags, F_NUM)) != 0) return r;
+    if ((r = dalloc(ctx, &ctx->d_sp, ctx->species.size())) != 0) return r;
+    if ((r = dalloc(ctx, &ctx->d_walls, std::max<size_t>(ctx->walls.size(), 1))) != 0) return r;
+    HIPCHK(hipMemcpy(ctx->d_sp, ctx->species.data(), sizeof(dem_pair_params) * ctx->species.size(),
+                     hipMemcpyHostToDevice));
+    if (!ctx->walls.empty())
+        HIPCHK(hipMemcpy(ctx->d_walls, ctx->walls.data(), sizeof(dem_wall) * ctx->walls.size(), hipMemcpyHostToDevice));
+    for (const auto& wl : ctx->walls)
+        if (wl.species < 0 || wl.species >= ctx->ns) return set_err(ctx, DEM_ERR_ARG, "wall species out of range");
+    ctx->n = n;
+    r = ctx->ns_springs == 9 ? alloc_particles<9>(ctx, n) : alloc_particles<3>(ctx, n);
+    if (r) return r;
+
+    // --- state
+    std::vector<double> h4(4 * n);
+    auto put4 = [&](double4* dst, auto fill) -> hipError_t {
+        for (int64_t i = 0; i < n; ++i) fill(i, &h4[4 * i]);
+        return hipMemcpy(dst, h4.data(), sizeof(double) * 4 * n, hipMemcpyHostToDevice);
+    };
+    double4 *dposr, *dvel, *domg, *dquat, *dfrc, *dtrq;
+    uint32_t *did, *dorig, *dspc;
+    if (ctx->ns_springs == 9) {
+        dposr = ctx->d9.posr; dvel = ctx->d9.vel; domg = ctx->d9.omg; dquat = ctx->d9.quat;
+        dfrc = ctx->d9.frc; dtrq = ctx->d9.trq; did = ctx->d9.id; dorig = ctx->d9.orig; dspc = ctx->d9.spc;
+    } else {
+        dposr = ctx->d3.posr; dvel = ctx->d3.vel; domg = ctx->d3.omg; dquat = ctx->d3.quat;
+        dfrc = ctx->d3.frc; dtrq = ctx->d3.trq; did = ctx->d3.id; dorig = ctx->d3.orig; dspc = ctx->d3.spc;
+    }
+    HIPCHK(put4(dposr, [&](int64_t i, double* o) { o[0] = x[3 * i]; o[1] = x[3 * i + 1]; o[2] = x[3 * i + 2]; o[3] = radius[i]; }));
+    HIPCHK(put4(dvel, [&](int64_t i, double* o) { o[0] = v[3 * i]; o[1] = v[3 * i + 1]; o[2] = v[3 * i + 2]; o[3] = inv_mass[i]; }));
+    HIPCHK(put4(domg, [&](int64_t i, double* o) {
+        o[0] = w ? w[3 * i] : 0.0; o[1] = w ? w[3 * i + 1] : 0.0; o[2] = w ? w[3 * i + 2] : 0.0; o[3] = inv_inertia[6 * i];
+    }));
+    HIPCHK(put4(dquat, [&](int64_t i, double* o) {
+        if (q) { for (int k = 0; k < 4; ++k) o[k] = q[4 * i + k]; } else { o[0] = 1.0; o[1] = o[2] = o[3] = 0.0; }
+    }));
+    HIPCHK(put4(dfrc, [&](int64_t i, double* o) {
+        o[0] = f ? f[3 * i] : 0.0; o[1] = f ? f[3 * i + 1] : 0.0; o[2] = f ? f[3 * i + 2] : 0.0; o[3] = 0.0;
+    }));
+    HIPCHK(put4(dtrq, [&](int64_t i, double* o) {
+        o[0] = t ? t[3 * i] : 0.0; o[1] = t ? t[3 * i + 1] : 0.0; o[2] = t ? t[3 * i + 2] : 0.0; o[3] = 0.0;
+    }));
+    std::vector<uint32_t> orig(n);
+    for (int64_t i = 0; i < n; ++i) orig[i] = (uint32_t)i;
+    {
+        const uint32_t nn = (uint32_t)n;
+        uint32_t* st = ctx->ns_springs == 9 ? ctx->d9.start : ctx->d3.start;
+        HIPCHK(hipMemcpy(st + ctx->nbuckets, &nn, sizeof nn, hipMemcpyHostToDevice));
+    }
+    HIPCHK(hipMemcpy(did, id, sizeof(uint32_t) * n, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(dorig, orig.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(dspc, spc.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice));
+    int flags[F_NUM] = {0};
+    flags[F_REBUILD] = 1;
+    HIPCHK(hipMemcpy(ctx->d_flags, flags, sizeof flags, hipMemcpyHostToDevice));
+    if (ctx->ns_springs == 9) {
+        hipLaunchKernelGGL(k_zero_masks<9>, dim3(grid_for(n)), dim3(kBlock), 0, ctx->stream, ctx->d9, p);
+        hipLaunchKernelGGL(k_hash_clear_all<9>, dim3(grid_for(ctx->d9.hcap)), dim3(kBlock), 0, ctx->stream, ctx->d9);
+    } else {
+        hipLaunchKernelGGL(k_zero_masks<3>, dim3(grid_for(n)), dim3(kBlock), 0, ctx->stream, ctx->d3, p);
+        hipLaunchKernelGGL(k_hash_clear_all<3>, dim3(grid_for(ctx->d3.hcap)), dim3(kBlock), 0, ctx->stream, ctx->d3);
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    ctx->cur = 0;
+    ctx->uploaded = true;
+    return DEM_OK;
+}
+
+int dem_upload_contacts(dem_ctx* ctx, int64_t n, const uint32_t* p_id, const uint32_t* i_id, const double* spring,
+                        const uint32_t* ts)
+{
+    if (!ctx || n < 0) return DEM_ERR_ARG;
+    if (!ctx->uploaded) return set_err(ctx, DEM_ERR_STATE, "upload particles before contacts");
+    if (n == 0) return DEM_OK;
+    if (!p_id || !i_id || !spring || !ts) return set_err(ctx, DEM_ERR_ARG, "upload_contacts: null array");
+    for (int64_t k = 0; k < n; ++k) {
+        if (i_id[k] & DEM_WALL_KEY) {
+            if ((int)(i_id[k] & ~DEM_WALL_KEY) >= (int)ctx->walls.size())
+                return set_err(ctx, DEM_ERR_ARG, "upload_contacts: wall index out of range");
+        } else if (p_id[k] >= i_id[k]) {
+            return set_err(ctx, DEM_ERR_ARG, "upload_contacts: p_id must be the lower particle id");
+        }
+    }
+    HIPCHK(hipSetDevice(ctx->device));
+    uint32_t *dp = nullptr, *di = nullptr, *dt = nullptr;
+    double* ds = nullptr;
+    HIPCHK(hipMalloc(&dp, sizeof(uint32_t) * n));
+    HIPCHK(hipMalloc(&di, sizeof(uint32_t) * n));
+    HIPCHK(hipMalloc(&dt, sizeof(uint32_t) * n));
+    HIPCHK(hipMalloc(&ds, sizeof(double) * 9 * n));
+    HIPCHK(hipMemcpy(dp, p_id, sizeof(uint32_t) * n, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(di, i_id, sizeof(uint32_t) * n, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(dt, ts, sizeof(uint32_t) * n, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ds, spring, sizeof(double) * 9 * n, hipMemcpyHostToDevice));
+    // the pending rebuild stores the active records into the hash table
+    // first; store the old ones now, then add the uploaded ones
+    int one = 1;
+    HIPCHK(hipMemcpy(ctx->d_flags + F_REBUILD, &one, sizeof one, hipMemcpyHostToDevice));
+    if (ctx->ns_springs == 9) {
+        hipLaunchKernelGGL(k_hash_upload<9>, dim3(grid_for(n)), dim3(kBlock), 0, ctx->stream, ctx->d9, (int)n, dp, di, ds, dt);
+    } else {
+        hipLaunchKernelGGL(k_hash_upload<3>, dim3(grid_for(n)), dim3(kBlock), 0, ctx->stream, ctx->d3, (int)n, dp, di, ds, dt);
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    (void)hipFree(dp);
+    (void)hipFree(di);
+    (void)hipFree(dt);
+    (void)hipFree(ds);
+    return DEM_OK;
+}
+
+int dem_compute_forces(dem_ctx* ctx)
+{
+    int r = ready(ctx);
+    if (r) return r;
+    HIPCHK(hipSetDevice(ctx->device));
+    refresh_params(ctx);
+    return ctx->ns_springs == 9 ? run_forces<9>(ctx) : run_forces<3>(ctx);
+}
+
+int dem_step(dem_ctx* ctx, int64_t n_steps)
+{
+    int r = ready(ctx);
+    if (r) return r;
+    if (n_steps < 0) return set_err(ctx, DEM_ERR_ARG, "negative step count");
+    HIPCHK(hipSetDevice(ctx->device));
+    refresh_params(ctx);
+    return ctx->ns_springs == 9 ? run_steps<9>(ctx, n_steps) : run_steps<3>(ctx, n_steps);
+}
+
+int dem_download_particles(dem_ctx* ctx, double* x, double* v, double* w, double* q, double* f, double* t)
+{
+    if (!ctx) return DEM_ERR_ARG;
+    if (!ctx->uploaded) return set_err(ctx, DEM_ERR_STATE, "no particles uploaded");
+    HIPCHK(hipSetDevice(ctx->device));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    const int64_t n = ctx->n;
+    const bool nine = ctx->ns_springs == 9;
+    double4* src[6] = {nine ? ctx->d9.posr : ctx->d3.posr, nine ? ctx->d9.vel : ctx->d3.vel,
+                       nine ? ctx->d9.omg : ctx->d3.omg, nine ? ctx->d9.quat : ctx->d3.quat,
+                       nine ? ctx->d9.frc : ctx->d3.frc, nine ? ctx->d9.trq : ctx->d3.trq};
+    double* dst[6] = {x, v, w, q, f, t};
+    std::vector<uint32_t> orig(n);
+    HIPCHK(hipMemcpy(orig.data(), nine ? ctx->d9.orig : ctx->d3.orig, sizeof(uint32_t) * n, hipMemcpyDeviceToHost));
+    std::vector<double> h(4 * n);
+    for (int a = 0; a < 6; ++a) {
+        if (!dst[a]) continue;
+        HIPCHK(hipMemcpy(h.data(), src[a], sizeof(double) * 4 * n, hipMemcpyDeviceToHost));
+        const int nc = a == 3 ? 4 : 3;
+        for (int64_t i = 0; i < n; ++i)
+            for (int c = 0; c < nc; ++c) dst[a][(size_t)nc * orig[i] + c] = h[4 * i + c];
+    }
+    return DEM_OK;
+}
+
+int dem_contact_count(dem_ctx* ctx, int64_t* n)
+{
+    if (!ctx || !n) return DEM_ERR_ARG;
+    if (!ctx->uploaded) {
+        *n = 0;
+        return DEM_OK;
+    }
+    HIPCHK(hipSetDevice(ctx->device));
+    int zero[2] = {0, 0};
+    HIPCHK(hipMemcpyAsync(ctx->d_flags + F_COUNT_A, zero, sizeof zero, hipMemcpyHostToDevice, ctx->stream));
+    if (ctx->ns_springs == 9)
+        hipLaunchKernelGGL(k_count_contacts<9>, dim3(grid_for(ctx->n)), dim3(kBlock), 0, ctx->stream, ctx->d9, ctx->prm, ctx->cur);
+    else
+        hipLaunchKernelGGL(k_count_contacts<3>, dim3(grid_for(ctx->n)), dim3(kBlock), 0, ctx->stream, ctx->d3, ctx->prm, ctx->cur);
+    int c[2];
+    HIPCHK(hipMemcpyAsync(c, ctx->d_flags + F_COUNT_A, sizeof c, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    *n = (int64_t)c[0] + c[1];
+    return DEM_OK;
+}
+
+int dem_download_contacts(dem_ctx* ctx, int64_t capacity, int64_t* n_out, uint32_t* p_id, uint32_t* i_id,
+                          double* spring, uint32_t* time_stamp)
+{
+    if (!ctx || !n_out || capacity < 0) return DEM_ERR_ARG;
+    int64_t total = 0;
+    int r = dem_contact_count(ctx, &total);
+    if (r) return r;
+    if (total > capacity) return set_err(ctx, DEM_ERR_CAPACITY, "download_contacts: capacity too small");
+    *n_out = total;
+    if (total == 0) return DEM_OK;
+    uint32_t *dp = nullptr, *di = nullptr, *dt = nullptr;
+    double* ds = nullptr;
+    int* dc = nullptr;
+    HIPCHK(hipMalloc(&dp, sizeof(uint32_t) * total));
+    HIPCHK(hipMalloc(&di, sizeof(uint32_t) * total));
+    HIPCHK(hipMalloc(&dt, sizeof(uint32_t) * total));
+    HIPCHK(hipMalloc(&ds, sizeof(double) * 9 * total));
+    HIPCHK(hipMalloc(&dc, sizeof(int)));
+    HIPCHK(hipMemsetAsync(dc, 0, sizeof(int), ctx->stream));
+    if (ctx->ns_springs == 9)
+        hipLaunchKernelGGL(k_collect_contacts<9>, dim3(grid_for(ctx->n)), dim3(kBlock), 0, ctx->stream, ctx->d9, ctx->prm,
+                           ctx->cur, dp, di, ds, dt, dc, (int)total);
+    else
+        hipLaunchKernelGGL(k_collect_contacts<3>, dim3(grid_for(ctx->n)), dim3(kBlock), 0, ctx->stream, ctx->d3, ctx->prm,
+                           ctx->cur, dp, di, ds, dt, dc, (int)total);
+    std::vector<uint32_t> hp(total), hi(total), ht(total);
+    std::vector<double> hs(9 * total);
+    HIPCHK(hipMemcpyAsync(hp.data(), dp, sizeof(uint32_t) * total, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(hi.data(), di, sizeof(uint32_t) * total, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(ht.data(), dt, sizeof(uint32_t) * total, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(hs.data(), ds, sizeof(double) * 9 * total, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    (void)hipFree(dp);
+    (void)hipFree(di);
+    (void)hipFree(dt);
+    (void)hipFree(ds);
+    (void)hipFree(dc);
+    std::vector<int64_t> order(total);
+    for (int64_t k = 0; k < total; ++k) order[k] = k;
+    std::sort(order.begin(), order.end(), [&](int64_t a, int64_t b) {
+        return hp[a] != hp[b] ? hp[a] < hp[b] : hi[a] < hi[b];
+    });
+    for (int64_t k = 0; k < total; ++k) {
+        const int64_t o = order[k];
+        if (p_id) p_id[k] = hp[o];
+        if (i_id) i_id[k] = hi[o];
+        if (time_stamp) time_stamp[k] = ht[o];
+        if (spring)
+            for (int a = 0; a < 9; ++a) spring[9 * k + a] = hs[9 * o + a];
+    }
+    return DEM_OK;
+}
+
+int dem_get_stats(dem_ctx* ctx, dem_stats* s)
+{
+    if (!ctx || !s) return DEM_ERR_ARG;
+    std::memset(s, 0, sizeof *s);
+    s->n_particles = ctx->n;
+    s->n_steps = ctx->nt;
+    s->time = ctx->time;
+    if (!ctx->uploaded) return DEM_OK;
+    int r = check_flags(ctx);
+    if (r) return r;
+    s->n_rebuilds = ctx->last_rebuilds;
+    s->max_list_length = ctx->max_list;
+    s->grid_levels = ctx->prm.levels;
+    s->skin = ctx->prm.skin;
+    for (int l = 0; l < ctx->prm.levels; ++l) s->cell_size[l] = ctx->prm.h[l];
+    int64_t nc = 0;
+    r = dem_contact_count(ctx, &nc);
+    if (r) return r;
+    s->n_contacts = nc;
+    int c[2];
+    HIPCHK(hipMemcpy(c, ctx->d_flags + F_COUNT_A, sizeof c, hipMemcpyDeviceToHost));
+    s->n_wall_contacts = c[1];
+    // list entries
+    std::vector<int32_t> cnt(ctx->n);
+    HIPCHK(hipMemcpy(cnt.data(), ctx->ns_springs == 9 ? ctx->d9.ncount : ctx->d3.ncount, sizeof(int32_t) * ctx->n,
+                     hipMemcpyDeviceToHost));
+    int64_t tot = 0;
+    for (auto v : cnt) tot += v;
+    s->n_list_entries = tot;
+    return DEM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,305 @@
+"""ctypes binding of the DEM engine's C ABI (``include/mercurydpm_dem.h``).
+
+``Engine`` wraps one ``dem_ctx``.  The product library is
+``mercurydpm_fork_amd/lib/libmercurydpm_dem.so`` (HIP, gfx950); loading fails
+loudly when it has not been built -- there is no CPU fallback.
+
+``Engine`` accepts any library that exports the ABI's call sequence under a
+symbol prefix, so test code can drive a checker with the same calls; the
+package itself only ever loads the HIP library.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass, field
+from typing import Optional, Sequence
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_DIR = os.path.join(_HERE, "lib")
+HIP_LIB_PATH = os.path.join(LIB_DIR, "libmercurydpm_dem.so")
+
+DEM_WALL_KEY = 0x80000000
+
+NORMAL_LINEAR_VISCOELASTIC = 0
+NORMAL_HERTZIAN_VISCOELASTIC = 1
+FRICTION_NONE = 0
+FRICTION_SLIDING = 1
+FRICTION_FULL = 2
+FRICTION_MINDLIN = 3
+WALL_INFINITE = 0
+WALL_CYLINDER = 1
+
+ERRORS = {-1: "DEM_ERR_ARG", -2: "DEM_ERR_STATE", -3: "DEM_ERR_DEVICE", -4: "DEM_ERR_CAPACITY",
+          -5: "DEM_ERR_UNSUPPORTED"}
+
+
+class DemError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class PairParams(C.Structure):
+    _fields_ = [("normal_law", C.c_int32), ("friction_law", C.c_int32),
+                ("constant_restitution", C.c_int32), ("reserved", C.c_int32),
+                ("stiffness", C.c_double), ("dissipation", C.c_double),
+                ("elastic_modulus", C.c_double), ("sliding_stiffness", C.c_double),
+                ("sliding_dissipation", C.c_double), ("sliding_mu", C.c_double),
+                ("sliding_mu_static", C.c_double), ("shear_modulus", C.c_double),
+                ("rolling_stiffness", C.c_double), ("rolling_dissipation", C.c_double),
+                ("rolling_mu", C.c_double), ("rolling_mu_static", C.c_double),
+                ("torsion_stiffness", C.c_double), ("torsion_dissipation", C.c_double),
+                ("torsion_mu", C.c_double), ("torsion_mu_static", C.c_double),
+                ("interaction_distance", C.c_double)]
+
+
+class Wall(C.Structure):
+    _fields_ = [("type", C.c_int32), ("species", C.c_int32), ("id", C.c_uint32),
+                ("reserved", C.c_int32), ("position", C.c_double * 3),
+                ("dist_coef", C.c_double * 3), ("axis", C.c_double * 3), ("radius", C.c_double)]
+
+
+class Periodic(C.Structure):
+    _fields_ = [("axis", C.c_int32), ("reserved", C.c_int32), ("left", C.c_double),
+                ("right", C.c_double), ("shift", C.c_double)]
+
+
+class Config(C.Structure):
+    _fields_ = [("device", C.c_int32), ("max_neighbors", C.c_int32), ("skin", C.c_double),
+                ("grid_levels", C.c_int32), ("reserved", C.c_int32)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("n_particles", C.c_int64), ("n_steps", C.c_int64), ("n_rebuilds", C.c_int64),
+                ("n_contacts", C.c_int64), ("n_wall_contacts", C.c_int64),
+                ("n_list_entries", C.c_int64), ("max_list_length", C.c_int32),
+                ("grid_levels", C.c_int32), ("time", C.c_double), ("skin", C.c_double),
+                ("cell_size", C.c_double * 4)]
+
+
+_PTR = C.c_void_p
+_I64 = C.c_int64
+_I32 = C.c_int32
+
+
+def _declare(lib, prefix: str, has_config: bool):
+    def f(name, res, *args):
+        fn = getattr(lib, prefix + name)
+        fn.restype = res
+        fn.argtypes = list(args)
+        return fn
+
+    f("create", _I32, C.POINTER(_PTR), *( [C.POINTER(Config)] if has_config else []))
+    f("destroy", _I32, _PTR)
+    f("last_error", C.c_char_p, _PTR)
+    f("set_species", _I32, _PTR, _I32, _PTR)
+    f("set_walls", _I32, _PTR, _I32, _PTR)
+    f("set_periodic", _I32, _PTR, _I32, _PTR)
+    f("set_gravity", _I32, _PTR, _PTR)
+    f("set_background_drag", _I32, _PTR, C.c_double)
+    f("set_timestep", _I32, _PTR, C.c_double)
+    f("set_rotation", _I32, _PTR, _I32)
+    f("set_time", _I32, _PTR, C.c_double, _I64)
+    f("upload_particles", _I32, _PTR, _I64, *([_PTR] * 11))
+    f("upload_contacts", _I32, _PTR, _I64, _PTR, _PTR, _PTR, _PTR)
+    f("compute_forces", _I32, _PTR)
+    f("step", _I32, _PTR, _I64)
+    f("download_particles", _I32, _PTR, *([_PTR] * 6))
+    f("contact_count", _I32, _PTR, C.POINTER(_I64))
+    f("download_contacts", _I32, _PTR, _I64, C.POINTER(_I64), _PTR, _PTR, _PTR, _PTR)
+
+
+_hip_lib = None
+
+
+def load_hip_library(path: Optional[str] = None):
+    """Load the HIP engine; raises if it is missing (no fallback)."""
+    global _hip_lib
+    if _hip_lib is not None and path is None:
+        return _hip_lib
+    p = path or HIP_LIB_PATH
+    if not os.path.exists(p):
+        raise RuntimeError(
+            f"HIP engine library not found at {p}; run __graft_entry__.build() (hipcc, gfx950)")
+    lib = C.CDLL(p)
+    _declare(lib, "dem_", True)
+    lib.dem_get_stats.restype = _I32
+    lib.dem_get_stats.argtypes = [_PTR, C.POINTER(Stats)]
+    lib.dem_abi_version.restype = _I32
+    lib.dem_abi_version.argtypes = []
+    if path is None:
+        _hip_lib = lib
+    return lib
+
+
+def _ptr(a: Optional[np.ndarray]):
+    if a is None:
+        return None
+    return a.ctypes.data_as(C.c_void_p)
+
+
+@dataclass
+class State:
+    """Per-particle state in handler order (numpy arrays)."""
+    id: np.ndarray
+    species: np.ndarray
+    radius: np.ndarray
+    inv_mass: np.ndarray
+    inv_inertia: np.ndarray
+    x: np.ndarray
+    v: np.ndarray
+    w: np.ndarray
+    q: np.ndarray
+    f: Optional[np.ndarray] = None
+    t: Optional[np.ndarray] = None
+
+    @property
+    def n(self) -> int:
+        return int(self.id.shape[0])
+
+
+@dataclass
+class Contacts:
+    p_id: np.ndarray
+    i_id: np.ndarray
+    spring: np.ndarray  # (n, 9)
+    ts: np.ndarray
+
+    @property
+    def n(self) -> int:
+        return int(self.p_id.shape[0])
+
+    def keys(self):
+        return set(zip(self.p_id.tolist(), self.i_id.tolist()))
+
+
+class Engine:
+    """One simulation context behind the C ABI."""
+
+    def __init__(self, lib=None, prefix: str = "dem_", device: int = 0, max_neighbors: int = 0,
+                 skin: float = 0.0, grid_levels: int = 0):
+        self.lib = lib if lib is not None else load_hip_library()
+        self.prefix = prefix
+        self._has_stats = hasattr(self.lib, prefix + "get_stats") and prefix == "dem_"
+        h = _PTR()
+        if prefix == "dem_":
+            cfg = Config(device, max_neighbors, skin, grid_levels, 0)
+            rc = self._fn("create")(C.byref(h), C.byref(cfg))
+        else:
+            rc = self._fn("create")(C.byref(h))
+        if rc != 0:
+            raise DemError(rc, "create failed")
+        self.h = h
+        self.n = 0
+
+    def _fn(self, name):
+        return getattr(self.lib, self.prefix + name)
+
+    def _check(self, rc: int):
+        if rc != 0:
+            msg = self._fn("last_error")(self.h)
+            raise DemError(rc, msg.decode() if msg else "")
+
+    def close(self):
+        if getattr(self, "h", None):
+            self._fn("destroy")(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # ----------------------------------------------------------------- setup
+    def set_species(self, table: Sequence[PairParams], n_species: int):
+        arr = (PairParams * len(table))(*table)
+        self._check(self._fn("set_species")(self.h, n_species, C.cast(arr, C.c_void_p)))
+
+    def set_walls(self, walls: Sequence[Wall]):
+        arr = (Wall * max(1, len(walls)))(*walls)
+        self._check(self._fn("set_walls")(self.h, len(walls), C.cast(arr, C.c_void_p)))
+
+    def set_periodic(self, bnds: Sequence[Periodic]):
+        arr = (Periodic * max(1, len(bnds)))(*bnds)
+        self._check(self._fn("set_periodic")(self.h, len(bnds), C.cast(arr, C.c_void_p)))
+
+    def set_gravity(self, g):
+        a = np.ascontiguousarray(g, dtype=np.float64)
+        self._check(self._fn("set_gravity")(self.h, _ptr(a)))
+
+    def set_background_drag(self, d: float):
+        self._check(self._fn("set_background_drag")(self.h, float(d)))
+
+    def set_timestep(self, dt: float):
+        self._check(self._fn("set_timestep")(self.h, float(dt)))
+
+    def set_rotation(self, on: bool):
+        self._check(self._fn("set_rotation")(self.h, 1 if on else 0))
+
+    def set_time(self, t: float, nt: int):
+        self._check(self._fn("set_time")(self.h, float(t), int(nt)))
+
+    def upload(self, s: State):
+        def c(a, dt):
+            return None if a is None else np.ascontiguousarray(a, dtype=dt)
+        arrs = [c(s.id, np.uint32), c(s.species, np.int32), c(s.radius, np.float64),
+                c(s.inv_mass, np.float64), c(s.inv_inertia, np.float64), c(s.x, np.float64),
+                c(s.v, np.float64), c(s.w, np.float64), c(s.q, np.float64), c(s.f, np.float64),
+                c(s.t, np.float64)]
+        self._keep = arrs
+        self._check(self._fn("upload_particles")(self.h, s.n, *[_ptr(a) for a in arrs]))
+        self.n = s.n
+
+    def upload_contacts(self, ct: Contacts):
+        p = np.ascontiguousarray(ct.p_id, dtype=np.uint32)
+        i = np.ascontiguousarray(ct.i_id, dtype=np.uint32)
+        sp = np.ascontiguousarray(ct.spring, dtype=np.float64)
+        ts = np.ascontiguousarray(ct.ts, dtype=np.uint32)
+        self._check(self._fn("upload_contacts")(self.h, ct.n, _ptr(p), _ptr(i), _ptr(sp), _ptr(ts)))
+
+    # ------------------------------------------------------------------ run
+    def compute_forces(self):
+        self._check(self._fn("compute_forces")(self.h))
+
+    def step(self, n: int = 1):
+        self._check(self._fn("step")(self.h, int(n)))
+
+    # ------------------------------------------------------------- download
+    def download(self):
+        n = self.n
+        x = np.empty((n, 3)); v = np.empty((n, 3)); w = np.empty((n, 3))
+        q = np.empty((n, 4)); f = np.empty((n, 3)); t = np.empty((n, 3))
+        self._check(self._fn("download_particles")(self.h, _ptr(x), _ptr(v), _ptr(w), _ptr(q),
+                                                   _ptr(f), _ptr(t)))
+        return dict(x=x, v=v, w=w, q=q, f=f, t=t)
+
+    def contact_count(self) -> int:
+        n = _I64()
+        self._check(self._fn("contact_count")(self.h, C.byref(n)))
+        return int(n.value)
+
+    def contacts(self) -> Contacts:
+        cap = max(1, self.contact_count())
+        p = np.empty(cap, np.uint32); i = np.empty(cap, np.uint32)
+        sp = np.empty((cap, 9)); ts = np.empty(cap, np.uint32)
+        n = _I64()
+        self._check(self._fn("download_contacts")(self.h, cap, C.byref(n), _ptr(p), _ptr(i),
+                                                  _ptr(sp), _ptr(ts)))
+        k = int(n.value)
+        return Contacts(p[:k].copy(), i[:k].copy(), sp[:k].copy(), ts[:k].copy())
+
+    def stats(self) -> Stats:
+        s = Stats()
+        if self._has_stats:
+            self._check(self.lib.dem_get_stats(self.h, C.byref(s)))
+        return s

@@ -1,0 +1,1122 @@
+/*
+ * dem_oracle.c -- CPU oracle for the DEM time-step engine.  TEST
+ * INFRASTRUCTURE ONLY: loaded by tests/, by __graft_entry__.smoke() and by
+ * bench.py's cpu_baseline leg as the checker; never by the product library.
+ *
+ * A plain-C restatement of the reference's serial CPU path, keeping its object
+ * model rather than the GPU's: real particles followed by periodic ghost
+ * particles appended each step, an interaction pool with per-particle
+ * interaction lists, InteractionHandler::getInteraction lookups,
+ * removeObjectKeepingPeriodics remapping and eraseOldInteractions.  Every
+ * function cites the reference file:line it restates (paths relative to the
+ * MercuryDPM source tree).  Arithmetic follows the reference's operation order
+ * (Vec3D operators, Kernel/Math/Vector.h:127-250, Vector.cc:77-188); build
+ * with -ffp-contract=off so no multiply-add is fused.
+ *
+ * Broad phase: the reference enumerates candidate pairs with HGrid
+ * (Mercury3D.cc:73-355).  HGrid is complete for pairs with
+ * d^2 < (r1+r2)^2 and evaluates each such pair exactly once, so the oracle
+ * uses a hashed uniform grid (cell = largest diameter) that also visits each
+ * pair once; only the order of force summation differs from HGrid.
+ *
+ * Pinning (tests/test_oracle_golden.py): TangentialSpringUnitTest,
+ * PeriodicWallsUnitTest (inline gold, 1e-10), InclinedPlaneUnitTest closed
+ * form, HertzContactRestitutionUnitTest, MindlinSelfTest gold .fstat,
+ * free-flight closed form, two-body collision time / restitution closed forms,
+ * brute-force O(N^2) pair sets, shifted-copy invariance of periodic images,
+ * cylinder-vs-plane equivalence.
+ */
+#include "dem_oracle.h"
+
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+typedef struct { double X, Y, Z; } V3;
+
+static inline V3 v3(double x, double y, double z) { V3 r = {x, y, z}; return r; }
+static inline V3 vadd(V3 a, V3 b) { return v3(a.X + b.X, a.Y + b.Y, a.Z + b.Z); }
+static inline V3 vsub(V3 a, V3 b) { return v3(a.X - b.X, a.Y - b.Y, a.Z - b.Z); }
+static inline V3 vmul(V3 a, double s) { return v3(a.X * s, a.Y * s, a.Z * s); }
+static inline V3 vdiv(V3 a, double s) { return v3(a.X / s, a.Y / s, a.Z / s); }
+static inline V3 vneg(V3 a) { return v3(-a.X, -a.Y, -a.Z); }
+static inline double vdot(V3 a, V3 b) { return a.X * b.X + a.Y * b.Y + a.Z * b.Z; }
+static inline V3 vcross(V3 a, V3 b)
+{
+    return v3(a.Y * b.Z - a.Z * b.Y, a.Z * b.X - a.X * b.Z, a.X * b.Y - a.Y * b.X);
+}
+static inline double vlen2(V3 a) { return a.X * a.X + a.Y * a.Y + a.Z * a.Z; }
+static inline double vlen(V3 a) { return sqrt(vlen2(a)); }
+static inline double vcomp(V3 a, int k) { return k == 0 ? a.X : (k == 1 ? a.Y : a.Z); }
+
+typedef struct {
+    uint32_t id;
+    int32_t species;
+    double radius, invMass;
+    double invI[6]; /* XX XY XZ YY YZ ZZ */
+    V3 pos, vel, omega;
+    double q[4];
+    V3 force, torque;
+    double fscale, tscale; /* sum of |contact force| and |contact torque| (tolerance scale) */
+    int32_t from;   /* -1: real; else handler index of the real particle */
+    int32_t *inter; /* interaction pool indices (BaseInteractable::interactions_) */
+    int32_t nInter, capInter;
+} Particle;
+
+typedef struct {
+    int32_t alive;
+    int32_t P, I;   /* particle indices; I is a wall index when IisWall */
+    int32_t IisWall;
+    uint32_t ts;
+    const dem_pair_params* sp;
+    V3 normal, contact, force, torque, relVel;
+    double overlap, distance, absNormalForce, normalRelVel;
+    V3 slidingSpring, rollingSpring, torsionSpring;
+} Inter;
+
+struct oracle_ctx {
+    char err[512];
+    int nSpecies;
+    dem_pair_params* species;
+    int nWalls;
+    dem_wall* walls;
+    int nPeriodic;
+    dem_periodic periodic[3];
+    V3 gravity;
+    double drag, dt, time;
+    int rotation;
+    int64_t nt;
+    /* particle handler: nReal real particles followed by ghosts */
+    Particle* p;
+    int64_t nReal, nP, capP;
+    uint32_t nextId;
+    double largestRadius; /* ParticleHandler::getLargestParticle()->getMaxInteractionRadius() */
+    /* interaction handler */
+    Inter* in;
+    int64_t nIn, capIn;
+    int32_t* freeList;
+    int64_t nFree, capFree;
+    int64_t nAlive;
+    /* hashed grid scratch */
+    int64_t* head;
+    int64_t* next;
+    int64_t* cell; /* 3 ints per particle, packed */
+    int64_t nBuckets;
+};
+
+static int fail(oracle_ctx* c, int code, const char* msg)
+{
+    if (c) snprintf(c->err, sizeof c->err, "%s", msg);
+    return code;
+}
+
+/* ------------------------------------------------------------------ lists */
+
+static void listAdd(Particle* p, int32_t k)
+{
+    if (p->nInter == p->capInter) {
+        p->capInter = p->capInter ? 2 * p->capInter : 8;
+        p->inter = (int32_t*)realloc(p->inter, sizeof(int32_t) * (size_t)p->capInter);
+    }
+    p->inter[p->nInter++] = k;
+}
+
+static void listRemove(Particle* p, int32_t k)
+{
+    /* std::vector::erase(find(...)) keeps the order (BaseInteractable::removeInteraction) */
+    for (int32_t a = 0; a < p->nInter; ++a) {
+        if (p->inter[a] == k) {
+            memmove(p->inter + a, p->inter + a + 1, sizeof(int32_t) * (size_t)(p->nInter - a - 1));
+            p->nInter--;
+            return;
+        }
+    }
+}
+
+/* InteractionHandler::addObject (InteractionHandler.cc:87-95) */
+static int32_t interNew(oracle_ctx* c, int32_t P, int32_t I, int32_t IisWall)
+{
+    int32_t k;
+    if (c->nFree > 0) {
+        k = c->freeList[--c->nFree];
+    } else {
+        if (c->nIn == c->capIn) {
+            c->capIn = c->capIn ? 2 * c->capIn : 1024;
+            c->in = (Inter*)realloc(c->in, sizeof(Inter) * (size_t)c->capIn);
+        }
+        k = (int32_t)c->nIn++;
+    }
+    Inter* x = &c->in[k];
+    memset(x, 0, sizeof *x);
+    x->alive = 1;
+    x->P = P;
+    x->I = I;
+    x->IisWall = IisWall;
+    listAdd(&c->p[P], k);
+    if (!IisWall) listAdd(&c->p[I], k);
+    c->nAlive++;
+    return k;
+}
+
+/* BaseHandler::removeObject + BaseInteraction destructor (unlinks both ends) */
+static void interRemove(oracle_ctx* c, int32_t k)
+{
+    Inter* x = &c->in[k];
+    if (!x->alive) return;
+    listRemove(&c->p[x->P], k);
+    if (!x->IisWall) listRemove(&c->p[x->I], k);
+    x->alive = 0;
+    if (c->nFree == c->capFree) {
+        c->capFree = c->capFree ? 2 * c->capFree : 1024;
+        c->freeList = (int32_t*)realloc(c->freeList, sizeof(int32_t) * (size_t)c->capFree);
+    }
+    c->freeList[c->nFree++] = k;
+    c->nAlive--;
+}
+
+/* BaseInteraction::setI (BaseInteraction.cc:435-441) */
+static void interSetI(oracle_ctx* c, int32_t k, int32_t I)
+{
+    Inter* x = &c->in[k];
+    listRemove(&c->p[x->I], k);
+    x->I = I;
+    listAdd(&c->p[I], k);
+}
+
+/* InteractionHandler::getExistingInteraction (InteractionHandler.cc:102-112):
+ * scans P's list for an interaction whose I is the given object. */
+static int32_t getExisting(oracle_ctx* c, int32_t P, int32_t I, int32_t IisWall)
+{
+    const Particle* p = &c->p[P];
+    for (int32_t a = 0; a < p->nInter; ++a) {
+        const Inter* x = &c->in[p->inter[a]];
+        if (x->IisWall == IisWall && x->I == I) return p->inter[a];
+    }
+    return -1;
+}
+
+static const dem_pair_params* mixed(const oracle_ctx* c, int32_t a, int32_t b)
+{
+    return &c->species[a * c->nSpecies + b];
+}
+
+/* InteractionHandler::getInteraction (InteractionHandler.cc:146-176) */
+static int32_t getInteraction(oracle_ctx* c, int32_t P, int32_t I, int32_t IisWall, uint32_t ts)
+{
+    const dem_pair_params* sp =
+        IisWall ? mixed(c, c->p[P].species, c->walls[I].species) : mixed(c, c->p[P].species, c->p[I].species);
+    int32_t k = getExisting(c, P, I, IisWall);
+    if (k < 0) k = interNew(c, P, I, IisWall);
+    c->in[k].ts = ts;
+    c->in[k].sp = sp;
+    return k;
+}
+
+/* ------------------------------------------------------------- geometry */
+
+/* Quaternion::rotateInverseInertiaTensor (Quaternion.cc:677-703) */
+static void rotateInvI(const double* q, const double* I, double* o)
+{
+    const double q0 = q[0], q1 = q[1], q2 = q[2], q3 = q[3];
+    const double XX = I[0], XY = I[1], XZ = I[2], YY = I[3], YZ = I[4], ZZ = I[5];
+    double a = 1 - 2 * q2 * q2 - 2 * q3 * q3;
+    double b = 2 * q1 * q2 - 2 * q0 * q3;
+    double cc = 2 * q1 * q3 + 2 * q0 * q2;
+    double d = 2 * q1 * q2 + 2 * q0 * q3;
+    double e = 1 - 2 * q1 * q1 - 2 * q3 * q3;
+    double f = 2 * q2 * q3 - 2 * q0 * q1;
+    double g = 2 * q1 * q3 - 2 * q0 * q2;
+    double h = 2 * q2 * q3 + 2 * q0 * q1;
+    double i = 1 - 2 * q1 * q1 - 2 * q2 * q2;
+    o[0] = XX * a * a + 2 * XY * a * b + 2 * XZ * a * cc + YY * b * b + 2 * YZ * b * cc + ZZ * cc * cc;
+    o[1] = d * (XX * a + XY * b + XZ * cc) + e * (XY * a + YY * b + YZ * cc) + f * (XZ * a + YZ * b + ZZ * cc);
+    o[2] = g * (XX * a + XY * b + XZ * cc) + h * (XY * a + YY * b + YZ * cc) + i * (XZ * a + YZ * b + ZZ * cc);
+    o[3] = XX * d * d + 2 * XY * d * e + 2 * XZ * d * f + YY * e * e + 2 * YZ * e * f + ZZ * f * f;
+    o[4] = g * (XX * d + XY * e + XZ * f) + h * (XY * d + YY * e + YZ * f) + i * (XZ * d + YZ * e + ZZ * f);
+    o[5] = XX * g * g + 2 * XY * g * h + 2 * XZ * g * i + YY * h * h + 2 * YZ * h * i + ZZ * i * i;
+}
+
+/* operator*(MatrixSymmetric3D, Vec3D) (MatrixSymmetric.cc:132-137) */
+static V3 symMul(const double* A, V3 b)
+{
+    return v3(A[0] * b.X + A[1] * b.Y + A[2] * b.Z, A[1] * b.X + A[3] * b.Y + A[4] * b.Z,
+              A[2] * b.X + A[4] * b.Y + A[5] * b.Z);
+}
+
+/* BaseInteractable::rotate -> Quaternion::updateAngularDisplacement
+ * (Quaternion.cc:423-438, normalise 162-170) */
+static void rotateQ(double* q, V3 v)
+{
+    const double q0 = q[0], q1 = q[1], q2 = q[2], q3 = q[3];
+    double d0 = -q1 * v.X - q2 * v.Y - q3 * v.Z;
+    double d1 = q0 * v.X + q3 * v.Y - q2 * v.Z;
+    double d2 = -q3 * v.X + q0 * v.Y + q1 * v.Z;
+    double d3 = q2 * v.X - q1 * v.Y + q0 * v.Z;
+    q[0] += d0 * 0.5;
+    q[1] += d1 * 0.5;
+    q[2] += d2 * 0.5;
+    q[3] += d3 * 0.5;
+    double l2 = q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3];
+    double l = sqrt(l2);
+    q[0] /= l;
+    q[1] /= l;
+    q[2] /= l;
+    q[3] /= l;
+}
+
+/* BaseInteractable::getVelocityAtContact (BaseInteractable.cc:375-378) */
+static V3 velAtContact(V3 vel, V3 pos, V3 omega, V3 c) { return vsub(vel, vcross(vsub(c, pos), omega)); }
+
+/* PeriodicBoundary::getDistance (PeriodicBoundary.cc:207-212) for an
+ * axis-aligned normal: dot(position, normal) = the axis component. */
+static double periodicDistance(const dem_periodic* b, V3 pos)
+{
+    double d = vdot(pos, v3(b->axis == 0 ? 1.0 : 0.0, b->axis == 1 ? 1.0 : 0.0, b->axis == 2 ? 1.0 : 0.0));
+    double l = d - b->left, r = b->right - d;
+    return l < r ? l : r;
+}
+
+/* PeriodicBoundary::shiftPosition (PeriodicBoundary.cc:219-229) with
+ * isClosestToLeftBoundary (286-290); move(shift) adds shift_ (all three
+ * components, two of which are zero). */
+static void periodicShift(const dem_periodic* b, V3* pos)
+{
+    V3 n = v3(b->axis == 0 ? 1.0 : 0.0, b->axis == 1 ? 1.0 : 0.0, b->axis == 2 ? 1.0 : 0.0);
+    V3 s = vmul(n, b->shift);
+    double d = vdot(*pos, n);
+    if (b->right - d > d - b->left)
+        *pos = vadd(*pos, s);
+    else
+        *pos = vadd(*pos, vneg(s));
+}
+
+/* ----------------------------------------------------------- force laws */
+
+static double effRadius(const oracle_ctx* c, const Inter* x)
+{
+    /* BaseInteraction::getEffectiveRadius (BaseInteraction.cc:793-799):
+     * curvature 1/r for particles, 0 for walls (BaseInteractable.h:414) */
+    double inv = 1.0 / c->p[x->P].radius + (x->IisWall ? 0.0 : 1.0 / c->p[x->I].radius);
+    return 1.0 / inv;
+}
+
+static double effMass(const oracle_ctx* c, const Inter* x)
+{
+    /* BaseInteraction::getEffectiveMass (BaseInteraction.cc:810-816) */
+    double inv = c->p[x->P].invMass + (x->IisWall ? 0.0 : c->p[x->I].invMass);
+    return 1.0 / inv;
+}
+
+static void partnerKinematics(const oracle_ctx* c, const Inter* x, V3* vel, V3* pos, V3* omega)
+{
+    if (x->IisWall) {
+        /* static walls: zero velocity, position p0 (origin for a cylinder) */
+        const dem_wall* w = &c->walls[x->I];
+        *vel = v3(0.0, 0.0, 0.0);
+        *omega = v3(0.0, 0.0, 0.0);
+        *pos = w->type == DEM_WALL_INFINITE ? v3(w->position[0], w->position[1], w->position[2]) : v3(0.0, 0.0, 0.0);
+    } else {
+        const Particle* q = &c->p[x->I];
+        *vel = q->vel;
+        *pos = q->pos;
+        *omega = q->omega;
+    }
+}
+
+/* LinearViscoelasticInteraction::computeNormalForce (LinearViscoelasticInteraction.cc:105-132)
+ * HertzianViscoelasticInteraction::computeNormalForce (HertzianViscoelasticInteraction.cc:106-141) */
+static void normalForce(const oracle_ctx* c, Inter* x)
+{
+    const Particle* P = &c->p[x->P];
+    V3 iv, ip, iw;
+    partnerKinematics(c, x, &iv, &ip, &iw);
+    x->relVel = vsub(velAtContact(P->vel, P->pos, P->omega, x->contact), velAtContact(iv, ip, iw, x->contact));
+    x->normalRelVel = vdot(x->relVel, x->normal);
+    if (x->overlap > 0) {
+        double fn;
+        if (x->sp->normal_law == DEM_NORMAL_LINEAR_VISCOELASTIC) {
+            fn = x->sp->stiffness * x->overlap - x->sp->dissipation * x->normalRelVel;
+            if (x->sp->constant_restitution) fn *= 2.0 * effMass(c, x);
+        } else {
+            double stiffness = 4. / 3. * x->sp->elastic_modulus * sqrt(effRadius(c, x) * x->overlap);
+            double dc = x->sp->dissipation * sqrt(effMass(c, x) * stiffness);
+            fn = stiffness * x->overlap - dc * x->normalRelVel;
+        }
+        x->absNormalForce = fabs(fn);
+        x->force = vmul(x->normal, fn);
+        x->torque = v3(0.0, 0.0, 0.0);
+    } else {
+        x->absNormalForce = 0.0;
+        x->force = v3(0.0, 0.0, 0.0);
+        x->torque = v3(0.0, 0.0, 0.0);
+    }
+}
+
+/* SlidingFrictionInteraction::computeFrictionForce + computeSlidingSpring
+ * (SlidingFrictionInteraction.cc:106-196) */
+static void slidingFriction(oracle_ctx* c, Inter* x)
+{
+    const dem_pair_params* s = x->sp;
+    if (x->absNormalForce == 0.0) return;
+    if (s->sliding_mu != 0.0) {
+        V3 vt = vsub(x->relVel, vmul(x->normal, x->normalRelVel));
+        if (s->sliding_stiffness != 0.0) {
+            V3 ssv;
+            if (x->IisWall) {
+                ssv = vt;
+            } else {
+                V3 dv = vsub(c->p[x->P].vel, c->p[x->I].vel);
+                ssv = vsub(vt, vdiv(vmul(x->normal, vdot(x->slidingSpring, dv)), x->distance));
+            }
+            x->slidingSpring = vadd(x->slidingSpring, vmul(ssv, c->dt));
+            V3 ft = vsub(vmul(x->slidingSpring, -s->sliding_stiffness), vmul(vt, s->sliding_dissipation));
+            if (s->constant_restitution) ft = vmul(ft, 2.0 * effMass(c, x));
+            double ftsq = vlen2(ft);
+            double lim = s->sliding_mu_static * x->absNormalForce;
+            if (ftsq <= lim * lim) {
+                x->force = vadd(x->force, ft);
+            } else {
+                ft = vmul(ft, s->sliding_mu * x->absNormalForce / sqrt(ftsq));
+                x->force = vadd(x->force, ft);
+                x->slidingSpring =
+                    vdiv(vneg(vadd(ft, vmul(vt, s->sliding_dissipation))), s->sliding_stiffness);
+            }
+        } else {
+            double vtsq = vlen2(vt);
+            double lim = s->sliding_mu_static * x->absNormalForce;
+            V3 ft;
+            if (vtsq * (s->sliding_dissipation * s->sliding_dissipation) <= lim * lim)
+                ft = vmul(vt, -s->sliding_dissipation);
+            else
+                ft = vmul(vt, -(s->sliding_mu * x->absNormalForce / sqrt(vtsq)));
+            x->force = vadd(x->force, ft);
+        }
+    }
+}
+
+/* FrictionInteraction::computeFrictionForce (FrictionInteraction.cc:108-222) */
+static void fullFriction(oracle_ctx* c, Inter* x)
+{
+    slidingFriction(c, x);
+    const dem_pair_params* s = x->sp;
+    if (x->absNormalForce == 0.0) return;
+    const Particle* P = &c->p[x->P];
+    V3 iv, ip, iw;
+    partnerKinematics(c, x, &iv, &ip, &iw);
+    if (s->rolling_mu != 0.0 && s->rolling_stiffness != 0.0) {
+        double effD = 2.0 * effRadius(c, x);
+        V3 rrv = vmul(vcross(x->normal, vsub(P->omega, iw)), -effD);
+        V3 rsv;
+        if (x->IisWall) {
+            rsv = rrv;
+        } else {
+            V3 relv = vsub(P->vel, iv);
+            rsv = vsub(rrv, vmul(x->normal, vdot(x->rollingSpring, relv) / x->distance));
+        }
+        x->rollingSpring = vadd(x->rollingSpring, vmul(rsv, c->dt));
+        V3 rf = vsub(vmul(x->rollingSpring, -s->rolling_stiffness), vmul(rrv, s->rolling_dissipation));
+        if (s->constant_restitution) rf = vmul(rf, 2.0 * effMass(c, x));
+        double rfsq = vlen2(rf);
+        double lim = s->rolling_mu_static * x->absNormalForce;
+        if (!(rfsq <= lim * lim)) {
+            rf = vmul(rf, s->rolling_mu * x->absNormalForce / sqrt(rfsq));
+            x->rollingSpring = vdiv(vneg(vadd(rf, vmul(rrv, s->rolling_dissipation))), s->rolling_stiffness);
+        }
+        x->torque = vadd(x->torque, vmul(vcross(x->normal, rf), effD));
+    }
+    if (s->torsion_mu != 0.0 && s->torsion_stiffness != 0.0) {
+        double effD = 2.0 * effRadius(c, x);
+        V3 trv = vmul(x->normal, effD * vdot(x->normal, vsub(P->omega, iw)));
+        V3 tsv = trv;
+        x->torsionSpring = vmul(x->normal, vdot(vadd(x->torsionSpring, vmul(tsv, c->dt)), x->normal));
+        V3 tf = vsub(vmul(x->torsionSpring, -s->torsion_stiffness), vmul(trv, s->torsion_dissipation));
+        if (s->constant_restitution) tf = vmul(tf, 2.0 * effMass(c, x));
+        double tfsq = vlen2(tf);
+        double lim = s->torsion_mu_static * x->absNormalForce;
+        if (!(tfsq <= lim * lim)) {
+            tf = vmul(tf, s->torsion_mu * x->absNormalForce / sqrt(tfsq));
+            x->torsionSpring = vdiv(vneg(vadd(tf, vmul(trv, s->torsion_dissipation))), s->torsion_stiffness);
+        }
+        x->torque = vadd(x->torque, vmul(tf, effD));
+    }
+}
+
+/* MindlinInteraction::computeFrictionForce (MindlinInteraction.cc:279-344),
+ * updateTangentialStiffnessZero (212-216) */
+static void mindlinFriction(oracle_ctx* c, Inter* x)
+{
+    const dem_pair_params* s = x->sp;
+    if (x->absNormalForce == 0.0) return;
+    if (s->sliding_mu != 0.0) {
+        V3 vt = vsub(x->relVel, vmul(x->normal, x->normalRelVel));
+        double springLength = vlen(x->slidingSpring);
+        if (springLength > 1e-10) {
+            x->slidingSpring = vsub(x->slidingSpring, vmul(x->normal, vdot(x->slidingSpring, x->normal)));
+            x->slidingSpring = vmul(x->slidingSpring, springLength / vlen(x->slidingSpring));
+        }
+        x->slidingSpring = vadd(x->slidingSpring, vmul(vt, c->dt));
+        double r1 = 1.0 * effRadius(c, x);
+        double kt0 = 8 * s->shear_modulus * sqrt(r1 * x->overlap);
+        double sdc = s->sliding_dissipation * sqrt(effMass(c, x) * kt0);
+        V3 ft = vmul(x->slidingSpring, -2.0 / 3.0 * kt0);
+        if (vlen(ft) < s->sliding_mu * x->absNormalForce) {
+            ft = vsub(ft, vmul(vt, sdc));
+            x->force = vadd(x->force, ft);
+        } else {
+            ft = vmul(ft, s->sliding_mu * x->absNormalForce / vlen(ft));
+            x->force = vadd(x->force, ft);
+            x->slidingSpring = vneg(vdiv(ft, 2.0 / 3.0 * kt0));
+        }
+    }
+}
+
+/* Interaction<Normal,Friction,Adhesive>::computeForce (Interaction.h:309-316) */
+static void computeForce(oracle_ctx* c, Inter* x)
+{
+    normalForce(c, x);
+    switch (x->sp->friction_law) {
+    case DEM_FRICTION_SLIDING: slidingFriction(c, x); break;
+    case DEM_FRICTION_FULL: fullFriction(c, x); break;
+    case DEM_FRICTION_MINDLIN: mindlinFriction(c, x); break;
+    default: break;
+    }
+}
+
+/* reverseHistory (SlidingFrictionInteraction.cc:293-298,
+ * FrictionInteraction.cc:270-277, MindlinInteraction.cc:415-420): sliding and
+ * torsion springs change sign, the rolling spring does not. */
+static void reverseHistory(Inter* x)
+{
+    x->slidingSpring = vneg(x->slidingSpring);
+    x->torsionSpring = vneg(x->torsionSpring);
+}
+
+/* ----------------------------------------------------- per-pair drivers */
+
+static inline int isFixed(const Particle* p) { return p->invMass == 0.0; }
+
+/* DPMBase::computeInternalForce (DPMBase.cc:3113-3162) with
+ * BaseParticle::getInteractionWith (BaseParticle.cc:688-710) */
+static void computeInternalForce(oracle_ctx* c, int64_t a, int64_t b)
+{
+    Particle* P1 = &c->p[a];
+    Particle* P2 = &c->p[b];
+    if (isFixed(P1) && isFixed(P2)) return;
+    if (P1->from >= 0 && P2->from >= 0) return;
+    int64_t iI, iJ;
+    if (P1->id > P2->id) { iI = b; iJ = a; } else { iI = a; iJ = b; }
+    Particle* PI = &c->p[iI];
+    Particle* PJ = &c->p[iJ];
+    V3 branch = vsub(PI->pos, PJ->pos);
+    double d2 = vlen2(branch);
+    double sumR = PJ->radius + PI->radius + mixed(c, PJ->species, PI->species)->interaction_distance;
+    if (d2 >= sumR * sumR) return;
+    int32_t k = getInteraction(c, (int32_t)iI, (int32_t)iJ, 0, (uint32_t)(c->nt + 1));
+    PI = &c->p[iI];
+    PJ = &c->p[iJ];
+    Inter* x = &c->in[k];
+    double distance = sqrt(d2);
+    x->normal = vdiv(branch, distance);
+    x->overlap = PI->radius + PJ->radius - distance;
+    x->distance = distance;
+    x->contact = vsub(PI->pos, vmul(x->normal, PI->radius - 0.5 * x->overlap));
+    computeForce(c, x);
+    PI->force = vadd(PI->force, x->force);
+    PJ->force = vadd(PJ->force, vneg(x->force));
+    PI->fscale += vlen(x->force);
+    PJ->fscale += vlen(x->force);
+    if (c->rotation) {
+        V3 tI = vsub(x->torque, vcross(vsub(PI->pos, x->contact), x->force));
+        V3 tJ = vadd(vneg(x->torque), vcross(vsub(PJ->pos, x->contact), x->force));
+        PI->torque = vadd(PI->torque, tI);
+        PJ->torque = vadd(PJ->torque, tJ);
+        PI->tscale += vlen(tI);
+        PJ->tscale += vlen(tJ);
+    }
+}
+
+/* InfiniteWall::getDistanceAndNormal (InfiniteWall.cc:151-175, distance via
+ * Quaternion::getDistance, Quaternion.cc:658-669); CylindricalWall::
+ * getDistanceAndNormal (CylindricalWall.cc:72-83). */
+static int wallDistanceAndNormal(const dem_wall* w, const Particle* p, double wallRadius, double* dist, V3* n)
+{
+    if (w->type == DEM_WALL_INFINITE) {
+        double d = w->dist_coef[0] * (w->position[0] - p->pos.X) +
+                   (w->dist_coef[1] * (w->position[1] - p->pos.Y) + w->dist_coef[2] * (w->position[2] - p->pos.Z));
+        *dist = d;
+        if (d >= wallRadius) return 0;
+        *n = v3(w->axis[0], w->axis[1], w->axis[2]);
+        return 1;
+    }
+    double R = sqrt(p->pos.X * p->pos.X + p->pos.Y * p->pos.Y);
+    double d = w->radius - R;
+    *dist = d;
+    if (d >= p->radius) return 0;
+    V3 nr = v3(p->pos.X, p->pos.Y, 0.0);
+    *n = vdiv(nr, vlen(nr));
+    return 1;
+}
+
+/* DPMBase::computeForcesDueToWalls (DPMBase.cc:3188-3215) with
+ * BaseWall::getInteractionWith (BaseWall.cc:367-430) */
+static void computeForcesDueToWall(oracle_ctx* c, int64_t pi, int32_t wi)
+{
+    Particle* p = &c->p[pi];
+    if (p->from >= 0) return;
+    const dem_wall* w = &c->walls[wi];
+    double wallRadius = p->radius + mixed(c, p->species, w->species)->interaction_distance;
+    double distance;
+    V3 normal;
+    if (!wallDistanceAndNormal(w, p, wallRadius, &distance, &normal)) return;
+    double overlap = p->radius - distance;
+    int32_t k = getInteraction(c, (int32_t)pi, wi, 1, (uint32_t)(c->nt + 1));
+    p = &c->p[pi];
+    Inter* x = &c->in[k];
+    x->normal = vneg(normal);
+    x->distance = distance;
+    x->overlap = overlap;
+    x->contact = vsub(p->pos, vmul(x->normal, p->radius - 0.5 * x->overlap));
+    computeForce(c, x);
+    p->force = vadd(p->force, x->force);
+    p->fscale += vlen(x->force);
+    if (c->rotation) {
+        V3 t = vsub(x->torque, vcross(vsub(p->pos, x->contact), x->force));
+        p->torque = vadd(p->torque, t);
+        p->tscale += vlen(t);
+    }
+}
+
+/* DPMBase::computeExternalForces (DPMBase.cc:3168-3179) */
+static void computeExternalForces(oracle_ctx* c, Particle* p)
+{
+    if (!isFixed(p)) {
+        double mass = 1.0 / p->invMass;
+        V3 fe = vsub(vmul(c->gravity, mass), vmul(p->vel, c->drag));
+        p->force = vadd(p->force, fe);
+        p->fscale += vlen(fe);
+    }
+}
+
+/* ------------------------------------------------------------ broad phase */
+
+static uint64_t hashCell(int64_t x, int64_t y, int64_t z)
+{
+    uint64_t h = (uint64_t)x * 73856093ull ^ (uint64_t)y * 19349663ull ^ (uint64_t)z * 83492791ull;
+    return h;
+}
+
+/* Mercury3D::computeInternalForces over the whole handler: every pair with
+ * d^2 < (r1+r2)^2 is passed to computeInternalForce exactly once (in the
+ * pass of its lower handler index), followed by computeExternalForces
+ * (DPMBase.cc:3421-3431). */
+static void pairLoop(oracle_ctx* c)
+{
+    double h = 2.0 * c->largestRadius;
+    if (!(h > 0)) h = 1.0;
+    int64_t nb = 1;
+    while (nb < 2 * c->nP) nb <<= 1;
+    if (nb != c->nBuckets) {
+        free(c->head);
+        c->head = (int64_t*)malloc(sizeof(int64_t) * (size_t)nb);
+        c->nBuckets = nb;
+    }
+    for (int64_t b = 0; b < nb; ++b) c->head[b] = -1;
+    c->next = (int64_t*)realloc(c->next, sizeof(int64_t) * (size_t)(c->nP > 0 ? c->nP : 1));
+    c->cell = (int64_t*)realloc(c->cell, sizeof(int64_t) * 3 * (size_t)(c->nP > 0 ? c->nP : 1));
+    for (int64_t i = c->nP - 1; i >= 0; --i) {
+        int64_t cx = (int64_t)floor(c->p[i].pos.X / h);
+        int64_t cy = (int64_t)floor(c->p[i].pos.Y / h);
+        int64_t cz = (int64_t)floor(c->p[i].pos.Z / h);
+        c->cell[3 * i] = cx;
+        c->cell[3 * i + 1] = cy;
+        c->cell[3 * i + 2] = cz;
+        uint64_t b = hashCell(cx, cy, cz) & (uint64_t)(nb - 1);
+        c->next[i] = c->head[b];
+        c->head[b] = i;
+    }
+    for (int64_t i = 0; i < c->nP; ++i) {
+        int64_t cx = c->cell[3 * i], cy = c->cell[3 * i + 1], cz = c->cell[3 * i + 2];
+        for (int dz = -1; dz <= 1; ++dz)
+            for (int dy = -1; dy <= 1; ++dy)
+                for (int dx = -1; dx <= 1; ++dx) {
+                    int64_t tx = cx + dx, ty = cy + dy, tz = cz + dz;
+                    uint64_t b = hashCell(tx, ty, tz) & (uint64_t)(nb - 1);
+                    for (int64_t j = c->head[b]; j >= 0; j = c->next[j]) {
+                        if (j <= i) continue;
+                        if (c->cell[3 * j] != tx || c->cell[3 * j + 1] != ty || c->cell[3 * j + 2] != tz) continue;
+                        computeInternalForce(c, i, j);
+                    }
+                }
+        computeExternalForces(c, &c->p[i]);
+    }
+}
+
+/* -------------------------------------------------------- periodic images */
+
+static void ensureParticleCapacity(oracle_ctx* c, int64_t n)
+{
+    if (n <= c->capP) return;
+    int64_t cap = c->capP ? c->capP : 64;
+    while (cap < n) cap *= 2;
+    c->p = (Particle*)realloc(c->p, sizeof(Particle) * (size_t)cap);
+    c->capP = cap;
+}
+
+/* PeriodicBoundary::createGhostParticle (PeriodicBoundary.cc:314-333) with
+ * copyInteractionsForPeriodicParticles (BaseInteractable.cc:386-393) and
+ * BaseInteraction::copySwitchPointer (BaseInteraction.cc:341-359).  Copies of
+ * wall interactions (P becomes the wall) are never evaluated and are deleted
+ * again by removeObjectKeepingPeriodics, so they are not created here. */
+static void createGhost(oracle_ctx* c, int64_t real, const dem_periodic* b)
+{
+    ensureParticleCapacity(c, c->nP + 1);
+    Particle* src = &c->p[real];
+    Particle* g = &c->p[c->nP];
+    *g = *src;
+    g->inter = NULL;
+    g->nInter = g->capInter = 0;
+    int64_t gi = c->nP;
+    c->nP++;
+    src = &c->p[real];
+    int32_t n0 = src->nInter;
+    for (int32_t a = 0; a < n0; ++a) {
+        int32_t k = c->p[real].inter[a];
+        Inter copy = c->in[k];
+        if (copy.IisWall) continue;
+        if (copy.P == (int32_t)real) {
+            reverseHistory(&copy);
+            copy.P = copy.I;
+        }
+        int32_t nk = interNew(c, copy.P, (int32_t)gi, 0);
+        Inter* x = &c->in[nk];
+        copy.I = (int32_t)gi;
+        copy.alive = 1;
+        *x = copy;
+    }
+    g = &c->p[gi];
+    periodicShift(b, &g->pos);
+    int64_t from = real;
+    while (c->p[from].from >= 0) from = c->p[from].from;
+    g->from = (int32_t)from;
+    g->id = c->nextId++;
+}
+
+/* DPMBase::checkAndDuplicatePeriodicParticles (DPMBase.cc:5043-5072) ->
+ * PeriodicBoundary::createPeriodicParticles / createPeriodicParticle
+ * (PeriodicBoundary.cc:304-364) */
+static void createGhosts(oracle_ctx* c)
+{
+    for (int bi = 0; bi < c->nPeriodic; ++bi) {
+        const dem_periodic* b = &c->periodic[bi];
+        int64_t n = c->nP;
+        for (int64_t i = 0; i < n; ++i) {
+            double maxDistance = c->p[i].radius + c->largestRadius;
+            if (periodicDistance(b, c->p[i].pos) < maxDistance) createGhost(c, i, b);
+        }
+    }
+}
+
+/* InteractionHandler::removeObjectKeepingPeriodics (InteractionHandler.cc:316-371) */
+static void removeKeepingPeriodics(oracle_ctx* c, int32_t k)
+{
+    Inter* x = &c->in[k];
+    if (!x->IisWall) {
+        int32_t P = x->P, I = x->I;
+        int32_t realI = c->p[I].from;
+        if (realI >= 0 && c->p[P].from < 0) {
+            if (P < realI) {
+                int32_t other = getExisting(c, P, realI, 0);
+                if (other < 0) other = getExisting(c, realI, P, 0);
+                if (other >= 0) {
+                    if (c->in[other].ts < c->in[k].ts) {
+                        interSetI(c, k, realI);
+                        interRemove(c, other);
+                        return;
+                    }
+                    interRemove(c, k);
+                    return;
+                }
+                interSetI(c, k, realI);
+                return;
+            }
+        }
+    }
+    interRemove(c, k);
+}
+
+/* DPMBase::removeDuplicatePeriodicParticles (DPMBase.cc:5000-5036) */
+static void removeGhosts(oracle_ctx* c)
+{
+    while (c->nP > c->nReal) {
+        Particle* g = &c->p[c->nP - 1];
+        while (g->nInter > 0) {
+            removeKeepingPeriodics(c, g->inter[0]);
+            g = &c->p[c->nP - 1];
+        }
+        free(g->inter);
+        c->nP--;
+    }
+}
+
+/* ------------------------------------------------------------ time step */
+
+/* DPMBase::computeAllForces (DPMBase.cc:3397-3526) */
+static void computeAllForces(oracle_ctx* c)
+{
+    for (int64_t i = 0; i < c->nP; ++i) {
+        c->p[i].force = v3(0.0, 0.0, 0.0);
+        c->p[i].torque = v3(0.0, 0.0, 0.0);
+        c->p[i].fscale = 0.0;
+        c->p[i].tscale = 0.0;
+    }
+    pairLoop(c);
+    for (int32_t w = 0; w < c->nWalls; ++w)
+        for (int64_t i = 0; i < c->nP; ++i)
+            if (!isFixed(&c->p[i]) && c->p[i].from < 0) computeForcesDueToWall(c, i, w);
+}
+
+/* BaseParticle::integrateBeforeForceComputation (BaseParticle.cc:718-752);
+ * fixed particles: BaseInteractable::integrateBeforeForceComputation without
+ * prescribed motion (BaseInteractable.cc:538-609). */
+static void integrateBefore(oracle_ctx* c)
+{
+    const double dt = c->dt;
+    for (int64_t i = 0; i < c->nP; ++i) {
+        Particle* p = &c->p[i];
+        if (p->invMass == 0.0) {
+            p->pos = vadd(p->pos, vmul(p->vel, dt));
+            rotateQ(p->q, vmul(p->omega, dt));
+        } else {
+            p->vel = vadd(p->vel, vmul(vmul(vmul(p->force, p->invMass), 0.5), dt));
+            p->pos = vadd(p->pos, vmul(p->vel, dt));
+            if (c->rotation) {
+                double R[6];
+                rotateInvI(p->q, p->invI, R);
+                p->omega = vadd(p->omega, vmul(vmul(symMul(R, p->torque), 0.5), dt));
+                rotateQ(p->q, vmul(p->omega, dt));
+            }
+        }
+    }
+}
+
+/* BaseParticle::integrateAfterForceComputation (BaseParticle.cc:760-776) */
+static void integrateAfter(oracle_ctx* c)
+{
+    const double dt = c->dt;
+    for (int64_t i = 0; i < c->nP; ++i) {
+        Particle* p = &c->p[i];
+        if (p->invMass == 0.0) continue;
+        p->vel = vadd(p->vel, vmul(vmul(vmul(p->force, p->invMass), 0.5), dt));
+        if (c->rotation) {
+            double R[6];
+            rotateInvI(p->q, p->invI, R);
+            p->omega = vadd(p->omega, vmul(vmul(symMul(R, p->torque), 0.5), dt));
+        }
+    }
+}
+
+/* DPMBase::checkInteractionWithBoundaries (DPMBase.cc:3272-3300) ->
+ * PeriodicBoundary::checkBoundaryAfterParticlesMove (PeriodicBoundary.cc:374-391) */
+static void wrapPeriodic(oracle_ctx* c)
+{
+    for (int bi = 0; bi < c->nPeriodic; ++bi)
+        for (int64_t i = 0; i < c->nP; ++i)
+            if (periodicDistance(&c->periodic[bi], c->p[i].pos) < 0) periodicShift(&c->periodic[bi], &c->p[i].pos);
+}
+
+/* InteractionHandler::eraseOldInteractions (InteractionHandler.cc:377-393) */
+static void eraseOld(oracle_ctx* c, uint32_t nt)
+{
+    for (int64_t k = 0; k < c->nIn; ++k)
+        if (c->in[k].alive && c->in[k].ts <= nt) interRemove(c, (int32_t)k);
+}
+
+/* --------------------------------------------------------------- the API */
+
+int oracle_dem_create(oracle_ctx** out)
+{
+    if (!out) return DEM_ERR_ARG;
+    oracle_ctx* c = (oracle_ctx*)calloc(1, sizeof(oracle_ctx));
+    c->dt = 0.0;
+    c->rotation = 1;
+    *out = c;
+    return DEM_OK;
+}
+
+int oracle_dem_destroy(oracle_ctx* c)
+{
+    if (!c) return DEM_OK;
+    for (int64_t i = 0; i < c->nP; ++i) free(c->p[i].inter);
+    free(c->p);
+    free(c->in);
+    free(c->freeList);
+    free(c->species);
+    free(c->walls);
+    free(c->head);
+    free(c->next);
+    free(c->cell);
+    free(c);
+    return DEM_OK;
+}
+
+const char* oracle_dem_last_error(const oracle_ctx* c) { return c ? c->err : "null context"; }
+
+int oracle_dem_set_species(oracle_ctx* c, int32_t n, const dem_pair_params* t)
+{
+    if (!c || n <= 0 || !t) return fail(c, DEM_ERR_ARG, "set_species: bad arguments");
+    for (int32_t k = 0; k < n * n; ++k)
+        if (t[k].interaction_distance != 0.0) return fail(c, DEM_ERR_UNSUPPORTED, "interaction_distance must be 0");
+    free(c->species);
+    c->species = (dem_pair_params*)malloc(sizeof(dem_pair_params) * (size_t)(n * n));
+    memcpy(c->species, t, sizeof(dem_pair_params) * (size_t)(n * n));
+    c->nSpecies = n;
+    return DEM_OK;
+}
+
+int oracle_dem_set_walls(oracle_ctx* c, int32_t n, const dem_wall* w)
+{
+    if (!c || n < 0 || (n > 0 && !w)) return fail(c, DEM_ERR_ARG, "set_walls: bad arguments");
+    free(c->walls);
+    c->walls = NULL;
+    if (n > 0) {
+        c->walls = (dem_wall*)malloc(sizeof(dem_wall) * (size_t)n);
+        memcpy(c->walls, w, sizeof(dem_wall) * (size_t)n);
+    }
+    c->nWalls = n;
+    return DEM_OK;
+}
+
+int oracle_dem_set_periodic(oracle_ctx* c, int32_t n, const dem_periodic* b)
+{
+    if (!c || n < 0 || n > 3 || (n > 0 && !b)) return fail(c, DEM_ERR_ARG, "set_periodic: bad arguments");
+    for (int32_t i = 0; i < n; ++i) c->periodic[i] = b[i];
+    c->nPeriodic = n;
+    return DEM_OK;
+}
+
+int oracle_dem_set_gravity(oracle_ctx* c, const double g[3])
+{
+    c->gravity = v3(g[0], g[1], g[2]);
+    return DEM_OK;
+}
+int oracle_dem_set_background_drag(oracle_ctx* c, double d)
+{
+    c->drag = d;
+    return DEM_OK;
+}
+int oracle_dem_set_timestep(oracle_ctx* c, double dt)
+{
+    c->dt = dt;
+    return DEM_OK;
+}
+int oracle_dem_set_rotation(oracle_ctx* c, int32_t r)
+{
+    c->rotation = r ? 1 : 0;
+    return DEM_OK;
+}
+int oracle_dem_set_time(oracle_ctx* c, double t, int64_t nt)
+{
+    c->time = t;
+    c->nt = nt;
+    return DEM_OK;
+}
+
+int oracle_dem_upload_particles(oracle_ctx* c, int64_t n, const uint32_t* id, const int32_t* species,
+                                const double* radius, const double* inv_mass, const double* inv_inertia,
+                                const double* x, const double* v, const double* w, const double* q,
+                                const double* f, const double* t)
+{
+    if (!c || n < 0) return fail(c, DEM_ERR_ARG, "upload_particles: bad arguments");
+    for (int64_t i = 0; i < c->nP; ++i) free(c->p[i].inter);
+    c->nP = c->nReal = 0;
+    c->nIn = c->nFree = c->nAlive = 0;
+    ensureParticleCapacity(c, n + 1);
+    c->largestRadius = 0.0;
+    uint32_t maxId = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        if (i > 0 && id[i] <= id[i - 1]) return fail(c, DEM_ERR_ARG, "ids must increase with the handler index");
+        if (species[i] < 0 || species[i] >= c->nSpecies) return fail(c, DEM_ERR_ARG, "species index out of range");
+        Particle* p = &c->p[i];
+        memset(p, 0, sizeof *p);
+        p->id = id[i];
+        p->species = species[i];
+        p->radius = radius[i];
+        p->invMass = inv_mass[i];
+        for (int k = 0; k < 6; ++k) p->invI[k] = inv_inertia[6 * i + k];
+        p->pos = v3(x[3 * i], x[3 * i + 1], x[3 * i + 2]);
+        p->vel = v3(v[3 * i], v[3 * i + 1], v[3 * i + 2]);
+        p->omega = w ? v3(w[3 * i], w[3 * i + 1], w[3 * i + 2]) : v3(0, 0, 0);
+        if (q) {
+            for (int k = 0; k < 4; ++k) p->q[k] = q[4 * i + k];
+        } else {
+            p->q[0] = 1.0;
+        }
+        p->force = f ? v3(f[3 * i], f[3 * i + 1], f[3 * i + 2]) : v3(0, 0, 0);
+        p->torque = t ? v3(t[3 * i], t[3 * i + 1], t[3 * i + 2]) : v3(0, 0, 0);
+        p->from = -1;
+        if (p->radius > c->largestRadius) c->largestRadius = p->radius;
+        if (id[i] > maxId) maxId = id[i];
+    }
+    c->nP = c->nReal = n;
+    c->nextId = maxId + 1;
+    return DEM_OK;
+}
+
+static int64_t findById(const oracle_ctx* c, uint32_t id)
+{
+    int64_t lo = 0, hi = c->nReal - 1;
+    while (lo <= hi) {
+        int64_t mid = (lo + hi) / 2;
+        if (c->p[mid].id == id) return mid;
+        if (c->p[mid].id < id) lo = mid + 1; else hi = mid - 1;
+    }
+    return -1;
+}
+
+int oracle_dem_upload_contacts(oracle_ctx* c, int64_t n, const uint32_t* p_id, const uint32_t* i_id,
+                               const double* spring, const uint32_t* ts)
+{
+    for (int64_t k = 0; k < n; ++k) {
+        int64_t P = findById(c, p_id[k]);
+        if (P < 0) return fail(c, DEM_ERR_ARG, "upload_contacts: unknown P id");
+        int32_t isWall = (i_id[k] & DEM_WALL_KEY) != 0;
+        int64_t I;
+        if (isWall) {
+            I = (int64_t)(i_id[k] & ~DEM_WALL_KEY);
+            if (I >= c->nWalls) return fail(c, DEM_ERR_ARG, "upload_contacts: wall index out of range");
+        } else {
+            I = findById(c, i_id[k]);
+            if (I < 0 || p_id[k] >= i_id[k]) return fail(c, DEM_ERR_ARG, "upload_contacts: bad I id");
+        }
+        int32_t x = interNew(c, (int32_t)P, (int32_t)I, isWall);
+        Inter* in = &c->in[x];
+        in->ts = ts[k];
+        in->sp = isWall ? mixed(c, c->p[P].species, c->walls[I].species) : mixed(c, c->p[P].species, c->p[I].species);
+        in->slidingSpring = v3(spring[9 * k], spring[9 * k + 1], spring[9 * k + 2]);
+        in->rollingSpring = v3(spring[9 * k + 3], spring[9 * k + 4], spring[9 * k + 5]);
+        in->torsionSpring = v3(spring[9 * k + 6], spring[9 * k + 7], spring[9 * k + 8]);
+    }
+    return DEM_OK;
+}
+
+static int checkReady(oracle_ctx* c)
+{
+    if (!c->species) return fail(c, DEM_ERR_STATE, "species not set");
+    return DEM_OK;
+}
+
+/* the force pass of DPMBase::initialiseSolve (DPMBase.cc:4260-4275) */
+int oracle_dem_compute_forces(oracle_ctx* c)
+{
+    int r = checkReady(c);
+    if (r) return r;
+    createGhosts(c);
+    computeAllForces(c);
+    removeGhosts(c);
+    return DEM_OK;
+}
+
+/* DPMBase::computeOneTimeStep (DPMBase.cc:4315-4389) */
+int oracle_dem_step(oracle_ctx* c, int64_t n)
+{
+    int r = checkReady(c);
+    if (r) return r;
+    for (int64_t s = 0; s < n; ++s) {
+        integrateBefore(c);
+        wrapPeriodic(c);
+        createGhosts(c);
+        computeAllForces(c);
+        removeGhosts(c);
+        integrateAfter(c);
+        eraseOld(c, (uint32_t)c->nt);
+        c->time += c->dt;
+        c->nt++;
+    }
+    return DEM_OK;
+}
+
+int oracle_dem_download_particles(oracle_ctx* c, double* x, double* v, double* w, double* q, double* f,
+                                  double* t)
+{
+    for (int64_t i = 0; i < c->nReal; ++i) {
+        const Particle* p = &c->p[i];
+        if (x) { x[3 * i] = p->pos.X; x[3 * i + 1] = p->pos.Y; x[3 * i + 2] = p->pos.Z; }
+        if (v) { v[3 * i] = p->vel.X; v[3 * i + 1] = p->vel.Y; v[3 * i + 2] = p->vel.Z; }
+        if (w) { w[3 * i] = p->omega.X; w[3 * i + 1] = p->omega.Y; w[3 * i + 2] = p->omega.Z; }
+        if (q) for (int k = 0; k < 4; ++k) q[4 * i + k] = p->q[k];
+        if (f) { f[3 * i] = p->force.X; f[3 * i + 1] = p->force.Y; f[3 * i + 2] = p->force.Z; }
+        if (t) { t[3 * i] = p->torque.X; t[3 * i + 1] = p->torque.Y; t[3 * i + 2] = p->torque.Z; }
+    }
+    return DEM_OK;
+}
+
+int oracle_dem_contact_count(oracle_ctx* c, int64_t* n)
+{
+    *n = c->nAlive;
+    return DEM_OK;
+}
+
+typedef struct { uint32_t p, i; int32_t k; } Key;
+static int keyCmp(const void* a, const void* b)
+{
+    const Key* x = (const Key*)a;
+    const Key* y = (const Key*)b;
+    if (x->p != y->p) return x->p < y->p ? -1 : 1;
+    if (x->i != y->i) return x->i < y->i ? -1 : 1;
+    return 0;
+}
+
+int oracle_dem_download_contacts(oracle_ctx* c, int64_t cap, int64_t* nOut, uint32_t* p_id, uint32_t* i_id,
+                                 double* spring, uint32_t* ts)
+{
+    Key* keys = (Key*)malloc(sizeof(Key) * (size_t)(c->nAlive > 0 ? c->nAlive : 1));
+    int64_t n = 0;
+    for (int64_t k = 0; k < c->nIn; ++k) {
+        const Inter* x = &c->in[k];
+        if (!x->alive) continue;
+        keys[n].p = c->p[x->P].id;
+        keys[n].i = x->IisWall ? (DEM_WALL_KEY | (uint32_t)x->I) : c->p[x->I].id;
+        keys[n].k = (int32_t)k;
+        n++;
+    }
+    qsort(keys, (size_t)n, sizeof(Key), keyCmp);
+    if (n > cap) {
+        free(keys);
+        return fail(c, DEM_ERR_CAPACITY, "download_contacts: capacity too small");
+    }
+    for (int64_t a = 0; a < n; ++a) {
+        const Inter* x = &c->in[keys[a].k];
+        p_id[a] = keys[a].p;
+        i_id[a] = keys[a].i;
+        if (spring) {
+            double* s = spring + 9 * a;
+            s[0] = x->slidingSpring.X; s[1] = x->slidingSpring.Y; s[2] = x->slidingSpring.Z;
+            s[3] = x->rollingSpring.X; s[4] = x->rollingSpring.Y; s[5] = x->rollingSpring.Z;
+            s[6] = x->torsionSpring.X; s[7] = x->torsionSpring.Y; s[8] = x->torsionSpring.Z;
+        }
+        if (ts) ts[a] = x->ts;
+    }
+    *nOut = n;
+    free(keys);
+    return DEM_OK;
+}
+
+int oracle_dem_get_time(oracle_ctx* c, double* time, int64_t* nt)
+{
+    *time = c->time;
+    *nt = c->nt;
+    return DEM_OK;
+}
+
+/* Test diagnostic: per particle, the sum of the magnitudes of the terms added
+ * to its force and torque in the last force pass (the scale of the summation
+ * round-off the engine-versus-oracle tolerance is expressed in). */
+int oracle_dem_force_scale(oracle_ctx* c, double* fs, double* ts)
+{
+    for (int64_t i = 0; i < c->nReal; ++i) {
+        if (fs) fs[i] = c->p[i].fscale;
+        if (ts) ts[i] = c->p[i].tscale;
+    }
+    return DEM_OK;
+}

@@ -1,0 +1,185 @@
+"""Pins the CPU oracle against the reference's own known answers, closed forms
+and an independent brute-force contact search.  CPU only."""
+import math
+
+import numpy as np
+import pytest
+
+import oracle_lib
+from bruteforce import brute_force_contacts
+from mercurydpm_fork_amd import dem
+from mercurydpm_fork_amd import scenarios as S
+from mercurydpm_fork_amd.species import PI, Species, cylindrical_wall, infinite_wall
+
+
+def run_oracle(sc, n_steps=None):
+    e = oracle_lib.engine()
+    S.apply(e, sc)
+    e.compute_forces()
+    e.step(sc.n_steps() if n_steps is None else n_steps)
+    return e
+
+
+def test_tangential_spring_known_answer():
+    """TangentialSpringUnitTest.cpp:89-101 (tolerance 1e-7)."""
+    st = run_oracle(S.tangential_spring_unit_test()).download()
+    g = S.TANGENTIAL_SPRING_GOLD
+    np.testing.assert_allclose(st["x"][0], g["x0"], atol=1e-7, rtol=0)
+    np.testing.assert_allclose(st["v"][0], g["v0"], atol=1e-7, rtol=0)
+    np.testing.assert_allclose(st["x"][1], g["x1"], atol=1e-7, rtol=0)
+    np.testing.assert_allclose(st["v"][1], g["v1"], atol=1e-7, rtol=0)
+    np.testing.assert_allclose(st["w"][1], g["w1"], atol=1e-7, rtol=0)
+
+
+def test_periodic_walls_gold():
+    """PeriodicWallsUnitTest.cpp:150-361: 14 discs, periodic in x and y,
+    positions and velocities at t = 0.11 within 1e-10."""
+    st = run_oracle(S.periodic_walls_unit_test()).download()
+    for k, (x, v) in enumerate(S.PERIODIC_WALLS_GOLD):
+        np.testing.assert_allclose(st["x"][k], x, atol=1e-10, rtol=0, err_msg=f"E{k} position")
+        np.testing.assert_allclose(st["v"][k], v, atol=1e-10, rtol=0, err_msg=f"E{k} velocity")
+
+
+def test_inclined_plane_rolling():
+    """InclinedPlaneUnitTest.cpp:80-83: v_x(1) = 1.81167, v_y = 0.00234107
+    (1e-4); rolling without slip gives 5/7 g sin(theta) t = 1.8117."""
+    st = run_oracle(S.inclined_plane_unit_test()).download()
+    assert abs(st["v"][0][0] - 1.81167) < 1e-4
+    assert abs(st["v"][0][1] - 0.00234107) < 1e-4
+    assert abs(st["v"][0][0] - 5.0 / 7.0 * 9.8 * math.sin(15 * PI / 180)) < 2e-3
+
+
+def test_hertz_restitution():
+    """HertzContactRestitutionUnitTest.cpp:56-61: restitution 0.8 (1e-2)."""
+    st = run_oracle(S.hertz_restitution_unit_test()).download()
+    assert abs(st["v"][1][0] - 0.8) < 1e-2
+    assert abs(st["v"][0][0] + 0.8) < 1e-2
+
+
+def test_linear_collision_time_and_restitution():
+    """Two equal spheres, head-on, linear viscoelastic: the contact lasts the
+    species' collision time (LinearViscoelasticNormalSpecies::getCollisionTime)
+    and the rebound speed is e times the impact speed."""
+    s = Species(density=6. / PI)
+    s.set_collision_time_and_restitution(0.005, 0.88, 1.0)
+    st = S.make_state([(0, 0, 0), (1.0 + 1e-9, 0, 0)], [(0.05, 0, 0), (-0.05, 0, 0)], [0.5, 0.5], [0, 0], [s])
+    sc = S.Scenario("pair", [s], [], [], (0.0, 0.0, 0.0), 0.005 / 2000, st)
+    e = oracle_lib.engine()
+    S.apply(e, sc)
+    e.compute_forces()
+    touching = 0
+    for _ in range(4000):
+        e.step(1)
+        if e.contact_count():
+            touching += 1
+    v = e.download()["v"]
+    assert abs(v[1][0] / 0.05 - 0.88) < 2e-3
+    assert abs(touching * sc.dt - 0.005) < 3 * sc.dt
+
+
+def test_free_flight_closed_form():
+    """Velocity Verlet integrates constant gravity exactly:
+    x(t) = x0 + v0 t + g t^2 / 2."""
+    s = Species(density=1.0)
+    s.stiffness = 1e3
+    st = S.make_state([(0.0, 0.0, 10.0)], [(0.3, -0.2, 1.0)], [0.5], [0], [s])
+    sc = S.Scenario("free", [s], [], [], (0.0, 0.0, -9.81), 1e-3, st)
+    e = oracle_lib.engine()
+    S.apply(e, sc)
+    e.compute_forces()
+    e.step(1000)
+    x = e.download()["x"][0]
+    t = 1.0
+    np.testing.assert_allclose(x, [0.3 * t, -0.2 * t, 10.0 + t - 0.5 * 9.81 * t * t], atol=1e-9)
+
+
+@pytest.mark.parametrize("laws", ["lvf", "full", "hvm"])
+@pytest.mark.parametrize("periodic", [(True, True), (False, False)])
+def test_contact_set_matches_brute_force(laws, periodic):
+    """The stored interactions after a step are exactly the pairs an O(N^2)
+    search finds with the reference's predicate d^2 < (r1 + r2)^2, evaluated
+    on the positions of the force pass (periodic images by the ghost rules of
+    PeriodicBoundary.cc:304-364) -- the HGridUnitTest pattern."""
+    sc = S.mixture(n=300, seed=11, laws=laws, periodic=periodic, walls=("bottom", "top", "cylinder"))
+    e = run_oracle(sc, n_steps=3)
+    st = e.download()
+    got = e.contacts().keys()
+    want = brute_force_contacts(sc, st["x"])
+    assert got == want
+    assert len(want) > 50
+
+
+def test_cylinder_equals_plane_for_one_sphere():
+    """A sphere touching a CylindricalWall feels the force of an InfiniteWall
+    tangent to the cylinder at the contact (CylindricalWall.cc:72-83 versus
+    InfiniteWall.cc:151-175)."""
+    s = Species(density=1.0, friction_law=dem.FRICTION_SLIDING)
+    s.set_collision_time_and_restitution(0.01, 0.8, 1.0)
+    s.sliding_stiffness = 2. / 7 * s.stiffness
+    s.set_sliding_friction(0.4)
+    R = 5.0
+    pos = [(R - 0.49, 0.0, 1.0)]
+    vel = [(0.2, 0.0, -0.1)]  # stays in the plane y = 0, where both normals are e_x
+    out = []
+    for wall in (cylindrical_wall(R), infinite_wall((1, 0, 0), (R, 0, 0))):
+        st = S.make_state(pos, vel, [0.5], [0], [s])
+        sc = S.Scenario("w", [s], [wall], [], (0, 0, -1.0), 1e-4, st)
+        e = run_oracle(sc, n_steps=20)
+        out.append(e.download())
+    np.testing.assert_allclose(out[0]["f"], out[1]["f"], rtol=1e-12, atol=1e-12)
+    np.testing.assert_allclose(out[0]["x"], out[1]["x"], rtol=0, atol=1e-15)
+
+
+def test_periodic_shift_invariance():
+    """Moving every sphere by a fraction of the periodic box changes only the
+    rounding of the result (PeriodicWallsWithSlidingFrictionUnitTest's
+    shifted-copy check).  The spheres start without contacts, so no contact
+    meets the step-0 time-stamp rule tested below."""
+    sc = S.mixture(n=200, seed=5, laws="lvf", periodic=(True, True), walls=("bottom",), inflate=1.0, vmax=2.0)
+    a = run_oracle(sc, n_steps=300).download()
+    x = sc.state.x.copy()
+    x[:, 0] = (x[:, 0] + 3.0) % 8.0
+    x[:, 1] = (x[:, 1] + 5.0) % 8.0
+    sc.state.x = x
+    b = run_oracle(sc, n_steps=300).download()
+    np.testing.assert_allclose(b["v"], a["v"], atol=1e-9, rtol=0)
+    np.testing.assert_allclose(b["f"], a["f"], atol=1e-6, rtol=1e-9)
+
+
+def _pair_state(periodic_pair):
+    s = Species(density=1.0, friction_law=dem.FRICTION_SLIDING)
+    s.set_collision_time_and_restitution(0.01, 0.8, 1.0)
+    s.sliding_stiffness = 2. / 7 * s.stiffness
+    s.set_sliding_friction(0.5)
+    if periodic_pair:
+        pos = [(4.0, 0.2, 1.0), (4.3, 7.4, 1.0)]   # touching across y = 0
+    else:
+        pos = [(4.0, 4.2, 1.0), (4.3, 3.4, 1.0)]   # the same pair, direct
+    vel = [(0.5, 0.0, 0.1), (-0.5, 0.0, -0.1)]
+    st = S.make_state(pos, vel, [0.45, 0.45], [0, 0], [s])
+    per = [S.periodic_boundary(0, 0.0, 8.0), S.periodic_boundary(1, 0.0, 8.0)]
+    return S.Scenario("pair", [s], [], per, (0.0, 0.0, 0.0), 1e-4, st)
+
+
+def test_step0_periodic_contact_keeps_initial_spring():
+    """Time stamps of the initial force pass and of step 0 are both 1
+    (DPMBase.cc:3146 uses getNumberOfTimeSteps()+1, numberOfTimeSteps_ = 0 in
+    both).  removeObjectKeepingPeriodics keeps the ghost's copy only when its
+    stamp is newer (InteractionHandler.cc:339-350), so a periodic contact
+    present at t = 0 keeps the spring of the initial pass after step 0, while
+    the same contact seen directly is advanced by step 0."""
+    res = {}
+    for periodic_pair in (True, False):
+        e = oracle_lib.engine()
+        S.apply(e, _pair_state(periodic_pair))
+        e.compute_forces()
+        s0 = e.contacts().spring[0][:3].copy()
+        e.step(1)
+        c = e.contacts()
+        res[periodic_pair] = (s0, c.spring[0][:3].copy(), int(c.ts[0]))
+    s0, s1, ts = res[True]
+    np.testing.assert_array_equal(s1, s0)
+    assert ts == 1
+    d0, d1, dts = res[False]
+    assert np.max(np.abs(d1 - d0)) > 0.5 * np.max(np.abs(d0))
+    assert dts == 1
