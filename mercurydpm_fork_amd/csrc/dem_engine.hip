@@ -54,7 +54,11 @@ enum Flag {
     F_REBUILDS = 5,
     F_COUNT_A = 6,
     F_COUNT_B = 7,
-    F_NUM = 8
+    F_ORPH_PARITY = 8,  // which orphan buffer is current
+    F_ORPH_COUNT0 = 9,
+    F_ORPH_COUNT1 = 10,
+    F_ORPH_OVERFLOW = 11,
+    F_NUM = 12
 };
 
 template <int NS>
@@ -72,6 +76,7 @@ struct Params {
     double dt, drag, g[3];
     double rmax, skin, thr2;
     uint32_t nt;
+    uint32_t alive_min; // stored interactions with an older stamp have been erased
     // grid
     double h[kMaxLevels];       // cell size of a non-periodic axis
     double rmaxl[kMaxLevels];   // largest radius on each level
@@ -104,7 +109,13 @@ struct Dev {
     // hash
     unsigned long long* hkey;
     Rec<NS>* hval;
+    uint8_t* hmatched;
     uint64_t hcap;
+    // interactions whose pair left the Verlet lists at a rebuild while their
+    // time stamp still keeps them alive (see k_orphans)
+    unsigned long long* okey[2];
+    Rec<NS>* orec[2];
+    int ocap;
     // tables
     const dem_pair_params* sp;
     const dem_wall* walls;
@@ -260,6 +271,7 @@ __device__ inline bool hash_find(const Dev<NS>& d, uint64_t key, Rec<NS>& r)
         unsigned long long k = d.hkey[h];
         if (k == key) {
             r = d.hval[h];
+            d.hmatched[h] = 1;
             return true;
         }
         if (k == kEmptyKey) return false;
@@ -289,6 +301,13 @@ __global__ void __launch_bounds__(kBlock) k_hash_store(Dev<NS> d, Params p, int 
         wm &= wm - 1;
         const uint64_t key = ((uint64_t)idi << 32) | (DEM_WALL_KEY | (uint32_t)w);
         hash_insert(d, key, d.wrec[cur][(size_t)i * p.nwalls + w]);
+    }
+    // orphans still alive at this step (time stamp >= step counter) go back in
+    const int par = d.flags[F_ORPH_PARITY];
+    const int no = d.flags[F_ORPH_COUNT0 + par];
+    for (int k = i; k < no && k < d.ocap; k += p.n) {
+        const Rec<NS> r = d.orec[par][k];
+        if (r.ts >= p.alive_min) hash_insert(d, d.okey[par][k], r);
     }
 }
 
@@ -588,12 +607,43 @@ __global__ void __launch_bounds__(kBlock) k_rematch(Dev<NS> d, Params p, int cur
     d.wmask[cur][i] = wm;
 }
 
+// Interactions that found no slot in the new lists: their pair is now
+// farther apart than the skin, so they are not touched in this step.  The
+// reference keeps such an interaction until eraseOldInteractions sees a time
+// stamp not newer than the step counter (InteractionHandler.cc:377-393); that
+// is the case for every record except those stamped ahead of the counter
+// (the initial force pass stamps 1 and step 0 erases only stamps <= 0).  The
+// survivors go to the orphan buffer, which is downloaded with the contacts and
+// re-inserted at the next rebuild.
+template <int NS>
+__global__ void __launch_bounds__(kBlock) k_orphans(Dev<NS> d, Params p)
+{
+    if (!rebuild_pending(d.flags)) return;
+    const uint64_t h = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (h >= d.hcap) return;
+    const unsigned long long key = d.hkey[h];
+    if (key == kEmptyKey || d.hmatched[h]) return;
+    const Rec<NS> r = d.hval[h];
+    if (p.erase && !(r.ts > p.nt)) return;
+    const int par = d.flags[F_ORPH_PARITY] ^ 1;
+    const int o = atomicAdd(&d.flags[F_ORPH_COUNT0 + par], 1);
+    if (o >= d.ocap) {
+        atomicOr(&d.flags[F_ORPH_OVERFLOW], 1);
+        return;
+    }
+    d.okey[par][o] = key;
+    d.orec[par][o] = r;
+}
+
 template <int NS>
 __global__ void __launch_bounds__(kBlock) k_hash_clear(Dev<NS> d)
 {
     if (!rebuild_pending(d.flags)) return;
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < d.hcap) d.hkey[i] = kEmptyKey;
+    if (i < d.hcap) {
+        d.hkey[i] = kEmptyKey;
+        d.hmatched[i] = 0;
+    }
 }
 
 __global__ void k_rebuild_done(int* flags)
@@ -601,6 +651,9 @@ __global__ void k_rebuild_done(int* flags)
     if (flags[F_REBUILD]) {
         flags[F_REBUILD] = 0;
         flags[F_REBUILDS] += 1;
+        const int par = flags[F_ORPH_PARITY];
+        flags[F_ORPH_COUNT0 + par] = 0;
+        flags[F_ORPH_PARITY] = par ^ 1;
     }
 }
 
@@ -835,8 +888,12 @@ __global__ void __launch_bounds__(kBlock) k_count_contacts(Dev<NS> d, Params p, 
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= p.n) return;
-    const int c = __popcll(d.mask[cur][i]);
+    int c = __popcll(d.mask[cur][i]);
     const int w = p.nwalls > 0 ? __popc(d.wmask[cur][i]) : 0;
+    const int par = d.flags[F_ORPH_PARITY];
+    const int no = min(d.flags[F_ORPH_COUNT0 + par], d.ocap);
+    for (int k = i; k < no; k += p.n)
+        if (d.orec[par][k].ts >= p.alive_min) ++c;
     if (c) atomicAdd(&d.flags[F_COUNT_A], c);
     if (w) atomicAdd(&d.flags[F_COUNT_B], w);
 }
@@ -860,6 +917,21 @@ __global__ void __launch_bounds__(kBlock) k_collect_contacts(Dev<NS> d, Params p
         iid[o] = d.id[d.nbr[s]];
         for (int a = 0; a < 9; ++a) spring[9 * (size_t)o + a] = a < NS ? d.rec[cur][s].s[a] : 0.0;
         ts[o] = d.rec[cur][s].ts;
+    }
+    {
+        const int par = d.flags[F_ORPH_PARITY];
+        const int no = min(d.flags[F_ORPH_COUNT0 + par], d.ocap);
+        for (int k = i; k < no; k += p.n) {
+            const Rec<NS> r = d.orec[par][k];
+            if (r.ts < p.alive_min) continue;
+            const int o = atomicAdd(counter, 1);
+            if (o >= cap) continue;
+            const unsigned long long key = d.okey[par][k];
+            pid[o] = (uint32_t)(key >> 32);
+            iid[o] = (uint32_t)key;
+            for (int a = 0; a < 9; ++a) spring[9 * (size_t)o + a] = a < NS ? r.s[a] : 0.0;
+            ts[o] = r.ts;
+        }
     }
     if (p.nwalls > 0) {
         uint32_t wm = d.wmask[cur][i];
@@ -896,7 +968,10 @@ template <int NS>
 __global__ void k_hash_clear_all(Dev<NS> d)
 {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < d.hcap) d.hkey[i] = kEmptyKey;
+    if (i < d.hcap) {
+        d.hkey[i] = kEmptyKey;
+        d.hmatched[i] = 0;
+    }
 }
 
 template <int NS>
@@ -959,6 +1034,7 @@ struct dem_ctx {
     std::vector<uint32_t> scan_sizes;
     uint32_t nbuckets = 0;
     int64_t last_rebuilds = 0;
+    uint32_t alive_min = 0;
     int max_list = 0;
 };
 
@@ -1030,7 +1106,9 @@ int alloc_particles(dem_ctx* ctx, int64_t n)
     A(d.key, n) A(d.perm, n);
     A(d.count, ctx->nbuckets) A(d.cursor, ctx->nbuckets) A(d.start, (size_t)ctx->nbuckets + 1);
     d.hcap = next_pow2(std::max<int64_t>(4096, 8 * n + 4 * n * (int64_t)ctx->walls.size()));
-    A(d.hkey, d.hcap) A(d.hval, d.hcap);
+    A(d.hkey, d.hcap) A(d.hval, d.hcap) A(d.hmatched, d.hcap);
+    d.ocap = (int)std::min<int64_t>(std::max<int64_t>(1024, n), 1 << 22);
+    A(d.okey[0], d.ocap) A(d.okey[1], d.ocap) A(d.orec[0], d.ocap) A(d.orec[1], d.ocap);
 #undef A
     // scan hierarchy: level 0 scans count into start; every further level
     // scans the block sums of the level below in place
@@ -1095,6 +1173,7 @@ void launch_rebuild(dem_ctx* ctx)
     hipLaunchKernelGGL(k_copy_back<NS>, dim3(gn), dim3(kBlock), 0, s, d, p);
     hipLaunchKernelGGL(k_build_lists<NS>, dim3(gn), dim3(kBlock), 0, s, d, p);
     hipLaunchKernelGGL(k_rematch<NS>, dim3(gn), dim3(kBlock), 0, s, d, p, ctx->cur);
+    hipLaunchKernelGGL(k_orphans<NS>, dim3(grid_for(d.hcap)), dim3(kBlock), 0, s, d, p);
     hipLaunchKernelGGL(k_hash_clear<NS>, dim3(grid_for(d.hcap)), dim3(kBlock), 0, s, d);
     hipLaunchKernelGGL(k_rebuild_done, dim3(1), dim3(1), 0, s, d.flags);
 }
@@ -1129,6 +1208,7 @@ int check_flags(dem_ctx* ctx)
                        "Verlet list overflow: a particle has " + std::to_string(f[F_MAX_LIST]) +
                            " neighbours, more than max_neighbors=" + std::to_string(ctx->maxn_cfg));
     if (f[F_HASH_OVERFLOW]) return set_err(ctx, DEM_ERR_CAPACITY, "contact hash table overflow");
+    if (f[F_ORPH_OVERFLOW]) return set_err(ctx, DEM_ERR_CAPACITY, "orphaned-interaction buffer overflow");
     if (f[F_LIST_ASYM]) return set_err(ctx, DEM_ERR_STATE, "internal error: asymmetric Verlet list");
     return DEM_OK;
 }
@@ -1138,10 +1218,15 @@ int run_steps(dem_ctx* ctx, int64_t nsteps)
 {
     for (int64_t s = 0; s < nsteps; ++s) {
         launch_kick<NS>(ctx, s > 0 ? 1 : 0, 1);
+        ctx->prm.erase = 1;
+        ctx->prm.nt = (uint32_t)ctx->nt;
+        ctx->prm.alive_min = ctx->alive_min;
         launch_rebuild<NS>(ctx);
         launch_force<NS>(ctx, 1);
         ctx->time += ctx->dt;
         ctx->nt += 1;
+        ctx->alive_min = (uint32_t)ctx->nt;
+        ctx->prm.alive_min = ctx->alive_min;
     }
     if (nsteps > 0) launch_kick<NS>(ctx, 1, 0);
     HIPCHK(hipGetLastError());
@@ -1151,6 +1236,9 @@ int run_steps(dem_ctx* ctx, int64_t nsteps)
 template <int NS>
 int run_forces(dem_ctx* ctx)
 {
+    ctx->prm.erase = 0;
+    ctx->prm.nt = (uint32_t)ctx->nt;
+    ctx->prm.alive_min = ctx->alive_min;
     launch_rebuild<NS>(ctx);
     launch_force<NS>(ctx, 0);
     HIPCHK(hipGetLastError());
@@ -1452,6 +1540,7 @@ int dem_upload_particles(dem_ctx* ctx, int64_t n, const uint32_t* id, const int3
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(ctx->stream));
     ctx->cur = 0;
+    ctx->alive_min = 0;
     ctx->uploaded = true;
     return DEM_OK;
 }
@@ -1482,10 +1571,13 @@ int dem_upload_contacts(dem_ctx* ctx, int64_t n, const uint32_t* p_id, const uin
     HIPCHK(hipMemcpy(di, i_id, sizeof(uint32_t) * n, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(dt, ts, sizeof(uint32_t) * n, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(ds, spring, sizeof(double) * 9 * n, hipMemcpyHostToDevice));
-    // the pending rebuild stores the active records into the hash table
-    // first; store the old ones now, then add the uploaded ones
     int one = 1;
     HIPCHK(hipMemcpy(ctx->d_flags + F_REBUILD, &one, sizeof one, hipMemcpyHostToDevice));
+    // uploaded records replace the current history
+    if (ctx->ns_springs == 9)
+        hipLaunchKernelGGL(k_zero_masks<9>, dim3(grid_for(ctx->n)), dim3(kBlock), 0, ctx->stream, ctx->d9, ctx->prm);
+    else
+        hipLaunchKernelGGL(k_zero_masks<3>, dim3(grid_for(ctx->n)), dim3(kBlock), 0, ctx->stream, ctx->d3, ctx->prm);
     if (ctx->ns_springs == 9) {
         hipLaunchKernelGGL(k_hash_upload<9>, dim3(grid_for(n)), dim3(kBlock), 0, ctx->stream, ctx->d9, (int)n, dp, di, ds, dt);
     } else {

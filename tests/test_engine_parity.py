@@ -38,6 +38,17 @@ def assert_contacts_equal(g, o):
     return co.n
 
 
+def assert_contacts_close(g, o, rtol=1e-6):
+    """After many steps the force summation order has perturbed the state at
+    round-off level: same contact set and time stamps, springs close."""
+    cg, co = g.contacts(), o.contacts()
+    assert cg.keys() == co.keys(), (len(cg.keys() ^ co.keys()), sorted(cg.keys() ^ co.keys())[:10])
+    np.testing.assert_array_equal(cg.ts, co.ts)
+    scale = np.max(np.abs(co.spring)) if co.n else 1.0
+    np.testing.assert_allclose(cg.spring, co.spring, rtol=rtol, atol=rtol * scale)
+    return co.n
+
+
 def assert_forces_close(g, o, tol=FORCE_TOL):
     a, b = g.download(), o.download()
     fs, ts = oracle_lib.force_scale(o)
@@ -57,15 +68,26 @@ CASES = [
 ]
 
 
+def reseeded(sc, st):
+    """The scenario with the particle state replaced by a downloaded one."""
+    return S.Scenario(sc.name, sc.species, sc.walls, sc.periodic, sc.gravity, sc.dt,
+                      dem.State(id=sc.state.id, species=sc.state.species, radius=sc.state.radius,
+                                inv_mass=sc.state.inv_mass, inv_inertia=sc.state.inv_inertia,
+                                x=st["x"], v=st["v"], w=st["w"], q=st["q"], f=st["f"], t=st["t"]))
+
+
 @pytest.mark.parametrize("case", CASES, ids=lambda c: f"{c['laws']}-{'P' if c['periodic'][0] else 'np'}-{len(c['walls'])}w")
 def test_first_pass_and_one_step(case):
-    sc = S.mixture(n=400, seed=21, **case)
+    """The force pass of initialiseSolve, then one computeOneTimeStep from
+    the oracle's state after that pass (so both start from identical forces)."""
+    sc = S.mixture(n=400, seed=21, lx=7.0, ly=7.0, lz=7.0, inflate=1.12, **case)
     g, o = make_pair(sc)
     g.compute_forces()
     o.compute_forces()
     n0 = assert_contacts_equal(g, o)
     assert n0 > 100
     assert_forces_close(g, o)
+    g, o = make_pair(reseeded(sc, o.download()), o.contacts())
     g.step(1)
     o.step(1)
     assert_contacts_equal(g, o)
@@ -73,6 +95,11 @@ def test_first_pass_and_one_step(case):
     np.testing.assert_array_equal(a["x"], b["x"])
     np.testing.assert_array_equal(a["q"], b["q"])
     np.testing.assert_allclose(a["v"], b["v"], rtol=0, atol=1e-12)
+    g.step(20)
+    o.step(20)
+    assert_contacts_close(g, o)
+    a, b = g.download(), o.download()
+    np.testing.assert_allclose(a["x"], b["x"], rtol=0, atol=1e-9)
 
 
 @pytest.mark.parametrize("case", CASES[:4], ids=lambda c: f"{c['laws']}-{'P' if c['periodic'][0] else 'np'}")
@@ -89,11 +116,7 @@ def test_reseeded_steps(case):
         done = target
         st = ref.download()
         ct = ref.contacts()
-        s2 = S.Scenario(sc.name, sc.species, sc.walls, sc.periodic, sc.gravity, sc.dt,
-                        dem.State(id=sc.state.id, species=sc.state.species, radius=sc.state.radius,
-                                  inv_mass=sc.state.inv_mass, inv_inertia=sc.state.inv_inertia,
-                                  x=st["x"], v=st["v"], w=st["w"], q=st["q"], f=st["f"], t=st["t"]))
-        g, o = make_pair(s2, ct)
+        g, o = make_pair(reseeded(sc, st), ct)
         for e in (g, o):
             e.set_time(done * sc.dt, done)
         g.step(1)
@@ -113,7 +136,7 @@ def test_many_steps_with_frequent_rebuilds():
     for _ in range(6):
         g.step(50)
         o.step(50)
-        assert_contacts_equal(g, o)
+        assert_contacts_close(g, o)
         a, b = g.download(), o.download()
         np.testing.assert_allclose(a["x"], b["x"], rtol=0, atol=1e-9)
     assert g.stats().n_rebuilds > 100
@@ -174,6 +197,6 @@ def test_box_lvf_vs_oracle():
     o.compute_forces()
     g.step(400)
     o.step(400)
-    assert_contacts_equal(g, o)
+    assert_contacts_close(g, o)
     a, b = g.download(), o.download()
     np.testing.assert_allclose(a["x"], b["x"], rtol=0, atol=1e-10)
