@@ -1,0 +1,181 @@
+#!/usr/bin/env python
+"""Throughput of the DEM time step on one MI355X.
+
+Workload: the polydisperse periodic chute -- 2,000,000 flowing spheres with
+d ~ U[0.5, 1.5] (three grid levels) over a rough base of fixed spheres,
+tilted 24 degrees, periodic in x and y, HertzianViscoelasticMindlin with
+e = 0.8 and mu = 0.5, dt = t_c / 50, FP64.  The spheres start on a
+non-overlapping lattice and are first settled into a dense bed (untimed:
+strong gravity straight down, then the same with background drag), then the
+chute gravity is switched on and, after the warm-up steps, K steps are timed.
+
+Prints one JSON line: particle-steps/s (flowing spheres x steps / s), the
+roofline of the dominant kernel (the force kernel, HIP-event timed on the
+engine's stream) and a CPU baseline (the oracle, one thread, on a bounded
+sample of the same state).
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=500)
+    ap.add_argument("--n", type=int, default=2_000_000, help="flowing spheres")
+    ap.add_argument("--settle-steps", type=int, default=8000, help="untimed settling under strong gravity")
+    ap.add_argument("--settle-gravity", type=float, default=100.0)
+    ap.add_argument("--damp-steps", type=int, default=2000, help="untimed settling with background drag")
+    ap.add_argument("--damp-drag", type=float, default=15.0)
+    ap.add_argument("--save-state", default=None, help="write the settled state (npz) before timing")
+    ap.add_argument("--load-state", default=None, help="start from a state written by --save-state")
+    ap.add_argument("--cpu-steps", type=int, default=3, help="oracle steps in the CPU baseline (0: skip)")
+    ap.add_argument("--skin", type=float, default=0.0)
+    ap.add_argument("--seed", type=int, default=3)
+    return ap.parse_args()
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU: the engine runs a simulation on a single MI355X; "
+              f"--gpus {a.gpus} is not supported (multi-GPU slab decomposition is a design only)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from mercurydpm_fork_amd import dem
+    from mercurydpm_fork_amd import scenarios as S
+
+    t_gen = time.time()
+    sc = S.chute_hvm(n_flow=a.n, seed=a.seed)
+    n_total = sc.state.n
+    eng = dem.Engine(device=0, skin=a.skin)
+    chute_g = sc.gravity
+    S.apply(eng, sc)
+    eng.compute_forces()
+    print(f"# generated {n_total} spheres ({sc.info['n_flowing']} flowing, {sc.info['n_base']} base, "
+          f"footprint {sc.info['footprint']:.1f}) in {time.time() - t_gen:.1f}s", file=sys.stderr, flush=True)
+
+    if a.load_state:
+        z = np.load(a.load_state)
+        s2 = S.Scenario(sc.name, sc.species, sc.walls, sc.periodic, chute_g, sc.dt,
+                        dem.State(id=sc.state.id, species=sc.state.species, radius=sc.state.radius,
+                                  inv_mass=sc.state.inv_mass, inv_inertia=sc.state.inv_inertia,
+                                  x=z["x"], v=z["v"], w=z["w"], q=z["q"], f=z["f"], t=z["t"]))
+        eng = dem.Engine(device=0, skin=a.skin)
+        S.apply(eng, s2, dem.Contacts(z["p_id"], z["i_id"], z["spring"], z["ts"]))
+        eng.set_time(float(z["time"]), int(z["nt"]))
+        a.settle_steps = 0
+    # settle into a dense bed (untimed): gravity straight down, then damped
+    if a.settle_steps > 0:
+        t0 = time.time()
+        eng.set_gravity((0.0, 0.0, -a.settle_gravity))
+        total = a.settle_steps + a.damp_steps
+        done = 0
+        while done < total:
+            if done == a.settle_steps:
+                eng.set_background_drag(a.damp_drag)
+            k = min(2000, (a.settle_steps if done < a.settle_steps else total) - done)
+            eng.step(k)
+            done += k
+            print(f"# settle {done}/{total} steps, {eng.contact_count()} contacts, "
+                  f"{time.time() - t0:.1f}s", file=sys.stderr, flush=True)
+        eng.set_gravity(chute_g)
+        eng.set_background_drag(0.0)
+
+    if a.save_state:
+        d = eng.download()
+        ct = eng.contacts()
+        st0 = eng.stats()
+        np.savez(a.save_state, x=d["x"], v=d["v"], w=d["w"], q=d["q"], f=d["f"], t=d["t"], p_id=ct.p_id,
+                 i_id=ct.i_id, spring=ct.spring, ts=ct.ts, time=st0.time, nt=st0.n_steps)
+        print(f"# saved state to {a.save_state}", file=sys.stderr, flush=True)
+    eng.step(a.warmup)
+    c0 = eng.contact_count()
+    torch.cuda.synchronize()
+    eng.set_profiling(True)
+    t0 = time.perf_counter()
+    eng.step(a.steps)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    st = eng.stats()
+    c1 = st.n_contacts
+    elapsed = t1 - t0
+    n_flow = sc.info["n_flowing"]
+    value = n_flow * a.steps / elapsed
+    # contacts per particle over the window (pairs counted once + wall contacts)
+    cbar = 0.5 * (c0 + c1) / n_total
+    force_ms = st.t_force_ms / max(st.n_timed_steps, 1)
+    b_force = 144.0 + 64.0 * cbar        # algorithmic bytes per particle in the force kernel
+    achieved = b_force * n_total / (force_ms * 1e-3) / 1e9
+    b_step = 264.0 + 64.0 * cbar         # algorithmic bytes per particle-step
+    step_gbs = b_step * n_total * a.steps / elapsed / 1e9
+    print(f"# timed {a.steps} steps in {elapsed:.3f}s: force {force_ms:.3f} ms, kick "
+          f"{st.t_kick_ms / max(st.n_timed_steps, 1):.3f} ms, rebuild {st.t_rebuild_ms / max(st.n_timed_steps, 1):.3f} ms "
+          f"per step; {st.n_rebuilds} rebuilds total; C/N {cbar:.3f}; max list {st.max_list_length}",
+          file=sys.stderr, flush=True)
+
+    cpu = None
+    if a.cpu_steps > 0:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_lib
+        d = eng.download()
+        ct = eng.contacts()
+        s2 = S.Scenario(sc.name, sc.species, sc.walls, sc.periodic, chute_g, sc.dt,
+                        dem.State(id=sc.state.id, species=sc.state.species, radius=sc.state.radius,
+                                  inv_mass=sc.state.inv_mass, inv_inertia=sc.state.inv_inertia,
+                                  x=d["x"], v=d["v"], w=d["w"], q=d["q"], f=d["f"], t=d["t"]))
+        o = oracle_lib.engine()
+        S.apply(o, s2, ct)
+        o.set_time(0.0, int(st.n_steps))
+        tc0 = time.perf_counter()
+        o.step(a.cpu_steps)
+        tc1 = time.perf_counter()
+        cpu = {"value": n_flow * a.cpu_steps / (tc1 - tc0), "unit": "particle-steps/s", "cores": 1,
+               "kind": "port",
+               "sample": f"oracle/dem_oracle.c (serial restatement of the reference step), {a.cpu_steps} steps "
+                         f"of the same {n_total}-sphere chute state the GPU timed, on 1 host core"}
+
+    out = {
+        "metric": "particle-steps/sec at 2M polydisperse spheres",
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": 1e3 * elapsed / a.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded rock-salt lattice packing settled under gravity)",
+        "config": {"workload": "polydisperse periodic chute, HertzianViscoelasticMindlin e=0.8 mu=0.5, 24 deg",
+                   "flowing_spheres": n_flow, "base_spheres": sc.info["n_base"],
+                   "diameter": "U[0.5,1.5]", "footprint": round(sc.info["footprint"], 3),
+                   "dt": sc.dt, "contacts_per_particle": round(cbar, 4),
+                   "rebuilds_in_window": None, "step_hbm_gbs_algorithmic": round(step_gbs, 2),
+                   "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                     "kernel": "k_force", "kernel_ms": round(force_ms, 4),
+                     "bytes_per_particle": round(b_force, 2)},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

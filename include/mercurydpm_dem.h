@@ -161,6 +161,12 @@ typedef struct dem_stats {
     double time;                  /* DPMBase::getTime */
     double skin;
     double cell_size[4];          /* per level */
+    /* HIP-event times on the engine's stream, accumulated since the last
+     * dem_set_profiling(ctx, 1) (zero when profiling is off) */
+    int64_t n_timed_steps;
+    double t_force_ms;            /* pair/wall/external force kernel */
+    double t_kick_ms;             /* velocity-Verlet kick/drift kernels */
+    double t_rebuild_ms;          /* grid sort + Verlet-list rebuild sequence */
 } dem_stats;
 
 /* Create/destroy a context bound to one GPU.  No counterpart object in the
@@ -246,6 +252,11 @@ int dem_download_contacts(dem_ctx* ctx, int64_t capacity, int64_t* n_out, uint32
                           uint32_t* i_id, double* spring, uint32_t* time_stamp);
 
 int dem_get_stats(dem_ctx* ctx, dem_stats* stats);
+
+/* Per-phase HIP-event timing of dem_step (no counterpart in the reference,
+ * whose only timer is the wall clock of Time::toc, MercuryTime.h:70-73).
+ * Enabling resets the accumulated times. */
+int dem_set_profiling(dem_ctx* ctx, int32_t on);
 
 #ifdef __cplusplus
 }

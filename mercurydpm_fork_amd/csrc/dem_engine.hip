@@ -72,6 +72,7 @@ struct Params {
     int n, maxn, nwalls, ns, levels, rotation, nper;
     int erase;
     int paxis[3];
+    int axis_pb[3]; // periodic boundary index acting on each axis, -1 if none
     double pleft[3], pright[3], pshift[3];
     double dt, drag, g[3];
     double rmax, skin, thr2;
@@ -133,11 +134,14 @@ __device__ inline uint64_t mix64(uint64_t k)
     return k;
 }
 
+// Spatial hash of a cell (the prime-multiply/xor hash of Teschner et al.,
+// "Optimized Spatial Hashing for Collision Detection of Deformable Objects",
+// 2003; HGrid::computeHashBucketIndex, HGrid.h:76-84, uses the same family).
+// Colliding cells share a bucket; the list build tells them apart by the cell
+// coordinates stored with each particle.
 __device__ inline uint32_t cell_hash(int cx, int cy, int cz)
 {
-    uint64_t k = ((uint64_t)(uint32_t)cx * 0x9E3779B97F4A7C15ull) ^ ((uint64_t)(uint32_t)cy * 0xC2B2AE3D27D4EB4Full) ^
-                 ((uint64_t)(uint32_t)cz * 0x165667B19E3779F9ull);
-    return (uint32_t)mix64(k);
+    return ((uint32_t)cx * 73856093u) ^ ((uint32_t)cy * 19349663u) ^ ((uint32_t)cz * 83492791u);
 }
 
 __device__ inline double comp(V3 v, int a) { return a == 0 ? v.x : (a == 1 ? v.y : v.z); }
@@ -174,6 +178,12 @@ __device__ inline V3 periodic_shift(const Params& p, int b, V3 pos)
     if (p.pright[b] - d > d - p.pleft[b]) return vadd(pos, s);
     return vadd(pos, vneg(s));
 }
+
+// Rebuild kernels run with a bounded grid and stride over their range, so a
+// launch with no rebuild pending retires in a few microseconds.
+constexpr int kRebuildBlocks = 2048;
+#define GRID_STRIDE(i, n) \
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < (uint64_t)(n); i += (uint64_t)gridDim.x * blockDim.x)
 
 __device__ inline bool rebuild_pending(const int* flags) { return __atomic_load_n(&flags[F_REBUILD], __ATOMIC_RELAXED) != 0; }
 
@@ -281,31 +291,38 @@ __device__ inline bool hash_find(const Dev<NS>& d, uint64_t key, Rec<NS>& r)
 }
 
 template <int NS>
-__global__ void __launch_bounds__(kBlock) k_hash_store(Dev<NS> d, Params p, int cur)
+__global__ void __launch_bounds__(kBlock) k_rb_begin(Dev<NS> d, Params p, int cur, uint32_t nbuckets)
 {
     if (!rebuild_pending(d.flags)) return;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= p.n) return;
-    const uint32_t idi = d.id[i];
-    unsigned long long m = d.mask[cur][i];
-    while (m) {
-        const int k = __ffsll((long long)m) - 1;
-        m &= m - 1;
-        const uint32_t j = d.nbr[(size_t)i * p.maxn + k];
-        const uint64_t key = ((uint64_t)idi << 32) | d.id[j];
-        hash_insert(d, key, d.rec[cur][(size_t)i * p.maxn + k]);
+    GRID_STRIDE(b, nbuckets)
+    {
+        d.count[b] = 0;
+        d.cursor[b] = 0;
     }
-    uint32_t wm = d.wmask[cur][i];
-    while (wm) {
-        const int w = __ffs(wm) - 1;
-        wm &= wm - 1;
-        const uint64_t key = ((uint64_t)idi << 32) | (DEM_WALL_KEY | (uint32_t)w);
-        hash_insert(d, key, d.wrec[cur][(size_t)i * p.nwalls + w]);
+    GRID_STRIDE(i, p.n)
+    {
+        const uint32_t idi = d.id[i];
+        unsigned long long m = d.mask[cur][i];
+        while (m) {
+            const int k = __ffsll((long long)m) - 1;
+            m &= m - 1;
+            const uint32_t j = d.nbr[i * p.maxn + k];
+            const uint64_t key = ((uint64_t)idi << 32) | d.id[j];
+            hash_insert(d, key, d.rec[cur][i * p.maxn + k]);
+        }
+        uint32_t wm = d.wmask[cur][i];
+        while (wm) {
+            const int w = __ffs(wm) - 1;
+            wm &= wm - 1;
+            const uint64_t key = ((uint64_t)idi << 32) | (DEM_WALL_KEY | (uint32_t)w);
+            hash_insert(d, key, d.wrec[cur][i * p.nwalls + w]);
+        }
     }
-    // orphans still alive at this step (time stamp >= step counter) go back in
+    // orphans still alive at this step (time stamp >= alive_min) go back in
     const int par = d.flags[F_ORPH_PARITY];
-    const int no = d.flags[F_ORPH_COUNT0 + par];
-    for (int k = i; k < no && k < d.ocap; k += p.n) {
+    const int no = min(d.flags[F_ORPH_COUNT0 + par], d.ocap);
+    GRID_STRIDE(k, no)
+    {
         const Rec<NS> r = d.orec[par][k];
         if (r.ts >= p.alive_min) hash_insert(d, d.okey[par][k], r);
     }
@@ -314,22 +331,12 @@ __global__ void __launch_bounds__(kBlock) k_hash_store(Dev<NS> d, Params p, int 
 // ------------------------------------------------------------------------
 // Rebuild: counting sort by (level, hashed cell), then by id in a bucket.
 // ------------------------------------------------------------------------
-template <int NS>
-__global__ void k_zero_u32(Dev<NS> d, uint32_t* a, uint32_t n)
-{
-    if (!rebuild_pending(d.flags)) return;
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) a[i] = 0;
-}
-
 __device__ inline int4 cell_of(const Params& p, int lvl, V3 x)
 {
     int c[3];
     for (int a = 0; a < 3; ++a) {
         const double xa = comp(x, a);
-        int pb = -1;
-        for (int b = 0; b < p.nper; ++b)
-            if (p.paxis[b] == a) pb = b;
+        const int pb = p.axis_pb[a];
         if (pb >= 0) {
             const int nc = p.ncell[lvl][a];
             int t = (int)floor((xa - p.pleft[pb]) / p.hc[lvl][a]);
@@ -347,14 +354,16 @@ template <int NS>
 __global__ void __launch_bounds__(kBlock) k_keys(Dev<NS> d, Params p)
 {
     if (!rebuild_pending(d.flags)) return;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= p.n) return;
-    const int lvl = (int)(d.spc[i] >> 16);
-    const int4 c = cell_of(p, lvl, xyz(d.posr[i]));
-    const uint32_t key = p.nboff[lvl] + (cell_hash(c.y, c.z, c.w) & (p.nb[lvl] - 1));
-    d.key[i] = key;
-    d.cellc[i] = c;
-    atomicAdd(&d.count[key], 1u);
+    GRID_STRIDE(i, p.n)
+    {
+        const int lvl = (int)(d.spc[i] >> 16);
+        int4 c = cell_of(p, lvl, xyz(d.posr[i]));
+        const uint32_t key = p.nboff[lvl] + (cell_hash(c.y, c.z, c.w) & (p.nb[lvl] - 1));
+        d.key[i] = key;
+        if (d.vel[i].w == 0.0) c.x |= 0x100; // fixed particle
+        d.cellc[i] = c;
+        atomicAdd(&d.count[key], 1u);
+    }
 }
 
 // exclusive scan, 256 threads x 8 elements per block
@@ -396,39 +405,39 @@ template <int NS>
 __global__ void __launch_bounds__(kBlock) k_scan_add(Dev<NS> d, uint32_t* out, const uint32_t* add, uint32_t n)
 {
     if (!rebuild_pending(d.flags)) return;
-    const uint32_t idx = blockIdx.x * kBlock + threadIdx.x;
-    const uint32_t tile = idx / kScanTile;
-    if (idx < n) out[idx] += add[tile];
+    GRID_STRIDE(idx, n) out[idx] += add[idx / kScanTile];
 }
 
 template <int NS>
 __global__ void __launch_bounds__(kBlock) k_scatter(Dev<NS> d, Params p)
 {
     if (!rebuild_pending(d.flags)) return;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= p.n) return;
-    const uint32_t key = d.key[i];
-    const uint32_t pos = d.start[key] + atomicAdd(&d.cursor[key], 1u);
-    d.perm[pos] = (uint32_t)i;
+    GRID_STRIDE(i, p.n)
+    {
+        const uint32_t key = d.key[i];
+        const uint32_t pos = d.start[key] + atomicAdd(&d.cursor[key], 1u);
+        d.perm[pos] = (uint32_t)i;
+    }
 }
 
 template <int NS>
 __global__ void __launch_bounds__(kBlock) k_sort_buckets(Dev<NS> d, uint32_t nbuckets, uint32_t n)
 {
     if (!rebuild_pending(d.flags)) return;
-    const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= nbuckets) return;
-    const uint32_t s = d.start[b];
-    const uint32_t e = (b + 1 < nbuckets) ? d.start[b + 1] : n;
-    for (uint32_t a = s + 1; a < e; ++a) {
-        const uint32_t v = d.perm[a];
-        const uint32_t vid = d.id[v];
-        uint32_t c = a;
-        while (c > s && d.id[d.perm[c - 1]] > vid) {
-            d.perm[c] = d.perm[c - 1];
-            --c;
+    GRID_STRIDE(b, nbuckets)
+    {
+        const uint32_t s = d.start[b];
+        const uint32_t e = d.start[b + 1];
+        for (uint32_t a = s + 1; a < e; ++a) {
+            const uint32_t v = d.perm[a];
+            const uint32_t vid = d.id[v];
+            uint32_t c = a;
+            while (c > s && d.id[d.perm[c - 1]] > vid) {
+                d.perm[c] = d.perm[c - 1];
+                --c;
+            }
+            d.perm[c] = v;
         }
-        d.perm[c] = v;
     }
 }
 
@@ -436,38 +445,40 @@ template <int NS>
 __global__ void __launch_bounds__(kBlock) k_gather(Dev<NS> d, Params p)
 {
     if (!rebuild_pending(d.flags)) return;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= p.n) return;
-    const uint32_t o = d.perm[i];
-    d.t_posr[i] = d.posr[o];
-    d.t_vel[i] = d.vel[o];
-    d.t_omg[i] = d.omg[o];
-    d.t_quat[i] = d.quat[o];
-    d.t_frc[i] = d.frc[o];
-    d.t_trq[i] = d.trq[o];
-    d.t_id[i] = d.id[o];
-    d.t_orig[i] = d.orig[o];
-    d.t_spc[i] = d.spc[o];
-    d.t_cellc[i] = d.cellc[o];
+    GRID_STRIDE(i, p.n)
+    {
+        const uint32_t o = d.perm[i];
+        d.t_posr[i] = d.posr[o];
+        d.t_vel[i] = d.vel[o];
+        d.t_omg[i] = d.omg[o];
+        d.t_quat[i] = d.quat[o];
+        d.t_frc[i] = d.frc[o];
+        d.t_trq[i] = d.trq[o];
+        d.t_id[i] = d.id[o];
+        d.t_orig[i] = d.orig[o];
+        d.t_spc[i] = d.spc[o];
+        d.t_cellc[i] = d.cellc[o];
+    }
 }
 
 template <int NS>
 __global__ void __launch_bounds__(kBlock) k_copy_back(Dev<NS> d, Params p)
 {
     if (!rebuild_pending(d.flags)) return;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= p.n) return;
-    d.posr[i] = d.t_posr[i];
-    d.vel[i] = d.t_vel[i];
-    d.omg[i] = d.t_omg[i];
-    d.quat[i] = d.t_quat[i];
-    d.frc[i] = d.t_frc[i];
-    d.trq[i] = d.t_trq[i];
-    d.id[i] = d.t_id[i];
-    d.orig[i] = d.t_orig[i];
-    d.spc[i] = d.t_spc[i];
-    d.cellc[i] = d.t_cellc[i];
-    d.disp[i] = make_double4(0.0, 0.0, 0.0, 0.0);
+    GRID_STRIDE(i, p.n)
+    {
+        d.posr[i] = d.t_posr[i];
+        d.vel[i] = d.t_vel[i];
+        d.omg[i] = d.t_omg[i];
+        d.quat[i] = d.t_quat[i];
+        d.frc[i] = d.t_frc[i];
+        d.trq[i] = d.t_trq[i];
+        d.id[i] = d.t_id[i];
+        d.orig[i] = d.t_orig[i];
+        d.spc[i] = d.t_spc[i];
+        d.cellc[i] = d.t_cellc[i];
+        d.disp[i] = make_double4(0.0, 0.0, 0.0, 0.0);
+    }
 }
 
 // ------------------------------------------------------------------------
@@ -493,71 +504,69 @@ template <int NS>
 __global__ void __launch_bounds__(kBlock) k_build_lists(Dev<NS> d, Params p)
 {
     if (!rebuild_pending(d.flags)) return;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= p.n) return;
-    const double4 pr = d.posr[i];
-    const V3 xi = xyz(pr);
-    const bool fixed_i = d.vel[i].w == 0.0;
-    const size_t base = (size_t)i * p.maxn;
-    int cnt = 0;
-    bool overflow = false;
-    for (int m = 0; m < p.levels; ++m) {
-        const double R = pr.w + p.rmaxl[m] + p.skin;
-        int lo[3], hi[3], nc[3];
-        for (int a = 0; a < 3; ++a) {
-            const double xa = comp(xi, a);
-            int pb = -1;
-            for (int b = 0; b < p.nper; ++b)
-                if (p.paxis[b] == a) pb = b;
-            if (pb >= 0) {
-                nc[a] = p.ncell[m][a];
-                lo[a] = (int)floor((xa - R - p.pleft[pb]) / p.hc[m][a]);
-                hi[a] = (int)floor((xa + R - p.pleft[pb]) / p.hc[m][a]);
-                if (hi[a] - lo[a] + 1 >= nc[a]) {
-                    lo[a] = 0;
-                    hi[a] = nc[a] - 1;
+    GRID_STRIDE(ii, p.n)
+    {
+        const int i = (int)ii;
+        const double4 pr = d.posr[i];
+        const V3 xi = xyz(pr);
+        const bool fixed_i = (d.cellc[i].x & 0x100) != 0;
+        const size_t base = (size_t)i * p.maxn;
+        int cnt = 0;
+        for (int m = 0; m < p.levels; ++m) {
+            const double R = pr.w + p.rmaxl[m] + p.skin;
+            int lo[3], hi[3], nc[3];
+            for (int a = 0; a < 3; ++a) {
+                const double xa = comp(xi, a);
+                const int pb = p.axis_pb[a];
+                if (pb >= 0) {
+                    nc[a] = p.ncell[m][a];
+                    lo[a] = (int)floor((xa - R - p.pleft[pb]) / p.hc[m][a]);
+                    hi[a] = (int)floor((xa + R - p.pleft[pb]) / p.hc[m][a]);
+                    if (hi[a] - lo[a] + 1 >= nc[a]) {
+                        lo[a] = 0;
+                        hi[a] = nc[a] - 1;
+                    }
+                } else {
+                    nc[a] = 0;
+                    lo[a] = (int)floor((xa - R) / p.h[m]);
+                    hi[a] = (int)floor((xa + R) / p.h[m]);
                 }
-            } else {
-                nc[a] = 0;
-                lo[a] = (int)floor((xa - R) / p.h[m]);
-                hi[a] = (int)floor((xa + R) / p.h[m]);
             }
-        }
-        for (int tz = lo[2]; tz <= hi[2]; ++tz) {
-            int cz = tz;
-            if (nc[2]) cz = ((tz % nc[2]) + nc[2]) % nc[2];
-            for (int ty = lo[1]; ty <= hi[1]; ++ty) {
-                int cy = ty;
-                if (nc[1]) cy = ((ty % nc[1]) + nc[1]) % nc[1];
-                for (int tx = lo[0]; tx <= hi[0]; ++tx) {
-                    int cx = tx;
-                    if (nc[0]) cx = ((tx % nc[0]) + nc[0]) % nc[0];
-                    const uint32_t b = p.nboff[m] + (cell_hash(cx, cy, cz) & (p.nb[m] - 1));
-                    const uint32_t s = d.start[b];
-                    const uint32_t e = d.start[b + 1];
-                    for (uint32_t q = s; q < e; ++q) {
-                        if (q == (uint32_t)i) continue;
-                        const int4 cq = d.cellc[q];
-                        if (cq.x != m || cq.y != cx || cq.z != cy || cq.w != cz) continue;
-                        if (fixed_i && d.vel[q].w == 0.0) continue;
-                        const double4 pq = d.posr[q];
-                        const V3 dl = min_image_delta(p, xi, xyz(pq));
-                        const double cut = pr.w + pq.w + p.skin;
-                        if (vlen2(dl) < cut * cut) {
-                            if (cnt < p.maxn)
-                                d.nbr[base + cnt] = q;
-                            else
-                                overflow = true;
-                            ++cnt;
+            const uint32_t mask = p.nb[m] - 1;
+            for (int tz = lo[2]; tz <= hi[2]; ++tz) {
+                int cz = tz;
+                if (nc[2]) cz = ((tz % nc[2]) + nc[2]) % nc[2];
+                for (int ty = lo[1]; ty <= hi[1]; ++ty) {
+                    int cy = ty;
+                    if (nc[1]) cy = ((ty % nc[1]) + nc[1]) % nc[1];
+                    const uint32_t hyz = ((uint32_t)cy * 19349663u) ^ ((uint32_t)cz * 83492791u);
+                    for (int tx = lo[0]; tx <= hi[0]; ++tx) {
+                        int cx = tx;
+                        if (nc[0]) cx = ((tx % nc[0]) + nc[0]) % nc[0];
+                        const uint32_t b = p.nboff[m] + ((((uint32_t)cx * 73856093u) ^ hyz) & mask);
+                        const uint32_t s = d.start[b];
+                        const uint32_t e = d.start[b + 1];
+                        for (uint32_t q = s; q < e; ++q) {
+                            const int4 cq = d.cellc[q];
+                            if ((cq.x & 0xFF) != m || cq.y != cx || cq.z != cy || cq.w != cz) continue;
+                            if (q == (uint32_t)i) continue;
+                            if (fixed_i && (cq.x & 0x100)) continue;
+                            const double4 pq = d.posr[q];
+                            const V3 dl = min_image_delta(p, xi, xyz(pq));
+                            const double cut = pr.w + pq.w + p.skin;
+                            if (vlen2(dl) < cut * cut) {
+                                if (cnt < p.maxn) d.nbr[base + cnt] = q;
+                                ++cnt;
+                            }
                         }
                     }
                 }
             }
         }
+        if (cnt > p.maxn) atomicOr(&d.flags[F_LIST_OVERFLOW], 1);
+        atomicMax(&d.flags[F_MAX_LIST], cnt);
+        d.ncount[i] = cnt < p.maxn ? cnt : p.maxn;
     }
-    if (overflow) atomicOr(&d.flags[F_LIST_OVERFLOW], 1);
-    atomicMax(&d.flags[F_MAX_LIST], cnt);
-    d.ncount[i] = cnt < p.maxn ? cnt : p.maxn;
 }
 
 // Reader slots find the writer's slot; writer slots and wall slots take their
@@ -566,45 +575,47 @@ template <int NS>
 __global__ void __launch_bounds__(kBlock) k_rematch(Dev<NS> d, Params p, int cur)
 {
     if (!rebuild_pending(d.flags)) return;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= p.n) return;
-    const uint32_t idi = d.id[i];
-    const int cnt = d.ncount[i];
-    const size_t base = (size_t)i * p.maxn;
-    unsigned long long m = 0;
-    for (int k = 0; k < cnt; ++k) {
-        const uint32_t j = d.nbr[base + k];
-        const uint32_t idj = d.id[j];
-        if (idi < idj) {
-            Rec<NS> r;
-            if (hash_find(d, ((uint64_t)idi << 32) | idj, r)) {
-                d.rec[cur][base + k] = r;
-                m |= 1ull << k;
-            }
-            d.rev[base + k] = 0xFF;
-        } else {
-            const int cj = d.ncount[j];
-            const size_t bj = (size_t)j * p.maxn;
-            int kk = -1;
-            for (int a = 0; a < cj; ++a)
-                if (d.nbr[bj + a] == (uint32_t)i) {
-                    kk = a;
-                    break;
+    GRID_STRIDE(ii, p.n)
+    {
+        const int i = (int)ii;
+        const uint32_t idi = d.id[i];
+        const int cnt = d.ncount[i];
+        const size_t base = (size_t)i * p.maxn;
+        unsigned long long m = 0;
+        for (int k = 0; k < cnt; ++k) {
+            const uint32_t j = d.nbr[base + k];
+            const uint32_t idj = d.id[j];
+            if (idi < idj) {
+                Rec<NS> r;
+                if (hash_find(d, ((uint64_t)idi << 32) | idj, r)) {
+                    d.rec[cur][base + k] = r;
+                    m |= 1ull << k;
                 }
-            if (kk < 0) atomicOr(&d.flags[F_LIST_ASYM], 1);
-            d.rev[base + k] = (uint8_t)(kk < 0 ? 0xFF : kk);
+                d.rev[base + k] = 0xFF;
+            } else {
+                const int cj = d.ncount[j];
+                const size_t bj = (size_t)j * p.maxn;
+                int kk = -1;
+                for (int a = 0; a < cj; ++a)
+                    if (d.nbr[bj + a] == (uint32_t)i) {
+                        kk = a;
+                        break;
+                    }
+                if (kk < 0) atomicOr(&d.flags[F_LIST_ASYM], 1);
+                d.rev[base + k] = (uint8_t)(kk < 0 ? 0xFF : kk);
+            }
         }
-    }
-    d.mask[cur][i] = m;
-    uint32_t wm = 0;
-    for (int w = 0; w < p.nwalls; ++w) {
-        Rec<NS> r;
-        if (hash_find(d, ((uint64_t)idi << 32) | (DEM_WALL_KEY | (uint32_t)w), r)) {
-            d.wrec[cur][(size_t)i * p.nwalls + w] = r;
-            wm |= 1u << w;
+        d.mask[cur][i] = m;
+        uint32_t wm = 0;
+        for (int w = 0; w < p.nwalls; ++w) {
+            Rec<NS> r;
+            if (hash_find(d, ((uint64_t)idi << 32) | (DEM_WALL_KEY | (uint32_t)w), r)) {
+                d.wrec[cur][(size_t)i * p.nwalls + w] = r;
+                wm |= 1u << w;
+            }
         }
+        d.wmask[cur][i] = wm;
     }
-    d.wmask[cur][i] = wm;
 }
 
 // Interactions that found no slot in the new lists: their pair is now
@@ -619,30 +630,25 @@ template <int NS>
 __global__ void __launch_bounds__(kBlock) k_orphans(Dev<NS> d, Params p)
 {
     if (!rebuild_pending(d.flags)) return;
-    const uint64_t h = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (h >= d.hcap) return;
-    const unsigned long long key = d.hkey[h];
-    if (key == kEmptyKey || d.hmatched[h]) return;
-    const Rec<NS> r = d.hval[h];
-    if (p.erase && !(r.ts > p.nt)) return;
     const int par = d.flags[F_ORPH_PARITY] ^ 1;
-    const int o = atomicAdd(&d.flags[F_ORPH_COUNT0 + par], 1);
-    if (o >= d.ocap) {
-        atomicOr(&d.flags[F_ORPH_OVERFLOW], 1);
-        return;
-    }
-    d.okey[par][o] = key;
-    d.orec[par][o] = r;
-}
-
-template <int NS>
-__global__ void __launch_bounds__(kBlock) k_hash_clear(Dev<NS> d)
-{
-    if (!rebuild_pending(d.flags)) return;
-    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < d.hcap) {
-        d.hkey[i] = kEmptyKey;
-        d.hmatched[i] = 0;
+    GRID_STRIDE(h, d.hcap)
+    {
+        const unsigned long long key = d.hkey[h];
+        if (key == kEmptyKey) continue;
+        if (!d.hmatched[h]) {
+            const Rec<NS> r = d.hval[h];
+            if (!p.erase || r.ts > p.nt) {
+                const int o = atomicAdd(&d.flags[F_ORPH_COUNT0 + par], 1);
+                if (o < d.ocap) {
+                    d.okey[par][o] = key;
+                    d.orec[par][o] = r;
+                } else {
+                    atomicOr(&d.flags[F_ORPH_OVERFLOW], 1);
+                }
+            }
+        }
+        d.hkey[h] = kEmptyKey;
+        d.hmatched[h] = 0;
     }
 }
 
@@ -1035,6 +1041,11 @@ struct dem_ctx {
     uint32_t nbuckets = 0;
     int64_t last_rebuilds = 0;
     uint32_t alive_min = 0;
+    // profiling
+    bool profiling = false;
+    std::vector<hipEvent_t> events;
+    int64_t timed_steps = 0;
+    double t_force = 0, t_kick = 0, t_rebuild = 0;
     int max_list = 0;
 };
 
@@ -1149,8 +1160,8 @@ void launch_scan(dem_ctx* ctx)
     }
     for (size_t k = L - 1; k-- > 0;) {
         const uint32_t n = ctx->scan_sizes[k];
-        hipLaunchKernelGGL(k_scan_add<NS>, dim3(grid_for(n)), dim3(kBlock), 0, s, d, ctx->scan_levels[k],
-                           ctx->scan_levels[k + 1], n);
+        hipLaunchKernelGGL(k_scan_add<NS>, dim3(std::min<unsigned>(grid_for(n), kRebuildBlocks)), dim3(kBlock), 0,
+                           s, d, ctx->scan_levels[k], ctx->scan_levels[k + 1], n);
     }
 }
 
@@ -1160,21 +1171,17 @@ void launch_rebuild(dem_ctx* ctx)
     Dev<NS>& d = dev<NS>(ctx);
     hipStream_t s = ctx->stream;
     const Params& p = ctx->prm;
-    const unsigned gn = grid_for(ctx->n);
-    hipLaunchKernelGGL(k_hash_store<NS>, dim3(gn), dim3(kBlock), 0, s, d, p, ctx->cur);
-    hipLaunchKernelGGL(k_zero_u32<NS>, dim3(grid_for(ctx->nbuckets)), dim3(kBlock), 0, s, d, d.count, ctx->nbuckets);
-    hipLaunchKernelGGL(k_zero_u32<NS>, dim3(grid_for(ctx->nbuckets)), dim3(kBlock), 0, s, d, d.cursor, ctx->nbuckets);
-    hipLaunchKernelGGL(k_keys<NS>, dim3(gn), dim3(kBlock), 0, s, d, p);
+    const dim3 g(kRebuildBlocks), b(kBlock);
+    hipLaunchKernelGGL(k_rb_begin<NS>, g, b, 0, s, d, p, ctx->cur, ctx->nbuckets);
+    hipLaunchKernelGGL(k_keys<NS>, g, b, 0, s, d, p);
     launch_scan<NS>(ctx);
-    hipLaunchKernelGGL(k_scatter<NS>, dim3(gn), dim3(kBlock), 0, s, d, p);
-    hipLaunchKernelGGL(k_sort_buckets<NS>, dim3(grid_for(ctx->nbuckets)), dim3(kBlock), 0, s, d, ctx->nbuckets,
-                       (uint32_t)ctx->n);
-    hipLaunchKernelGGL(k_gather<NS>, dim3(gn), dim3(kBlock), 0, s, d, p);
-    hipLaunchKernelGGL(k_copy_back<NS>, dim3(gn), dim3(kBlock), 0, s, d, p);
-    hipLaunchKernelGGL(k_build_lists<NS>, dim3(gn), dim3(kBlock), 0, s, d, p);
-    hipLaunchKernelGGL(k_rematch<NS>, dim3(gn), dim3(kBlock), 0, s, d, p, ctx->cur);
-    hipLaunchKernelGGL(k_orphans<NS>, dim3(grid_for(d.hcap)), dim3(kBlock), 0, s, d, p);
-    hipLaunchKernelGGL(k_hash_clear<NS>, dim3(grid_for(d.hcap)), dim3(kBlock), 0, s, d);
+    hipLaunchKernelGGL(k_scatter<NS>, g, b, 0, s, d, p);
+    hipLaunchKernelGGL(k_sort_buckets<NS>, g, b, 0, s, d, ctx->nbuckets, (uint32_t)ctx->n);
+    hipLaunchKernelGGL(k_gather<NS>, g, b, 0, s, d, p);
+    hipLaunchKernelGGL(k_copy_back<NS>, g, b, 0, s, d, p);
+    hipLaunchKernelGGL(k_build_lists<NS>, g, b, 0, s, d, p);
+    hipLaunchKernelGGL(k_rematch<NS>, g, b, 0, s, d, p, ctx->cur);
+    hipLaunchKernelGGL(k_orphans<NS>, g, b, 0, s, d, p);
     hipLaunchKernelGGL(k_rebuild_done, dim3(1), dim3(1), 0, s, d.flags);
 }
 
@@ -1213,22 +1220,69 @@ int check_flags(dem_ctx* ctx)
     return DEM_OK;
 }
 
+int ensure_events(dem_ctx* ctx, size_t n)
+{
+    while (ctx->events.size() < n) {
+        hipEvent_t e;
+        HIPCHK(hipEventCreate(&e));
+        ctx->events.push_back(e);
+    }
+    return DEM_OK;
+}
+
 template <int NS>
 int run_steps(dem_ctx* ctx, int64_t nsteps)
 {
-    for (int64_t s = 0; s < nsteps; ++s) {
-        launch_kick<NS>(ctx, s > 0 ? 1 : 0, 1);
-        ctx->prm.erase = 1;
-        ctx->prm.nt = (uint32_t)ctx->nt;
-        ctx->prm.alive_min = ctx->alive_min;
-        launch_rebuild<NS>(ctx);
-        launch_force<NS>(ctx, 1);
-        ctx->time += ctx->dt;
-        ctx->nt += 1;
-        ctx->alive_min = (uint32_t)ctx->nt;
-        ctx->prm.alive_min = ctx->alive_min;
+    // profiling: events e[4s..4s+3] bracket kick | rebuild | force of step s
+    const bool prof = ctx->profiling;
+    constexpr int64_t kChunk = 256;
+    for (int64_t s0 = 0; s0 < nsteps; s0 += kChunk) {
+        const int64_t n = std::min(kChunk, nsteps - s0);
+        if (prof) {
+            int r = ensure_events(ctx, (size_t)(4 * n + 2));
+            if (r) return r;
+        }
+        for (int64_t k = 0; k < n; ++k) {
+            const int64_t s = s0 + k;
+            if (prof) HIPCHK(hipEventRecord(ctx->events[4 * k], ctx->stream));
+            launch_kick<NS>(ctx, s > 0 ? 1 : 0, 1);
+            if (prof) HIPCHK(hipEventRecord(ctx->events[4 * k + 1], ctx->stream));
+            ctx->prm.erase = 1;
+            ctx->prm.nt = (uint32_t)ctx->nt;
+            ctx->prm.alive_min = ctx->alive_min;
+            launch_rebuild<NS>(ctx);
+            if (prof) HIPCHK(hipEventRecord(ctx->events[4 * k + 2], ctx->stream));
+            launch_force<NS>(ctx, 1);
+            if (prof) HIPCHK(hipEventRecord(ctx->events[4 * k + 3], ctx->stream));
+            ctx->time += ctx->dt;
+            ctx->nt += 1;
+            ctx->alive_min = (uint32_t)ctx->nt;
+            ctx->prm.alive_min = ctx->alive_min;
+        }
+        if (s0 + n == nsteps) {
+            if (prof) HIPCHK(hipEventRecord(ctx->events[4 * n], ctx->stream));
+            launch_kick<NS>(ctx, 1, 0);
+            if (prof) HIPCHK(hipEventRecord(ctx->events[4 * n + 1], ctx->stream));
+        }
+        if (prof) {
+            HIPCHK(hipStreamSynchronize(ctx->stream));
+            for (int64_t k = 0; k < n; ++k) {
+                float a = 0, b = 0, c = 0;
+                HIPCHK(hipEventElapsedTime(&a, ctx->events[4 * k], ctx->events[4 * k + 1]));
+                HIPCHK(hipEventElapsedTime(&b, ctx->events[4 * k + 1], ctx->events[4 * k + 2]));
+                HIPCHK(hipEventElapsedTime(&c, ctx->events[4 * k + 2], ctx->events[4 * k + 3]));
+                ctx->t_kick += a;
+                ctx->t_rebuild += b;
+                ctx->t_force += c;
+            }
+            if (s0 + n == nsteps) {
+                float a = 0;
+                HIPCHK(hipEventElapsedTime(&a, ctx->events[4 * n], ctx->events[4 * n + 1]));
+                ctx->t_kick += a;
+            }
+            ctx->timed_steps += n;
+        }
     }
-    if (nsteps > 0) launch_kick<NS>(ctx, 1, 0);
     HIPCHK(hipGetLastError());
     return check_flags(ctx);
 }
@@ -1295,6 +1349,7 @@ int dem_destroy(dem_ctx* ctx)
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     free_all(ctx);
+    for (hipEvent_t e : ctx->events) (void)hipEventDestroy(e);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return DEM_OK;
@@ -1440,7 +1495,9 @@ int dem_upload_particles(dem_ctx* ctx, int64_t n, const uint32_t* id, const int3
     p.rmax = rmax;
     p.skin = skin;
     p.thr2 = (0.5 * skin) * (0.5 * skin) * 0.999;
+    for (int a = 0; a < 3; ++a) p.axis_pb[a] = -1;
     for (int b = 0; b < p.nper; ++b) {
+        p.axis_pb[ctx->periodic[b].axis] = b;
         p.paxis[b] = ctx->periodic[b].axis;
         p.pleft[b] = ctx->periodic[b].left;
         p.pright[b] = ctx->periodic[b].right;
@@ -1710,6 +1767,15 @@ int dem_download_contacts(dem_ctx* ctx, int64_t capacity, int64_t* n_out, uint32
     return DEM_OK;
 }
 
+int dem_set_profiling(dem_ctx* ctx, int32_t on)
+{
+    if (!ctx) return DEM_ERR_ARG;
+    ctx->profiling = on != 0;
+    ctx->timed_steps = 0;
+    ctx->t_force = ctx->t_kick = ctx->t_rebuild = 0.0;
+    return DEM_OK;
+}
+
 int dem_get_stats(dem_ctx* ctx, dem_stats* s)
 {
     if (!ctx || !s) return DEM_ERR_ARG;
@@ -1717,6 +1783,10 @@ int dem_get_stats(dem_ctx* ctx, dem_stats* s)
     s->n_particles = ctx->n;
     s->n_steps = ctx->nt;
     s->time = ctx->time;
+    s->n_timed_steps = ctx->timed_steps;
+    s->t_force_ms = ctx->t_force;
+    s->t_kick_ms = ctx->t_kick;
+    s->t_rebuild_ms = ctx->t_rebuild;
     if (!ctx->uploaded) return DEM_OK;
     int r = check_flags(ctx);
     if (r) return r;

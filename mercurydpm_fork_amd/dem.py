@@ -77,7 +77,8 @@ class Stats(C.Structure):
                 ("n_contacts", C.c_int64), ("n_wall_contacts", C.c_int64),
                 ("n_list_entries", C.c_int64), ("max_list_length", C.c_int32),
                 ("grid_levels", C.c_int32), ("time", C.c_double), ("skin", C.c_double),
-                ("cell_size", C.c_double * 4)]
+                ("cell_size", C.c_double * 4), ("n_timed_steps", C.c_int64), ("t_force_ms", C.c_double),
+                ("t_kick_ms", C.c_double), ("t_rebuild_ms", C.c_double)]
 
 
 _PTR = C.c_void_p
@@ -130,6 +131,8 @@ def load_hip_library(path: Optional[str] = None):
     lib.dem_get_stats.argtypes = [_PTR, C.POINTER(Stats)]
     lib.dem_abi_version.restype = _I32
     lib.dem_abi_version.argtypes = []
+    lib.dem_set_profiling.restype = _I32
+    lib.dem_set_profiling.argtypes = [_PTR, _I32]
     if path is None:
         _hip_lib = lib
     return lib
@@ -297,6 +300,9 @@ class Engine:
                                                   _ptr(sp), _ptr(ts)))
         k = int(n.value)
         return Contacts(p[:k].copy(), i[:k].copy(), sp[:k].copy(), ts[:k].copy())
+
+    def set_profiling(self, on: bool):
+        self._check(self.lib.dem_set_profiling(self.h, 1 if on else 0))
 
     def stats(self) -> Stats:
         s = Stats()

@@ -16,7 +16,7 @@ import numpy as np
 
 from . import dem
 from .species import (PI, Species, cylindrical_wall, infinite_wall, periodic_boundary, species_table,
-                      sphere_inv_mass)
+                      sphere_inv_mass, sphere_inv_mass_array)
 
 
 @dataclass
@@ -79,15 +79,12 @@ def make_state(pos, vel, radius, species_idx, species: List[Species], fixed=None
     radius = np.asarray(radius, dtype=np.float64)
     species_idx = np.asarray(species_idx, dtype=np.int32)
     fixed = np.zeros(n, bool) if fixed is None else np.asarray(fixed, bool)
-    inv_mass = np.empty(n)
+    dens = np.array([sp.density for sp in species], dtype=np.float64)[species_idx]
+    im, ii = sphere_inv_mass_array(radius, dens, dims)
+    inv_mass = np.where(fixed, 0.0, im)
     inv_i = np.zeros((n, 6))
-    for k in range(n):
-        if fixed[k]:
-            inv_mass[k] = 0.0
-            continue
-        im, ii = sphere_inv_mass(float(radius[k]), species[species_idx[k]].density, dims)
-        inv_mass[k] = im
-        inv_i[k, 0] = inv_i[k, 3] = inv_i[k, 5] = ii
+    for c in (0, 3, 5):
+        inv_i[:, c] = np.where(fixed, 0.0, ii)
     vel = vel.copy()
     vel[fixed] = 0.0
     w = np.zeros((n, 3)) if omega is None else np.asarray(omega, dtype=np.float64).reshape(n, 3).copy()
@@ -300,15 +297,47 @@ def lattice_packing(n, lx, ly, z0, a, radii, rng):
     return np.stack([x, y, z], axis=1)
 
 
+def rocksalt_packing(d, lx, ly, z0, rng, jitter=0.004):
+    """Non-overlapping polydisperse packing on a rock-salt lattice: the
+    smaller half of the diameters sits on one sublattice, the larger half on
+    the other, so nearest neighbours are always a small and a large sphere and
+    the spacing only needs to exceed (max small + max large) / 2.  Returns
+    centres in the order of the diameters given."""
+    n = len(d)
+    order = np.argsort(d, kind="stable")
+    small = order[: n // 2]
+    large = order[n // 2:]
+    a = 0.5 * (float(d[small].max()) + float(d[large].max())) + 2.0 * math.sqrt(3.0) * jitter + 1e-3
+    nx = int(math.floor(lx / a)) // 2 * 2
+    ny = int(math.floor(ly / a)) // 2 * 2
+    ax, ay = lx / nx, ly / ny
+    per_layer = nx * ny
+    layers = int(math.ceil(2 * max(len(small), len(large)) / per_layer)) + 1
+    iz, iy, ix = np.meshgrid(np.arange(layers), np.arange(ny), np.arange(nx), indexing="ij")
+    ix, iy, iz = ix.ravel(), iy.ravel(), iz.ravel()
+    parity = (ix + iy + iz) % 2
+    sites_a = np.flatnonzero(parity == 0)[: len(small)]
+    sites_b = np.flatnonzero(parity == 1)[: len(large)]
+    pos = np.empty((n, 3))
+    for idx, sites in ((small, sites_a), (large, sites_b)):
+        perm = rng.permutation(len(idx))
+        st = sites[perm]
+        pos[idx, 0] = (ix[st] + 0.5) * ax
+        pos[idx, 1] = (iy[st] + 0.5) * ay
+        pos[idx, 2] = z0 + (iz[st] + 0.5) * a
+    pos += rng.uniform(-jitter, jitter, pos.shape)
+    return pos
+
+
 def chute_hvm(n_flow=2_000_000, seed=3, seed_base=4, angle_deg=24.0, l=None, depth=30.0,
-              base_d=1.0, lattice=1.55) -> Scenario:
+              base_d=1.0) -> Scenario:
     """Polydisperse periodic chute: n_flow spheres with d ~ U[0.5, 1.5],
     tilted by angle_deg (gravity (sin, 0, -cos), |g| = 1, the SilbertPeriodic
     convention of Chute::setChuteAngleAndMagnitudeOfGravity, Chute.cc:789-805),
     periodic in x and y, a rough base of fixed spheres (d = base_d, square
     lattice with random heights) over an InfiniteWall.  HertzianViscoelastic-
     Mindlin with e = 0.8 and mu = 0.5, dt = t_c / 50.  The flowing spheres
-    start on a jittered lattice (loose, non-overlapping); the footprint is
+    start on a rock-salt lattice (loose, non-overlapping); the footprint is
     chosen so that the bed would be `depth` diameters deep at packing
     fraction 0.55."""
     rng = np.random.default_rng(seed)
@@ -329,7 +358,7 @@ def chute_hvm(n_flow=2_000_000, seed=3, seed_base=4, angle_deg=24.0, l=None, dep
     bz = rngb.uniform(-0.25, 0.25, bx.size) * base_d
     base = np.stack([bx, by, bz], axis=1)
     r_base = np.full(bx.size, 0.5 * base_d)
-    flow = lattice_packing(n_flow, l, l, 0.75 * base_d, lattice, r_flow, rng)
+    flow = rocksalt_packing(d, l, l, 1.0 * base_d, rng)
     pos = np.concatenate([base, flow])
     rad = np.concatenate([r_base, r_flow])
     fixed = np.concatenate([np.ones(bx.size, bool), np.zeros(n_flow, bool)])

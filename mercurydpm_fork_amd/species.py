@@ -55,6 +55,20 @@ def sphere_inv_mass(radius: float, density: float, dims: int = 3):
     return inv_mass, inv_i
 
 
+def sphere_inv_mass_array(radius, density, dims: int = 3):
+    """Element-wise sphere_inv_mass for numpy arrays (same operation order)."""
+    import numpy as np
+    r = np.asarray(radius, dtype=np.float64)
+    rho = np.asarray(density, dtype=np.float64)
+    if dims == 3:
+        inv_mass = 1.0 / (4.0 / 3.0 * PI * r * r * r * rho)
+        inv_i = 1.0 / (.4 * (1.0 / inv_mass) * (r * r))
+    else:
+        inv_mass = 1.0 / (PI * r * r * rho)
+        inv_i = 1.0 / (.5 * (1.0 / inv_mass) * (r * r))
+    return inv_mass, inv_i
+
+
 def volume_from_radius(radius: float, dims: int = 3) -> float:
     """BaseParticle::getVolume (BaseParticle.cc:147-166)."""
     return 4.0 / 3.0 * PI * radius * radius * radius if dims == 3 else PI * radius * radius
