@@ -185,6 +185,13 @@ constexpr int kRebuildBlocks = 2048;
 #define GRID_STRIDE(i, n) \
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < (uint64_t)(n); i += (uint64_t)gridDim.x * blockDim.x)
 
+// Verlet-list slots are stored k-major (slot of entry k of particle i is
+// k * n + i) so that a wavefront reading entry k of 64 consecutive particles
+// touches one contiguous segment.  An entry is the partner's sorted index with
+// bit 31 set when this particle has the lower id (it writes the record).
+constexpr uint32_t kWriterBit = 0x80000000u;
+__host__ __device__ inline size_t lslot(int k, int n, int i) { return (size_t)k * n + i; }
+
 __device__ inline bool rebuild_pending(const int* flags) { return __atomic_load_n(&flags[F_REBUILD], __ATOMIC_RELAXED) != 0; }
 
 // ------------------------------------------------------------------------
@@ -306,9 +313,9 @@ __global__ void __launch_bounds__(kBlock) k_rb_begin(Dev<NS> d, Params p, int cu
         while (m) {
             const int k = __ffsll((long long)m) - 1;
             m &= m - 1;
-            const uint32_t j = d.nbr[i * p.maxn + k];
+            const uint32_t j = d.nbr[lslot(k, p.n, (int)i)] & ~kWriterBit;
             const uint64_t key = ((uint64_t)idi << 32) | d.id[j];
-            hash_insert(d, key, d.rec[cur][i * p.maxn + k]);
+            hash_insert(d, key, d.rec[cur][lslot(k, p.n, (int)i)]);
         }
         uint32_t wm = d.wmask[cur][i];
         while (wm) {
@@ -510,7 +517,7 @@ __global__ void __launch_bounds__(kBlock) k_build_lists(Dev<NS> d, Params p)
         const double4 pr = d.posr[i];
         const V3 xi = xyz(pr);
         const bool fixed_i = (d.cellc[i].x & 0x100) != 0;
-        const size_t base = (size_t)i * p.maxn;
+        const uint32_t idi = d.id[i];
         int cnt = 0;
         for (int m = 0; m < p.levels; ++m) {
             const double R = pr.w + p.rmaxl[m] + p.skin;
@@ -555,7 +562,7 @@ __global__ void __launch_bounds__(kBlock) k_build_lists(Dev<NS> d, Params p)
                             const V3 dl = min_image_delta(p, xi, xyz(pq));
                             const double cut = pr.w + pq.w + p.skin;
                             if (vlen2(dl) < cut * cut) {
-                                if (cnt < p.maxn) d.nbr[base + cnt] = q;
+                                if (cnt < p.maxn) d.nbr[lslot(cnt, p.n, i)] = q | (idi < d.id[q] ? kWriterBit : 0u);
                                 ++cnt;
                             }
                         }
@@ -580,29 +587,28 @@ __global__ void __launch_bounds__(kBlock) k_rematch(Dev<NS> d, Params p, int cur
         const int i = (int)ii;
         const uint32_t idi = d.id[i];
         const int cnt = d.ncount[i];
-        const size_t base = (size_t)i * p.maxn;
         unsigned long long m = 0;
         for (int k = 0; k < cnt; ++k) {
-            const uint32_t j = d.nbr[base + k];
-            const uint32_t idj = d.id[j];
-            if (idi < idj) {
+            const size_t sl = lslot(k, p.n, i);
+            const uint32_t e = d.nbr[sl];
+            const uint32_t j = e & ~kWriterBit;
+            if (e & kWriterBit) {
                 Rec<NS> r;
-                if (hash_find(d, ((uint64_t)idi << 32) | idj, r)) {
-                    d.rec[cur][base + k] = r;
+                if (hash_find(d, ((uint64_t)idi << 32) | d.id[j], r)) {
+                    d.rec[cur][sl] = r;
                     m |= 1ull << k;
                 }
-                d.rev[base + k] = 0xFF;
+                d.rev[sl] = 0xFF;
             } else {
                 const int cj = d.ncount[j];
-                const size_t bj = (size_t)j * p.maxn;
                 int kk = -1;
                 for (int a = 0; a < cj; ++a)
-                    if (d.nbr[bj + a] == (uint32_t)i) {
+                    if ((d.nbr[lslot(a, p.n, (int)j)] & ~kWriterBit) == (uint32_t)i) {
                         kk = a;
                         break;
                     }
                 if (kk < 0) atomicOr(&d.flags[F_LIST_ASYM], 1);
-                d.rev[base + k] = (uint8_t)(kk < 0 ? 0xFF : kk);
+                d.rev[sl] = (uint8_t)(kk < 0 ? 0xFF : kk);
             }
         }
         d.mask[cur][i] = m;
@@ -675,58 +681,73 @@ __global__ void k_rebuild_done(int* flags)
 // this pass are dropped when their time stamp is not newer than the step
 // counter (eraseOldInteractions, InteractionHandler.cc:377-393).
 // ------------------------------------------------------------------------
-template <int NS>
-__global__ void __launch_bounds__(kBlock) k_force(Dev<NS> d, Params p, int cur)
-{
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= p.n) return;
-    const double4 pr = d.posr[i];
-    const double4 vl = d.vel[i];
-    const double4 om = d.omg[i];
-    const uint32_t idi = d.id[i];
-    const int si = (int)(d.spc[i] & 0xFFFF);
-    const bool fixed_i = vl.w == 0.0;
-    Body bi;
-    bi.pos = xyz(pr);
-    bi.vel = xyz(vl);
-    bi.omega = xyz(om);
-    bi.radius = pr.w;
-    bi.inv_mass = vl.w;
-    V3 F = zero3(), T = zero3();
-    const uint32_t ts_new = p.nt + 1;
-    const Rec<NS>* rin = d.rec[cur];
-    Rec<NS>* rout = d.rec[cur ^ 1];
-    const unsigned long long m_in = d.mask[cur][i];
-    unsigned long long m_out = 0, touched = 0;
-    const int cnt = d.ncount[i];
-    const size_t base = (size_t)i * p.maxn;
+// The kernel works per wavefront of 64 particles ("owners", one per lane):
+//  scan   -- every lane walks its Verlet list and tests the contact
+//            predicate; contacts are compacted into an LDS queue with a
+//            wavefront ballot + prefix count (mbcnt), so
+//  process -- the force law runs with all 64 lanes busy, one queued contact
+//            each, instead of diverging over list entries that are not in
+//            contact.  Contributions go to LDS and each owner adds its own in
+//            queue order, i.e. in list order: the sum is deterministic.
+// Both particles of a pair evaluate it (no atomics); the lower id writes the
+// history record.
+constexpr int kWaves = kBlock / 64;
 
-    for (int k = 0; k < cnt; ++k) {
-        const uint32_t j = d.nbr[base + k];
+__device__ inline void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ inline uint32_t lane_prefix(unsigned long long ballot)
+{
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(ballot >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)ballot, 0u));
+}
+
+// queue entry: j (32) | k (6) << 32 | owner lane (6) << 38 | image shift
+// code per periodic boundary (2 bits each: 1 = +shift, 2 = -shift) << 44 |
+// img << 50 | i_is_P << 51 | writer << 52
+struct OwnerLds {
+    double px[64], py[64], pz[64], vx[64], vy[64], vz[64], wx[64], wy[64], wz[64], r[64], im[64];
+    unsigned long long m_in[64];
+    uint32_t id[64];
+    int32_t sp[64];
+    int32_t row[64];
+    unsigned long long q[128];
+    double cf[6][64];
+};
+
+template <int NS>
+__device__ inline void process_batch(Dev<NS>& d, const Params& p, int cur, OwnerLds& L, int nb, int lane)
+{
+    if (lane < nb) {
+        const unsigned long long e = L.q[lane];
+        const uint32_t j = (uint32_t)e;
+        const int k = (int)((e >> 32) & 63);
+        const int o = (int)((e >> 38) & 63);
+        const int codes = (int)((e >> 44) & 63);
+        const bool img = (e >> 50) & 1;
+        const bool i_is_P = (e >> 51) & 1;
+        const bool writer = (e >> 52) & 1;
+        Body bi;
+        bi.pos = mk(L.px[o], L.py[o], L.pz[o]);
+        bi.vel = mk(L.vx[o], L.vy[o], L.vz[o]);
+        bi.omega = mk(L.wx[o], L.wy[o], L.wz[o]);
+        bi.radius = L.r[o];
+        bi.inv_mass = L.im[o];
         const double4 pq = d.posr[j];
         V3 xj = xyz(pq);
-        bool img = false, exists = true;
         for (int b = 0; b < p.nper; ++b) {
-            const int a = p.paxis[b];
-            const double c = comp(xj, a) - comp(bi.pos, a);
-            if (c > 0.5 * p.pshift[b] || c < -0.5 * p.pshift[b]) {
-                img = true;
-                if (!(periodic_distance(p, b, xj) < pq.w + p.rmax)) exists = false;
-                xj = periodic_shift(p, b, xj);
+            const int c = (codes >> (2 * b)) & 3;
+            if (c) {
+                // PeriodicBoundary::shiftPosition: position += shift_ or += -shift_
+                const int a = p.paxis[b];
+                V3 n = mk(a == 0 ? 1.0 : 0.0, a == 1 ? 1.0 : 0.0, a == 2 ? 1.0 : 0.0);
+                V3 sh = vmul(n, p.pshift[b]);
+                xj = vadd(xj, c == 1 ? sh : vneg(sh));
             }
         }
-        if (img && !exists) continue;
-        const uint32_t idj = d.id[j];
-        const bool i_is_P = img || idi < idj;
-        const V3 posP = i_is_P ? bi.pos : xj;
-        const V3 posI = i_is_P ? xj : bi.pos;
-        const double rP = i_is_P ? pr.w : pq.w;
-        const double rI = i_is_P ? pq.w : pr.w;
-        const V3 branch = vsub(posP, posI);
-        const double d2 = vlen2(branch);
-        const double sum_r = rI + rP;
-        if (d2 >= sum_r * sum_r) continue;
-
         const double4 vq = d.vel[j];
         const double4 oq = d.omg[j];
         Body bj;
@@ -737,29 +758,34 @@ __global__ void __launch_bounds__(kBlock) k_force(Dev<NS> d, Params p, int cur)
         bj.inv_mass = vq.w;
         const Body& P = i_is_P ? bi : bj;
         const Body& I = i_is_P ? bj : bi;
+        const V3 branch = vsub(P.pos, I.pos);
+        const double d2 = vlen2(branch);
         ContactGeom g;
         const double distance = sqrt(d2);
         g.normal = vdiv(branch, distance);
-        g.overlap = rP + rI - distance;
+        g.overlap = P.radius + I.radius - distance;
         g.distance = distance;
-        g.contact = vsub(posP, vmul(g.normal, rP - 0.5 * g.overlap));
+        g.contact = vsub(P.pos, vmul(g.normal, P.radius - 0.5 * g.overlap));
         g.i_is_wall = false;
-
-        const bool writer = idi < idj;
+        const int i = L.row[o];
+        const size_t slot = lslot(k, p.n, i);
+        const Rec<NS>* rin = d.rec[cur];
         double spring[NS];
         for (int a = 0; a < NS; ++a) spring[a] = 0.0;
+        bool had = false;
         if (writer) {
-            if ((m_in >> k) & 1ull)
-                for (int a = 0; a < NS; ++a) spring[a] = rin[base + k].s[a];
+            had = (L.m_in[o] >> k) & 1ull;
+            if (had)
+                for (int a = 0; a < NS; ++a) spring[a] = rin[slot].s[a];
         } else {
-            const uint32_t kk = d.rev[base + k];
+            const uint32_t kk = d.rev[slot];
             if (kk != 0xFF && ((d.mask[cur][j] >> kk) & 1ull)) {
-                const Rec<NS>& r = rin[(size_t)j * p.maxn + kk];
+                const Rec<NS>& r = rin[lslot((int)kk, p.n, (int)j)];
                 for (int a = 0; a < NS; ++a) spring[a] = r.s[a];
             }
             if (img) {
-                // the stored frame has the lower id as P; this image evaluation
-                // has the higher id as P (BaseInteraction::reverseHistory)
+                // the stored frame has the lower id as P; this image
+                // evaluation has the higher id as P (reverseHistory)
                 spring[0] = -spring[0];
                 spring[1] = -spring[1];
                 spring[2] = -spring[2];
@@ -771,39 +797,166 @@ __global__ void __launch_bounds__(kBlock) k_force(Dev<NS> d, Params p, int cur)
             }
         }
         const int sj = (int)(d.spc[j] & 0xFFFF);
-        const dem_pair_params sp = d.sp[si * p.ns + sj];
+        const dem_pair_params& sp = d.sp[L.sp[o] * p.ns + sj];
         V3 f, tc;
         contact_force<NS>(sp, P, I, g, spring, p.dt, f, tc);
+        V3 cF, cT = zero3();
         if (i_is_P) {
-            F = vadd(F, f);
-            if (p.rotation) T = vadd(T, vsub(tc, vcross(vsub(posP, g.contact), f)));
+            cF = f;
+            if (p.rotation) cT = vsub(tc, vcross(vsub(P.pos, g.contact), f));
         } else {
-            F = vadd(F, vneg(f));
-            if (p.rotation) T = vadd(T, vadd(vneg(tc), vcross(vsub(posI, g.contact), f)));
+            cF = vneg(f);
+            if (p.rotation) cT = vadd(vneg(tc), vcross(vsub(I.pos, g.contact), f));
         }
+        L.cf[0][lane] = cF.x;
+        L.cf[1][lane] = cF.y;
+        L.cf[2][lane] = cF.z;
+        L.cf[3][lane] = cT.x;
+        L.cf[4][lane] = cT.y;
+        L.cf[5][lane] = cT.z;
         if (writer) {
             Rec<NS> r;
             for (int a = 0; a < NS; ++a) r.s[a] = spring[a];
-            r.ts = ts_new;
+            r.ts = p.nt + 1;
             r.pad = 0;
             // An image contact is stored through the ghost's copy of the
             // interaction, which replaces the real-real record only when its
             // time stamp is newer (InteractionHandler.cc:339-350); with equal
             // stamps (the initial force pass and step 0 both stamp 1) the
             // previous record survives unchanged.
-            if (img && ((m_in >> k) & 1ull) && rin[base + k].ts >= ts_new) r = rin[base + k];
-            rout[base + k] = r;
-            m_out |= 1ull << k;
-            touched |= 1ull << k;
+            if (img && had && rin[slot].ts >= r.ts) r = rin[slot];
+            d.rec[cur ^ 1][slot] = r;
         }
     }
+}
+
+template <int NS>
+__global__ void __launch_bounds__(kBlock) k_force(Dev<NS> d, Params p, int cur)
+{
+    __shared__ OwnerLds lds[kWaves];
+    const int lane = threadIdx.x & 63;
+    OwnerLds& L = lds[threadIdx.x >> 6];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = i < p.n;
+    const int ii = valid ? i : 0;
+    const double4 pr = d.posr[ii];
+    const double4 vl = d.vel[ii];
+    const double4 om = d.omg[ii];
+    const uint32_t idi = d.id[ii];
+    const int si = (int)(d.spc[ii] & 0xFFFF);
+    const bool fixed_i = vl.w == 0.0;
+    Body bi;
+    bi.pos = xyz(pr);
+    bi.vel = xyz(vl);
+    bi.omega = xyz(om);
+    bi.radius = pr.w;
+    bi.inv_mass = vl.w;
+    const unsigned long long m_in = valid ? d.mask[cur][ii] : 0ull;
+    L.px[lane] = pr.x;
+    L.py[lane] = pr.y;
+    L.pz[lane] = pr.z;
+    L.vx[lane] = vl.x;
+    L.vy[lane] = vl.y;
+    L.vz[lane] = vl.z;
+    L.wx[lane] = om.x;
+    L.wy[lane] = om.y;
+    L.wz[lane] = om.z;
+    L.r[lane] = pr.w;
+    L.im[lane] = vl.w;
+    L.m_in[lane] = m_in;
+    L.id[lane] = idi;
+    L.sp[lane] = si;
+    L.row[lane] = ii;
+    const int cnt = valid ? d.ncount[ii] : 0;
+    unsigned long long touched = 0;
+    V3 F = zero3(), T = zero3();
+    int qn = 0;
+    wave_sync();
+
+    for (int k = 0;; ++k) {
+        const bool active = k < cnt;
+        if (!__any(active)) break;
+        bool contact = false;
+        unsigned long long entry = 0;
+        if (active) {
+            const uint32_t ej = d.nbr[lslot(k, p.n, ii)];
+            const uint32_t j = ej & ~kWriterBit;
+            const bool writer = (ej & kWriterBit) != 0;
+            const double4 pq = d.posr[j];
+            V3 xj = xyz(pq);
+            bool img = false, exists = true;
+            int codes = 0;
+            for (int b = 0; b < p.nper; ++b) {
+                const int a = p.paxis[b];
+                const double c = comp(xj, a) - comp(bi.pos, a);
+                if (c > 0.5 * p.pshift[b] || c < -0.5 * p.pshift[b]) {
+                    img = true;
+                    if (!(periodic_distance(p, b, xj) < pq.w + p.rmax)) exists = false;
+                    const double dd = comp(xj, a);
+                    codes |= (p.pright[b] - dd > dd - p.pleft[b] ? 1 : 2) << (2 * b);
+                    xj = periodic_shift(p, b, xj);
+                }
+            }
+            if (!img || exists) {
+                const bool i_is_P = img || writer;
+                const V3 posP = i_is_P ? bi.pos : xj;
+                const V3 posI = i_is_P ? xj : bi.pos;
+                const double rP = i_is_P ? pr.w : pq.w;
+                const double rI = i_is_P ? pq.w : pr.w;
+                const V3 branch = vsub(posP, posI);
+                const double d2 = vlen2(branch);
+                const double sum_r = rI + rP;
+                if (d2 < sum_r * sum_r) {
+                    contact = true;
+                    entry = (unsigned long long)j | ((unsigned long long)k << 32) | ((unsigned long long)lane << 38) |
+                            ((unsigned long long)codes << 44) | ((unsigned long long)img << 50) |
+                            ((unsigned long long)i_is_P << 51) | ((unsigned long long)writer << 52);
+                    if (writer) touched |= 1ull << k;
+                }
+            }
+        }
+        const unsigned long long ballot = __ballot(contact);
+        if (contact) L.q[qn + lane_prefix(ballot)] = entry;
+        qn += __popcll(ballot);
+        if (qn >= 64) {
+            wave_sync();
+            process_batch<NS>(d, p, cur, L, 64, lane);
+            wave_sync();
+            for (int a = 0; a < 64; ++a) {
+                if ((int)((L.q[a] >> 38) & 63) == lane) {
+                    F = vadd(F, mk(L.cf[0][a], L.cf[1][a], L.cf[2][a]));
+                    T = vadd(T, mk(L.cf[3][a], L.cf[4][a], L.cf[5][a]));
+                }
+            }
+            wave_sync();
+            qn -= 64;
+            if (lane < qn) L.q[lane] = L.q[64 + lane];
+            wave_sync();
+        }
+    }
+    if (qn > 0) {
+        wave_sync();
+        process_batch<NS>(d, p, cur, L, qn, lane);
+        wave_sync();
+        for (int a = 0; a < qn; ++a) {
+            if ((int)((L.q[a] >> 38) & 63) == lane) {
+                F = vadd(F, mk(L.cf[0][a], L.cf[1][a], L.cf[2][a]));
+                T = vadd(T, mk(L.cf[3][a], L.cf[4][a], L.cf[5][a]));
+            }
+        }
+    }
+    if (!valid) return;
+    const Rec<NS>* rin = d.rec[cur];
+    Rec<NS>* rout = d.rec[cur ^ 1];
+    const uint32_t ts_new = p.nt + 1;
+    unsigned long long m_out = touched;
     unsigned long long carry = m_in & ~touched;
     while (carry) {
         const int k = __ffsll((long long)carry) - 1;
         carry &= carry - 1;
-        const Rec<NS> r = rin[base + k];
+        const Rec<NS> r = rin[lslot(k, p.n, i)];
         if (!p.erase || r.ts > p.nt) {
-            rout[base + k] = r;
+            rout[lslot(k, p.n, i)] = r;
             m_out |= 1ull << k;
         }
     }
@@ -918,9 +1071,9 @@ __global__ void __launch_bounds__(kBlock) k_collect_contacts(Dev<NS> d, Params p
         m &= m - 1;
         const int o = atomicAdd(counter, 1);
         if (o >= cap) continue;
-        const size_t s = (size_t)i * p.maxn + k;
+        const size_t s = lslot(k, p.n, i);
         pid[o] = idi;
-        iid[o] = d.id[d.nbr[s]];
+        iid[o] = d.id[d.nbr[s] & ~kWriterBit];
         for (int a = 0; a < 9; ++a) spring[9 * (size_t)o + a] = a < NS ? d.rec[cur][s].s[a] : 0.0;
         ts[o] = d.rec[cur][s].ts;
     }
@@ -1466,7 +1619,7 @@ int dem_upload_particles(dem_ctx* ctx, int64_t n, const uint32_t* id, const int3
         rmin = std::min(rmin, radius[i]);
         rmax = std::max(rmax, radius[i]);
     }
-    const double skin = ctx->skin_cfg > 0 ? ctx->skin_cfg : 0.1 * rmin;
+    const double skin = ctx->skin_cfg > 0 ? ctx->skin_cfg : 0.2 * rmin;
     // levels (EXPONENTIAL cell-size distribution of MercuryBase::hGridRebuild,
     // MercuryBase.cc:204-306, between the smallest and largest diameter)
     int L = ctx->levels_cfg;
