@@ -1,7 +1,9 @@
 """Builds the package's native code in-tree:
 
 * lib/libmercurydpm_dem.so  -- HIP engine for gfx950 (csrc/dem_engine.hip)
-* lib/libmercurydpm_host.so -- C++ host mirror of the reference API (host/)
+* lib/libmercurydpm_host.so -- C++ host API keeping the reference's driver
+  interface (host/Mercury3D.h)
+* tests/drivers/bin/*        -- drivers written against that API
 
 Run ``python -m mercurydpm_fork_amd.build`` or ``__graft_entry__.build()``.
 """
@@ -43,8 +45,37 @@ def build_engine():
     return out
 
 
+CXX = os.environ.get("CXX", "g++")
+HOST_FLAGS = ["-std=c++17", "-O2", "-fPIC", "-Wall", "-Wextra", "-ffp-contract=off"]
+DRIVER_DIR = os.path.join(ROOT, "tests", "drivers")
+
+
+def build_host():
+    """C++ host API (host/Mercury3D.cc) linked against the engine."""
+    out = os.path.join(LIB, "libmercurydpm_host.so")
+    _run([CXX, *HOST_FLAGS, "-shared", "-o", out, os.path.join(HERE, "host", "Mercury3D.cc"),
+          "-L" + LIB, "-lmercurydpm_dem", "-Wl,-rpath,$ORIGIN"])
+    return out
+
+
+def build_drivers():
+    """Driver programs written against the host API (tests/drivers/*.cpp)."""
+    outs = []
+    bindir = os.path.join(DRIVER_DIR, "bin")
+    os.makedirs(bindir, exist_ok=True)
+    for f in sorted(os.listdir(DRIVER_DIR)):
+        if not f.endswith(".cpp"):
+            continue
+        out = os.path.join(bindir, f[:-4])
+        _run([CXX, *HOST_FLAGS, "-I" + os.path.join(HERE, "host"), "-o", out, os.path.join(DRIVER_DIR, f),
+              "-L" + LIB, "-lmercurydpm_host", "-lmercurydpm_dem",
+              "-Wl,-rpath,$ORIGIN/../../../mercurydpm_fork_amd/lib"])
+        outs.append(out)
+    return outs
+
+
 def build_all():
-    return [build_engine()]
+    return [build_engine(), build_host(), *build_drivers()]
 
 
 if __name__ == "__main__":

@@ -1,0 +1,411 @@
+// Mercury3D.cc -- implementation of the C++ host API (see Mercury3D.h).
+#include "Mercury3D.h"
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+
+// mathsFunc::log (Kernel/Math/ExtendedMath.cc:104-129)
+double mathsFunc::log(double Power)
+{
+    const double E = 2.71828182845905;
+    double P = Power, N = 0.0, A, L, R;
+    while (P >= E) {
+        P /= E;
+        N++;
+    }
+    N += (P / E);
+    P = Power;
+    do {
+        A = N;
+        L = (P / (std::exp(N - 1.0)));
+        R = ((N - 1.0) * E);
+        N = ((L + R) / E);
+    } while (N < A);
+    return N;
+}
+
+// ParticleSpecies::getMassFromRadius with the 3-D volume of
+// BaseParticle::getVolume (BaseParticle.cc:147-166)
+double ParticleSpecies::getMassFromRadius(double radius) const
+{
+    return getDensity() * (4.0 / 3.0 * constants::pi * radius * radius * radius);
+}
+
+// LinearViscoelasticNormalSpecies::setCollisionTimeAndRestitutionCoefficient
+// (LinearViscoelasticNormalSpecies.cc:225-245)
+void LinearViscoelasticSpecies::setCollisionTimeAndRestitutionCoefficient(double tc, double eps, double mass)
+{
+    if (!(tc > 0) || !(eps >= 0 && eps <= 1) || !(mass > 0))
+        throw std::invalid_argument("setCollisionTimeAndRestitutionCoefficient: tc > 0, 0 <= eps <= 1, mass > 0");
+    if (params.constant_restitution) mass = 1;
+    if (eps == 0.0) {
+        params.stiffness = .5 * mass * mathsFunc::square(constants::pi / tc);
+        params.dissipation = std::sqrt(2.0 * mass * params.stiffness);
+    } else {
+        params.dissipation = -mass / tc * mathsFunc::log(eps);
+        params.stiffness =
+            .5 * mass * (mathsFunc::square(constants::pi / tc) + mathsFunc::square(params.dissipation / mass));
+    }
+}
+
+// setStiffnessAndRestitutionCoefficient / setRestitutionCoefficient
+// (LinearViscoelasticNormalSpecies.cc:207-222)
+void LinearViscoelasticSpecies::setStiffnessAndRestitutionCoefficient(double k, double eps, double mass)
+{
+    if (params.constant_restitution) mass = 1;
+    params.stiffness = k;
+    if (eps == 0.0) {
+        params.dissipation = std::sqrt(2.0 * mass * k);
+    } else {
+        const double logEps = std::log(eps);
+        params.dissipation = -std::sqrt(2.0 * mass * k / (constants::sqr_pi + mathsFunc::square(logEps))) * logEps;
+    }
+}
+
+// LinearViscoelasticNormalSpecies::getCollisionTime (cc:137-166)
+double LinearViscoelasticSpecies::getCollisionTime(double mass) const
+{
+    if (params.constant_restitution) mass = 1;
+    const double tosqrt = params.stiffness / (.5 * mass) - mathsFunc::square(params.dissipation / mass);
+    return constants::pi / std::sqrt(tosqrt);
+}
+
+double LinearViscoelasticSpecies::getRestitutionCoefficient(double mass) const
+{
+    if (params.constant_restitution) mass = 1;
+    return std::exp(-params.dissipation / mass * getCollisionTime(mass));
+}
+
+// HertzianViscoelasticNormalSpecies::setEffectiveElasticModulusAndRestitutionCoefficient
+// (HertzianViscoelasticNormalSpecies.cc:109-135)
+void HertzianViscoelasticMindlinSpecies::setEffectiveElasticModulusAndRestitutionCoefficient(double E, double rest)
+{
+    if (E < 0 || rest < 0 || rest > 1) throw std::invalid_argument("elastic modulus >= 0, 0 <= restitution <= 1");
+    params.elastic_modulus = E;
+    if (rest > 0.0) {
+        const double logRestSquared = std::log(rest) * std::log(rest);
+        params.dissipation = std::sqrt(5.0 * logRestSquared / (logRestSquared + constants::sqr_pi));
+    } else {
+        params.dissipation = std::sqrt(5.0);
+    }
+}
+
+double HertzianViscoelasticMindlinSpecies::getCollisionTime(double diameter, double density, double v) const
+{
+    return 2.214 * std::pow(mathsFunc::square(density / params.elastic_modulus) / v, 0.2) * diameter;
+}
+
+// BaseSpecies::average (BaseSpecies.cc:110-115)
+static double average(double a, double b) { return (a == 0 || b == 0) ? 0.0 : (2. / (1.0 / a + 1.0 / b)); }
+
+static dem_pair_params mixParams(const dem_pair_params& a, const dem_pair_params& b)
+{
+    dem_pair_params m = a;
+    m.stiffness = average(a.stiffness, b.stiffness);
+    m.dissipation = average(a.dissipation, b.dissipation);
+    m.elastic_modulus = average(a.elastic_modulus, b.elastic_modulus);
+    m.sliding_stiffness = average(a.sliding_stiffness, b.sliding_stiffness);
+    m.sliding_dissipation = average(a.sliding_dissipation, b.sliding_dissipation);
+    m.sliding_mu = average(a.sliding_mu, b.sliding_mu);
+    m.sliding_mu_static = average(a.sliding_mu_static, b.sliding_mu_static);
+    m.shear_modulus = average(a.shear_modulus, b.shear_modulus);
+    m.rolling_stiffness = average(a.rolling_stiffness, b.rolling_stiffness);
+    m.rolling_dissipation = average(a.rolling_dissipation, b.rolling_dissipation);
+    m.rolling_mu = average(a.rolling_mu, b.rolling_mu);
+    m.rolling_mu_static = average(a.rolling_mu_static, b.rolling_mu_static);
+    m.torsion_stiffness = average(a.torsion_stiffness, b.torsion_stiffness);
+    m.torsion_dissipation = average(a.torsion_dissipation, b.torsion_dissipation);
+    m.torsion_mu = average(a.torsion_mu, b.torsion_mu);
+    m.torsion_mu_static = average(a.torsion_mu_static, b.torsion_mu_static);
+    return m;
+}
+
+dem_pair_params* SpeciesHandler::getMixedObject(unsigned a, unsigned b)
+{
+    const unsigned n = getNumberOfObjects();
+    if (a >= n || b >= n) throw std::out_of_range("getMixedObject: no such species");
+    if (a == b) return &objects_[a]->params;
+    if (mixed_.size() != (size_t)n * n) mixed_.assign((size_t)n * n, {false, dem_pair_params{}});
+    const unsigned lo = std::min(a, b), hi = std::max(a, b);
+    auto& e = mixed_[(size_t)lo * n + hi];
+    if (!e.first) {
+        e.second = mixParams(objects_[hi]->params, objects_[lo]->params);
+        e.first = true;
+    }
+    return &e.second;
+}
+
+std::vector<dem_pair_params> SpeciesHandler::table()
+{
+    const unsigned n = getNumberOfObjects();
+    std::vector<dem_pair_params> t((size_t)n * n);
+    for (unsigned a = 0; a < n; ++a)
+        for (unsigned b = 0; b < n; ++b) t[(size_t)a * n + b] = *getMixedObject(a, b);
+    return t;
+}
+
+bool SpeciesHandler::useAngularDOFs() const
+{
+    for (const auto& s : objects_)
+        if (s->params.friction_law != DEM_FRICTION_NONE) return true;
+    return false;
+}
+
+BaseParticle* ParticleHandler::copyAndAddObject(const BaseParticle& p)
+{
+    auto q = std::make_unique<BaseParticle>(p);
+    q->id_ = nextId_++;
+    q->index_ = getNumberOfObjects();
+    if (!q->species_) throw std::invalid_argument("particle has no species");
+    if (q->fixed_) {
+        q->invMass_ = 0.0;
+        q->invInertia_ = 0.0;
+    } else {
+        // BaseParticle::computeMass (BaseParticle.cc:874-883)
+        const double r = q->radius_, rho = q->species_->getDensity();
+        if (dims_ == 3) {
+            q->invMass_ = 1.0 / (4.0 / 3.0 * constants::pi * r * r * r * rho);
+            q->invInertia_ = 1.0 / (.4 * (1.0 / q->invMass_) * mathsFunc::square(r));
+        } else {
+            q->invMass_ = 1.0 / (constants::pi * r * r * rho);
+            q->invInertia_ = 1.0 / (.5 * (1.0 / q->invMass_) * mathsFunc::square(r));
+        }
+    }
+    objects_.push_back(std::move(q));
+    return objects_.back().get();
+}
+
+// Quaternion::setOrientationViaNormal (Quaternion.cc:543-566) followed by
+// the coefficients of Quaternion::getDistance (658-669) and getAxis (501-512)
+dem_wall InfiniteWall::toEngine() const
+{
+    double nx = normal_.X, ny = normal_.Y, nz = normal_.Z;
+    const double l = std::sqrt(nx * nx + ny * ny + nz * nz);
+    nx /= l;
+    ny /= l;
+    nz /= l;
+    double q0, q1, q2, q3;
+    if (nx <= -1 + 1e-14) {
+        q0 = 0;
+        q1 = 0;
+        q2 = 1;
+        q3 = 0;
+    } else {
+        q0 = 1.0 + nx;
+        q1 = 0;
+        q2 = -(0.0 + nz);
+        q3 = 0.0 + ny;
+        const double ql = std::sqrt(q0 * q0 + q1 * q1 + q2 * q2 + q3 * q3);
+        q0 /= ql;
+        q1 /= ql;
+        q2 /= ql;
+        q3 /= ql;
+    }
+    dem_wall w{};
+    w.type = DEM_WALL_INFINITE;
+    w.species = species_ ? (int32_t)species_->getIndex() : 0;
+    w.id = id_;
+    w.position[0] = position_.X;
+    w.position[1] = position_.Y;
+    w.position[2] = position_.Z;
+    const double q00 = q0 * q0, q02 = 2 * q0 * q2, q03 = 2 * q0 * q3, q11 = q1 * q1, q12 = 2 * q1 * q2,
+                 q13 = 2 * q1 * q3, q22 = q2 * q2, q33 = q3 * q3;
+    w.dist_coef[0] = q00 + q11 - q22 - q33;
+    w.dist_coef[1] = q12 + q03;
+    w.dist_coef[2] = q13 - q02;
+    const double a12 = q1 * q2, a30 = q3 * q0, a13 = q1 * q3, a20 = q2 * q0;
+    w.axis[0] = q00 - q33 + q11 - q22;
+    w.axis[1] = 2.0 * (a12 + a30);
+    w.axis[2] = 2.0 * (a13 - a20);
+    return w;
+}
+
+void InfiniteWall::set(const Vec3D& normal, const Vec3D& point)
+{
+    normal_ = normal;
+    position_ = point;
+}
+
+dem_wall CylindricalWall::toEngine() const
+{
+    dem_wall w{};
+    w.type = DEM_WALL_CYLINDER;
+    w.species = species_ ? (int32_t)species_->getIndex() : 0;
+    w.id = id_;
+    w.radius = radius_;
+    return w;
+}
+
+void PeriodicBoundary::set(const Vec3D& normal, double left, double right)
+{
+    int axis = -1;
+    if (normal.Y == 0 && normal.Z == 0 && normal.X != 0) axis = 0;
+    if (normal.X == 0 && normal.Z == 0 && normal.Y != 0) axis = 1;
+    if (normal.X == 0 && normal.Y == 0 && normal.Z != 0) axis = 2;
+    if (axis < 0) throw std::invalid_argument("PeriodicBoundary: the normal must be a coordinate axis");
+    const double nn = axis == 0 ? normal.X : (axis == 1 ? normal.Y : normal.Z);
+    const double scale = 1. / std::sqrt(nn * nn);
+    if (nn < 0) throw std::invalid_argument("PeriodicBoundary: the normal must point along +axis");
+    b_.axis = axis;
+    b_.left = left * scale;
+    b_.right = right * scale;
+    if (!(b_.right > b_.left)) throw std::invalid_argument("PeriodicBoundary: left must be smaller than right");
+    b_.shift = (nn * scale) * (b_.right - b_.left);
+}
+
+// ------------------------------------------------------------------ DPMBase
+
+DPMBase::DPMBase() = default;
+
+DPMBase::~DPMBase()
+{
+    if (ctx_) dem_destroy(ctx_);
+}
+
+bool DPMBase::getRotation() const { return rotationSet_ ? rotation_ : speciesHandler.useAngularDOFs(); }
+
+void DPMBase::check(int rc, const char* what)
+{
+    if (rc != DEM_OK) {
+        // the reference's logger(ERROR, ...) ends the program
+        // (Logger.cc:194); here the error is thrown so a driver may catch it
+        throw std::runtime_error(std::string(what) + ": " + dem_last_error(ctx_));
+    }
+}
+
+dem_stats DPMBase::getEngineStats()
+{
+    dem_stats s{};
+    if (ctx_) check(dem_get_stats(ctx_, &s), "dem_get_stats");
+    return s;
+}
+
+// DPMBase::initialiseSolve (DPMBase.cc:4180-4277): set up, upload, force pass
+void DPMBase::initialiseSolve()
+{
+    nt_ = 0;
+    time_ = 0.0;
+    setupInitialConditions();
+    if (dt_ <= 0) throw std::runtime_error("DPMBase::solve: time step not set");
+    if (ctx_) dem_destroy(ctx_);
+    ctx_ = nullptr;
+    dem_config cfg{};
+    cfg.device = device_;
+    cfg.skin = skin_;
+    cfg.grid_levels = grid_levels_;
+    check(dem_create(&ctx_, &cfg), "dem_create");
+    const auto tab = speciesHandler.table();
+    check(dem_set_species(ctx_, (int32_t)speciesHandler.getNumberOfObjects(), tab.data()), "dem_set_species");
+    std::vector<dem_wall> walls;
+    for (auto& w : wallHandler) walls.push_back(w->toEngine());
+    check(dem_set_walls(ctx_, (int32_t)walls.size(), walls.data()), "dem_set_walls");
+    std::vector<dem_periodic> per;
+    for (auto& b : boundaryHandler) per.push_back(b->toEngine());
+    check(dem_set_periodic(ctx_, (int32_t)per.size(), per.data()), "dem_set_periodic");
+    const double g[3] = {gravity_.X, gravity_.Y, gravity_.Z};
+    check(dem_set_gravity(ctx_, g), "dem_set_gravity");
+    check(dem_set_background_drag(ctx_, drag_), "dem_set_background_drag");
+    check(dem_set_timestep(ctx_, dt_), "dem_set_timestep");
+    check(dem_set_rotation(ctx_, getRotation() ? 1 : 0), "dem_set_rotation");
+    check(dem_set_time(ctx_, 0.0, 0), "dem_set_time");
+    const size_t n = particleHandler.getNumberOfObjects();
+    std::vector<uint32_t> id(n);
+    std::vector<int32_t> sp(n);
+    std::vector<double> r(n), im(n), ii(6 * n, 0.0), x(3 * n), v(3 * n), w(3 * n), q(4 * n), f(3 * n), t(3 * n);
+    for (size_t k = 0; k < n; ++k) {
+        const BaseParticle* p = particleHandler.getObject((unsigned)k);
+        id[k] = p->id_;
+        sp[k] = (int32_t)p->getIndSpecies();
+        r[k] = p->radius_;
+        im[k] = p->invMass_;
+        ii[6 * k] = ii[6 * k + 3] = ii[6 * k + 5] = p->invInertia_;
+        const Vec3D* vs[5] = {&p->position_, &p->velocity_, &p->angularVelocity_, &p->force_, &p->torque_};
+        double* ds[5] = {&x[3 * k], &v[3 * k], &w[3 * k], &f[3 * k], &t[3 * k]};
+        for (int a = 0; a < 5; ++a) {
+            ds[a][0] = vs[a]->X;
+            ds[a][1] = vs[a]->Y;
+            ds[a][2] = vs[a]->Z;
+        }
+        for (int a = 0; a < 4; ++a) q[4 * k + a] = p->q_[a];
+    }
+    check(dem_upload_particles(ctx_, (int64_t)n, id.data(), sp.data(), r.data(), im.data(), ii.data(), x.data(),
+                               v.data(), w.data(), q.data(), f.data(), t.data()),
+          "dem_upload_particles");
+    check(dem_compute_forces(ctx_), "dem_compute_forces");
+    synced_ = false;
+}
+
+void DPMBase::computeOneTimeStep()
+{
+    check(dem_step(ctx_, 1), "dem_step");
+    time_ += dt_;
+    ++nt_;
+    synced_ = false;
+}
+
+void DPMBase::solve()
+{
+    initialiseSolve();
+    // the number of steps DPMBase::solve takes: time_ accumulates dt_ while
+    // time_ < timeMax (DPMBase.cc:4287-4290)
+    while (time_ < tMax_ && continueSolve()) {
+        int64_t batch = 0;
+        double t = time_;
+        while (t < tMax_ && batch < (int64_t)hookStride_) {
+            t += dt_;
+            ++batch;
+        }
+        actionsBeforeTimeStep();
+        if (batch == 1) {
+            computeOneTimeStep();
+        } else {
+            check(dem_step(ctx_, batch), "dem_step");
+            for (int64_t k = 0; k < batch; ++k) time_ += dt_;
+            nt_ += batch;
+            synced_ = false;
+        }
+        actionsAfterTimeStep();
+    }
+    synchroniseHandlers();
+    actionsAfterSolve();
+}
+
+void DPMBase::synchroniseHandlers()
+{
+    if (!ctx_ || synced_) return;
+    const size_t n = particleHandler.getNumberOfObjects();
+    std::vector<double> x(3 * n), v(3 * n), w(3 * n), q(4 * n), f(3 * n), t(3 * n);
+    check(dem_download_particles(ctx_, x.data(), v.data(), w.data(), q.data(), f.data(), t.data()),
+          "dem_download_particles");
+    for (size_t k = 0; k < n; ++k) {
+        BaseParticle* p = particleHandler.getObject((unsigned)k);
+        p->position_ = Vec3D(x[3 * k], x[3 * k + 1], x[3 * k + 2]);
+        p->velocity_ = Vec3D(v[3 * k], v[3 * k + 1], v[3 * k + 2]);
+        p->angularVelocity_ = Vec3D(w[3 * k], w[3 * k + 1], w[3 * k + 2]);
+        p->force_ = Vec3D(f[3 * k], f[3 * k + 1], f[3 * k + 2]);
+        p->torque_ = Vec3D(t[3 * k], t[3 * k + 1], t[3 * k + 2]);
+        for (int a = 0; a < 4; ++a) p->q_[a] = q[4 * k + a];
+    }
+    int64_t nc = 0;
+    check(dem_contact_count(ctx_, &nc), "dem_contact_count");
+    std::vector<uint32_t> pid(nc), iid(nc), ts(nc);
+    std::vector<double> spring(9 * nc);
+    int64_t got = 0;
+    check(dem_download_contacts(ctx_, nc, &got, pid.data(), iid.data(), spring.data(), ts.data()),
+          "dem_download_contacts");
+    auto& objs = interactionHandler.objects();
+    objs.clear();
+    for (int64_t k = 0; k < got; ++k) {
+        Interaction c;
+        c.P = pid[k];
+        c.I = iid[k];
+        c.isWall = (iid[k] & DEM_WALL_KEY) != 0;
+        c.slidingSpring = Vec3D(spring[9 * k], spring[9 * k + 1], spring[9 * k + 2]);
+        c.rollingSpring = Vec3D(spring[9 * k + 3], spring[9 * k + 4], spring[9 * k + 5]);
+        c.torsionSpring = Vec3D(spring[9 * k + 6], spring[9 * k + 7], spring[9 * k + 8]);
+        c.timeStamp = ts[k];
+        objs.push_back(c);
+    }
+    synced_ = true;
+}

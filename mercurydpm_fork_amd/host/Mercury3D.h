@@ -1,0 +1,472 @@
+// Mercury3D.h -- C++ host API of the GPU DEM engine, keeping the driver-facing
+// interface of MercuryDPM (DPMBase/Mercury3D, SpeciesHandler, ParticleHandler,
+// WallHandler, BoundaryHandler, InteractionHandler) so a driver written against
+// the reference compiles against this header and runs its time loop on one
+// MI355X through the C ABI of include/mercurydpm_dem.h.
+//
+// Scope: the per-timestep hot path.  Species: LinearViscoelastic,
+// LinearViscoelasticSlidingFriction, LinearViscoelasticFriction (rolling and
+// torsion), HertzianViscoelasticMindlin.  Walls: InfiniteWall,
+// CylindricalWall.  Boundaries: axis-aligned PeriodicBoundary.  Particles:
+// SphericalParticle.  Output files, restart and CG are not part of it.
+//
+// The handlers are the set-up and output source of truth, as in the
+// reference; inside DPMBase::solve() the device state is authoritative and is
+// copied back into the handlers when solve() returns, or whenever
+// synchroniseHandlers() is called.
+#pragma once
+
+#include <cmath>
+#include <cstdint>
+#include <functional>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/mercurydpm_dem.h"
+
+namespace constants {
+const double pi = 3.1415926535897932384626433832795;        // ExtendedMath.h:44
+const double sqr_pi = 9.8696044010893586188344909998762;    // ExtendedMath.h:46
+}  // namespace constants
+
+namespace mathsFunc {
+// mathsFunc::log (Kernel/Math/ExtendedMath.cc:104-129), used by the species
+// setters; restated so that derived stiffnesses match the reference's bits.
+double log(double x);
+inline double square(double x) { return x * x; }
+}  // namespace mathsFunc
+
+// ---------------------------------------------------------------- Vec3D ---
+struct Vec3D {
+    double X = 0, Y = 0, Z = 0;
+    Vec3D() = default;
+    Vec3D(double x, double y, double z) : X(x), Y(y), Z(z) {}
+    Vec3D operator+(const Vec3D& a) const { return Vec3D(X + a.X, Y + a.Y, Z + a.Z); }
+    Vec3D operator-(const Vec3D& a) const { return Vec3D(X - a.X, Y - a.Y, Z - a.Z); }
+    Vec3D operator*(double a) const { return Vec3D(X * a, Y * a, Z * a); }
+    Vec3D operator/(double a) const { return Vec3D(X / a, Y / a, Z / a); }
+    Vec3D operator-() const { return Vec3D(-X, -Y, -Z); }
+    friend Vec3D operator*(double a, const Vec3D& b) { return Vec3D(b.X * a, b.Y * a, b.Z * a); }
+    static double dot(const Vec3D& a, const Vec3D& b) { return a.X * b.X + a.Y * b.Y + a.Z * b.Z; }
+    double getLengthSquared() const { return X * X + Y * Y + Z * Z; }
+    double getLength() const { return std::sqrt(getLengthSquared()); }
+    // Vec3D::isEqualTo (Vector.cc): squared distance within tol^2
+    bool isEqualTo(const Vec3D& o, double tol) const { return (*this - o).getLengthSquared() <= tol * tol; }
+};
+
+// ------------------------------------------------------------- Species ---
+class SpeciesHandler;
+
+// A ParticleSpecies with its normal and friction force laws
+// (Species<NormalForceSpecies, FrictionForceSpecies> in the reference).
+class ParticleSpecies {
+public:
+    virtual ~ParticleSpecies() = default;
+    void setDensity(double rho) { density_ = rho; }
+    double getDensity() const { return density_; }
+    unsigned getIndex() const { return index_; }
+    // mass of a sphere of this species (ParticleSpecies::getMassFromRadius)
+    double getMassFromRadius(double radius) const;
+    dem_pair_params params{};  // resolved engine parameters
+
+protected:
+    friend class SpeciesHandler;
+    double density_ = 1.0;
+    unsigned index_ = 0;
+    SpeciesHandler* handler_ = nullptr;
+};
+
+// LinearViscoelasticNormalSpecies setters (LinearViscoelasticNormalSpecies.cc)
+class LinearViscoelasticSpecies : public ParticleSpecies {
+public:
+    LinearViscoelasticSpecies() { params.normal_law = DEM_NORMAL_LINEAR_VISCOELASTIC; }
+    void setStiffness(double k) { params.stiffness = k; }
+    double getStiffness() const { return params.stiffness; }
+    void setDissipation(double d) { params.dissipation = d; }
+    double getDissipation() const { return params.dissipation; }
+    void setConstantRestitution(bool c) { params.constant_restitution = c ? 1 : 0; }
+    void setCollisionTimeAndRestitutionCoefficient(double tc, double eps, double mass);
+    void setStiffnessAndRestitutionCoefficient(double k, double eps, double mass);
+    double getCollisionTime(double mass) const;
+    double getRestitutionCoefficient(double mass) const;
+};
+
+// SlidingFrictionSpecies setters (SlidingFrictionSpecies.cc)
+class LinearViscoelasticSlidingFrictionSpecies : public LinearViscoelasticSpecies {
+public:
+    LinearViscoelasticSlidingFrictionSpecies() { params.friction_law = DEM_FRICTION_SLIDING; }
+    void setSlidingStiffness(double k) { params.sliding_stiffness = k; }
+    double getSlidingStiffness() const { return params.sliding_stiffness; }
+    void setSlidingDissipation(double d) { params.sliding_dissipation = d; }
+    double getSlidingDissipation() const { return params.sliding_dissipation; }
+    void setSlidingFrictionCoefficient(double mu)
+    {
+        params.sliding_mu = mu;
+        params.sliding_mu_static = mu;
+    }
+    void setSlidingFrictionCoefficientStatic(double mu) { params.sliding_mu_static = mu; }
+};
+
+// FrictionSpecies setters (FrictionSpecies.cc): sliding + rolling + torsion
+class LinearViscoelasticFrictionSpecies : public LinearViscoelasticSlidingFrictionSpecies {
+public:
+    LinearViscoelasticFrictionSpecies() { params.friction_law = DEM_FRICTION_FULL; }
+    void setRollingStiffness(double k) { params.rolling_stiffness = k; }
+    void setRollingDissipation(double d) { params.rolling_dissipation = d; }
+    void setRollingFrictionCoefficient(double mu)
+    {
+        params.rolling_mu = mu;
+        params.rolling_mu_static = mu;
+    }
+    void setRollingFrictionCoefficientStatic(double mu) { params.rolling_mu_static = mu; }
+    void setTorsionStiffness(double k) { params.torsion_stiffness = k; }
+    void setTorsionDissipation(double d) { params.torsion_dissipation = d; }
+    void setTorsionFrictionCoefficient(double mu)
+    {
+        params.torsion_mu = mu;
+        params.torsion_mu_static = mu;
+    }
+    void setTorsionFrictionCoefficientStatic(double mu) { params.torsion_mu_static = mu; }
+};
+
+// HertzianViscoelasticNormalSpecies + MindlinSpecies setters
+class HertzianViscoelasticMindlinSpecies : public ParticleSpecies {
+public:
+    HertzianViscoelasticMindlinSpecies()
+    {
+        params.normal_law = DEM_NORMAL_HERTZIAN_VISCOELASTIC;
+        params.friction_law = DEM_FRICTION_MINDLIN;
+    }
+    void setEffectiveElasticModulus(double E) { params.elastic_modulus = E; }
+    double getEffectiveElasticModulus() const { return params.elastic_modulus; }
+    void setDissipation(double d) { params.dissipation = d; }
+    double getDissipation() const { return params.dissipation; }
+    void setEffectiveElasticModulusAndRestitutionCoefficient(double E, double rest);
+    void setEffectiveShearModulus(double G) { params.shear_modulus = G; }
+    void setEffectiveElasticModulusAndEffectiveShearModulus(double E, double G)
+    {
+        params.elastic_modulus = E;
+        params.shear_modulus = G;
+    }
+    void setSlidingDissipation(double d) { params.sliding_dissipation = d; }
+    void setSlidingFrictionCoefficient(double mu)
+    {
+        params.sliding_mu = mu;
+        params.sliding_mu_static = mu;
+    }
+    void setSlidingFrictionCoefficientStatic(double mu) { params.sliding_mu_static = mu; }
+    // HertzianViscoelasticNormalSpecies::getCollisionTime (cc:290-294)
+    double getCollisionTime(double diameter, double density, double relativeVelocity) const;
+};
+
+// ------------------------------------------------------------ Particles ---
+class ParticleHandler;
+
+class BaseParticle {
+public:
+    virtual ~BaseParticle() = default;
+    void setPosition(const Vec3D& x) { position_ = x; }
+    void setVelocity(const Vec3D& v) { velocity_ = v; }
+    void setAngularVelocity(const Vec3D& w) { angularVelocity_ = w; }
+    void setRadius(double r) { radius_ = r; }
+    void setSpecies(const ParticleSpecies* s) { species_ = s; }
+    // BaseParticle::fixParticle (BaseParticle.cc:171-183)
+    void fixParticle() { fixed_ = true; velocity_ = Vec3D(); angularVelocity_ = Vec3D(); }
+    const Vec3D& getPosition() const { return position_; }
+    const Vec3D& getVelocity() const { return velocity_; }
+    const Vec3D& getAngularVelocity() const { return angularVelocity_; }
+    const Vec3D& getForce() const { return force_; }
+    const Vec3D& getTorque() const { return torque_; }
+    const double* getOrientation() const { return q_; }
+    double getRadius() const { return radius_; }
+    double getInvMass() const { return invMass_; }
+    double getMass() const { return 1.0 / invMass_; }
+    bool isFixed() const { return invMass_ == 0.0; }
+    unsigned getId() const { return id_; }
+    unsigned getIndex() const { return index_; }
+    const ParticleSpecies* getSpecies() const { return species_; }
+    unsigned getIndSpecies() const { return species_ ? species_->getIndex() : 0; }
+
+protected:
+    friend class ParticleHandler;
+    friend class DPMBase;
+    Vec3D position_, velocity_, angularVelocity_, force_, torque_;
+    double q_[4] = {1, 0, 0, 0};
+    double radius_ = 1.0;  // BaseParticle default radius (BaseParticle.cc:36)
+    double invMass_ = 0.0, invInertia_ = 0.0;
+    bool fixed_ = false;
+    unsigned id_ = 0, index_ = 0;
+    const ParticleSpecies* species_ = nullptr;
+};
+
+class SphericalParticle : public BaseParticle {};
+
+// ------------------------------------------------------------- Walls ------
+class BaseWall {
+public:
+    virtual ~BaseWall() = default;
+    void setSpecies(const ParticleSpecies* s) { species_ = s; }
+    const ParticleSpecies* getSpecies() const { return species_; }
+    unsigned getId() const { return id_; }
+    Vec3D getForce() const { return force_; }
+    virtual dem_wall toEngine() const = 0;
+
+protected:
+    friend class WallHandler;
+    friend class DPMBase;
+    const ParticleSpecies* species_ = nullptr;
+    unsigned id_ = 0;
+    Vec3D force_;
+};
+
+// InfiniteWall (InfiniteWall.cc): orientation via normal, position p0
+class InfiniteWall : public BaseWall {
+public:
+    InfiniteWall() = default;
+    InfiniteWall(const Vec3D& normal, const Vec3D& point, const ParticleSpecies* s)
+    {
+        set(normal, point);
+        setSpecies(s);
+    }
+    void set(const Vec3D& normal, const Vec3D& point);
+    dem_wall toEngine() const override;
+
+private:
+    Vec3D normal_ = Vec3D(1, 0, 0), position_;
+};
+
+// CylindricalWall (CylindricalWall.cc): axis = z through the origin
+class CylindricalWall : public BaseWall {
+public:
+    void set(double radius) { radius_ = radius; }
+    double getRadius() const { return radius_; }
+    dem_wall toEngine() const override;
+
+private:
+    double radius_ = 1.0;
+};
+
+// ---------------------------------------------------------- Boundaries ----
+class PeriodicBoundary {
+public:
+    // PeriodicBoundary::set (PeriodicBoundary.cc:84-94); the normal must be a
+    // coordinate axis
+    void set(const Vec3D& normal, double left, double right);
+    dem_periodic toEngine() const { return b_; }
+
+private:
+    dem_periodic b_{};
+};
+
+// ------------------------------------------------------------ Handlers ----
+template <class T>
+class HandlerBase {
+public:
+    unsigned getNumberOfObjects() const { return (unsigned)objects_.size(); }
+    unsigned getSize() const { return getNumberOfObjects(); }
+    T* getObject(unsigned i) { return objects_.at(i).get(); }
+    const T* getObject(unsigned i) const { return objects_.at(i).get(); }
+    T* getLastObject() { return objects_.back().get(); }
+    typename std::vector<std::unique_ptr<T>>::iterator begin() { return objects_.begin(); }
+    typename std::vector<std::unique_ptr<T>>::iterator end() { return objects_.end(); }
+    void clear() { objects_.clear(); }
+
+protected:
+    std::vector<std::unique_ptr<T>> objects_;
+};
+
+class SpeciesHandler : public HandlerBase<ParticleSpecies> {
+public:
+    template <class S>
+    S* copyAndAddObject(const S& s)
+    {
+        auto p = std::make_unique<S>(s);
+        S* raw = p.get();
+        raw->index_ = getNumberOfObjects();
+        raw->handler_ = this;
+        objects_.push_back(std::move(p));
+        mixed_.clear();
+        return raw;
+    }
+    // SpeciesHandler::getMixedObject (SpeciesHandler.cc:922-945): defaults to
+    // the harmonic mean of every parameter (BaseSpecies::average); the
+    // returned parameters may be overridden before solve().
+    dem_pair_params* getMixedObject(unsigned a, unsigned b);
+    std::vector<dem_pair_params> table();
+    bool useAngularDOFs() const;
+
+private:
+    std::vector<std::pair<bool, dem_pair_params>> mixed_;
+};
+
+class ParticleHandler : public HandlerBase<BaseParticle> {
+public:
+    // ParticleHandler::copyAndAddObject: new monotone id, mass from species
+    // (BaseParticle::computeMass, BaseParticle.cc:874-883)
+    BaseParticle* copyAndAddObject(const BaseParticle& p);
+    void setParticleDimensions(unsigned d) { dims_ = d; }
+
+private:
+    unsigned nextId_ = 0;
+    unsigned dims_ = 3;
+};
+
+class WallHandler : public HandlerBase<BaseWall> {
+public:
+    template <class W>
+    W* copyAndAddObject(const W& w)
+    {
+        auto p = std::make_unique<W>(w);
+        W* raw = p.get();
+        raw->id_ = nextId_++;
+        objects_.push_back(std::move(p));
+        return raw;
+    }
+
+private:
+    unsigned nextId_ = 0;
+};
+
+class BoundaryHandler : public HandlerBase<PeriodicBoundary> {
+public:
+    PeriodicBoundary* copyAndAddObject(const PeriodicBoundary& b)
+    {
+        objects_.push_back(std::make_unique<PeriodicBoundary>(b));
+        return objects_.back().get();
+    }
+};
+
+// One stored interaction (BaseInteraction): P/I ids, springs, time stamp.
+struct Interaction {
+    unsigned P, I;  // I = DEM_WALL_KEY | wall index for a wall
+    bool isWall;
+    Vec3D slidingSpring, rollingSpring, torsionSpring;
+    unsigned timeStamp;
+};
+
+class InteractionHandler {
+public:
+    unsigned getNumberOfObjects() const { return (unsigned)objects_.size(); }
+    const Interaction& getObject(unsigned i) const { return objects_.at(i); }
+    std::vector<Interaction>& objects() { return objects_; }
+
+private:
+    std::vector<Interaction> objects_;
+};
+
+// ------------------------------------------------------------- DPMBase ----
+class DPMBase {
+public:
+    DPMBase();
+    virtual ~DPMBase();
+    DPMBase(const DPMBase&) = delete;
+    DPMBase& operator=(const DPMBase&) = delete;
+
+    SpeciesHandler speciesHandler;
+    ParticleHandler particleHandler;
+    WallHandler wallHandler;
+    BoundaryHandler boundaryHandler;
+    InteractionHandler interactionHandler;
+
+    void setName(const std::string& n) { name_ = n; }
+    const std::string& getName() const { return name_; }
+    void setTimeStep(double dt) { dt_ = dt; }
+    double getTimeStep() const { return dt_; }
+    void setTimeMax(double t) { tMax_ = t; }
+    double getTimeMax() const { return tMax_; }
+    double getTime() const { return time_; }
+    unsigned getNumberOfTimeSteps() const { return (unsigned)nt_; }
+    void setGravity(const Vec3D& g) { gravity_ = g; }
+    const Vec3D& getGravity() const { return gravity_; }
+    void setBackgroundDrag(double d) { drag_ = d; }
+    double getBackgroundDrag() const { return drag_; }
+    void setRotation(bool r) { rotationSet_ = true; rotation_ = r; }
+    bool getRotation() const;
+    void setXMin(double v) { min_.X = v; }
+    void setYMin(double v) { min_.Y = v; }
+    void setZMin(double v) { min_.Z = v; }
+    void setXMax(double v) { max_.X = v; }
+    void setYMax(double v) { max_.Y = v; }
+    void setZMax(double v) { max_.Z = v; }
+    void setMin(const Vec3D& v) { min_ = v; }
+    void setMax(const Vec3D& v) { max_ = v; }
+    void setMin(double x, double y, double z) { min_ = Vec3D(x, y, z); }
+    void setMax(double x, double y, double z) { max_ = Vec3D(x, y, z); }
+    double getXMin() const { return min_.X; }
+    double getYMin() const { return min_.Y; }
+    double getZMin() const { return min_.Z; }
+    double getXMax() const { return max_.X; }
+    double getYMax() const { return max_.Y; }
+    double getZMax() const { return max_.Z; }
+    void setSystemDimensions(unsigned d) { systemDims_ = d; }
+    void setParticleDimensions(unsigned d) { particleHandler.setParticleDimensions(d); }
+    // accepted for source compatibility; output files are not written
+    void setSaveCount(unsigned) {}
+    void setFileType(int) {}
+    // engine options
+    void setDevice(int d) { device_ = d; }
+    void setVerletSkin(double s) { skin_ = s; }
+
+    virtual void setupInitialConditions() {}
+    virtual void actionsBeforeTimeStep() {}
+    virtual void actionsAfterTimeStep() {}
+    virtual void actionsAfterSolve() {}
+    // return false to stop the time loop (DPMBase::continueSolve)
+    virtual bool continueSolve() const { return true; }
+
+    // DPMBase::solve (DPMBase.cc:4279-4295): initialiseSolve (set-up, upload,
+    // first force pass), then computeOneTimeStep until time >= timeMax.
+    void solve();
+    void solve(int, char**) { solve(); }
+    // One time step on the device (DPMBase::computeOneTimeStep,
+    // DPMBase.cc:4315-4389).  Steps are issued in batches when the
+    // before/after hooks are not overridden (see setHookStride).
+    virtual void computeOneTimeStep();
+    // Hooks run every `stride` steps (1 = every step, as in the reference);
+    // larger strides let the device run batches between host round trips.
+    void setHookStride(unsigned s) { hookStride_ = s ? s : 1; }
+    // Copy device state (particles, forces, interactions) into the handlers.
+    void synchroniseHandlers();
+    dem_stats getEngineStats();
+
+protected:
+    int grid_levels_ = 0;
+
+private:
+    void initialiseSolve();
+    void check(int rc, const char* what);
+    std::string name_ = "out";
+    double dt_ = 0, tMax_ = 0, time_ = 0, drag_ = 0;
+    int64_t nt_ = 0;
+    Vec3D gravity_ = Vec3D(0, 0, 0);  // DPMBase constructor defaults (DPMBase.cc:240-262)
+    Vec3D min_, max_;
+    bool rotation_ = true, rotationSet_ = false;
+    unsigned systemDims_ = 3;
+    unsigned hookStride_ = 1;
+    int device_ = 0;
+    double skin_ = 0;
+    dem_ctx* ctx_ = nullptr;
+    bool synced_ = true;
+};
+
+// MercuryBase / Mercury3D: the HGrid problem classes.  Their broad phase is
+// the engine's multilevel cell grid; setHGridMaxLevels maps to its levels.
+class MercuryBase : public DPMBase {
+public:
+    void setHGridMaxLevels(unsigned n) { grid_levels_ = (int)n; }
+};
+
+class Mercury3D : public MercuryBase {};
+using GpuMercury3D = Mercury3D;
+
+// Mercury2D: discs of particle dimension 2 (Mercury2D.cc:61-62)
+class Mercury2D : public MercuryBase {
+public:
+    Mercury2D()
+    {
+        setSystemDimensions(2);
+        setParticleDimensions(2);
+    }
+};
