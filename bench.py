@@ -42,6 +42,8 @@ def parse():
     ap.add_argument("--cpu-steps", type=int, default=3, help="oracle steps in the CPU baseline (0: skip)")
     ap.add_argument("--skin", type=float, default=0.0)
     ap.add_argument("--seed", type=int, default=3)
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="only copy this many bytes device-to-device (calibrates HBM counters) and exit")
     return ap.parse_args()
 
 
@@ -55,6 +57,15 @@ def main():
 
     import numpy as np
     import torch
+
+    if a.copy_probe > 0:
+        src = torch.ones(a.copy_probe // 8, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": a.copy_probe}))
+        return
 
     from mercurydpm_fork_amd import dem
     from mercurydpm_fork_amd import scenarios as S

@@ -83,8 +83,9 @@ struct Params {
     double rmaxl[kMaxLevels];   // largest radius on each level
     double hc[kMaxLevels][3];   // cell size per axis (periodic axes: L / ncell)
     int ncell[kMaxLevels][3];   // cells per periodic axis (0 = not periodic)
-    uint32_t nb[kMaxLevels];    // buckets per level (power of two)
+    uint32_t nb[kMaxLevels];    // buckets per level (nbx * nby * nbz)
     uint32_t nboff[kMaxLevels]; // bucket offset of each level
+    int nbd[kMaxLevels][3];     // bucket grid dimensions per level
 };
 
 template <int NS>
@@ -134,14 +135,39 @@ __device__ inline uint64_t mix64(uint64_t k)
     return k;
 }
 
-// Spatial hash of a cell (the prime-multiply/xor hash of Teschner et al.,
-// "Optimized Spatial Hashing for Collision Detection of Deformable Objects",
-// 2003; HGrid::computeHashBucketIndex, HGrid.h:76-84, uses the same family).
-// Colliding cells share a bucket; the list build tells them apart by the cell
-// coordinates stored with each particle.
-__device__ inline uint32_t cell_hash(int cx, int cy, int cz)
+// Bucket of a cell: its linear index in a per-level box of nbd[0] x nbd[1] x
+// nbd[2] cells, coordinates taken modulo the box.  Along a periodic axis the
+// box spans the whole period, so there is no aliasing; along an open axis it
+// spans the extent seen at upload with a margin, and cells beyond alias onto
+// buckets of cells far away (told apart by the cell coordinates stored with
+// each particle).  Sorting by this index keeps neighbouring cells, and hence
+// the partners gathered by the force kernel, close in memory.
+__device__ inline int wrap_mod(int c, int n)
 {
-    return ((uint32_t)cx * 73856093u) ^ ((uint32_t)cy * 19349663u) ^ ((uint32_t)cz * 83492791u);
+    int r = c % n;
+    return r < 0 ? r + n : r;
+}
+
+__device__ inline uint32_t cell_bucket(const int nbd[3], int cx, int cy, int cz)
+{
+    return ((uint32_t)wrap_mod(cz, nbd[2]) * (uint32_t)nbd[1] + (uint32_t)wrap_mod(cy, nbd[1])) * (uint32_t)nbd[0] +
+           (uint32_t)wrap_mod(cx, nbd[0]);
+}
+
+// Blocks are dealt round-robin to the 8 XCDs.  Remap them so that each XCD
+// (and its L2) takes runs of kXcdChunk consecutive blocks, i.e. spatially
+// compact groups of particles, while the runs stay interleaved over the whole
+// range so that every XCD gets a similar mix of small and large spheres.
+// gridDim.x must be a multiple of 8 * kXcdChunk.
+#ifndef DEM_XCD_CHUNK
+#define DEM_XCD_CHUNK 1
+#endif
+constexpr int kXcdChunk = DEM_XCD_CHUNK;
+__host__ __device__ constexpr unsigned xcd_grid(unsigned g) { return (g + 8 * kXcdChunk - 1) / (8 * kXcdChunk) * (8 * kXcdChunk); }
+__device__ inline int xcd_block()
+{
+    const int b = (int)blockIdx.x, x = b & 7, r = b >> 3;
+    return ((r / kXcdChunk) * 8 + x) * kXcdChunk + r % kXcdChunk;
 }
 
 __device__ inline double comp(V3 v, int a) { return a == 0 ? v.x : (a == 1 ? v.y : v.z); }
@@ -182,6 +208,9 @@ __device__ inline V3 periodic_shift(const Params& p, int b, V3 pos)
 // Rebuild kernels run with a bounded grid and stride over their range, so a
 // launch with no rebuild pending retires in a few microseconds.
 constexpr int kRebuildBlocks = 2048;
+static_assert(kRebuildBlocks % (8 * kXcdChunk) == 0, "rebuild grid must suit the XCD remap");
+#define GRID_STRIDE_XCD(i, n) \
+    for (uint64_t i = (uint64_t)xcd_block() * blockDim.x + threadIdx.x; i < (uint64_t)(n); i += (uint64_t)gridDim.x * blockDim.x)
 #define GRID_STRIDE(i, n) \
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < (uint64_t)(n); i += (uint64_t)gridDim.x * blockDim.x)
 
@@ -365,7 +394,7 @@ __global__ void __launch_bounds__(kBlock) k_keys(Dev<NS> d, Params p)
     {
         const int lvl = (int)(d.spc[i] >> 16);
         int4 c = cell_of(p, lvl, xyz(d.posr[i]));
-        const uint32_t key = p.nboff[lvl] + (cell_hash(c.y, c.z, c.w) & (p.nb[lvl] - 1));
+        const uint32_t key = p.nboff[lvl] + cell_bucket(p.nbd[lvl], c.y, c.z, c.w);
         d.key[i] = key;
         if (d.vel[i].w == 0.0) c.x |= 0x100; // fixed particle
         d.cellc[i] = c;
@@ -511,7 +540,7 @@ template <int NS>
 __global__ void __launch_bounds__(kBlock) k_build_lists(Dev<NS> d, Params p)
 {
     if (!rebuild_pending(d.flags)) return;
-    GRID_STRIDE(ii, p.n)
+    GRID_STRIDE_XCD(ii, p.n)
     {
         const int i = (int)ii;
         const double4 pr = d.posr[i];
@@ -539,18 +568,16 @@ __global__ void __launch_bounds__(kBlock) k_build_lists(Dev<NS> d, Params p)
                     hi[a] = (int)floor((xa + R) / p.h[m]);
                 }
             }
-            const uint32_t mask = p.nb[m] - 1;
             for (int tz = lo[2]; tz <= hi[2]; ++tz) {
                 int cz = tz;
                 if (nc[2]) cz = ((tz % nc[2]) + nc[2]) % nc[2];
                 for (int ty = lo[1]; ty <= hi[1]; ++ty) {
                     int cy = ty;
                     if (nc[1]) cy = ((ty % nc[1]) + nc[1]) % nc[1];
-                    const uint32_t hyz = ((uint32_t)cy * 19349663u) ^ ((uint32_t)cz * 83492791u);
                     for (int tx = lo[0]; tx <= hi[0]; ++tx) {
                         int cx = tx;
                         if (nc[0]) cx = ((tx % nc[0]) + nc[0]) % nc[0];
-                        const uint32_t b = p.nboff[m] + ((((uint32_t)cx * 73856093u) ^ hyz) & mask);
+                        const uint32_t b = p.nboff[m] + cell_bucket(p.nbd[m], cx, cy, cz);
                         const uint32_t s = d.start[b];
                         const uint32_t e = d.start[b + 1];
                         for (uint32_t q = s; q < e; ++q) {
@@ -708,21 +735,37 @@ __device__ inline uint32_t lane_prefix(unsigned long long ballot)
 // queue entry: j (32) | k (6) << 32 | owner lane (6) << 38 | image shift
 // code per periodic boundary (2 bits each: 1 = +shift, 2 = -shift) << 44 |
 // img << 50 | i_is_P << 51 | writer << 52
+#ifndef DEM_SCAN_UNROLL
+#define DEM_SCAN_UNROLL 1
+#endif
+#ifndef DEM_OWNER_LDS
+#define DEM_OWNER_LDS 1
+#endif
+// List entries each lane loads before testing them (memory-level parallelism).
+constexpr int kScanU = DEM_SCAN_UNROLL;
+// Minimum waves per SIMD the register allocation of the force kernel must allow.
+#if defined(DEM_FORCE_WAVES) && DEM_FORCE_WAVES > 0
+#define FORCE_WAVES_ATTR __attribute__((amdgpu_waves_per_eu(DEM_FORCE_WAVES)))
+#else
+#define FORCE_WAVES_ATTR
+#endif
+constexpr int kQueue = 64 + 64 * kScanU;
 struct OwnerLds {
+#if DEM_OWNER_LDS
     double px[64], py[64], pz[64], vx[64], vy[64], vz[64], wx[64], wy[64], wz[64], r[64], im[64];
+#endif
     unsigned long long m_in[64];
-    uint32_t id[64];
     int32_t sp[64];
     int32_t row[64];
-    unsigned long long q[128];
+    unsigned long long q[kQueue];
     double cf[6][64];
 };
 
 template <int NS>
-__device__ inline void process_batch(Dev<NS>& d, const Params& p, int cur, OwnerLds& L, int nb, int lane)
+__device__ inline void process_batch(Dev<NS>& d, const Params& p, int cur, OwnerLds& L, int base, int nb, int lane)
 {
     if (lane < nb) {
-        const unsigned long long e = L.q[lane];
+        const unsigned long long e = L.q[base + lane];
         const uint32_t j = (uint32_t)e;
         const int k = (int)((e >> 32) & 63);
         const int o = (int)((e >> 38) & 63);
@@ -730,12 +773,24 @@ __device__ inline void process_batch(Dev<NS>& d, const Params& p, int cur, Owner
         const bool img = (e >> 50) & 1;
         const bool i_is_P = (e >> 51) & 1;
         const bool writer = (e >> 52) & 1;
+        const int i = L.row[o];
         Body bi;
+#if DEM_OWNER_LDS
         bi.pos = mk(L.px[o], L.py[o], L.pz[o]);
         bi.vel = mk(L.vx[o], L.vy[o], L.vz[o]);
         bi.omega = mk(L.wx[o], L.wy[o], L.wz[o]);
         bi.radius = L.r[o];
         bi.inv_mass = L.im[o];
+#else
+        {
+            const double4 a = d.posr[i], b = d.vel[i], c = d.omg[i];
+            bi.pos = xyz(a);
+            bi.vel = xyz(b);
+            bi.omega = xyz(c);
+            bi.radius = a.w;
+            bi.inv_mass = b.w;
+        }
+#endif
         const double4 pq = d.posr[j];
         V3 xj = xyz(pq);
         for (int b = 0; b < p.nper; ++b) {
@@ -767,7 +822,6 @@ __device__ inline void process_batch(Dev<NS>& d, const Params& p, int cur, Owner
         g.distance = distance;
         g.contact = vsub(P.pos, vmul(g.normal, P.radius - 0.5 * g.overlap));
         g.i_is_wall = false;
-        const int i = L.row[o];
         const size_t slot = lslot(k, p.n, i);
         const Rec<NS>* rin = d.rec[cur];
         double spring[NS];
@@ -831,18 +885,17 @@ __device__ inline void process_batch(Dev<NS>& d, const Params& p, int cur, Owner
 }
 
 template <int NS>
-__global__ void __launch_bounds__(kBlock) k_force(Dev<NS> d, Params p, int cur)
+__global__ void __launch_bounds__(kBlock) FORCE_WAVES_ATTR k_force(Dev<NS> d, Params p, int cur)
 {
     __shared__ OwnerLds lds[kWaves];
     const int lane = threadIdx.x & 63;
     OwnerLds& L = lds[threadIdx.x >> 6];
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = xcd_block() * blockDim.x + threadIdx.x;
     const bool valid = i < p.n;
     const int ii = valid ? i : 0;
     const double4 pr = d.posr[ii];
     const double4 vl = d.vel[ii];
     const double4 om = d.omg[ii];
-    const uint32_t idi = d.id[ii];
     const int si = (int)(d.spc[ii] & 0xFFFF);
     const bool fixed_i = vl.w == 0.0;
     Body bi;
@@ -852,6 +905,7 @@ __global__ void __launch_bounds__(kBlock) k_force(Dev<NS> d, Params p, int cur)
     bi.radius = pr.w;
     bi.inv_mass = vl.w;
     const unsigned long long m_in = valid ? d.mask[cur][ii] : 0ull;
+#if DEM_OWNER_LDS
     L.px[lane] = pr.x;
     L.py[lane] = pr.y;
     L.pz[lane] = pr.z;
@@ -863,8 +917,8 @@ __global__ void __launch_bounds__(kBlock) k_force(Dev<NS> d, Params p, int cur)
     L.wz[lane] = om.z;
     L.r[lane] = pr.w;
     L.im[lane] = vl.w;
+#endif
     L.m_in[lane] = m_in;
-    L.id[lane] = idi;
     L.sp[lane] = si;
     L.row[lane] = ii;
     const int cnt = valid ? d.ncount[ii] : 0;
@@ -873,78 +927,88 @@ __global__ void __launch_bounds__(kBlock) k_force(Dev<NS> d, Params p, int cur)
     int qn = 0;
     wave_sync();
 
-    for (int k = 0;; ++k) {
-        const bool active = k < cnt;
-        if (!__any(active)) break;
-        bool contact = false;
-        unsigned long long entry = 0;
-        if (active) {
-            const uint32_t ej = d.nbr[lslot(k, p.n, ii)];
-            const uint32_t j = ej & ~kWriterBit;
-            const bool writer = (ej & kWriterBit) != 0;
-            const double4 pq = d.posr[j];
-            V3 xj = xyz(pq);
-            bool img = false, exists = true;
-            int codes = 0;
-            for (int b = 0; b < p.nper; ++b) {
-                const int a = p.paxis[b];
-                const double c = comp(xj, a) - comp(bi.pos, a);
-                if (c > 0.5 * p.pshift[b] || c < -0.5 * p.pshift[b]) {
-                    img = true;
-                    if (!(periodic_distance(p, b, xj) < pq.w + p.rmax)) exists = false;
-                    const double dd = comp(xj, a);
-                    codes |= (p.pright[b] - dd > dd - p.pleft[b] ? 1 : 2) << (2 * b);
-                    xj = periodic_shift(p, b, xj);
-                }
-            }
-            if (!img || exists) {
-                const bool i_is_P = img || writer;
-                const V3 posP = i_is_P ? bi.pos : xj;
-                const V3 posI = i_is_P ? xj : bi.pos;
-                const double rP = i_is_P ? pr.w : pq.w;
-                const double rI = i_is_P ? pq.w : pr.w;
-                const V3 branch = vsub(posP, posI);
-                const double d2 = vlen2(branch);
-                const double sum_r = rI + rP;
-                if (d2 < sum_r * sum_r) {
-                    contact = true;
-                    entry = (unsigned long long)j | ((unsigned long long)k << 32) | ((unsigned long long)lane << 38) |
-                            ((unsigned long long)codes << 44) | ((unsigned long long)img << 50) |
-                            ((unsigned long long)i_is_P << 51) | ((unsigned long long)writer << 52);
-                    if (writer) touched |= 1ull << k;
-                }
-            }
-        }
-        const unsigned long long ballot = __ballot(contact);
-        if (contact) L.q[qn + lane_prefix(ballot)] = entry;
-        qn += __popcll(ballot);
-        if (qn >= 64) {
+    // process the queued contacts in batches of 64 (or the remainder when
+    // `all`), each owner adding its contributions in queue order
+    auto drain = [&](bool all) {
+        int head = 0;
+        while (qn - head >= 64 || (all && qn > head)) {
+            const int nb = qn - head < 64 ? qn - head : 64;
             wave_sync();
-            process_batch<NS>(d, p, cur, L, 64, lane);
+            process_batch<NS>(d, p, cur, L, head, nb, lane);
             wave_sync();
-            for (int a = 0; a < 64; ++a) {
-                if ((int)((L.q[a] >> 38) & 63) == lane) {
+            for (int a = 0; a < nb; ++a) {
+                if ((int)((L.q[head + a] >> 38) & 63) == lane) {
                     F = vadd(F, mk(L.cf[0][a], L.cf[1][a], L.cf[2][a]));
                     T = vadd(T, mk(L.cf[3][a], L.cf[4][a], L.cf[5][a]));
                 }
             }
+            head += nb;
+        }
+        if (head > 0) {
+            // fewer than 64 left, and head >= 64: source and target are disjoint
             wave_sync();
-            qn -= 64;
-            if (lane < qn) L.q[lane] = L.q[64 + lane];
+            qn -= head;
+            if (lane < qn) L.q[lane] = L.q[head + lane];
             wave_sync();
         }
-    }
-    if (qn > 0) {
-        wave_sync();
-        process_batch<NS>(d, p, cur, L, qn, lane);
-        wave_sync();
-        for (int a = 0; a < qn; ++a) {
-            if ((int)((L.q[a] >> 38) & 63) == lane) {
-                F = vadd(F, mk(L.cf[0][a], L.cf[1][a], L.cf[2][a]));
-                T = vadd(T, mk(L.cf[3][a], L.cf[4][a], L.cf[5][a]));
+    };
+
+    for (int k0 = 0; __any(k0 < cnt); k0 += kScanU) {
+        uint32_t ej[kScanU];
+        double4 pq[kScanU];
+#pragma unroll
+        for (int u = 0; u < kScanU; ++u) ej[u] = k0 + u < cnt ? d.nbr[lslot(k0 + u, p.n, ii)] : 0u;
+#pragma unroll
+        for (int u = 0; u < kScanU; ++u)
+            pq[u] = k0 + u < cnt ? d.posr[ej[u] & ~kWriterBit] : make_double4(0.0, 0.0, 0.0, 0.0);
+#pragma unroll
+        for (int u = 0; u < kScanU; ++u) {
+            const int k = k0 + u;
+            bool contact = false;
+            unsigned long long entry = 0;
+            if (k < cnt) {
+                const uint32_t j = ej[u] & ~kWriterBit;
+                const bool writer = (ej[u] & kWriterBit) != 0;
+                V3 xj = xyz(pq[u]);
+                bool img = false, exists = true;
+                int codes = 0;
+                for (int b = 0; b < p.nper; ++b) {
+                    const int a = p.paxis[b];
+                    const double c = comp(xj, a) - comp(bi.pos, a);
+                    if (c > 0.5 * p.pshift[b] || c < -0.5 * p.pshift[b]) {
+                        img = true;
+                        if (!(periodic_distance(p, b, xj) < pq[u].w + p.rmax)) exists = false;
+                        const double dd = comp(xj, a);
+                        codes |= (p.pright[b] - dd > dd - p.pleft[b] ? 1 : 2) << (2 * b);
+                        xj = periodic_shift(p, b, xj);
+                    }
+                }
+                if (!img || exists) {
+                    const bool i_is_P = img || writer;
+                    const V3 posP = i_is_P ? bi.pos : xj;
+                    const V3 posI = i_is_P ? xj : bi.pos;
+                    const double rP = i_is_P ? pr.w : pq[u].w;
+                    const double rI = i_is_P ? pq[u].w : pr.w;
+                    const V3 branch = vsub(posP, posI);
+                    const double d2 = vlen2(branch);
+                    const double sum_r = rI + rP;
+                    if (d2 < sum_r * sum_r) {
+                        contact = true;
+                        entry = (unsigned long long)j | ((unsigned long long)k << 32) |
+                                ((unsigned long long)lane << 38) | ((unsigned long long)codes << 44) |
+                                ((unsigned long long)img << 50) | ((unsigned long long)i_is_P << 51) |
+                                ((unsigned long long)writer << 52);
+                        if (writer) touched |= 1ull << k;
+                    }
+                }
             }
+            const unsigned long long ballot = __ballot(contact);
+            if (contact) L.q[qn + lane_prefix(ballot)] = entry;
+            qn += __popcll(ballot);
         }
+        if (qn >= 64) drain(false);
     }
+    if (qn > 0) drain(true);
     if (!valid) return;
     const Rec<NS>* rin = d.rec[cur];
     Rec<NS>* rout = d.rec[cur ^ 1];
@@ -1344,7 +1408,8 @@ void launch_force(dem_ctx* ctx, int erase)
     Dev<NS>& d = dev<NS>(ctx);
     ctx->prm.erase = erase;
     ctx->prm.nt = (uint32_t)ctx->nt;
-    hipLaunchKernelGGL(k_force<NS>, dim3(grid_for(ctx->n)), dim3(kBlock), 0, ctx->stream, d, ctx->prm, ctx->cur);
+    const unsigned g = xcd_grid(grid_for(ctx->n));
+    hipLaunchKernelGGL(k_force<NS>, dim3(g), dim3(kBlock), 0, ctx->stream, d, ctx->prm, ctx->cur);
     ctx->cur ^= 1;
 }
 
@@ -1606,7 +1671,12 @@ int dem_upload_particles(dem_ctx* ctx, int64_t n, const uint32_t* id, const int3
     HIPCHK(hipSetDevice(ctx->device));
     // --- validate and derive the grid on the host
     double rmin = 1e300, rmax = 0.0;
+    double xmin[3] = {1e300, 1e300, 1e300}, xmax[3] = {-1e300, -1e300, -1e300};
     for (int64_t i = 0; i < n; ++i) {
+        for (int a = 0; a < 3; ++a) {
+            xmin[a] = std::min(xmin[a], x[3 * i + a]);
+            xmax[a] = std::max(xmax[a], x[3 * i + a]);
+        }
         if (i > 0 && id[i] <= id[i - 1]) return set_err(ctx, DEM_ERR_ARG, "ids must increase with the handler index");
         if (id[i] >= DEM_WALL_KEY) return set_err(ctx, DEM_ERR_ARG, "particle ids must be < 2^31");
         if (species[i] < 0 || species[i] >= ctx->ns) return set_err(ctx, DEM_ERR_ARG, "species index out of range");
@@ -1674,7 +1744,26 @@ int dem_upload_particles(dem_ctx* ctx, int64_t n, const uint32_t* id, const int3
             p.ncell[l][a] = nc;
             p.hc[l][a] = p.pshift[b] / nc;
         }
-        p.nb[l] = next_pow2(std::max<int64_t>(64, 2 * per_level[l]));
+        // bucket box: the full period along periodic axes, the extent of the
+        // uploaded positions with a 50% margin along open ones
+        for (int a = 0; a < 3; ++a) {
+            if (p.ncell[l][a] > 0) {
+                p.nbd[l][a] = p.ncell[l][a];
+            } else {
+                const double ext = xmax[a] - xmin[a];
+                p.nbd[l][a] = (int)std::ceil(1.5 * ext / p.h[l]) + 4;
+            }
+        }
+        // keep the bucket count within a few per particle of the level
+        const double cap = (double)std::max<int64_t>(4096, 16 * per_level[l] + 4096);
+        while ((double)p.nbd[l][0] * p.nbd[l][1] * p.nbd[l][2] > cap) {
+            int a = 0;
+            for (int b = 1; b < 3; ++b)
+                if (p.ncell[l][b] == 0 && (p.ncell[l][a] > 0 || p.nbd[l][b] > p.nbd[l][a])) a = b;
+            if (p.ncell[l][a] > 0 || p.nbd[l][a] <= 1) break;  // only open axes shrink
+            p.nbd[l][a] = (p.nbd[l][a] + 1) / 2;
+        }
+        p.nb[l] = (uint32_t)((uint64_t)p.nbd[l][0] * p.nbd[l][1] * p.nbd[l][2]);
         p.nboff[l] = off;
         off += p.nb[l];
     }
