@@ -214,12 +214,14 @@ static_assert(kRebuildBlocks % (8 * kXcdChunk) == 0, "rebuild grid must suit the
 #define GRID_STRIDE(i, n) \
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < (uint64_t)(n); i += (uint64_t)gridDim.x * blockDim.x)
 
-// Verlet-list slots are stored k-major (slot of entry k of particle i is
-// k * n + i) so that a wavefront reading entry k of 64 consecutive particles
-// touches one contiguous segment.  An entry is the partner's sorted index with
-// bit 31 set when this particle has the lower id (it writes the record).
+// Verlet-list slots are stored particle-major with a fixed stride (slot of
+// entry k of particle i is i * stride + k, stride = max_neighbors), so the
+// lists of the 64 particles of a wavefront lie in one compact range that the
+// force kernel walks as a flat sequence.  An entry is the partner's sorted
+// index with bit 31 set when this particle has the lower id (it writes the
+// record).
 constexpr uint32_t kWriterBit = 0x80000000u;
-__host__ __device__ inline size_t lslot(int k, int n, int i) { return (size_t)k * n + i; }
+__host__ __device__ inline size_t lslot(int k, int stride, int i) { return (size_t)i * stride + k; }
 
 __device__ inline bool rebuild_pending(const int* flags) { return __atomic_load_n(&flags[F_REBUILD], __ATOMIC_RELAXED) != 0; }
 
@@ -342,9 +344,9 @@ __global__ void __launch_bounds__(kBlock) k_rb_begin(Dev<NS> d, Params p, int cu
         while (m) {
             const int k = __ffsll((long long)m) - 1;
             m &= m - 1;
-            const uint32_t j = d.nbr[lslot(k, p.n, (int)i)] & ~kWriterBit;
+            const uint32_t j = d.nbr[lslot(k, p.maxn, (int)i)] & ~kWriterBit;
             const uint64_t key = ((uint64_t)idi << 32) | d.id[j];
-            hash_insert(d, key, d.rec[cur][lslot(k, p.n, (int)i)]);
+            hash_insert(d, key, d.rec[cur][lslot(k, p.maxn, (int)i)]);
         }
         uint32_t wm = d.wmask[cur][i];
         while (wm) {
@@ -589,7 +591,7 @@ __global__ void __launch_bounds__(kBlock) k_build_lists(Dev<NS> d, Params p)
                             const V3 dl = min_image_delta(p, xi, xyz(pq));
                             const double cut = pr.w + pq.w + p.skin;
                             if (vlen2(dl) < cut * cut) {
-                                if (cnt < p.maxn) d.nbr[lslot(cnt, p.n, i)] = q | (idi < d.id[q] ? kWriterBit : 0u);
+                                if (cnt < p.maxn) d.nbr[lslot(cnt, p.maxn, i)] = q | (idi < d.id[q] ? kWriterBit : 0u);
                                 ++cnt;
                             }
                         }
@@ -616,7 +618,7 @@ __global__ void __launch_bounds__(kBlock) k_rematch(Dev<NS> d, Params p, int cur
         const int cnt = d.ncount[i];
         unsigned long long m = 0;
         for (int k = 0; k < cnt; ++k) {
-            const size_t sl = lslot(k, p.n, i);
+            const size_t sl = lslot(k, p.maxn, i);
             const uint32_t e = d.nbr[sl];
             const uint32_t j = e & ~kWriterBit;
             if (e & kWriterBit) {
@@ -630,7 +632,7 @@ __global__ void __launch_bounds__(kBlock) k_rematch(Dev<NS> d, Params p, int cur
                 const int cj = d.ncount[j];
                 int kk = -1;
                 for (int a = 0; a < cj; ++a)
-                    if ((d.nbr[lslot(a, p.n, (int)j)] & ~kWriterBit) == (uint32_t)i) {
+                    if ((d.nbr[lslot(a, p.maxn, (int)j)] & ~kWriterBit) == (uint32_t)i) {
                         kk = a;
                         break;
                     }
@@ -734,14 +736,13 @@ __device__ inline uint32_t lane_prefix(unsigned long long ballot)
 
 // queue entry: j (32) | k (6) << 32 | owner lane (6) << 38 | image shift
 // code per periodic boundary (2 bits each: 1 = +shift, 2 = -shift) << 44 |
-// img << 50 | i_is_P << 51 | writer << 52
+// img << 50 | i_is_P << 51 | writer << 52 | slot of the pair in j's list
+// (0xFF: none) << 53
 #ifndef DEM_SCAN_UNROLL
-#define DEM_SCAN_UNROLL 1
+#define DEM_SCAN_UNROLL 4
 #endif
-#ifndef DEM_OWNER_LDS
-#define DEM_OWNER_LDS 1
-#endif
-// List entries each lane loads before testing them (memory-level parallelism).
+// Rounds of 64 list entries each lane loads before testing them
+// (memory-level parallelism).
 constexpr int kScanU = DEM_SCAN_UNROLL;
 // Minimum waves per SIMD the register allocation of the force kernel must allow.
 #if defined(DEM_FORCE_WAVES) && DEM_FORCE_WAVES > 0
@@ -751,15 +752,17 @@ constexpr int kScanU = DEM_SCAN_UNROLL;
 #endif
 constexpr int kQueue = 64 + 64 * kScanU;
 struct OwnerLds {
-#if DEM_OWNER_LDS
     double px[64], py[64], pz[64], vx[64], vy[64], vz[64], wx[64], wy[64], wz[64], r[64], im[64];
-#endif
     unsigned long long m_in[64];
     int32_t sp[64];
     int32_t row[64];
+    int32_t pre[64];    // exclusive prefix of the owners' list lengths
+    int32_t first[64];  // first queue position of each owner in a batch
     unsigned long long q[kQueue];
     double cf[6][64];
 };
+
+__device__ inline int q_owner(unsigned long long e) { return (int)((e >> 38) & 63); }
 
 template <int NS>
 __device__ inline void process_batch(Dev<NS>& d, const Params& p, int cur, OwnerLds& L, int base, int nb, int lane)
@@ -768,30 +771,38 @@ __device__ inline void process_batch(Dev<NS>& d, const Params& p, int cur, Owner
         const unsigned long long e = L.q[base + lane];
         const uint32_t j = (uint32_t)e;
         const int k = (int)((e >> 32) & 63);
-        const int o = (int)((e >> 38) & 63);
+        const int o = q_owner(e);
         const int codes = (int)((e >> 44) & 63);
         const bool img = (e >> 50) & 1;
         const bool i_is_P = (e >> 51) & 1;
         const bool writer = (e >> 52) & 1;
+        const uint32_t kk = (uint32_t)((e >> 53) & 0xFF);
         const int i = L.row[o];
+        // issue every load that depends only on the queue entry first: the
+        // partner's state and species and both candidate history records
+        const size_t slot = lslot(k, p.maxn, i);
+        const Rec<NS>* rin = d.rec[cur];
+        const double4 pq = d.posr[j];
+        const double4 vq = d.vel[j];
+        const double4 oq = d.omg[j];
+        const int sj = (int)(d.spc[j] & 0xFFFF);
+        const bool had = writer && ((L.m_in[o] >> k) & 1ull);
+        Rec<NS> hist;
+        bool has_hist = false;
+        if (had) {
+            hist = rin[slot];
+            has_hist = true;
+        } else if (!writer && kk != 0xFF) {
+            const unsigned long long mj = d.mask[cur][j];
+            hist = rin[lslot((int)kk, p.maxn, (int)j)];
+            has_hist = (mj >> kk) & 1ull;
+        }
         Body bi;
-#if DEM_OWNER_LDS
         bi.pos = mk(L.px[o], L.py[o], L.pz[o]);
         bi.vel = mk(L.vx[o], L.vy[o], L.vz[o]);
         bi.omega = mk(L.wx[o], L.wy[o], L.wz[o]);
         bi.radius = L.r[o];
         bi.inv_mass = L.im[o];
-#else
-        {
-            const double4 a = d.posr[i], b = d.vel[i], c = d.omg[i];
-            bi.pos = xyz(a);
-            bi.vel = xyz(b);
-            bi.omega = xyz(c);
-            bi.radius = a.w;
-            bi.inv_mass = b.w;
-        }
-#endif
-        const double4 pq = d.posr[j];
         V3 xj = xyz(pq);
         for (int b = 0; b < p.nper; ++b) {
             const int c = (codes >> (2 * b)) & 3;
@@ -803,8 +814,6 @@ __device__ inline void process_batch(Dev<NS>& d, const Params& p, int cur, Owner
                 xj = vadd(xj, c == 1 ? sh : vneg(sh));
             }
         }
-        const double4 vq = d.vel[j];
-        const double4 oq = d.omg[j];
         Body bj;
         bj.pos = xj;
         bj.vel = xyz(vq);
@@ -822,35 +831,20 @@ __device__ inline void process_batch(Dev<NS>& d, const Params& p, int cur, Owner
         g.distance = distance;
         g.contact = vsub(P.pos, vmul(g.normal, P.radius - 0.5 * g.overlap));
         g.i_is_wall = false;
-        const size_t slot = lslot(k, p.n, i);
-        const Rec<NS>* rin = d.rec[cur];
         double spring[NS];
-        for (int a = 0; a < NS; ++a) spring[a] = 0.0;
-        bool had = false;
-        if (writer) {
-            had = (L.m_in[o] >> k) & 1ull;
-            if (had)
-                for (int a = 0; a < NS; ++a) spring[a] = rin[slot].s[a];
-        } else {
-            const uint32_t kk = d.rev[slot];
-            if (kk != 0xFF && ((d.mask[cur][j] >> kk) & 1ull)) {
-                const Rec<NS>& r = rin[lslot((int)kk, p.n, (int)j)];
-                for (int a = 0; a < NS; ++a) spring[a] = r.s[a];
-            }
-            if (img) {
-                // the stored frame has the lower id as P; this image
-                // evaluation has the higher id as P (reverseHistory)
-                spring[0] = -spring[0];
-                spring[1] = -spring[1];
-                spring[2] = -spring[2];
-                if constexpr (NS == 9) {
-                    spring[6] = -spring[6];
-                    spring[7] = -spring[7];
-                    spring[8] = -spring[8];
-                }
+        for (int a = 0; a < NS; ++a) spring[a] = has_hist ? hist.s[a] : 0.0;
+        if (!writer && img) {
+            // the stored frame has the lower id as P; this image evaluation
+            // has the higher id as P (reverseHistory)
+            spring[0] = -spring[0];
+            spring[1] = -spring[1];
+            spring[2] = -spring[2];
+            if constexpr (NS == 9) {
+                spring[6] = -spring[6];
+                spring[7] = -spring[7];
+                spring[8] = -spring[8];
             }
         }
-        const int sj = (int)(d.spc[j] & 0xFFFF);
         const dem_pair_params& sp = d.sp[L.sp[o] * p.ns + sj];
         V3 f, tc;
         contact_force<NS>(sp, P, I, g, spring, p.dt, f, tc);
@@ -878,7 +872,7 @@ __device__ inline void process_batch(Dev<NS>& d, const Params& p, int cur, Owner
             // time stamp is newer (InteractionHandler.cc:339-350); with equal
             // stamps (the initial force pass and step 0 both stamp 1) the
             // previous record survives unchanged.
-            if (img && had && rin[slot].ts >= r.ts) r = rin[slot];
+            if (img && had && hist.ts >= r.ts) r = hist;
             d.rec[cur ^ 1][slot] = r;
         }
     }
@@ -891,6 +885,7 @@ __global__ void __launch_bounds__(kBlock) FORCE_WAVES_ATTR k_force(Dev<NS> d, Pa
     const int lane = threadIdx.x & 63;
     OwnerLds& L = lds[threadIdx.x >> 6];
     const int i = xcd_block() * blockDim.x + threadIdx.x;
+    const int wbase = i - lane;  // the wavefront's particles are wbase .. wbase + 63
     const bool valid = i < p.n;
     const int ii = valid ? i : 0;
     const double4 pr = d.posr[ii];
@@ -905,7 +900,7 @@ __global__ void __launch_bounds__(kBlock) FORCE_WAVES_ATTR k_force(Dev<NS> d, Pa
     bi.radius = pr.w;
     bi.inv_mass = vl.w;
     const unsigned long long m_in = valid ? d.mask[cur][ii] : 0ull;
-#if DEM_OWNER_LDS
+    const int cnt = valid ? d.ncount[ii] : 0;
     L.px[lane] = pr.x;
     L.py[lane] = pr.y;
     L.pz[lane] = pr.z;
@@ -917,30 +912,46 @@ __global__ void __launch_bounds__(kBlock) FORCE_WAVES_ATTR k_force(Dev<NS> d, Pa
     L.wz[lane] = om.z;
     L.r[lane] = pr.w;
     L.im[lane] = vl.w;
-#endif
     L.m_in[lane] = m_in;
     L.sp[lane] = si;
     L.row[lane] = ii;
-    const int cnt = valid ? d.ncount[ii] : 0;
+    // the wavefront's lists are walked as one flat sequence of
+    // total = sum(cnt) entries, entry e belonging to the owner o with
+    // pre[o] <= e < pre[o] + cnt[o]
+    int incl = cnt;
+    for (int sft = 1; sft < 64; sft <<= 1) {
+        const int t = __shfl_up(incl, sft, 64);
+        if (lane >= sft) incl += t;
+    }
+    const int total = __shfl(incl, 63, 64);
+    L.pre[lane] = incl - cnt;
     unsigned long long touched = 0;
     V3 F = zero3(), T = zero3();
     int qn = 0;
     wave_sync();
 
-    // process the queued contacts in batches of 64 (or the remainder when
-    // `all`), each owner adding its contributions in queue order
+    // Process the queued contacts in batches of 64 (or the remainder when
+    // `all`).  The queue is ordered by owner, then by list position, so each
+    // owner adds its contributions in list order: the sum is deterministic.
     auto drain = [&](bool all) {
         int head = 0;
         while (qn - head >= 64 || (all && qn > head)) {
             const int nb = qn - head < 64 ? qn - head : 64;
             wave_sync();
             process_batch<NS>(d, p, cur, L, head, nb, lane);
+            L.first[lane] = -1;
             wave_sync();
-            for (int a = 0; a < nb; ++a) {
-                if ((int)((L.q[head + a] >> 38) & 63) == lane) {
-                    F = vadd(F, mk(L.cf[0][a], L.cf[1][a], L.cf[2][a]));
-                    T = vadd(T, mk(L.cf[3][a], L.cf[4][a], L.cf[5][a]));
-                }
+            if (lane < nb) {
+                const int ow = q_owner(L.q[head + lane]);
+                if (lane == 0 || q_owner(L.q[head + lane - 1]) != ow) L.first[ow] = lane;
+            }
+            wave_sync();
+            for (int a = L.first[lane]; a >= 0 && a < nb; ++a) {
+                const unsigned long long qa = L.q[head + a];
+                if (q_owner(qa) != lane) break;
+                F = vadd(F, mk(L.cf[0][a], L.cf[1][a], L.cf[2][a]));
+                T = vadd(T, mk(L.cf[3][a], L.cf[4][a], L.cf[5][a]));
+                if ((qa >> 52) & 1) touched |= 1ull << ((qa >> 32) & 63);
             }
             head += nb;
         }
@@ -953,20 +964,38 @@ __global__ void __launch_bounds__(kBlock) FORCE_WAVES_ATTR k_force(Dev<NS> d, Pa
         }
     };
 
-    for (int k0 = 0; __any(k0 < cnt); k0 += kScanU) {
-        uint32_t ej[kScanU];
+    for (int e0 = 0; e0 < total; e0 += 64 * kScanU) {
+        uint32_t ej[kScanU], kr[kScanU];
+        int eo[kScanU], ek[kScanU];
         double4 pq[kScanU];
 #pragma unroll
-        for (int u = 0; u < kScanU; ++u) ej[u] = k0 + u < cnt ? d.nbr[lslot(k0 + u, p.n, ii)] : 0u;
+        for (int u = 0; u < kScanU; ++u) {
+            const int e = e0 + 64 * u + lane;
+            int o = 0;
+            if (e < total)
+                for (int sft = 32; sft > 0; sft >>= 1)
+                    if (L.pre[o + sft] <= e) o += sft;
+            eo[u] = o;
+            ek[u] = e - L.pre[o];
+            ej[u] = 0u;
+            kr[u] = 0xFFu;
+            if (e < total) {
+                const size_t sl = lslot(ek[u], p.maxn, wbase + o);
+                ej[u] = d.nbr[sl];
+                kr[u] = d.rev[sl];
+            }
+        }
 #pragma unroll
         for (int u = 0; u < kScanU; ++u)
-            pq[u] = k0 + u < cnt ? d.posr[ej[u] & ~kWriterBit] : make_double4(0.0, 0.0, 0.0, 0.0);
+            pq[u] = e0 + 64 * u + lane < total ? d.posr[ej[u] & ~kWriterBit] : make_double4(0.0, 0.0, 0.0, 0.0);
 #pragma unroll
         for (int u = 0; u < kScanU; ++u) {
-            const int k = k0 + u;
             bool contact = false;
             unsigned long long entry = 0;
-            if (k < cnt) {
+            if (e0 + 64 * u + lane < total) {
+                const int o = eo[u];
+                const V3 po = mk(L.px[o], L.py[o], L.pz[o]);
+                const double ro = L.r[o];
                 const uint32_t j = ej[u] & ~kWriterBit;
                 const bool writer = (ej[u] & kWriterBit) != 0;
                 V3 xj = xyz(pq[u]);
@@ -974,7 +1003,7 @@ __global__ void __launch_bounds__(kBlock) FORCE_WAVES_ATTR k_force(Dev<NS> d, Pa
                 int codes = 0;
                 for (int b = 0; b < p.nper; ++b) {
                     const int a = p.paxis[b];
-                    const double c = comp(xj, a) - comp(bi.pos, a);
+                    const double c = comp(xj, a) - comp(po, a);
                     if (c > 0.5 * p.pshift[b] || c < -0.5 * p.pshift[b]) {
                         img = true;
                         if (!(periodic_distance(p, b, xj) < pq[u].w + p.rmax)) exists = false;
@@ -985,20 +1014,19 @@ __global__ void __launch_bounds__(kBlock) FORCE_WAVES_ATTR k_force(Dev<NS> d, Pa
                 }
                 if (!img || exists) {
                     const bool i_is_P = img || writer;
-                    const V3 posP = i_is_P ? bi.pos : xj;
-                    const V3 posI = i_is_P ? xj : bi.pos;
-                    const double rP = i_is_P ? pr.w : pq[u].w;
-                    const double rI = i_is_P ? pq[u].w : pr.w;
+                    const V3 posP = i_is_P ? po : xj;
+                    const V3 posI = i_is_P ? xj : po;
+                    const double rP = i_is_P ? ro : pq[u].w;
+                    const double rI = i_is_P ? pq[u].w : ro;
                     const V3 branch = vsub(posP, posI);
                     const double d2 = vlen2(branch);
                     const double sum_r = rI + rP;
                     if (d2 < sum_r * sum_r) {
                         contact = true;
-                        entry = (unsigned long long)j | ((unsigned long long)k << 32) |
-                                ((unsigned long long)lane << 38) | ((unsigned long long)codes << 44) |
+                        entry = (unsigned long long)j | ((unsigned long long)ek[u] << 32) |
+                                ((unsigned long long)o << 38) | ((unsigned long long)codes << 44) |
                                 ((unsigned long long)img << 50) | ((unsigned long long)i_is_P << 51) |
-                                ((unsigned long long)writer << 52);
-                        if (writer) touched |= 1ull << k;
+                                ((unsigned long long)writer << 52) | ((unsigned long long)(kr[u] & 0xFF) << 53);
                     }
                 }
             }
@@ -1018,9 +1046,9 @@ __global__ void __launch_bounds__(kBlock) FORCE_WAVES_ATTR k_force(Dev<NS> d, Pa
     while (carry) {
         const int k = __ffsll((long long)carry) - 1;
         carry &= carry - 1;
-        const Rec<NS> r = rin[lslot(k, p.n, i)];
+        const Rec<NS> r = rin[lslot(k, p.maxn, i)];
         if (!p.erase || r.ts > p.nt) {
-            rout[lslot(k, p.n, i)] = r;
+            rout[lslot(k, p.maxn, i)] = r;
             m_out |= 1ull << k;
         }
     }
@@ -1135,7 +1163,7 @@ __global__ void __launch_bounds__(kBlock) k_collect_contacts(Dev<NS> d, Params p
         m &= m - 1;
         const int o = atomicAdd(counter, 1);
         if (o >= cap) continue;
-        const size_t s = lslot(k, p.n, i);
+        const size_t s = lslot(k, p.maxn, i);
         pid[o] = idi;
         iid[o] = d.id[d.nbr[s] & ~kWriterBit];
         for (int a = 0; a < 9; ++a) spring[9 * (size_t)o + a] = a < NS ? d.rec[cur][s].s[a] : 0.0;
