@@ -10,8 +10,8 @@ strong gravity straight down, then the same with background drag), then the
 chute gravity is switched on and, after the warm-up steps, K steps are timed.
 
 Prints one JSON line: particle-steps/s (flowing spheres x steps / s), the
-roofline of the dominant kernel (the force kernel, HIP-event timed on the
-engine's stream) and a CPU baseline (the oracle, one thread, on a bounded
+roofline of the dominant kernel (the force kernel, which also integrates the
+step; HIP-event timed on the engine's stream) and a CPU baseline (the oracle, one thread, on a bounded
 sample of the same state).
 """
 import argparse
@@ -133,12 +133,17 @@ def main():
     value = n_flow * a.steps / elapsed
     # contacts per particle over the window (pairs counted once + wall contacts)
     cbar = 0.5 * (c0 + c1) / n_total
+    lbar = st.n_list_entries / n_total  # Verlet-list entries per particle
     force_ms = st.t_force_ms / max(st.n_timed_steps, 1)
-    b_force = 144.0 + 64.0 * cbar        # algorithmic bytes per particle in the force kernel
+    # algorithmic HBM bytes per particle of the force kernel (which also
+    # integrates): own state in (posr, vel, omg, quat, disp: 160; mask, list
+    # length, species, wall mask: 20) and out (posr, vel, omg, quat, disp:
+    # 160; masks: 12), 5 B per list entry (partner index, reverse slot), and
+    # 64 B per pair in contact (history record in and out)
+    b_force = 352.0 + 5.0 * lbar + 64.0 * cbar
     achieved = b_force * n_total / (force_ms * 1e-3) / 1e9
-    b_step = 264.0 + 64.0 * cbar         # algorithmic bytes per particle-step
-    step_gbs = b_step * n_total * a.steps / elapsed / 1e9
-    print(f"# timed {a.steps} steps in {elapsed:.3f}s: force {force_ms:.3f} ms, kick "
+    step_gbs = b_force * n_total * a.steps / elapsed / 1e9
+    print(f"# timed {a.steps} steps in {elapsed:.3f}s: force+integration {force_ms:.3f} ms, kick "
           f"{st.t_kick_ms / max(st.n_timed_steps, 1):.3f} ms, rebuild {st.t_rebuild_ms / max(st.n_timed_steps, 1):.3f} ms "
           f"per step; {st.n_rebuilds} rebuilds total; C/N {cbar:.3f}; max list {st.max_list_length}",
           file=sys.stderr, flush=True)
@@ -181,6 +186,7 @@ def main():
                    "flowing_spheres": n_flow, "base_spheres": sc.info["n_base"],
                    "diameter": "U[0.5,1.5]", "footprint": round(sc.info["footprint"], 3),
                    "dt": sc.dt, "contacts_per_particle": round(cbar, 4),
+                   "list_entries_per_particle": round(lbar, 3),
                    "rebuilds_in_window": int(rebuilds_window), "step_hbm_gbs_algorithmic": round(step_gbs, 2),
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -144,7 +144,7 @@ typedef struct dem_periodic {
 typedef struct dem_config {
     int32_t device;          /* HIP device ordinal */
     int32_t max_neighbors;   /* Verlet-list capacity per particle; 0 = 64 */
-    double skin;             /* Verlet skin distance; <= 0 = 0.1 * min diameter */
+    double skin;             /* Verlet skin distance; <= 0 = 0.2 * min diameter */
     int32_t grid_levels;     /* cell-grid levels; 0 = automatic (<= 3) */
     int32_t reserved;
 } dem_config;

@@ -61,6 +61,11 @@ enum Flag {
     F_NUM = 12
 };
 
+// Integration fused into the force kernel: none (a force pass only), the
+// closing half-step only, or the closing half-step of this step and the
+// opening half-step and drift of the next.
+enum { KICK_NONE = 0, KICK_AFTER = 1, KICK_BOTH = 2 };
+
 template <int NS>
 struct Rec {
     double s[NS];
@@ -78,6 +83,7 @@ struct Params {
     double rmax, skin, thr2;
     uint32_t nt;
     uint32_t alive_min; // stored interactions with an older stamp have been erased
+    int kick;           // integration fused into the force pass (KICK_*)
     // grid
     double h[kMaxLevels];       // cell size of a non-periodic axis
     double rmaxl[kMaxLevels];   // largest radius on each level
@@ -226,39 +232,34 @@ __host__ __device__ inline size_t lslot(int k, int stride, int i) { return (size
 __device__ inline bool rebuild_pending(const int* flags) { return __atomic_load_n(&flags[F_REBUILD], __ATOMIC_RELAXED) != 0; }
 
 // ------------------------------------------------------------------------
-// Velocity Verlet.  One launch applies, in this order,
+// Velocity Verlet for one particle, applying in this order
 //  (after)  BaseParticle::integrateAfterForceComputation of the previous step
 //           (BaseParticle.cc:760-776),
 //  (before) BaseParticle::integrateBeforeForceComputation (BaseParticle.cc:
 //           718-752; fixed particles: BaseInteractable.cc:538-609), the
 //           periodic wrap of checkInteractionWithBoundaries (DPMBase.cc:3272,
-//           PeriodicBoundary.cc:374-391) and the Verlet-list displacement test.
+//           PeriodicBoundary.cc:374-391) and the Verlet-list displacement
+//           test, whose result it returns.
+// F and T are the particle's total force and torque of the last force pass.
 // ------------------------------------------------------------------------
-template <int NS>
-__global__ void __launch_bounds__(kBlock) k_kick_drift(Dev<NS> d, Params p, int after, int before)
+__device__ inline bool integrate(const Params& p, bool after, bool before, double4& pr, double4& vl, double4& om,
+                                 double4& qq, double4& ds, V3 F, V3 T)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= p.n) return;
-    double4 vl = d.vel[i];
-    double4 om = d.omg[i];
     const double inv_mass = vl.w;
     V3 v = xyz(vl);
     V3 w = xyz(om);
-    double4 qq = d.quat[i];
     double q[4] = {qq.x, qq.y, qq.z, qq.w};
     const double dt = p.dt;
+    bool flag = false;
     if (after && inv_mass != 0.0) {
-        const V3 F = xyz(d.frc[i]);
         v = vadd(v, vmul(vmul(vmul(F, inv_mass), 0.5), dt));
         if (p.rotation) {
-            const V3 T = xyz(d.trq[i]);
             double R[6];
             rotate_inv_inertia(q, om.w, R);
             w = vadd(w, vmul(vmul(sym_mul(R, T), 0.5), dt));
         }
     }
     if (before) {
-        double4 pr = d.posr[i];
         V3 x = xyz(pr);
         V3 step;
         if (inv_mass == 0.0) {
@@ -266,12 +267,10 @@ __global__ void __launch_bounds__(kBlock) k_kick_drift(Dev<NS> d, Params p, int 
             x = vadd(x, step);
             rotate_quaternion(q, vmul(w, dt));
         } else {
-            const V3 F = xyz(d.frc[i]);
             v = vadd(v, vmul(vmul(vmul(F, inv_mass), 0.5), dt));
             step = vmul(v, dt);
             x = vadd(x, step);
             if (p.rotation) {
-                const V3 T = xyz(d.trq[i]);
                 double R[6];
                 rotate_inv_inertia(q, om.w, R);
                 w = vadd(w, vmul(vmul(sym_mul(R, T), 0.5), dt));
@@ -280,17 +279,46 @@ __global__ void __launch_bounds__(kBlock) k_kick_drift(Dev<NS> d, Params p, int 
         }
         for (int b = 0; b < p.nper; ++b)
             if (periodic_distance(p, b, x) < 0) x = periodic_shift(p, b, x);
-        d.posr[i] = make_double4(x.x, x.y, x.z, pr.w);
-        d.quat[i] = make_double4(q[0], q[1], q[2], q[3]);
-        double4 ds = d.disp[i];
+        pr = make_double4(x.x, x.y, x.z, pr.w);
+        qq = make_double4(q[0], q[1], q[2], q[3]);
         ds.x += step.x;
         ds.y += step.y;
         ds.z += step.z;
-        d.disp[i] = ds;
-        if (ds.x * ds.x + ds.y * ds.y + ds.z * ds.z > p.thr2) atomicOr(&d.flags[F_REBUILD], 1);
+        flag = ds.x * ds.x + ds.y * ds.y + ds.z * ds.z > p.thr2;
     }
-    d.vel[i] = make_double4(v.x, v.y, v.z, inv_mass);
-    d.omg[i] = make_double4(w.x, w.y, w.z, om.w);
+    vl = make_double4(v.x, v.y, v.z, inv_mass);
+    om = make_double4(w.x, w.y, w.z, om.w);
+    return flag;
+}
+
+// Stand-alone integration step (the first half-step of a batch of steps, and
+// the closing half-step after a force pass that did not integrate).
+template <int NS>
+__global__ void __launch_bounds__(kBlock) k_kick_drift(Dev<NS> d, Params p, int after, int before)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= p.n) return;
+    double4 vl = d.vel[i];
+    double4 om = d.omg[i];
+    double4 qq = d.quat[i];
+    double4 pr = make_double4(0.0, 0.0, 0.0, 0.0), ds = pr;
+    V3 F = zero3(), T = zero3();
+    if (vl.w != 0.0) {
+        F = xyz(d.frc[i]);
+        if (p.rotation) T = xyz(d.trq[i]);
+    }
+    if (before) {
+        pr = d.posr[i];
+        ds = d.disp[i];
+    }
+    if (integrate(p, after, before, pr, vl, om, qq, ds, F, T)) atomicOr(&d.flags[F_REBUILD], 1);
+    if (before) {
+        d.posr[i] = pr;
+        d.quat[i] = qq;
+        d.disp[i] = ds;
+    }
+    d.vel[i] = vl;
+    d.omg[i] = om;
 }
 
 // ------------------------------------------------------------------------
@@ -1127,8 +1155,30 @@ __global__ void __launch_bounds__(kBlock) FORCE_WAVES_ATTR k_force(Dev<NS> d, Pa
         const double mass = 1.0 / vl.w;
         F = vadd(F, vsub(vmul(mk(p.g[0], p.g[1], p.g[2]), mass), vmul(bi.vel, p.drag)));
     }
-    d.frc[i] = make_double4(F.x, F.y, F.z, 0.0);
-    d.trq[i] = make_double4(T.x, T.y, T.z, 0.0);
+    if (p.kick != KICK_BOTH) {
+        d.frc[i] = make_double4(F.x, F.y, F.z, 0.0);
+        d.trq[i] = make_double4(T.x, T.y, T.z, 0.0);
+    }
+    if (p.kick != KICK_NONE) {
+        // the integration of this particle, written to the alternate state
+        // buffers: other wavefronts of this launch still read the current ones
+        double4 prn = pr, vln = vl, omn = om, qq = make_double4(0.0, 0.0, 0.0, 0.0), ds = qq;
+        const bool before = p.kick == KICK_BOTH;
+        if (before) {
+            qq = d.quat[i];
+            ds = d.disp[i];
+        } else if (p.rotation && !fixed_i) {
+            qq = d.quat[i];
+        }
+        if (integrate(p, true, before, prn, vln, omn, qq, ds, F, T)) atomicOr(&d.flags[F_REBUILD], 1);
+        d.t_posr[i] = prn;
+        d.t_vel[i] = vln;
+        d.t_omg[i] = omn;
+        if (before) {
+            d.quat[i] = qq;
+            d.disp[i] = ds;
+        }
+    }
 }
 
 // ------------------------------------------------------------------------
@@ -1431,14 +1481,21 @@ void launch_rebuild(dem_ctx* ctx)
 }
 
 template <int NS>
-void launch_force(dem_ctx* ctx, int erase)
+void launch_force(dem_ctx* ctx, int erase, int kick)
 {
     Dev<NS>& d = dev<NS>(ctx);
     ctx->prm.erase = erase;
     ctx->prm.nt = (uint32_t)ctx->nt;
+    ctx->prm.kick = kick;
     const unsigned g = xcd_grid(grid_for(ctx->n));
     hipLaunchKernelGGL(k_force<NS>, dim3(g), dim3(kBlock), 0, ctx->stream, d, ctx->prm, ctx->cur);
     ctx->cur ^= 1;
+    if (kick != KICK_NONE) {
+        // the integrated state went to the gather buffers: make it current
+        std::swap(d.posr, d.t_posr);
+        std::swap(d.vel, d.t_vel);
+        std::swap(d.omg, d.t_omg);
+    }
 }
 
 template <int NS>
@@ -1491,24 +1548,21 @@ int run_steps(dem_ctx* ctx, int64_t nsteps)
         for (int64_t k = 0; k < n; ++k) {
             const int64_t s = s0 + k;
             if (prof) HIPCHK(hipEventRecord(ctx->events[4 * k], ctx->stream));
-            launch_kick<NS>(ctx, s > 0 ? 1 : 0, 1);
+            // the first half-step and drift use the forces stored by the
+            // previous force pass; later ones are fused into the force kernel
+            if (s == 0) launch_kick<NS>(ctx, 0, 1);
             if (prof) HIPCHK(hipEventRecord(ctx->events[4 * k + 1], ctx->stream));
             ctx->prm.erase = 1;
             ctx->prm.nt = (uint32_t)ctx->nt;
             ctx->prm.alive_min = ctx->alive_min;
             launch_rebuild<NS>(ctx);
             if (prof) HIPCHK(hipEventRecord(ctx->events[4 * k + 2], ctx->stream));
-            launch_force<NS>(ctx, 1);
+            launch_force<NS>(ctx, 1, s + 1 < nsteps ? KICK_BOTH : KICK_AFTER);
             if (prof) HIPCHK(hipEventRecord(ctx->events[4 * k + 3], ctx->stream));
             ctx->time += ctx->dt;
             ctx->nt += 1;
             ctx->alive_min = (uint32_t)ctx->nt;
             ctx->prm.alive_min = ctx->alive_min;
-        }
-        if (s0 + n == nsteps) {
-            if (prof) HIPCHK(hipEventRecord(ctx->events[4 * n], ctx->stream));
-            launch_kick<NS>(ctx, 1, 0);
-            if (prof) HIPCHK(hipEventRecord(ctx->events[4 * n + 1], ctx->stream));
         }
         if (prof) {
             HIPCHK(hipStreamSynchronize(ctx->stream));
@@ -1520,11 +1574,6 @@ int run_steps(dem_ctx* ctx, int64_t nsteps)
                 ctx->t_kick += a;
                 ctx->t_rebuild += b;
                 ctx->t_force += c;
-            }
-            if (s0 + n == nsteps) {
-                float a = 0;
-                HIPCHK(hipEventElapsedTime(&a, ctx->events[4 * n], ctx->events[4 * n + 1]));
-                ctx->t_kick += a;
             }
             ctx->timed_steps += n;
         }
@@ -1540,7 +1589,7 @@ int run_forces(dem_ctx* ctx)
     ctx->prm.nt = (uint32_t)ctx->nt;
     ctx->prm.alive_min = ctx->alive_min;
     launch_rebuild<NS>(ctx);
-    launch_force<NS>(ctx, 0);
+    launch_force<NS>(ctx, 0, KICK_NONE);
     HIPCHK(hipGetLastError());
     return check_flags(ctx);
 }
@@ -1717,7 +1766,7 @@ int dem_upload_particles(dem_ctx* ctx, int64_t n, const uint32_t* id, const int3
         rmin = std::min(rmin, radius[i]);
         rmax = std::max(rmax, radius[i]);
     }
-    const double skin = ctx->skin_cfg > 0 ? ctx->skin_cfg : 0.2 * rmin;
+    const double skin = ctx->skin_cfg > 0 ? ctx->skin_cfg : 0.4 * rmin;
     // levels (EXPONENTIAL cell-size distribution of MercuryBase::hGridRebuild,
     // MercuryBase.cc:204-306, between the smallest and largest diameter)
     int L = ctx->levels_cfg;
