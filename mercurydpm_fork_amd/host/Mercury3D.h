@@ -145,6 +145,13 @@ public:
     double getDissipation() const { return params.dissipation; }
     void setEffectiveElasticModulusAndRestitutionCoefficient(double E, double rest);
     void setEffectiveShearModulus(double G) { params.shear_modulus = G; }
+    // HertzianViscoelasticNormalSpecies::setEffectiveElasticModulusAndPoissonRatio
+    // (cc:142-159); the shear modulus is the reference's E / 2 * (1 + nu)
+    void setEffectiveElasticModulusAndPoissonRatio(double E, double nu)
+    {
+        params.elastic_modulus = E;
+        params.shear_modulus = E / 2 * (1 + nu);
+    }
     void setEffectiveElasticModulusAndEffectiveShearModulus(double E, double G)
     {
         params.elastic_modulus = E;

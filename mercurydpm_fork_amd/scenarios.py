@@ -180,6 +180,30 @@ def hertz_restitution_unit_test() -> Scenario:
     return Scenario("HertzContactRestitutionUnitTest", [s], [], [], (0.0, 0.0, 0.0), dt, st, t_max=1.5)
 
 
+def oblique_impact_self_test() -> Scenario:
+    """Drivers/SelfTests/Interactions/ObliqueImpactSelfTest.cpp:31-78: a
+    HertzianViscoelasticMindlin sphere (E* = 1e5, nu = 0.3, mu = 0.1, no
+    dissipation) hits a flat wall at 10 m/s tangential, 1 m/s normal; the
+    reference's .fstat (SelfTestData/ObliqueImpactSelfTest.fstat) holds the
+    contact every 500 steps."""
+    s = Species(density=6.0 / PI, friction_law=dem.FRICTION_MINDLIN)
+    s.set_elastic_modulus_and_poisson_ratio(1e5, 0.3)
+    s.set_sliding_friction(0.1)
+    r = 0.005
+    w = infinite_wall((0, -1, 0), (0, 0.0, 0))
+    st = make_state([(0.0 + r, r + 0.000005, 0.0)], [(10.0, -1.0, 0.0)], [r], [0], [s])
+    return Scenario("ObliqueImpactSelfTest", [s], [w], [], (0.0, 0.0, 0.0), 0.25e-8, st, t_max=0.0005)
+
+
+OBLIQUE_IMPACT_SAVE_COUNT = 500
+
+
+def load_fstat(path):
+    """Rows of a .fstat fixture as a float array (columns as written by
+    BaseInteraction::writeToFStat, BaseInteraction.cc:487-531)."""
+    return np.loadtxt(path, comments="#", ndmin=2)
+
+
 # --------------------------------------------------------------------------
 # Seeded workloads
 # --------------------------------------------------------------------------

@@ -147,6 +147,15 @@ class Species:
             self.dissipation = math.sqrt(5.0)
         return self
 
+    def set_elastic_modulus_and_poisson_ratio(self, E: float, nu: float):
+        """setEffectiveElasticModulusAndPoissonRatio
+        (HertzianViscoelasticNormalSpecies.cc:142-159).  The shear modulus is
+        the reference's expression as written, E / 2 * (1 + nu)."""
+        self.normal_law = dem.NORMAL_HERTZIAN_VISCOELASTIC
+        self.elastic_modulus = E
+        self.shear_modulus = E / 2 * (1 + nu)
+        return self
+
     def hertz_collision_time(self, diameter: float, density: float, rel_velocity: float) -> float:
         r = density / self.elastic_modulus
         return 2.214 * math.pow(r * r / rel_velocity, 0.2) * diameter

@@ -21,7 +21,8 @@
  *
  * Pinning (tests/test_oracle_golden.py): TangentialSpringUnitTest,
  * PeriodicWallsUnitTest (inline gold, 1e-10), InclinedPlaneUnitTest closed
- * form, HertzContactRestitutionUnitTest, MindlinSelfTest gold .fstat,
+ * form, HertzContactRestitutionUnitTest, the 264 contact rows of
+ * ObliqueImpactSelfTest.fstat (Hertz-Mindlin sliding on a wall, 10 digits),
  * free-flight closed form, two-body collision time / restitution closed forms,
  * brute-force O(N^2) pair sets, shifted-copy invariance of periodic images,
  * cylinder-vs-plane equivalence.

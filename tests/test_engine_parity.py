@@ -200,3 +200,14 @@ def test_box_lvf_vs_oracle():
     assert_contacts_close(g, o)
     a, b = g.download(), o.download()
     np.testing.assert_allclose(a["x"], b["x"], rtol=0, atol=1e-10)
+
+
+def test_engine_oblique_impact_mindlin_fstat_gold():
+    """The reference's ObliqueImpactSelfTest.fstat (Hertz-Mindlin sliding
+    against a wall), every saved contact row to its 10 printed digits."""
+    from fstat_check import check_oblique_impact
+    sc = S.oblique_impact_self_test()
+    g = dem.Engine()
+    S.apply(g, sc)
+    g.compute_forces()
+    check_oblique_impact(g)

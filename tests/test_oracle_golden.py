@@ -10,6 +10,7 @@ from bruteforce import brute_force_contacts
 from mercurydpm_fork_amd import dem
 from mercurydpm_fork_amd import scenarios as S
 from mercurydpm_fork_amd.species import PI, Species, cylindrical_wall, infinite_wall
+from fstat_check import check_oblique_impact
 
 
 def run_oracle(sc, n_steps=None):
@@ -183,3 +184,13 @@ def test_step0_periodic_contact_keeps_initial_spring():
     d0, d1, dts = res[False]
     assert np.max(np.abs(d1 - d0)) > 0.5 * np.max(np.abs(d0))
     assert dts == 1
+
+
+def test_oblique_impact_mindlin_fstat_gold():
+    """ObliqueImpactSelfTest (SelfTestData/ObliqueImpactSelfTest.fstat): the
+    Hertz-Mindlin law with sliding against a wall, 264 saved contact rows."""
+    sc = S.oblique_impact_self_test()
+    e = oracle_lib.engine()
+    S.apply(e, sc)
+    e.compute_forces()
+    check_oblique_impact(e)
