@@ -45,7 +45,7 @@
 extern "C" {
 #endif
 
-#define DEM_ABI_VERSION 1
+#define DEM_ABI_VERSION 2
 
 /* error codes */
 #define DEM_OK 0
@@ -237,6 +237,27 @@ int dem_compute_forces(dem_ctx* ctx);
  * computeAllForces, removeDuplicatePeriodicParticles,
  * integrateAfterForceComputation, eraseOldInteractions, time_ += dt. */
 int dem_step(dem_ctx* ctx, int64_t n_steps);
+
+/* One DPMBase::computeOneTimeStep split where the reference calls
+ * actionsBeforeTimeStep (DPMBase.cc:4347), so that a host can prescribe
+ * particle motion there as drivers do:
+ *   dem_step_begin: integrateBeforeForceComputation and the periodic wrap of
+ *                   checkInteractionWithBoundaries;
+ *   dem_step_end:   checkAndDuplicatePeriodicParticles, hGridActions-
+ *                   BeforeTimeStep, computeAllForces, removeDuplicate-
+ *                   PeriodicParticles, integrateAfterForceComputation,
+ *                   eraseOldInteractions, time_ += dt.
+ * dem_step_begin + dem_step_end gives the same state as dem_step(ctx, 1). */
+int dem_step_begin(dem_ctx* ctx);
+int dem_step_end(dem_ctx* ctx);
+
+/* BaseInteractable::setPosition / setVelocity / setAngularVelocity /
+ * setOrientation for all n particles, arrays in handler order (x, v, w:
+ * 3 doubles per particle, q: 4); a NULL array leaves that quantity as it is.
+ * The stored interactions are kept, and the displacement counts towards the
+ * Verlet-list rebuild like any other motion. */
+int dem_set_kinematics(dem_ctx* ctx, int64_t n, const double* x, const double* v, const double* w,
+                       const double* q);
 
 /* Download state in handler order (any pointer may be NULL). */
 int dem_download_particles(dem_ctx* ctx, double* x, double* v, double* w, double* q,

@@ -41,6 +41,7 @@ namespace {
 
 constexpr int kMaxWalls = 32;
 constexpr int kMaxLevels = 4;
+constexpr int kMaxSlabs = 64;
 constexpr int kBlock = 256;
 constexpr uint64_t kEmptyKey = ~0ull;
 
@@ -58,7 +59,8 @@ enum Flag {
     F_ORPH_COUNT0 = 9,
     F_ORPH_COUNT1 = 10,
     F_ORPH_OVERFLOW = 11,
-    F_NUM = 12
+    F_DONE_BLOCKS = 12, // blocks of the last rebuild kernel that have finished
+    F_NUM = 13
 };
 
 // Integration fused into the force kernel: none (a force pass only), the
@@ -90,8 +92,11 @@ struct Params {
     double hc[kMaxLevels][3];   // cell size per axis (periodic axes: L / ncell)
     int ncell[kMaxLevels][3];   // cells per periodic axis (0 = not periodic)
     uint32_t nb[kMaxLevels];    // buckets per level (nbx * nby * nbz)
-    uint32_t nboff[kMaxLevels]; // bucket offset of each level
-    int nbd[kMaxLevels][3];     // bucket grid dimensions per level
+    int nbd[kMaxLevels][3];     // bucket box dimensions per level
+    uint32_t bstride[kMaxLevels][3]; // level-local bucket index = sum of wrapped coordinate x stride
+    int bax0;                   // slab axis (the longest)
+    int nslab;                  // slabs along it
+    int32_t sdelta[kMaxSlabs][kMaxLevels]; // bucket = level-local index + sdelta[slab][level]
 };
 
 template <int NS>
@@ -114,6 +119,7 @@ struct Dev {
     uint32_t* wmask[2];
     // grid
     uint32_t *key, *count, *start, *cursor, *perm;
+    uint32_t *tsum, *toff; // per-scan-tile bucket counts and their exclusive scan
     // hash
     unsigned long long* hkey;
     Rec<NS>* hval;
@@ -141,38 +147,48 @@ __device__ inline uint64_t mix64(uint64_t k)
     return k;
 }
 
-// Bucket of a cell: its linear index in a per-level box of nbd[0] x nbd[1] x
-// nbd[2] cells, coordinates taken modulo the box.  Along a periodic axis the
-// box spans the whole period, so there is no aliasing; along an open axis it
-// spans the extent seen at upload with a margin, and cells beyond alias onto
-// buckets of cells far away (told apart by the cell coordinates stored with
-// each particle).  Sorting by this index keeps neighbouring cells, and hence
-// the partners gathered by the force kernel, close in memory.
+// Bucket of a cell.  Cell coordinates are taken modulo a per-level box of
+// nbd[0] x nbd[1] x nbd[2] cells.  Along a periodic axis the box spans the
+// whole period, so there is no aliasing; along an open axis it spans the
+// extent seen at upload with a margin, and cells beyond alias onto buckets of
+// cells far away (told apart by the cell coordinates stored with each
+// particle).  The domain is cut into nslab slabs along its longest axis
+// (bax0); buckets are numbered slab by slab, level by level inside a slab, and
+// linearly inside a level (slow axis slowest).  Sorting by bucket therefore
+// keeps neighbouring cells -- of every level -- close in memory, and any
+// contiguous share of the sorted particles is a compact piece of the domain
+// with the full mix of sizes.
 __device__ inline int wrap_mod(int c, int n)
 {
     int r = c % n;
     return r < 0 ? r + n : r;
 }
 
-__device__ inline uint32_t cell_bucket(const int nbd[3], int cx, int cy, int cz)
+__device__ inline uint32_t cell_key(const Params& p, int l, int cx, int cy, int cz)
 {
-    return ((uint32_t)wrap_mod(cz, nbd[2]) * (uint32_t)nbd[1] + (uint32_t)wrap_mod(cy, nbd[1])) * (uint32_t)nbd[0] +
-           (uint32_t)wrap_mod(cx, nbd[0]);
+    const uint32_t wx = (uint32_t)wrap_mod(cx, p.nbd[l][0]);
+    const uint32_t wy = (uint32_t)wrap_mod(cy, p.nbd[l][1]);
+    const uint32_t wz = (uint32_t)wrap_mod(cz, p.nbd[l][2]);
+    const uint32_t ws = p.bax0 == 0 ? wx : (p.bax0 == 1 ? wy : wz);
+    const int slab = (int)((uint64_t)ws * (uint32_t)p.nslab / (uint32_t)p.nbd[l][p.bax0]);
+    return wx * p.bstride[l][0] + wy * p.bstride[l][1] + wz * p.bstride[l][2] + (uint32_t)p.sdelta[slab][l];
 }
 
 // Blocks are dealt round-robin to the 8 XCDs.  Remap them so that each XCD
-// (and its L2) takes runs of kXcdChunk consecutive blocks, i.e. spatially
-// compact groups of particles, while the runs stay interleaved over the whole
-// range so that every XCD gets a similar mix of small and large spheres.
-// gridDim.x must be a multiple of 8 * kXcdChunk.
+// (and its L2) works on one contiguous range of the sorted particles, i.e. on
+// a compact piece of the domain whose partners mostly lie in the same range
+// (kXcdChunk = 0), or on runs of kXcdChunk consecutive blocks interleaved
+// over the whole range.  gridDim.x must be a multiple of 8 * max(kXcdChunk, 1).
 #ifndef DEM_XCD_CHUNK
-#define DEM_XCD_CHUNK 1
+#define DEM_XCD_CHUNK 0
 #endif
 constexpr int kXcdChunk = DEM_XCD_CHUNK;
-__host__ __device__ constexpr unsigned xcd_grid(unsigned g) { return (g + 8 * kXcdChunk - 1) / (8 * kXcdChunk) * (8 * kXcdChunk); }
+constexpr unsigned kXcdQuantum = 8u * (kXcdChunk > 0 ? kXcdChunk : 1);
+__host__ __device__ constexpr unsigned xcd_grid(unsigned g) { return (g + kXcdQuantum - 1) / kXcdQuantum * kXcdQuantum; }
 __device__ inline int xcd_block()
 {
     const int b = (int)blockIdx.x, x = b & 7, r = b >> 3;
+    if (kXcdChunk == 0) return x * (int)(gridDim.x >> 3) + r;
     return ((r / kXcdChunk) * 8 + x) * kXcdChunk + r % kXcdChunk;
 }
 
@@ -214,7 +230,7 @@ __device__ inline V3 periodic_shift(const Params& p, int b, V3 pos)
 // Rebuild kernels run with a bounded grid and stride over their range, so a
 // launch with no rebuild pending retires in a few microseconds.
 constexpr int kRebuildBlocks = 2048;
-static_assert(kRebuildBlocks % (8 * kXcdChunk) == 0, "rebuild grid must suit the XCD remap");
+static_assert(kRebuildBlocks % kXcdQuantum == 0, "rebuild grid must suit the XCD remap");
 #define GRID_STRIDE_XCD(i, n) \
     for (uint64_t i = (uint64_t)xcd_block() * blockDim.x + threadIdx.x; i < (uint64_t)(n); i += (uint64_t)gridDim.x * blockDim.x)
 #define GRID_STRIDE(i, n) \
@@ -322,6 +338,32 @@ __global__ void __launch_bounds__(kBlock) k_kick_drift(Dev<NS> d, Params p, int 
 }
 
 // ------------------------------------------------------------------------
+// Rebuild: counting sort by grid bucket (cell_key), then by id in a bucket.
+// ------------------------------------------------------------------------
+__device__ inline int4 cell_of(const Params& p, int lvl, V3 x)
+{
+    int c[3];
+    for (int a = 0; a < 3; ++a) {
+        const double xa = comp(x, a);
+        const int pb = p.axis_pb[a];
+        if (pb >= 0) {
+            const int nc = p.ncell[lvl][a];
+            int t = (int)floor((xa - p.pleft[pb]) / p.hc[lvl][a]);
+            t %= nc;
+            if (t < 0) t += nc;
+            c[a] = t;
+        } else {
+            c[a] = (int)floor(xa / p.h[lvl]);
+        }
+    }
+    return make_int4(lvl, c[0], c[1], c[2]);
+}
+
+// bucket counts are scanned in tiles of 256 threads x 8 buckets
+constexpr int kScanPer = 8;
+constexpr int kScanTile = kBlock * kScanPer;
+
+// ------------------------------------------------------------------------
 // Rebuild: contact records -> hash table (keyed by particle ids).
 // ------------------------------------------------------------------------
 template <int NS>
@@ -357,16 +399,23 @@ __device__ inline bool hash_find(const Dev<NS>& d, uint64_t key, Rec<NS>& r)
 }
 
 template <int NS>
-__global__ void __launch_bounds__(kBlock) k_rb_begin(Dev<NS> d, Params p, int cur, uint32_t nbuckets)
+__global__ void __launch_bounds__(kBlock) k_rb_begin(Dev<NS> d, Params p, int cur)
 {
     if (!rebuild_pending(d.flags)) return;
-    GRID_STRIDE(b, nbuckets)
-    {
-        d.count[b] = 0;
-        d.cursor[b] = 0;
-    }
     GRID_STRIDE(i, p.n)
     {
+        // grid key of the particle; the bucket and tile counters were left
+        // at zero by the previous rebuild
+        {
+            const int lvl = (int)(d.spc[i] >> 16);
+            int4 c = cell_of(p, lvl, xyz(d.posr[i]));
+            const uint32_t key = cell_key(p, lvl, c.y, c.z, c.w);
+            d.key[i] = key;
+            if (d.vel[i].w == 0.0) c.x |= 0x100; // fixed particle
+            d.cellc[i] = c;
+            atomicAdd(&d.count[key], 1u);
+            atomicAdd(&d.tsum[key / kScanTile], 1u);
+        }
         const uint32_t idi = d.id[i];
         unsigned long long m = d.mask[cur][i];
         while (m) {
@@ -394,84 +443,65 @@ __global__ void __launch_bounds__(kBlock) k_rb_begin(Dev<NS> d, Params p, int cu
     }
 }
 
-// ------------------------------------------------------------------------
-// Rebuild: counting sort by (level, hashed cell), then by id in a bucket.
-// ------------------------------------------------------------------------
-__device__ inline int4 cell_of(const Params& p, int lvl, V3 x)
+// Exclusive scan of the bucket counts into start[] in two launches: one block
+// scans the per-tile totals (accumulated with the counts), then every tile
+// scans its own counts from its offset.
+__device__ inline uint32_t block_exclusive_scan(uint32_t t, uint32_t* s, uint32_t& total)
 {
-    int c[3];
-    for (int a = 0; a < 3; ++a) {
-        const double xa = comp(x, a);
-        const int pb = p.axis_pb[a];
-        if (pb >= 0) {
-            const int nc = p.ncell[lvl][a];
-            int t = (int)floor((xa - p.pleft[pb]) / p.hc[lvl][a]);
-            t %= nc;
-            if (t < 0) t += nc;
-            c[a] = t;
-        } else {
-            c[a] = (int)floor(xa / p.h[lvl]);
-        }
-    }
-    return make_int4(lvl, c[0], c[1], c[2]);
-}
-
-template <int NS>
-__global__ void __launch_bounds__(kBlock) k_keys(Dev<NS> d, Params p)
-{
-    if (!rebuild_pending(d.flags)) return;
-    GRID_STRIDE(i, p.n)
-    {
-        const int lvl = (int)(d.spc[i] >> 16);
-        int4 c = cell_of(p, lvl, xyz(d.posr[i]));
-        const uint32_t key = p.nboff[lvl] + cell_bucket(p.nbd[lvl], c.y, c.z, c.w);
-        d.key[i] = key;
-        if (d.vel[i].w == 0.0) c.x |= 0x100; // fixed particle
-        d.cellc[i] = c;
-        atomicAdd(&d.count[key], 1u);
-    }
-}
-
-// exclusive scan, 256 threads x 8 elements per block
-constexpr int kScanPer = 8;
-constexpr int kScanTile = kBlock * kScanPer;
-
-template <int NS>
-__global__ void __launch_bounds__(kBlock) k_scan_block(Dev<NS> d, const uint32_t* in, uint32_t* out, uint32_t* sums,
-                                                       uint32_t n)
-{
-    if (!rebuild_pending(d.flags)) return;
-    __shared__ uint32_t s[kBlock];
-    const uint32_t base = blockIdx.x * kScanTile + threadIdx.x * kScanPer;
-    uint32_t v[kScanPer];
-    uint32_t t = 0;
-    for (int k = 0; k < kScanPer; ++k) {
-        const uint32_t idx = base + k;
-        v[k] = idx < n ? in[idx] : 0u;
-        t += v[k];
-    }
     s[threadIdx.x] = t;
     __syncthreads();
     for (int off = 1; off < kBlock; off <<= 1) {
-        uint32_t x = threadIdx.x >= (unsigned)off ? s[threadIdx.x - off] : 0u;
+        const uint32_t x = threadIdx.x >= (unsigned)off ? s[threadIdx.x - off] : 0u;
         __syncthreads();
         s[threadIdx.x] += x;
         __syncthreads();
     }
-    uint32_t run = s[threadIdx.x] - t;
-    for (int k = 0; k < kScanPer; ++k) {
-        const uint32_t idx = base + k;
-        if (idx < n) out[idx] = run;
-        run += v[k];
-    }
-    if (threadIdx.x == kBlock - 1 && sums) sums[blockIdx.x] = s[kBlock - 1];
+    const uint32_t incl = s[threadIdx.x];
+    total = s[kBlock - 1];
+    __syncthreads();
+    return incl - t;
 }
 
 template <int NS>
-__global__ void __launch_bounds__(kBlock) k_scan_add(Dev<NS> d, uint32_t* out, const uint32_t* add, uint32_t n)
+__global__ void __launch_bounds__(kBlock) k_scan_tile_sums(Dev<NS> d, uint32_t ntiles)
 {
     if (!rebuild_pending(d.flags)) return;
-    GRID_STRIDE(idx, n) out[idx] += add[idx / kScanTile];
+    __shared__ uint32_t s[kBlock];
+    uint32_t carry = 0;
+    for (uint32_t base = 0; base < ntiles; base += kScanTile) {
+        const uint32_t b0 = base + threadIdx.x * kScanPer;
+        uint32_t v[kScanPer], t = 0;
+        for (int k = 0; k < kScanPer; ++k) {
+            v[k] = b0 + k < ntiles ? d.tsum[b0 + k] : 0u;
+            t += v[k];
+        }
+        uint32_t total;
+        uint32_t run = carry + block_exclusive_scan(t, s, total);
+        for (int k = 0; k < kScanPer; ++k) {
+            if (b0 + k < ntiles) d.toff[b0 + k] = run;
+            run += v[k];
+        }
+        carry += total;
+    }
+}
+
+template <int NS>
+__global__ void __launch_bounds__(kBlock) k_scan_tiles(Dev<NS> d, uint32_t nbuckets)
+{
+    if (!rebuild_pending(d.flags)) return;
+    __shared__ uint32_t s[kBlock];
+    const uint32_t b0 = blockIdx.x * kScanTile + threadIdx.x * kScanPer;
+    uint32_t v[kScanPer], t = 0;
+    for (int k = 0; k < kScanPer; ++k) {
+        v[k] = b0 + k < nbuckets ? d.count[b0 + k] : 0u;
+        t += v[k];
+    }
+    uint32_t total;
+    uint32_t run = d.toff[blockIdx.x] + block_exclusive_scan(t, s, total);
+    for (int k = 0; k < kScanPer; ++k) {
+        if (b0 + k < nbuckets) d.start[b0 + k] = run;
+        run += v[k];
+    }
 }
 
 template <int NS>
@@ -490,8 +520,12 @@ template <int NS>
 __global__ void __launch_bounds__(kBlock) k_sort_buckets(Dev<NS> d, uint32_t nbuckets, uint32_t n)
 {
     if (!rebuild_pending(d.flags)) return;
+    GRID_STRIDE(t, (nbuckets + kScanTile - 1) / kScanTile) d.tsum[t] = 0;
     GRID_STRIDE(b, nbuckets)
     {
+        // the counters are spent: leave them at zero for the next rebuild
+        d.count[b] = 0;
+        d.cursor[b] = 0;
         const uint32_t s = d.start[b];
         const uint32_t e = d.start[b + 1];
         for (uint32_t a = s + 1; a < e; ++a) {
@@ -527,26 +561,6 @@ __global__ void __launch_bounds__(kBlock) k_gather(Dev<NS> d, Params p)
     }
 }
 
-template <int NS>
-__global__ void __launch_bounds__(kBlock) k_copy_back(Dev<NS> d, Params p)
-{
-    if (!rebuild_pending(d.flags)) return;
-    GRID_STRIDE(i, p.n)
-    {
-        d.posr[i] = d.t_posr[i];
-        d.vel[i] = d.t_vel[i];
-        d.omg[i] = d.t_omg[i];
-        d.quat[i] = d.t_quat[i];
-        d.frc[i] = d.t_frc[i];
-        d.trq[i] = d.t_trq[i];
-        d.id[i] = d.t_id[i];
-        d.orig[i] = d.t_orig[i];
-        d.spc[i] = d.t_spc[i];
-        d.cellc[i] = d.t_cellc[i];
-        d.disp[i] = make_double4(0.0, 0.0, 0.0, 0.0);
-    }
-}
-
 // ------------------------------------------------------------------------
 // Rebuild: Verlet lists from the sorted multilevel grid.
 // ------------------------------------------------------------------------
@@ -573,10 +587,10 @@ __global__ void __launch_bounds__(kBlock) k_build_lists(Dev<NS> d, Params p)
     GRID_STRIDE_XCD(ii, p.n)
     {
         const int i = (int)ii;
-        const double4 pr = d.posr[i];
+        const double4 pr = d.t_posr[i];
         const V3 xi = xyz(pr);
-        const bool fixed_i = (d.cellc[i].x & 0x100) != 0;
-        const uint32_t idi = d.id[i];
+        const bool fixed_i = (d.t_cellc[i].x & 0x100) != 0;
+        const uint32_t idi = d.t_id[i];
         int cnt = 0;
         for (int m = 0; m < p.levels; ++m) {
             const double R = pr.w + p.rmaxl[m] + p.skin;
@@ -607,19 +621,19 @@ __global__ void __launch_bounds__(kBlock) k_build_lists(Dev<NS> d, Params p)
                     for (int tx = lo[0]; tx <= hi[0]; ++tx) {
                         int cx = tx;
                         if (nc[0]) cx = ((tx % nc[0]) + nc[0]) % nc[0];
-                        const uint32_t b = p.nboff[m] + cell_bucket(p.nbd[m], cx, cy, cz);
+                        const uint32_t b = cell_key(p, m, cx, cy, cz);
                         const uint32_t s = d.start[b];
                         const uint32_t e = d.start[b + 1];
                         for (uint32_t q = s; q < e; ++q) {
-                            const int4 cq = d.cellc[q];
+                            const int4 cq = d.t_cellc[q];
                             if ((cq.x & 0xFF) != m || cq.y != cx || cq.z != cy || cq.w != cz) continue;
                             if (q == (uint32_t)i) continue;
                             if (fixed_i && (cq.x & 0x100)) continue;
-                            const double4 pq = d.posr[q];
+                            const double4 pq = d.t_posr[q];
                             const V3 dl = min_image_delta(p, xi, xyz(pq));
                             const double cut = pr.w + pq.w + p.skin;
                             if (vlen2(dl) < cut * cut) {
-                                if (cnt < p.maxn) d.nbr[lslot(cnt, p.maxn, i)] = q | (idi < d.id[q] ? kWriterBit : 0u);
+                                if (cnt < p.maxn) d.nbr[lslot(cnt, p.maxn, i)] = q | (idi < d.t_id[q] ? kWriterBit : 0u);
                                 ++cnt;
                             }
                         }
@@ -642,7 +656,7 @@ __global__ void __launch_bounds__(kBlock) k_rematch(Dev<NS> d, Params p, int cur
     GRID_STRIDE(ii, p.n)
     {
         const int i = (int)ii;
-        const uint32_t idi = d.id[i];
+        const uint32_t idi = d.t_id[i];
         const int cnt = d.ncount[i];
         unsigned long long m = 0;
         for (int k = 0; k < cnt; ++k) {
@@ -651,7 +665,7 @@ __global__ void __launch_bounds__(kBlock) k_rematch(Dev<NS> d, Params p, int cur
             const uint32_t j = e & ~kWriterBit;
             if (e & kWriterBit) {
                 Rec<NS> r;
-                if (hash_find(d, ((uint64_t)idi << 32) | d.id[j], r)) {
+                if (hash_find(d, ((uint64_t)idi << 32) | d.t_id[j], r)) {
                     d.rec[cur][sl] = r;
                     m |= 1ull << k;
                 }
@@ -678,6 +692,18 @@ __global__ void __launch_bounds__(kBlock) k_rematch(Dev<NS> d, Params p, int cur
             }
         }
         d.wmask[cur][i] = wm;
+        // the sorted state becomes current
+        d.posr[i] = d.t_posr[i];
+        d.vel[i] = d.t_vel[i];
+        d.omg[i] = d.t_omg[i];
+        d.quat[i] = d.t_quat[i];
+        d.frc[i] = d.t_frc[i];
+        d.trq[i] = d.t_trq[i];
+        d.id[i] = d.t_id[i];
+        d.orig[i] = d.t_orig[i];
+        d.spc[i] = d.t_spc[i];
+        d.cellc[i] = d.t_cellc[i];
+        d.disp[i] = make_double4(0.0, 0.0, 0.0, 0.0);
     }
 }
 
@@ -713,16 +739,19 @@ __global__ void __launch_bounds__(kBlock) k_orphans(Dev<NS> d, Params p)
         d.hkey[h] = kEmptyKey;
         d.hmatched[h] = 0;
     }
-}
-
-__global__ void k_rebuild_done(int* flags)
-{
-    if (flags[F_REBUILD]) {
-        flags[F_REBUILD] = 0;
-        flags[F_REBUILDS] += 1;
-        const int par = flags[F_ORPH_PARITY];
-        flags[F_ORPH_COUNT0 + par] = 0;
-        flags[F_ORPH_PARITY] = par ^ 1;
+    // the last block to finish closes the rebuild: clear the flag, count
+    // the rebuild and make the new orphan buffer current
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        if (atomicAdd(&d.flags[F_DONE_BLOCKS], 1) == (int)gridDim.x - 1) {
+            __threadfence();
+            d.flags[F_DONE_BLOCKS] = 0;
+            d.flags[F_REBUILD] = 0;
+            d.flags[F_REBUILDS] += 1;
+            d.flags[F_ORPH_COUNT0 + (par ^ 1)] = 0;
+            d.flags[F_ORPH_PARITY] = par;
+        }
     }
 }
 
@@ -780,10 +809,9 @@ constexpr int kScanU = DEM_SCAN_UNROLL;
 #endif
 constexpr int kQueue = 64 + 64 * kScanU;
 struct OwnerLds {
-    double px[64], py[64], pz[64], vx[64], vy[64], vz[64], wx[64], wy[64], wz[64], r[64], im[64];
+    double px[64], py[64], pz[64], r[64];
     unsigned long long m_in[64];
     int32_t sp[64];
-    int32_t row[64];
     int32_t pre[64];    // exclusive prefix of the owners' list lengths
     int32_t first[64];  // first queue position of each owner in a batch
     unsigned long long q[kQueue];
@@ -793,7 +821,8 @@ struct OwnerLds {
 __device__ inline int q_owner(unsigned long long e) { return (int)((e >> 38) & 63); }
 
 template <int NS>
-__device__ inline void process_batch(Dev<NS>& d, const Params& p, int cur, OwnerLds& L, int base, int nb, int lane)
+__device__ inline void process_batch(Dev<NS>& d, const Params& p, int cur, OwnerLds& L, int wbase, int base, int nb,
+                                     int lane)
 {
     if (lane < nb) {
         const unsigned long long e = L.q[base + lane];
@@ -805,11 +834,14 @@ __device__ inline void process_batch(Dev<NS>& d, const Params& p, int cur, Owner
         const bool i_is_P = (e >> 51) & 1;
         const bool writer = (e >> 52) & 1;
         const uint32_t kk = (uint32_t)((e >> 53) & 0xFF);
-        const int i = L.row[o];
+        const int i = wbase + o;
         // issue every load that depends only on the queue entry first: the
-        // partner's state and species and both candidate history records
+        // owner's and the partner's velocities, the partner's state and
+        // species and both candidate history records
         const size_t slot = lslot(k, p.maxn, i);
         const Rec<NS>* rin = d.rec[cur];
+        const double4 vi = d.vel[i];
+        const double4 wi = d.omg[i];
         const double4 pq = d.posr[j];
         const double4 vq = d.vel[j];
         const double4 oq = d.omg[j];
@@ -827,10 +859,10 @@ __device__ inline void process_batch(Dev<NS>& d, const Params& p, int cur, Owner
         }
         Body bi;
         bi.pos = mk(L.px[o], L.py[o], L.pz[o]);
-        bi.vel = mk(L.vx[o], L.vy[o], L.vz[o]);
-        bi.omega = mk(L.wx[o], L.wy[o], L.wz[o]);
+        bi.vel = xyz(vi);
+        bi.omega = xyz(wi);
         bi.radius = L.r[o];
-        bi.inv_mass = L.im[o];
+        bi.inv_mass = vi.w;
         V3 xj = xyz(pq);
         for (int b = 0; b < p.nper; ++b) {
             const int c = (codes >> (2 * b)) & 3;
@@ -916,33 +948,19 @@ __global__ void __launch_bounds__(kBlock) FORCE_WAVES_ATTR k_force(Dev<NS> d, Pa
     const int wbase = i - lane;  // the wavefront's particles are wbase .. wbase + 63
     const bool valid = i < p.n;
     const int ii = valid ? i : 0;
-    const double4 pr = d.posr[ii];
-    const double4 vl = d.vel[ii];
-    const double4 om = d.omg[ii];
-    const int si = (int)(d.spc[ii] & 0xFFFF);
-    const bool fixed_i = vl.w == 0.0;
-    Body bi;
-    bi.pos = xyz(pr);
-    bi.vel = xyz(vl);
-    bi.omega = xyz(om);
-    bi.radius = pr.w;
-    bi.inv_mass = vl.w;
-    const unsigned long long m_in = valid ? d.mask[cur][ii] : 0ull;
     const int cnt = valid ? d.ncount[ii] : 0;
-    L.px[lane] = pr.x;
-    L.py[lane] = pr.y;
-    L.pz[lane] = pr.z;
-    L.vx[lane] = vl.x;
-    L.vy[lane] = vl.y;
-    L.vz[lane] = vl.z;
-    L.wx[lane] = om.x;
-    L.wy[lane] = om.y;
-    L.wz[lane] = om.z;
-    L.r[lane] = pr.w;
-    L.im[lane] = vl.w;
-    L.m_in[lane] = m_in;
-    L.sp[lane] = si;
-    L.row[lane] = ii;
+    {
+        // the owners' positions go to LDS for the contact phase (velocities
+        // are read where needed); the epilogue reads them back, so that they
+        // occupy no registers meanwhile
+        const double4 pr = d.posr[ii];
+        L.px[lane] = pr.x;
+        L.py[lane] = pr.y;
+        L.pz[lane] = pr.z;
+        L.r[lane] = pr.w;
+        L.m_in[lane] = valid ? d.mask[cur][ii] : 0ull;
+        L.sp[lane] = (int)(d.spc[ii] & 0xFFFF);
+    }
     // the wavefront's lists are walked as one flat sequence of
     // total = sum(cnt) entries, entry e belonging to the owner o with
     // pre[o] <= e < pre[o] + cnt[o]
@@ -966,7 +984,7 @@ __global__ void __launch_bounds__(kBlock) FORCE_WAVES_ATTR k_force(Dev<NS> d, Pa
         while (qn - head >= 64 || (all && qn > head)) {
             const int nb = qn - head < 64 ? qn - head : 64;
             wave_sync();
-            process_batch<NS>(d, p, cur, L, head, nb, lane);
+            process_batch<NS>(d, p, cur, L, wbase, head, nb, lane);
             L.first[lane] = -1;
             wave_sync();
             if (lane < nb) {
@@ -1066,6 +1084,19 @@ __global__ void __launch_bounds__(kBlock) FORCE_WAVES_ATTR k_force(Dev<NS> d, Pa
     }
     if (qn > 0) drain(true);
     if (!valid) return;
+    wave_sync();
+    const double4 pr = make_double4(L.px[lane], L.py[lane], L.pz[lane], L.r[lane]);
+    const double4 vl = d.vel[i];
+    const double4 om = d.omg[i];
+    const int si = L.sp[lane];
+    const unsigned long long m_in = L.m_in[lane];
+    const bool fixed_i = vl.w == 0.0;
+    Body bi;
+    bi.pos = xyz(pr);
+    bi.vel = xyz(vl);
+    bi.omega = xyz(om);
+    bi.radius = pr.w;
+    bi.inv_mass = vl.w;
     const Rec<NS>* rin = d.rec[cur];
     Rec<NS>* rout = d.rec[cur ^ 1];
     const uint32_t ts_new = p.nt + 1;
@@ -1275,6 +1306,33 @@ __global__ void k_hash_clear_all(Dev<NS> d)
     }
 }
 
+// BaseInteractable::setPosition/setVelocity/setAngularVelocity/setOrientation
+// from handler-ordered arrays (x, v, w: 3 doubles per particle, q: 4; null =
+// unchanged).  A moved particle adds its displacement to the Verlet-list
+// displacement like a drift does.
+template <int NS>
+__global__ void __launch_bounds__(kBlock) k_set_kinematics(Dev<NS> d, Params p, const double* x, const double* v,
+                                                           const double* w, const double* q)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= p.n) return;
+    const size_t o = d.orig[i];
+    if (x) {
+        const double4 pr = d.posr[i];
+        const V3 nx = mk(x[3 * o], x[3 * o + 1], x[3 * o + 2]);
+        double4 ds = d.disp[i];
+        ds.x += nx.x - pr.x;
+        ds.y += nx.y - pr.y;
+        ds.z += nx.z - pr.z;
+        d.disp[i] = ds;
+        if (ds.x * ds.x + ds.y * ds.y + ds.z * ds.z > p.thr2) atomicOr(&d.flags[F_REBUILD], 1);
+        d.posr[i] = make_double4(nx.x, nx.y, nx.z, pr.w);
+    }
+    if (v) d.vel[i] = make_double4(v[3 * o], v[3 * o + 1], v[3 * o + 2], d.vel[i].w);
+    if (w) d.omg[i] = make_double4(w[3 * o], w[3 * o + 1], w[3 * o + 2], d.omg[i].w);
+    if (q) d.quat[i] = make_double4(q[4 * o], q[4 * o + 1], q[4 * o + 2], q[4 * o + 3]);
+}
+
 template <int NS>
 __global__ void k_zero_masks(Dev<NS> d, Params p)
 {
@@ -1322,6 +1380,7 @@ struct dem_ctx {
     // particles
     int64_t n = 0;
     bool uploaded = false;
+    bool in_step = false; // between dem_step_begin and dem_step_end
     int cur = 0;
     Params prm{};
     // device buffers
@@ -1331,8 +1390,6 @@ struct dem_ctx {
     dem_pair_params* d_sp = nullptr;
     dem_wall* d_walls = nullptr;
     int* d_flags = nullptr;
-    std::vector<uint32_t*> scan_levels; // scan sums per recursion level
-    std::vector<uint32_t> scan_sizes;
     uint32_t nbuckets = 0;
     int64_t last_rebuilds = 0;
     uint32_t alive_min = 0;
@@ -1376,8 +1433,6 @@ void free_all(dem_ctx* ctx)
 {
     for (void* v : ctx->allocs) (void)hipFree(v);
     ctx->allocs.clear();
-    ctx->scan_levels.clear();
-    ctx->scan_sizes.clear();
     ctx->d_sp = nullptr;
     ctx->d_walls = nullptr;
     ctx->d_flags = nullptr;
@@ -1411,26 +1466,17 @@ int alloc_particles(dem_ctx* ctx, int64_t n)
     A(d.wrec[0], wslots) A(d.wrec[1], wslots) A(d.wmask[0], n) A(d.wmask[1], n);
     A(d.key, n) A(d.perm, n);
     A(d.count, ctx->nbuckets) A(d.cursor, ctx->nbuckets) A(d.start, (size_t)ctx->nbuckets + 1);
+    const size_t ntiles = ((size_t)ctx->nbuckets + kScanTile - 1) / kScanTile;
+    A(d.tsum, ntiles) A(d.toff, ntiles);
     d.hcap = next_pow2(std::max<int64_t>(4096, 8 * n + 4 * n * (int64_t)ctx->walls.size()));
     A(d.hkey, d.hcap) A(d.hval, d.hcap) A(d.hmatched, d.hcap);
     d.ocap = (int)std::min<int64_t>(std::max<int64_t>(1024, n), 1 << 22);
     A(d.okey[0], d.ocap) A(d.okey[1], d.ocap) A(d.orec[0], d.ocap) A(d.orec[1], d.ocap);
 #undef A
-    // scan hierarchy: level 0 scans count into start; every further level
-    // scans the block sums of the level below in place
-    ctx->scan_levels.clear();
-    ctx->scan_sizes.clear();
-    ctx->scan_levels.push_back(d.start);
-    ctx->scan_sizes.push_back(ctx->nbuckets);
-    uint32_t sz = ctx->nbuckets;
-    while (sz > (uint32_t)kScanTile) {
-        const uint32_t nb = (sz + kScanTile - 1) / kScanTile;
-        uint32_t* sums = nullptr;
-        if ((r = dalloc(ctx, &sums, nb)) != 0) return r;
-        ctx->scan_levels.push_back(sums);
-        ctx->scan_sizes.push_back(nb);
-        sz = nb;
-    }
+    // the rebuild leaves the bucket and tile counters at zero for the next one
+    HIPCHK(hipMemset(d.count, 0, sizeof(uint32_t) * ctx->nbuckets));
+    HIPCHK(hipMemset(d.cursor, 0, sizeof(uint32_t) * ctx->nbuckets));
+    HIPCHK(hipMemset(d.tsum, 0, sizeof(uint32_t) * ntiles));
     d.sp = ctx->d_sp;
     d.walls = ctx->d_walls;
     d.flags = ctx->d_flags;
@@ -1443,21 +1489,9 @@ template <int NS>
 void launch_scan(dem_ctx* ctx)
 {
     Dev<NS>& d = dev<NS>(ctx);
-    hipStream_t s = ctx->stream;
-    const size_t L = ctx->scan_levels.size();
-    for (size_t k = 0; k < L; ++k) {
-        const uint32_t n = ctx->scan_sizes[k];
-        const uint32_t nblk = (n + kScanTile - 1) / kScanTile;
-        uint32_t* in = k == 0 ? d.count : ctx->scan_levels[k];
-        uint32_t* out = ctx->scan_levels[k];
-        uint32_t* sums = k + 1 < L ? ctx->scan_levels[k + 1] : nullptr;
-        hipLaunchKernelGGL(k_scan_block<NS>, dim3(nblk), dim3(kBlock), 0, s, d, in, out, sums, n);
-    }
-    for (size_t k = L - 1; k-- > 0;) {
-        const uint32_t n = ctx->scan_sizes[k];
-        hipLaunchKernelGGL(k_scan_add<NS>, dim3(std::min<unsigned>(grid_for(n), kRebuildBlocks)), dim3(kBlock), 0,
-                           s, d, ctx->scan_levels[k], ctx->scan_levels[k + 1], n);
-    }
+    const uint32_t ntiles = (ctx->nbuckets + kScanTile - 1) / kScanTile;
+    hipLaunchKernelGGL(k_scan_tile_sums<NS>, dim3(1), dim3(kBlock), 0, ctx->stream, d, ntiles);
+    hipLaunchKernelGGL(k_scan_tiles<NS>, dim3(ntiles), dim3(kBlock), 0, ctx->stream, d, ctx->nbuckets);
 }
 
 template <int NS>
@@ -1467,17 +1501,14 @@ void launch_rebuild(dem_ctx* ctx)
     hipStream_t s = ctx->stream;
     const Params& p = ctx->prm;
     const dim3 g(kRebuildBlocks), b(kBlock);
-    hipLaunchKernelGGL(k_rb_begin<NS>, g, b, 0, s, d, p, ctx->cur, ctx->nbuckets);
-    hipLaunchKernelGGL(k_keys<NS>, g, b, 0, s, d, p);
+    hipLaunchKernelGGL(k_rb_begin<NS>, g, b, 0, s, d, p, ctx->cur);
     launch_scan<NS>(ctx);
     hipLaunchKernelGGL(k_scatter<NS>, g, b, 0, s, d, p);
     hipLaunchKernelGGL(k_sort_buckets<NS>, g, b, 0, s, d, ctx->nbuckets, (uint32_t)ctx->n);
     hipLaunchKernelGGL(k_gather<NS>, g, b, 0, s, d, p);
-    hipLaunchKernelGGL(k_copy_back<NS>, g, b, 0, s, d, p);
     hipLaunchKernelGGL(k_build_lists<NS>, g, b, 0, s, d, p);
     hipLaunchKernelGGL(k_rematch<NS>, g, b, 0, s, d, p, ctx->cur);
     hipLaunchKernelGGL(k_orphans<NS>, g, b, 0, s, d, p);
-    hipLaunchKernelGGL(k_rebuild_done, dim3(1), dim3(1), 0, s, d.flags);
 }
 
 template <int NS>
@@ -1592,6 +1623,47 @@ int run_forces(dem_ctx* ctx)
     launch_force<NS>(ctx, 0, KICK_NONE);
     HIPCHK(hipGetLastError());
     return check_flags(ctx);
+}
+
+// The second part of one computeOneTimeStep (after actionsBeforeTimeStep):
+// the same launches as a step of run_steps, closing with the half-kick only.
+template <int NS>
+int run_step_end(dem_ctx* ctx)
+{
+    ctx->prm.erase = 1;
+    ctx->prm.nt = (uint32_t)ctx->nt;
+    ctx->prm.alive_min = ctx->alive_min;
+    launch_rebuild<NS>(ctx);
+    launch_force<NS>(ctx, 1, KICK_AFTER);
+    ctx->time += ctx->dt;
+    ctx->nt += 1;
+    ctx->alive_min = (uint32_t)ctx->nt;
+    ctx->prm.alive_min = ctx->alive_min;
+    HIPCHK(hipGetLastError());
+    return check_flags(ctx);
+}
+
+template <int NS>
+int run_set_kinematics(dem_ctx* ctx, const double* x, const double* v, const double* w, const double* q)
+{
+    Dev<NS>& d = dev<NS>(ctx);
+    const size_t n = (size_t)ctx->n;
+    // stage the host arrays in the gather buffers, which hold nothing between
+    // steps and before the force pass of a step
+    const double* src[4] = {x, v, w, q};
+    double4* stage[4] = {d.t_posr, d.t_vel, d.t_omg, d.t_quat};
+    const double* dst[4] = {nullptr, nullptr, nullptr, nullptr};
+    for (int a = 0; a < 4; ++a) {
+        if (!src[a]) continue;
+        HIPCHK(hipMemcpyAsync(stage[a], src[a], sizeof(double) * (a == 3 ? 4 : 3) * n, hipMemcpyHostToDevice,
+                              ctx->stream));
+        dst[a] = (const double*)stage[a];
+    }
+    hipLaunchKernelGGL(k_set_kinematics<NS>, dim3(grid_for(n)), dim3(kBlock), 0, ctx->stream, d, ctx->prm, dst[0],
+                       dst[1], dst[2], dst[3]);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return DEM_OK;
 }
 
 int ready(dem_ctx* ctx)
@@ -1841,8 +1913,37 @@ int dem_upload_particles(dem_ctx* ctx, int64_t n, const uint32_t* id, const int3
             p.nbd[l][a] = (p.nbd[l][a] + 1) / 2;
         }
         p.nb[l] = (uint32_t)((uint64_t)p.nbd[l][0] * p.nbd[l][1] * p.nbd[l][2]);
-        p.nboff[l] = off;
-        off += p.nb[l];
+    }
+    // slabs along the longest axis of the coarsest level's box; inside a
+    // level the slab axis varies slowest, then the longer of the other two
+    {
+        const int lc = L - 1;
+        double ext[3];
+        for (int a = 0; a < 3; ++a) ext[a] = p.nbd[lc][a] * p.hc[lc][a];
+        int ax[3] = {0, 1, 2};
+        std::stable_sort(ax, ax + 3, [&](int u, int v) { return ext[u] > ext[v]; });
+        if (ext[ax[0]] == ext[ax[1]] && ax[1] > ax[0]) std::swap(ax[0], ax[1]);  // ties: y before x
+        p.bax0 = ax[0];
+        p.nslab = std::max(1, std::min(kMaxSlabs, p.nbd[lc][ax[0]]));
+        int64_t total = 0;
+        for (int l = 0; l < L; ++l) {
+            p.bstride[l][ax[2]] = 1;
+            p.bstride[l][ax[1]] = (uint32_t)p.nbd[l][ax[2]];
+            p.bstride[l][ax[0]] = (uint32_t)p.nbd[l][ax[2]] * (uint32_t)p.nbd[l][ax[1]];
+        }
+        for (int sl = 0; sl < p.nslab; ++sl)
+            for (int l = 0; l < L; ++l) {
+                const int64_t ns = p.nbd[l][ax[0]];
+                const int64_t c0 = (sl * ns + p.nslab - 1) / p.nslab, c1 = ((sl + 1) * ns + p.nslab - 1) / p.nslab;
+                const int64_t delta = total - c0 * (int64_t)p.bstride[l][ax[0]];
+                if (delta < INT32_MIN || delta > INT32_MAX) return set_err(ctx, DEM_ERR_CAPACITY, "too many grid buckets");
+                p.sdelta[sl][l] = (int32_t)delta;
+                total += (c1 - c0) * (int64_t)p.bstride[l][ax[0]];
+            }
+        if (total != (int64_t)p.nb[0] + (L > 1 ? p.nb[1] : 0) + (L > 2 ? p.nb[2] : 0) + (L > 3 ? p.nb[3] : 0) ||
+            total >= (int64_t)INT32_MAX)
+            return set_err(ctx, DEM_ERR_CAPACITY, "grid bucket layout");
+        off = (uint32_t)total;
     }
     ctx->nbuckets = off;
     refresh_params(ctx);
@@ -1918,6 +2019,7 @@ int dem_upload_particles(dem_ctx* ctx, int64_t n, const uint32_t* id, const int3
     ctx->cur = 0;
     ctx->alive_min = 0;
     ctx->uploaded = true;
+    ctx->in_step = false;
     return DEM_OK;
 }
 
@@ -1972,6 +2074,7 @@ int dem_compute_forces(dem_ctx* ctx)
 {
     int r = ready(ctx);
     if (r) return r;
+    if (ctx->in_step) return set_err(ctx, DEM_ERR_STATE, "dem_compute_forces inside dem_step_begin/dem_step_end");
     HIPCHK(hipSetDevice(ctx->device));
     refresh_params(ctx);
     return ctx->ns_springs == 9 ? run_forces<9>(ctx) : run_forces<3>(ctx);
@@ -1982,9 +2085,46 @@ int dem_step(dem_ctx* ctx, int64_t n_steps)
     int r = ready(ctx);
     if (r) return r;
     if (n_steps < 0) return set_err(ctx, DEM_ERR_ARG, "negative step count");
+    if (ctx->in_step) return set_err(ctx, DEM_ERR_STATE, "dem_step inside dem_step_begin/dem_step_end");
     HIPCHK(hipSetDevice(ctx->device));
     refresh_params(ctx);
     return ctx->ns_springs == 9 ? run_steps<9>(ctx, n_steps) : run_steps<3>(ctx, n_steps);
+}
+
+int dem_step_begin(dem_ctx* ctx)
+{
+    int r = ready(ctx);
+    if (r) return r;
+    if (ctx->in_step) return set_err(ctx, DEM_ERR_STATE, "dem_step_begin: a step is already open");
+    HIPCHK(hipSetDevice(ctx->device));
+    refresh_params(ctx);
+    if (ctx->ns_springs == 9)
+        launch_kick<9>(ctx, 0, 1);
+    else
+        launch_kick<3>(ctx, 0, 1);
+    HIPCHK(hipGetLastError());
+    ctx->in_step = true;
+    return DEM_OK;
+}
+
+int dem_step_end(dem_ctx* ctx)
+{
+    int r = ready(ctx);
+    if (r) return r;
+    if (!ctx->in_step) return set_err(ctx, DEM_ERR_STATE, "dem_step_end without dem_step_begin");
+    HIPCHK(hipSetDevice(ctx->device));
+    ctx->in_step = false;
+    return ctx->ns_springs == 9 ? run_step_end<9>(ctx) : run_step_end<3>(ctx);
+}
+
+int dem_set_kinematics(dem_ctx* ctx, int64_t n, const double* x, const double* v, const double* w, const double* q)
+{
+    if (!ctx) return DEM_ERR_ARG;
+    if (!ctx->uploaded) return set_err(ctx, DEM_ERR_STATE, "no particles uploaded");
+    if (n != ctx->n) return set_err(ctx, DEM_ERR_ARG, "dem_set_kinematics: n differs from the uploaded count");
+    HIPCHK(hipSetDevice(ctx->device));
+    refresh_params(ctx);
+    return ctx->ns_springs == 9 ? run_set_kinematics<9>(ctx, x, v, w, q) : run_set_kinematics<3>(ctx, x, v, w, q);
 }
 
 int dem_download_particles(dem_ctx* ctx, double* x, double* v, double* w, double* q, double* f, double* t)

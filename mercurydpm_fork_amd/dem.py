@@ -108,6 +108,9 @@ def _declare(lib, prefix: str, has_config: bool):
     f("upload_contacts", _I32, _PTR, _I64, _PTR, _PTR, _PTR, _PTR)
     f("compute_forces", _I32, _PTR)
     f("step", _I32, _PTR, _I64)
+    f("step_begin", _I32, _PTR)
+    f("step_end", _I32, _PTR)
+    f("set_kinematics", _I32, _PTR, _I64, _PTR, _PTR, _PTR, _PTR)
     f("download_particles", _I32, _PTR, *([_PTR] * 6))
     f("contact_count", _I32, _PTR, C.POINTER(_I64))
     f("download_contacts", _I32, _PTR, _I64, C.POINTER(_I64), _PTR, _PTR, _PTR, _PTR)
@@ -276,6 +279,22 @@ class Engine:
 
     def step(self, n: int = 1):
         self._check(self._fn("step")(self.h, int(n)))
+
+    def step_begin(self):
+        """First part of computeOneTimeStep, up to actionsBeforeTimeStep."""
+        self._check(self._fn("step_begin")(self.h))
+
+    def step_end(self):
+        """The rest of the step (forces, closing half-kick, erase, clock)."""
+        self._check(self._fn("step_end")(self.h))
+
+    def set_kinematics(self, x=None, v=None, w=None, q=None):
+        """Prescribed positions / velocities / angular velocities /
+        orientations of all particles (handler order); None = unchanged."""
+        arrs = []
+        for a, m in ((x, 3), (v, 3), (w, 3), (q, 4)):
+            arrs.append(None if a is None else np.ascontiguousarray(np.asarray(a, np.float64).reshape(self.n, m)))
+        self._check(self._fn("set_kinematics")(self.h, self.n, *[_ptr(a) for a in arrs]))
 
     # ------------------------------------------------------------- download
     def download(self):

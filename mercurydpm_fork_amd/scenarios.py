@@ -198,6 +198,98 @@ def oblique_impact_self_test() -> Scenario:
 OBLIQUE_IMPACT_SAVE_COUNT = 500
 
 
+# Drivers/UnitTests/MindlinSelfTest.cpp:36-58 with the set-ups of
+# Drivers/UnitTests/ContactLawTestHelpers.h.  The particles' motion is
+# prescribed in actionsBeforeTimeStep; `prescribe(t)` returns what those hooks
+# set at time t as (x, v, w, q) arrays in handler order (None = not set).
+MINDLIN_DISPLACEMENT = 0.015
+MINDLIN_TANGENTIAL_DISPLACEMENT = 0.0025
+MINDLIN_VELOCITY = 0.01
+MINDLIN_RADIUS = 0.25
+
+
+def mindlin_self_test_species() -> Species:
+    """MindlinSelfTest.cpp:44-48."""
+    s = Species(density=6. / PI, friction_law=dem.FRICTION_MINDLIN)
+    s.set_elastic_modulus_and_poisson_ratio(1e5, 0.3)
+    s.dissipation = 2.0
+    s.set_sliding_friction(0.1)
+    return s
+
+
+def mindlin_loading_test():
+    """loadingTest (ContactLawTestHelpers.h:21-84): a sphere pressed into a
+    flat wall and pulled back out; saved every step."""
+    d, vel, r = MINDLIN_DISPLACEMENT, MINDLIN_VELOCITY, MINDLIN_RADIUS
+    s = mindlin_self_test_species()
+    t_max = 2.0 * d / vel
+    w = infinite_wall((0, 0, -1), (0.0, 0.0, 0.0))
+    st = make_state([(0, 0, r)], [(0, 0, 0)], [r], [0], [s])
+    sc = Scenario("MindlinSelfTestLoading", [s], [w], [], (0.0, 0.0, 0.0), 2e-3 * t_max, st, t_max=t_max)
+
+    def prescribe(t):
+        if t <= d / vel:
+            v, x = (0, 0, vel), (0, 0, r - vel * t)
+        else:
+            v, x = (0, 0, -vel), (0, 0, r - d - d + vel * t)
+        return np.array([x], float), np.array([v], float), np.zeros((1, 3)), None
+
+    return sc, prescribe, 1
+
+
+def mindlin_normal_and_tangential_loading_test():
+    """normalAndTangentialLoadingTest (ContactLawTestHelpers.h:92-162): a
+    sphere pressed into a wall and moved back and forth tangentially."""
+    d, td, vel, r = MINDLIN_DISPLACEMENT, MINDLIN_TANGENTIAL_DISPLACEMENT, MINDLIN_VELOCITY, MINDLIN_RADIUS
+    s = mindlin_self_test_species()
+    t_max = 4.0 * td / vel
+    w = infinite_wall((0, 0, -1), (0.0, 0.0, 0.0))
+    st = make_state([(0, 0, r - d)], [(0, 0, 0)], [r], [0], [s])
+    sc = Scenario("MindlinSelfTestNormalAndTangentialLoading", [s], [w], [], (0.0, 0.0, 0.0), 4e-4 * t_max, st,
+                  t_max=t_max)
+
+    def prescribe(t):
+        move_right = int(t / (2.0 * td / vel) + 0.5) % 2 == 0
+        if move_right:
+            v, x = (-vel, 0, 0), (td - vel * t, 0, r - d)
+        else:
+            v, x = (vel, 0, 0), (-2 * td + vel * t, 0, r - d)
+        return np.array([x], float), np.array([v], float), np.zeros((1, 3)), None
+
+    return sc, prescribe, 1
+
+
+def mindlin_frame_independence_test():
+    """objectivenessTest (ContactLawTestHelpers.h:170-249): two spheres
+    loaded normally and tangentially, then the pair is rotated rigidly; saved
+    every 20 steps."""
+    d, td, vel, r = MINDLIN_DISPLACEMENT, MINDLIN_TANGENTIAL_DISPLACEMENT, MINDLIN_VELOCITY, MINDLIN_RADIUS
+    s = mindlin_self_test_species()
+    t_max = (td + 0.5 * PI * r) / vel
+    st = make_state([(0, r - d, 0), (0, -r + d, 0)], [(0, 0, 0), (0, 0, 0)], [r, r], [0, 0], [s])
+    sc = Scenario("MindlinSelfTestFrameIndependence", [s], [], [], (0.0, 0.0, 0.0), 1e-4 * t_max, st, t_max=t_max)
+
+    def prescribe(t):
+        if t <= td / vel:
+            x = [(-td + vel * t, r - d, 0), (td - vel * t, -r + d, 0)]
+            v = [(vel, 0, 0), (-vel, 0, 0)]
+            return np.array(x, float), np.array(v, float), np.zeros((2, 3)), None
+        angle = vel / (r - d) * (t - td / vel)
+        sn, cs = math.sin(angle), math.cos(angle)
+        k = vel / (r - d)
+        wp = (0 * k, 0 * k, -1 * k)
+        vp = (vel * cs, vel * -sn, vel * 0)
+        xp = ((r - d) * sn, (r - d) * cs, (r - d) * 0)
+        neg = lambda a: tuple(-c for c in a)
+        x = np.array([xp, neg(xp)], float)
+        v = np.array([vp, neg(vp)], float)
+        w = np.array([wp, neg(wp)], float)
+        q = np.array([(1.0, 0.0, 0.0, 0.0), (-1.0, -0.0, -0.0, -0.0)])
+        return x, v, w, q
+
+    return sc, prescribe, 20
+
+
 def load_fstat(path):
     """Rows of a .fstat fixture as a float array (columns as written by
     BaseInteraction::writeToFStat, BaseInteraction.cc:487-531)."""

@@ -1037,6 +1037,47 @@ int oracle_dem_step(oracle_ctx* c, int64_t n)
     return DEM_OK;
 }
 
+/* computeOneTimeStep split at actionsBeforeTimeStep (DPMBase.cc:4347): the
+ * first part integrates and wraps, the second does the rest of the step. */
+int oracle_dem_step_begin(oracle_ctx* c)
+{
+    int r = checkReady(c);
+    if (r) return r;
+    integrateBefore(c);
+    wrapPeriodic(c);
+    return DEM_OK;
+}
+
+int oracle_dem_step_end(oracle_ctx* c)
+{
+    int r = checkReady(c);
+    if (r) return r;
+    createGhosts(c);
+    computeAllForces(c);
+    removeGhosts(c);
+    integrateAfter(c);
+    eraseOld(c, (uint32_t)c->nt);
+    c->time += c->dt;
+    c->nt++;
+    return DEM_OK;
+}
+
+/* BaseInteractable::setPosition/setVelocity/setAngularVelocity and
+ * BaseParticle::setOrientation for every real particle (NULL = unchanged) */
+int oracle_dem_set_kinematics(oracle_ctx* c, int64_t n, const double* x, const double* v, const double* w,
+                              const double* q)
+{
+    if (n != c->nReal) return fail(c, DEM_ERR_ARG, "n differs from the uploaded count");
+    for (int64_t i = 0; i < n; ++i) {
+        Particle* p = &c->p[i];
+        if (x) p->pos = v3(x[3 * i], x[3 * i + 1], x[3 * i + 2]);
+        if (v) p->vel = v3(v[3 * i], v[3 * i + 1], v[3 * i + 2]);
+        if (w) p->omega = v3(w[3 * i], w[3 * i + 1], w[3 * i + 2]);
+        if (q) for (int k = 0; k < 4; ++k) p->q[k] = q[4 * i + k];
+    }
+    return DEM_OK;
+}
+
 int oracle_dem_download_particles(oracle_ctx* c, double* x, double* v, double* w, double* q, double* f,
                                   double* t)
 {

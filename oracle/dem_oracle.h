@@ -33,6 +33,10 @@ int oracle_dem_upload_contacts(oracle_ctx* c, int64_t n, const uint32_t* p_id, c
                                const double* spring, const uint32_t* ts);
 int oracle_dem_compute_forces(oracle_ctx* c);
 int oracle_dem_step(oracle_ctx* c, int64_t n);
+int oracle_dem_step_begin(oracle_ctx* c);
+int oracle_dem_step_end(oracle_ctx* c);
+int oracle_dem_set_kinematics(oracle_ctx* c, int64_t n, const double* x, const double* v, const double* w,
+                              const double* q);
 int oracle_dem_download_particles(oracle_ctx* c, double* x, double* v, double* w, double* q, double* f,
                                   double* t);
 int oracle_dem_contact_count(oracle_ctx* c, int64_t* n);

@@ -53,3 +53,90 @@ def check_oblique_impact(e):
     err = np.abs(got - want) / (np.abs(want) * 5e-9 + scale * 1e-9)
     assert err.max() <= 1.0, (np.unravel_index(err.argmax(), err.shape), got[err.argmax() // 6], want[err.argmax() // 6])
 
+
+
+def contact_row(st, ct, r, wall_normal=None):
+    """The .fstat quantities of the single stored interaction of `e`, on the
+    P side: contact point (3), overlap, tangential overlap, normal force and
+    |tangential force|; None without an interaction.  `wall_normal` is the
+    InfiniteWall normal for a particle-wall contact (wall through the origin);
+    otherwise the interaction is between particles 0 (P) and 1."""
+    if len(ct.p_id) == 0:
+        return None
+    x = st["x"]
+    if wall_normal is not None:
+        nw = np.asarray(wall_normal, float)
+        distance = float(np.dot(nw, -x[0]))
+        overlap = r[0] - distance
+        n = -nw
+    else:
+        b = x[0] - x[1]
+        dist = math.sqrt(float(np.dot(b, b)))
+        n = b / dist
+        overlap = r[0] + r[1] - dist
+    c = x[0] - n * (r[0] - 0.5 * overlap)
+    s = np.asarray(ct.spring[0][:3], float)
+    sl = math.sqrt(float(np.dot(s, s)))
+    t_ov = 0.0 if sl == 0.0 else sl * float(np.dot(s / sl, np.ones(3)))
+    F = st["f"][0]
+    fn = float(np.dot(F, n))
+    ft = float(np.linalg.norm(F - fn * n))
+    return (c[0], c[1], c[2], overlap, t_ov, fn, ft)
+
+
+def run_prescribed(e, sc, prescribe, save_count, wall_normal=None):
+    """DPMBase::solve with motion prescribed in actionsBeforeTimeStep: rows at
+    the start of every save_count-th step and after the last one (the
+    reference writes output at the start of computeOneTimeStep and once more
+    in finaliseSolve)."""
+    radius = np.asarray(sc.state.radius, float)
+    n = sc.n_steps()
+    rows = []
+    t = 0.0
+    for k in range(n + 1):
+        if k % save_count == 0 or k == n:
+            rows.append(contact_row(e.download(), e.contacts(), radius, wall_normal))
+        if k == n:
+            break
+        e.step_begin()
+        x, v, w, q = prescribe(t)
+        e.set_kinematics(x, v, w, q)
+        e.step_end()
+        t += sc.dt
+    return rows
+
+
+def compare_rows(rows, gold, pair):
+    """Every saved row against the gold file's P rows: 10 printed digits, with
+    an absolute floor of 1e-9 of the largest overlap for lengths and of the
+    largest normal force for forces (quantities that vanish in exact
+    arithmetic carry round-off of either sign there).  Rows whose overlap is
+    below 1e-12 of the largest (a contact touching to within a few ulp of the
+    position, whose existence and damping term turn on the last bit of the
+    prescribed position) are only required to exist; at most 1% of rows."""
+    want = gold[0::2] if pair else gold
+    assert len(rows) == len(want), (len(rows), len(want))
+    assert all(r is not None for r in rows)
+    got = np.array(rows)
+    want = want[:, 3:10]
+    length = np.max(np.abs(want[:, 3]))
+    force = np.max(np.abs(want[:, 5]))
+    marginal = np.abs(want[:, 3]) < 1e-12 * length
+    assert marginal.sum() <= 0.01 * len(want), int(marginal.sum())
+    got, want = got[~marginal], want[~marginal]
+    floor = np.array([length] * 5 + [force] * 2) * 1e-9
+    err = np.abs(got - want) / (np.abs(want) * 5e-9 + floor)
+    k, col = np.unravel_index(err.argmax(), err.shape)
+    assert err.max() <= 1.0, (int(k), int(col), got[k], want[k])
+
+
+def check_mindlin_self_test(make_engine, which):
+    import mercurydpm_fork_amd.scenarios as SC
+    sc, prescribe, save = getattr(SC, f"mindlin_{which}_test")()
+    e = make_engine()
+    S.apply(e, sc)
+    e.compute_forces()
+    wall = (0, 0, -1) if sc.walls else None
+    rows = run_prescribed(e, sc, prescribe, save, wall)
+    gold = S.load_fstat(os.path.join(os.path.dirname(__file__), "golden", f"mindlin_self_test_{which}.fstat"))
+    compare_rows(rows, gold, pair=wall is None)

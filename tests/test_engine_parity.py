@@ -211,3 +211,31 @@ def test_engine_oblique_impact_mindlin_fstat_gold():
     S.apply(g, sc)
     g.compute_forces()
     check_oblique_impact(g)
+
+
+@pytest.mark.parametrize("which", ["loading", "normal_and_tangential_loading", "frame_independence"])
+def test_engine_mindlin_self_test_fstat_gold(which):
+    """The reference's MindlinSelfTest*.fstat through dem_step_begin /
+    dem_set_kinematics / dem_step_end (motion prescribed where the reference
+    calls actionsBeforeTimeStep)."""
+    from fstat_check import check_mindlin_self_test
+    check_mindlin_self_test(dem.Engine, which)
+
+
+def test_engine_split_step_equals_step():
+    """dem_step_begin + dem_step_end is dem_step(1), bit for bit."""
+    sc = S.mixture(n=300, seed=5, laws="hvm", periodic=(True, True), walls=("bottom",))
+    a = dem.Engine()
+    b = dem.Engine()
+    S.apply(a, sc)
+    S.apply(b, sc)
+    a.compute_forces()
+    b.compute_forces()
+    for _ in range(20):
+        a.step(1)
+        b.step_begin()
+        b.step_end()
+    da, db = a.download(), b.download()
+    for k in ("x", "v", "w", "q", "f", "t"):
+        assert np.array_equal(da[k], db[k]), k
+    assert_contacts_equal(a, b)

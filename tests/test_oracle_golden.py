@@ -194,3 +194,13 @@ def test_oblique_impact_mindlin_fstat_gold():
     S.apply(e, sc)
     e.compute_forces()
     check_oblique_impact(e)
+
+
+@pytest.mark.parametrize("which", ["loading", "normal_and_tangential_loading", "frame_independence"])
+def test_mindlin_self_test_fstat_gold(which):
+    """MindlinSelfTest (Drivers/UnitTests/SelfTestData/MindlinSelfTest*.fstat):
+    Hertz-Mindlin loading / unloading against a wall, tangential cycling, and
+    a rigidly rotated particle pair (frame independence of the spring), with
+    the motion prescribed between the drift and the force pass."""
+    from fstat_check import check_mindlin_self_test
+    check_mindlin_self_test(oracle_lib.engine, which)
