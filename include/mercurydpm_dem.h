@@ -274,6 +274,34 @@ int dem_download_contacts(dem_ctx* ctx, int64_t capacity, int64_t* n_out, uint32
 
 int dem_get_stats(dem_ctx* ctx, dem_stats* stats);
 
+/* ---- coarse graining ----------------------------------------------------
+ * CG<CGCoordinates::Z, CGFunctions::Gauss, CGFields::StandardFields>
+ * evaluated on the current state, as cgHandler.evaluate() does in
+ * DPMBase::writeOutputFiles (DPMBase.cc:4084): CG::evaluate (CG.hcc:248-254)
+ * with evaluateParticle (CG.hcc:303-376) over the non-fixed particles and
+ * evaluateContact (CG.hcc:445-527) over the stored interactions, the Gauss
+ * function and its line integral (Gauss.hcc:79-241, Coordinates/Z.cc,
+ * Base_X_Y_Z.cc:32-49), StandardFields::setFields (StandardFields.cc:232-264)
+ * and the volume average over the x-y extent (CG.hcc:199-207).  Points are
+ * spaced evenly in z (Coordinates/Z.h:126-138).  The contact forces are those
+ * of the last force pass (the last step of a dem_step call, dem_step_end or
+ * dem_compute_forces). */
+typedef struct dem_cg_config {
+    int32_t n;                   /* evaluation points in z (CG::setNZ) */
+    int32_t particle_dimensions; /* 3 (default, 0) or 2: volume of a particle */
+    double width;                /* Gauss width (CG::setWidth); cutoff 3 x width */
+    double min[3], max[3];       /* CG domain; min[0..1] and max[0..1] give the averaged area */
+} dem_cg_config;
+
+/* Values per point, in this order (the columns of the reference's .stat
+ * file after time and z): VolumeFraction, Density, Momentum (3),
+ * MomentumFlux (xx xy xz yy yz zz), ContactStress (xx xy xz yx yy yz zx zy
+ * zz), InteractionForce (3), ParticleSize (6). */
+#define DEM_CG_FIELDS 29
+
+/* out: n x DEM_CG_FIELDS doubles. */
+int dem_cg_z(dem_ctx* ctx, const dem_cg_config* cfg, double* out);
+
 /* Per-phase HIP-event timing of dem_step (no counterpart in the reference,
  * whose only timer is the wall clock of Time::toc, MercuryTime.h:70-73).
  * Enabling resets the accumulated times. */

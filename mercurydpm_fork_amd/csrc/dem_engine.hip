@@ -86,6 +86,8 @@ struct Params {
     uint32_t nt;
     uint32_t alive_min; // stored interactions with an older stamp have been erased
     int kick;           // integration fused into the force pass (KICK_*)
+    int store_fc;       // keep the contact forces of this pass (for coarse graining)
+    double fc_stamp;    // stamp of this pass in fc / fw
     // grid
     double h[kMaxLevels];       // cell size of a non-periodic axis
     double rmaxl[kMaxLevels];   // largest radius on each level
@@ -117,6 +119,10 @@ struct Dev {
     unsigned long long* mask[2];
     Rec<NS>* wrec[2];
     uint32_t* wmask[2];
+    // contact forces of the last storing force pass (force on P; w = pass
+    // stamp), per list slot of the writer and per particle-wall slot
+    double4* fc;
+    double4* fw;
     // grid
     uint32_t *key, *count, *start, *cursor, *perm;
     uint32_t *tsum, *toff; // per-scan-tile bucket counts and their exclusive scan
@@ -755,6 +761,43 @@ __global__ void __launch_bounds__(kBlock) k_orphans(Dev<NS> d, Params p)
     }
 }
 
+// BaseWall::getDistanceNormalOverlap for the two wall types: InfiniteWall
+// (distance from Quaternion::getDistance coefficients, InfiniteWall.cc:
+// 151-175; normal Quaternion::getAxis) and CylindricalWall about the z axis
+// (CylindricalWall.cc:72-83).  n is the wall's normal.
+__device__ inline bool wall_contact(const dem_wall& wl, V3 x, double r, double interaction_distance, double& distance,
+                                    V3& n)
+{
+    if (wl.type == DEM_WALL_INFINITE) {
+        distance = wl.dist_coef[0] * (wl.position[0] - x.x) +
+                   (wl.dist_coef[1] * (wl.position[1] - x.y) + wl.dist_coef[2] * (wl.position[2] - x.z));
+        if (distance < r + interaction_distance) {
+            n = mk(wl.axis[0], wl.axis[1], wl.axis[2]);
+            return true;
+        }
+        return false;
+    }
+    const double R = sqrt(x.x * x.x + x.y * x.y);
+    distance = wl.radius - R;
+    if (distance < r) {
+        const V3 nr = mk(x.x, x.y, 0.0);
+        n = vdiv(nr, sqrt(vlen2(nr)));
+        return true;
+    }
+    return false;
+}
+
+// The interaction geometry of a wall contact (BaseWall::getInteractionWith,
+// BaseWall.cc:367-430): normal from the wall towards the particle.
+__device__ inline void wall_geometry(V3 x, double r, double distance, V3 n, ContactGeom& g)
+{
+    g.normal = vneg(n);
+    g.distance = distance;
+    g.overlap = r - distance;
+    g.contact = vsub(x, vmul(g.normal, r - 0.5 * g.overlap));
+    g.i_is_wall = true;
+}
+
 // ------------------------------------------------------------------------
 // Forces: DPMBase::computeAllForces (DPMBase.cc:3397-3526) for one particle:
 // pair contacts from its Verlet list (computeInternalForce, DPMBase.cc:
@@ -934,6 +977,7 @@ __device__ inline void process_batch(Dev<NS>& d, const Params& p, int cur, Owner
             // previous record survives unchanged.
             if (img && had && hist.ts >= r.ts) r = hist;
             d.rec[cur ^ 1][slot] = r;
+            if (p.store_fc) d.fc[slot] = make_double4(cF.x, cF.y, cF.z, p.fc_stamp);
         }
     }
 }
@@ -1123,33 +1167,11 @@ __global__ void __launch_bounds__(kBlock) FORCE_WAVES_ATTR k_force(Dev<NS> d, Pa
             bool contact = false;
             double distance = 0.0;
             V3 n = zero3();
-            if (!fixed_i) {
-                const double wall_r = pr.w + d.sp[si * p.ns + wl.species].interaction_distance;
-                if (wl.type == DEM_WALL_INFINITE) {
-                    distance = wl.dist_coef[0] * (wl.position[0] - bi.pos.x) +
-                               (wl.dist_coef[1] * (wl.position[1] - bi.pos.y) +
-                                wl.dist_coef[2] * (wl.position[2] - bi.pos.z));
-                    if (distance < wall_r) {
-                        contact = true;
-                        n = mk(wl.axis[0], wl.axis[1], wl.axis[2]);
-                    }
-                } else {
-                    const double R = sqrt(bi.pos.x * bi.pos.x + bi.pos.y * bi.pos.y);
-                    distance = wl.radius - R;
-                    if (distance < pr.w) {
-                        contact = true;
-                        const V3 nr = mk(bi.pos.x, bi.pos.y, 0.0);
-                        n = vdiv(nr, sqrt(vlen2(nr)));
-                    }
-                }
-            }
+            if (!fixed_i)
+                contact = wall_contact(wl, bi.pos, pr.w, d.sp[si * p.ns + wl.species].interaction_distance, distance, n);
             if (contact) {
                 ContactGeom g;
-                g.normal = vneg(n);
-                g.distance = distance;
-                g.overlap = pr.w - distance;
-                g.contact = vsub(bi.pos, vmul(g.normal, pr.w - 0.5 * g.overlap));
-                g.i_is_wall = true;
+                wall_geometry(bi.pos, pr.w, distance, n, g);
                 Body bw;
                 bw.pos = wl.type == DEM_WALL_INFINITE ? mk(wl.position[0], wl.position[1], wl.position[2]) : zero3();
                 bw.vel = zero3();
@@ -1163,6 +1185,7 @@ __global__ void __launch_bounds__(kBlock) FORCE_WAVES_ATTR k_force(Dev<NS> d, Pa
                 const dem_pair_params sp = d.sp[si * p.ns + wl.species];
                 V3 f, tc;
                 contact_force<NS>(sp, bi, bw, g, spring, p.dt, f, tc);
+                if (p.store_fc) d.fw[wbase + w] = make_double4(f.x, f.y, f.z, p.fc_stamp);
                 F = vadd(F, f);
                 if (p.rotation) T = vadd(T, vsub(tc, vcross(vsub(bi.pos, g.contact), f)));
                 Rec<NS> r;
@@ -1210,6 +1233,223 @@ __global__ void __launch_bounds__(kBlock) FORCE_WAVES_ATTR k_force(Dev<NS> d, Pa
             d.disp[i] = ds;
         }
     }
+}
+
+// ------------------------------------------------------------------------
+// Coarse graining: CG<CGCoordinates::Z, CGFunctions::Gauss,
+// CGFields::StandardFields> (CG.hcc:248-527, Gauss.hcc:79-241,
+// Coordinates/Z.cc, Base_X_Y_Z.cc:32-49, StandardFields.cc:188-264).
+// One thread per (evaluation point, chunk of particles) sums the chunk's
+// particle and contact terms at its point in a fixed order; a second pass
+// sums the chunks in order, so the result is deterministic.
+// ------------------------------------------------------------------------
+constexpr double kSqrtPi = 1.7724538509055160272981674833411; // ExtendedMath.h:45
+constexpr double kSqrt2 = 1.4142135623730950488016887242097;  // ExtendedMath.h:47
+constexpr int kCgFields = DEM_CG_FIELDS;
+
+struct CgParams {
+    int npts, chunk, nchunks, dims;
+    double w, cutoff, prefactor;
+    double min[3];
+    double start, delta, inv_delta, volume;
+    double stamp; // fc stamp of the force pass whose contacts are averaged
+};
+
+// static_cast<std::size_t> of a non-negative double (negative arguments,
+// undefined in the reference, are taken as 0), capped at n
+__device__ inline int cg_index(double x, int n) { return x <= 0.0 ? 0 : (x >= (double)n ? n : (int)x); }
+
+// Gauss::evaluateCGFunction with Z::getDistanceSquared
+__device__ inline double cg_phi(const CgParams& c, double pz, double z)
+{
+    const double d2 = (pz - z) * (pz - z);
+    return d2 > c.cutoff * c.cutoff ? 0.0 : c.prefactor * exp(-d2 / (2.0 * c.w * c.w));
+}
+
+// Base_X_Y_Z::getGaussIntegralPrefactor
+__device__ inline double cg_integral_prefactor(double distance, double width, double cutoff)
+{
+    const double ws2 = width * kSqrt2;
+    const double a = -cutoff;
+    const double b = cutoff + distance;
+    return 0.5 / (+erf(b / ws2) * b + ws2 / kSqrtPi * exp(-((b / ws2) * (b / ws2))) - erf(a / ws2) * a -
+                  ws2 / kSqrtPi * exp(-((a / ws2) * (a / ws2))));
+}
+
+enum { CG_NONE, CG_I_TO_P, CG_I_TO_CONTACT, CG_CONTACT_TO_P };
+
+// One interaction's contact stress and interaction force density at the
+// point z (CG::evaluateContact with Gauss::evaluateCGIntegral1D).  pz, iz:
+// positions of P and I (I: the contact point when I is fixed).
+__device__ inline void cg_contact_at(const CgParams& c, int k, double z, int type, V3 F, V3 n, double distance,
+                                     V3 contact, V3 xp, double iz, double acc[12])
+{
+    V3 branch, ifd = zero3();
+    if (type == CG_I_TO_P) {
+        branch = vmul(n, distance);                               // getIP
+    } else if (type == CG_I_TO_CONTACT) {
+        ifd = vneg(F);
+        branch = vsub(vadd(vmul(n, distance), contact), xp);      // getIC
+    } else {
+        ifd = F;
+        branch = vsub(xp, contact);                               // getCP
+    }
+    const double pZ = (xp.z - c.min[2]) * c.inv_delta - 0.5;
+    const double iZ = (iz - c.min[2]) * c.inv_delta - 0.5;
+    const double cZ = c.cutoff * c.inv_delta;
+    const int k0 = c.npts == 1 ? 0 : cg_index(fmax(0.0, fmin(pZ, iZ) - cZ + 1), c.npts);
+    const int k1 = c.npts == 1 ? 1 : cg_index(fmax(pZ, iZ) + cZ + 1, c.npts);
+    if (k < k0 || k >= k1) return;
+    const double normal_length = fabs(n.z);
+    double psi;
+    if (fabs(normal_length) < 1e-10) {
+        psi = cg_phi(c, contact.z, z);
+    } else {
+        const double nz = n.z / normal_length;
+        const double w_sqrt_2 = kSqrt2 * c.w;
+        // the prefactor is taken at the first point the reference reaches
+        // with the integral's end points inside the cut-off
+        double pre = 0.0;
+        bool found = false;
+        for (int kk = k0; kk <= k && !found; ++kk) {
+            const double zz = c.start + c.delta * (double)kk;
+            const double in = type != CG_CONTACT_TO_P ? (iz - zz) * nz : (contact.z - zz) * nz;
+            const double pn = type != CG_I_TO_CONTACT ? (xp.z - zz) * nz : (contact.z - zz) * nz;
+            if (in > c.cutoff || pn < -c.cutoff) continue;
+            pre = cg_integral_prefactor(pn - in, c.w, c.cutoff);
+            found = true;
+        }
+        const double in = type != CG_CONTACT_TO_P ? (iz - z) * nz : (contact.z - z) * nz;
+        const double pn = type != CG_I_TO_CONTACT ? (xp.z - z) * nz : (contact.z - z) * nz;
+        psi = (in > c.cutoff || pn < -c.cutoff) ? 0.0 : pre * (erf(pn / w_sqrt_2) - erf(in / w_sqrt_2));
+    }
+    const double cs[9] = {F.x * branch.x, F.x * branch.y, F.x * branch.z, F.y * branch.x, F.y * branch.y,
+                          F.y * branch.z, F.z * branch.x, F.z * branch.y, F.z * branch.z};
+    for (int a = 0; a < 9; ++a) acc[a] += cs[a] * psi;
+    if (type != CG_I_TO_P) {
+        const double phi = cg_phi(c, contact.z, z);
+        acc[9] += ifd.x * phi;
+        acc[10] += ifd.y * phi;
+        acc[11] += ifd.z * phi;
+    }
+}
+
+template <int NS>
+__global__ void __launch_bounds__(kBlock) k_cg_particles(Dev<NS> d, Params p, CgParams c, double* partial)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= c.npts * c.nchunks) return;
+    const int k = t % c.npts, chunk = t / c.npts;
+    const double z = c.start + c.delta * (double)k;
+    double acc[17];
+    for (int a = 0; a < 17; ++a) acc[a] = 0.0;
+    const int i0 = chunk * c.chunk, i1 = min(p.n, i0 + c.chunk);
+    for (int i = i0; i < i1; ++i) {
+        const double4 vl = d.vel[i];
+        if (vl.w == 0.0) continue; // StandardFields::evaluateFixedParticles is false
+        const double4 pr = d.posr[i];
+        const double idPos = (pr.z - c.min[2]) * c.inv_delta - 0.5;
+        const double idCutoff = c.cutoff * c.inv_delta;
+        if (k < cg_index(fmax(0.0, idPos - idCutoff + 1), c.npts) || k >= cg_index(idPos + idCutoff + 1, c.npts))
+            continue;
+        const double phi = cg_phi(c, pr.z, z);
+        const double r = pr.w, m = 1.0 / vl.w;
+        const double vol = c.dims == 3 ? 4.0 / 3.0 * 3.1415926535897932384626433832795 * r * r * r
+                                       : 3.1415926535897932384626433832795 * r * r;
+        acc[0] += vol * phi;
+        acc[1] += m * phi;
+        acc[2] += vl.x * m * phi;
+        acc[3] += vl.y * m * phi;
+        acc[4] += vl.z * m * phi;
+        acc[5] += vl.x * vl.x * m * phi;
+        acc[6] += vl.x * vl.y * m * phi;
+        acc[7] += vl.x * vl.z * m * phi;
+        acc[8] += vl.y * vl.y * m * phi;
+        acc[9] += vl.y * vl.z * m * phi;
+        acc[10] += vl.z * vl.z * m * phi;
+        double size = 1.0;
+        for (int a = 0; a < 6; ++a) {
+            acc[11 + a] += size * phi;
+            size *= r;
+        }
+    }
+    double* o = partial + ((size_t)chunk * c.npts + k) * kCgFields;
+    for (int a = 0; a < 11; ++a) o[a] = acc[a];
+    for (int a = 0; a < 6; ++a) o[23 + a] = acc[11 + a];
+}
+
+template <int NS>
+__global__ void __launch_bounds__(kBlock) k_cg_contacts(Dev<NS> d, Params p, CgParams c, int cur, double* partial)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= c.npts * c.nchunks) return;
+    const int k = t % c.npts, chunk = t / c.npts;
+    const double z = c.start + c.delta * (double)k;
+    double acc[12];
+    for (int a = 0; a < 12; ++a) acc[a] = 0.0;
+    const int i0 = chunk * c.chunk, i1 = min(p.n, i0 + c.chunk);
+    for (int i = i0; i < i1; ++i) {
+        const double4 pr = d.posr[i];
+        const V3 xp = xyz(pr);
+        const bool p_fixed = d.vel[i].w == 0.0;
+        // pairs, through the record in the lower id's (writer's) slot: P is
+        // the writer
+        unsigned long long m = d.mask[cur][i];
+        while (m) {
+            const int kk = __ffsll((long long)m) - 1;
+            m &= m - 1;
+            const size_t slot = lslot(kk, p.maxn, i);
+            const double4 f4 = d.fc[slot];
+            if (f4.w != c.stamp) continue; // not a contact of that pass
+            const uint32_t j = d.nbr[slot] & ~kWriterBit;
+            const double4 pq = d.posr[j];
+            V3 xj = xyz(pq);
+            for (int b = 0; b < p.nper; ++b) {
+                const int a = p.paxis[b];
+                const double dc = comp(xj, a) - comp(xp, a);
+                if (dc > 0.5 * p.pshift[b] || dc < -0.5 * p.pshift[b]) xj = periodic_shift(p, b, xj);
+            }
+            const V3 branch = vsub(xp, xj);
+            const double distance = sqrt(vlen2(branch));
+            const V3 n = vdiv(branch, distance);
+            const double overlap = pr.w + pq.w - distance;
+            const V3 contact = vsub(xp, vmul(n, pr.w - 0.5 * overlap));
+            const bool i_fixed = d.vel[j].w == 0.0;
+            const int type = p_fixed ? (i_fixed ? CG_NONE : CG_I_TO_CONTACT) : (i_fixed ? CG_CONTACT_TO_P : CG_I_TO_P);
+            if (type == CG_NONE) continue;
+            cg_contact_at(c, k, z, type, xyz(f4), n, distance, contact, xp, i_fixed ? contact.z : pq.z, acc);
+        }
+        // walls (BaseWall::isFixed is true): CONTACT_TO_P unless P is fixed
+        if (p.nwalls > 0 && !p_fixed) {
+            const uint32_t wm = d.wmask[cur][i];
+            for (int w = 0; w < p.nwalls; ++w) {
+                if (!((wm >> w) & 1u)) continue;
+                const double4 f4 = d.fw[(size_t)i * p.nwalls + w];
+                if (f4.w != c.stamp) continue;
+                double distance = 0.0;
+                V3 wn = zero3();
+                const int si = (int)(d.spc[i] & 0xFFFF);
+                if (!wall_contact(d.walls[w], xp, pr.w, d.sp[si * p.ns + d.walls[w].species].interaction_distance,
+                                  distance, wn))
+                    continue;
+                ContactGeom g;
+                wall_geometry(xp, pr.w, distance, wn, g);
+                cg_contact_at(c, k, z, CG_CONTACT_TO_P, xyz(f4), g.normal, g.distance, g.contact, xp, g.contact.z,
+                              acc);
+            }
+        }
+    }
+    double* o = partial + ((size_t)chunk * c.npts + k) * kCgFields;
+    for (int a = 0; a < 12; ++a) o[11 + a] = acc[a];
+}
+
+__global__ void __launch_bounds__(kBlock) k_cg_reduce(CgParams c, const double* partial, double* out)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= c.npts * kCgFields) return;
+    double s = 0.0;
+    for (int chunk = 0; chunk < c.nchunks; ++chunk) s += partial[(size_t)chunk * c.npts * kCgFields + t];
+    out[t] = s / c.volume; // CG::volumeAverageVariables
 }
 
 // ------------------------------------------------------------------------
@@ -1381,6 +1621,7 @@ struct dem_ctx {
     int64_t n = 0;
     bool uploaded = false;
     bool in_step = false; // between dem_step_begin and dem_step_end
+    int64_t fc_pass = 0;  // force passes that stored their contact forces
     int cur = 0;
     Params prm{};
     // device buffers
@@ -1461,9 +1702,9 @@ int alloc_particles(dem_ctx* ctx, int64_t n)
     A(d.t_id, n) A(d.t_orig, n) A(d.t_spc, n) A(d.t_cellc, n);
     const size_t slots = (size_t)n * ctx->maxn_cfg;
     A(d.nbr, slots) A(d.ncount, n) A(d.rev, slots);
-    A(d.rec[0], slots) A(d.rec[1], slots) A(d.mask[0], n) A(d.mask[1], n);
+    A(d.rec[0], slots) A(d.rec[1], slots) A(d.mask[0], n) A(d.mask[1], n) A(d.fc, slots);
     const size_t wslots = (size_t)n * std::max<size_t>(ctx->walls.size(), 1);
-    A(d.wrec[0], wslots) A(d.wrec[1], wslots) A(d.wmask[0], n) A(d.wmask[1], n);
+    A(d.wrec[0], wslots) A(d.wrec[1], wslots) A(d.wmask[0], n) A(d.wmask[1], n) A(d.fw, wslots);
     A(d.key, n) A(d.perm, n);
     A(d.count, ctx->nbuckets) A(d.cursor, ctx->nbuckets) A(d.start, (size_t)ctx->nbuckets + 1);
     const size_t ntiles = ((size_t)ctx->nbuckets + kScanTile - 1) / kScanTile;
@@ -1518,6 +1759,9 @@ void launch_force(dem_ctx* ctx, int erase, int kick)
     ctx->prm.erase = erase;
     ctx->prm.nt = (uint32_t)ctx->nt;
     ctx->prm.kick = kick;
+    // the last force pass of a call keeps its contact forces for dem_cg_z
+    ctx->prm.store_fc = kick != KICK_BOTH;
+    if (ctx->prm.store_fc) ctx->prm.fc_stamp = (double)++ctx->fc_pass;
     const unsigned g = xcd_grid(grid_for(ctx->n));
     hipLaunchKernelGGL(k_force<NS>, dim3(g), dim3(kBlock), 0, ctx->stream, d, ctx->prm, ctx->cur);
     ctx->cur ^= 1;
@@ -1684,6 +1928,48 @@ void refresh_params(dem_ctx* ctx)
 }
 
 }  // namespace
+
+template <int NS>
+int run_cg_z(dem_ctx* ctx, const dem_cg_config* cfg, double* out)
+{
+    Dev<NS>& d = dev<NS>(ctx);
+    CgParams c{};
+    c.npts = cfg->n;
+    c.dims = cfg->particle_dimensions ? cfg->particle_dimensions : 3;
+    c.w = cfg->width;
+    c.cutoff = 3.0 * c.w; // Gauss::setWidth
+    // Base_X_Y_Z::getGaussPrefactor (Base_X_Y_Z.cc:32-36)
+    c.prefactor = 1.0 / (kSqrt2 * kSqrtPi * c.w) / std::erf(c.cutoff / (kSqrt2 * c.w));
+    for (int a = 0; a < 3; ++a) c.min[a] = cfg->min[a];
+    c.delta = (cfg->max[2] - cfg->min[2]) / (double)c.npts;
+    c.start = cfg->min[2] + 0.5 * c.delta;
+    c.inv_delta = (double)c.npts / (cfg->max[2] - cfg->min[2]);
+    c.volume = (cfg->max[0] - cfg->min[0]) * (cfg->max[1] - cfg->min[1]);
+    c.stamp = (double)ctx->fc_pass;
+    c.chunk = 1024;
+    c.nchunks = (int)std::max<int64_t>(1, (ctx->n + c.chunk - 1) / c.chunk);
+    const size_t np = (size_t)c.nchunks * c.npts * kCgFields;
+    double* partial = nullptr;
+    double* dout = nullptr;
+    HIPCHK(hipMalloc(&partial, sizeof(double) * np));
+    if (hipMalloc(&dout, sizeof(double) * (size_t)c.npts * kCgFields) != hipSuccess) {
+        (void)hipFree(partial);
+        return set_err(ctx, DEM_ERR_DEVICE, "hipMalloc failed for the CG output");
+    }
+    const unsigned g1 = (unsigned)(((size_t)c.npts * c.nchunks + kBlock - 1) / kBlock);
+    hipLaunchKernelGGL(k_cg_particles<NS>, dim3(g1), dim3(kBlock), 0, ctx->stream, d, ctx->prm, c, partial);
+    hipLaunchKernelGGL(k_cg_contacts<NS>, dim3(g1), dim3(kBlock), 0, ctx->stream, d, ctx->prm, c, ctx->cur, partial);
+    hipLaunchKernelGGL(k_cg_reduce, dim3((unsigned)(((size_t)c.npts * kCgFields + kBlock - 1) / kBlock)), dim3(kBlock),
+                       0, ctx->stream, c, (const double*)partial, dout);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(out, dout, sizeof(double) * (size_t)c.npts * kCgFields, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    (void)hipFree(partial);
+    (void)hipFree(dout);
+    if (e != hipSuccess) return set_err(ctx, DEM_ERR_DEVICE, std::string("dem_cg_z: ") + hipGetErrorString(e));
+    return DEM_OK;
+}
 
 extern "C" {
 
@@ -2115,6 +2401,20 @@ int dem_step_end(dem_ctx* ctx)
     HIPCHK(hipSetDevice(ctx->device));
     ctx->in_step = false;
     return ctx->ns_springs == 9 ? run_step_end<9>(ctx) : run_step_end<3>(ctx);
+}
+
+int dem_cg_z(dem_ctx* ctx, const dem_cg_config* cfg, double* out)
+{
+    if (!ctx) return DEM_ERR_ARG;
+    if (!cfg || !out || cfg->n <= 0 || !(cfg->width > 0) || !(cfg->max[2] > cfg->min[2]))
+        return set_err(ctx, DEM_ERR_ARG, "dem_cg_z: bad configuration");
+    if (cfg->particle_dimensions != 0 && cfg->particle_dimensions != 2 && cfg->particle_dimensions != 3)
+        return set_err(ctx, DEM_ERR_ARG, "dem_cg_z: particle_dimensions must be 2 or 3");
+    if (!ctx->uploaded) return set_err(ctx, DEM_ERR_STATE, "no particles uploaded");
+    if (ctx->in_step) return set_err(ctx, DEM_ERR_STATE, "dem_cg_z inside dem_step_begin/dem_step_end");
+    if (ctx->fc_pass == 0) return set_err(ctx, DEM_ERR_STATE, "dem_cg_z before the first force pass");
+    HIPCHK(hipSetDevice(ctx->device));
+    return ctx->ns_springs == 9 ? run_cg_z<9>(ctx, cfg, out) : run_cg_z<3>(ctx, cfg, out);
 }
 
 int dem_set_kinematics(dem_ctx* ctx, int64_t n, const double* x, const double* v, const double* w, const double* q)

@@ -86,6 +86,19 @@ _I64 = C.c_int64
 _I32 = C.c_int32
 
 
+class CGConfig(C.Structure):
+    """dem_cg_config (include/mercurydpm_dem.h)."""
+    _fields_ = [("n", C.c_int32), ("particle_dimensions", C.c_int32), ("width", C.c_double),
+                ("min", C.c_double * 3), ("max", C.c_double * 3)]
+
+
+CG_FIELDS = 29
+CG_COLUMNS = (["VolumeFraction", "Density"] + [f"Momentum{a}" for a in "XYZ"] +
+              [f"MomentumFlux{a}" for a in ("XX", "XY", "XZ", "YY", "YZ", "ZZ")] +
+              [f"ContactStress{a}{b}" for a in "XYZ" for b in "XYZ"] +
+              [f"InteractionForce{a}" for a in "XYZ"] + [f"ParticleSize{k}" for k in range(6)])
+
+
 def _declare(lib, prefix: str, has_config: bool):
     def f(name, res, *args):
         fn = getattr(lib, prefix + name)
@@ -111,6 +124,7 @@ def _declare(lib, prefix: str, has_config: bool):
     f("step_begin", _I32, _PTR)
     f("step_end", _I32, _PTR)
     f("set_kinematics", _I32, _PTR, _I64, _PTR, _PTR, _PTR, _PTR)
+    f("cg_z", _I32, _PTR, C.POINTER(CGConfig), _PTR)
     f("download_particles", _I32, _PTR, *([_PTR] * 6))
     f("contact_count", _I32, _PTR, C.POINTER(_I64))
     f("download_contacts", _I32, _PTR, _I64, C.POINTER(_I64), _PTR, _PTR, _PTR, _PTR)
@@ -279,6 +293,14 @@ class Engine:
 
     def step(self, n: int = 1):
         self._check(self._fn("step")(self.h, int(n)))
+
+    def cg_z(self, n: int, width: float, lo, hi, particle_dimensions: int = 3) -> np.ndarray:
+        """CG<Z, Gauss, StandardFields> of the current state: an (n, 29)
+        array, columns as CG_COLUMNS, at z = lo[2] + (k + 1/2) (hi[2] - lo[2]) / n."""
+        cfg = CGConfig(int(n), int(particle_dimensions), float(width), (C.c_double * 3)(*lo), (C.c_double * 3)(*hi))
+        out = np.zeros((int(n), CG_FIELDS))
+        self._check(self._fn("cg_z")(self.h, C.byref(cfg), _ptr(out)))
+        return out
 
     def step_begin(self):
         """First part of computeOneTimeStep, up to actionsBeforeTimeStep."""

@@ -333,15 +333,53 @@ void DPMBase::initialiseSolve()
                                v.data(), w.data(), q.data(), f.data(), t.data()),
           "dem_upload_particles");
     check(dem_compute_forces(ctx_), "dem_compute_forces");
-    synced_ = false;
+    particlesSynced_ = interactionsSynced_ = false;
+    beforeHookMaybeOverridden_ = true;
 }
 
+// DPMBase::computeOneTimeStep (DPMBase.cc:4315-4389) with
+// actionsBeforeTimeStep between the drift and the force pass.
 void DPMBase::computeOneTimeStep()
 {
-    check(dem_step(ctx_, 1), "dem_step");
+    check(dem_step_begin(ctx_), "dem_step_begin");
+    runBeforeHook();
+    check(dem_step_end(ctx_), "dem_step_end");
     time_ += dt_;
     ++nt_;
-    synced_ = false;
+    particlesSynced_ = interactionsSynced_ = false;
+}
+
+void DPMBase::runBeforeHook()
+{
+    if (!beforeHookMaybeOverridden_) return;
+    synchroniseParticles();
+    beforeHookIsBase_ = false;
+    actionsBeforeTimeStep();
+    if (beforeHookIsBase_) {
+        beforeHookMaybeOverridden_ = false;
+        return;
+    }
+    pushKinematics();
+}
+
+// BaseInteractable::setPosition / setVelocity / setAngularVelocity /
+// setOrientation made in a hook reach the device
+void DPMBase::pushKinematics()
+{
+    const size_t n = particleHandler.getNumberOfObjects();
+    std::vector<double> x(3 * n), v(3 * n), w(3 * n), q(4 * n);
+    for (size_t k = 0; k < n; ++k) {
+        const BaseParticle* p = particleHandler.getObject((unsigned)k);
+        const Vec3D* vs[3] = {&p->position_, &p->velocity_, &p->angularVelocity_};
+        double* ds[3] = {&x[3 * k], &v[3 * k], &w[3 * k]};
+        for (int a = 0; a < 3; ++a) {
+            ds[a][0] = vs[a]->X;
+            ds[a][1] = vs[a]->Y;
+            ds[a][2] = vs[a]->Z;
+        }
+        for (int a = 0; a < 4; ++a) q[4 * k + a] = p->q_[a];
+    }
+    check(dem_set_kinematics(ctx_, (int64_t)n, x.data(), v.data(), w.data(), q.data()), "dem_set_kinematics");
 }
 
 void DPMBase::solve()
@@ -356,14 +394,16 @@ void DPMBase::solve()
             t += dt_;
             ++batch;
         }
-        actionsBeforeTimeStep();
         if (batch == 1) {
             computeOneTimeStep();
         } else {
+            // hooks every `batch` steps: the before-hook sees the state at
+            // the start of the batch
+            runBeforeHook();
             check(dem_step(ctx_, batch), "dem_step");
             for (int64_t k = 0; k < batch; ++k) time_ += dt_;
             nt_ += batch;
-            synced_ = false;
+            particlesSynced_ = interactionsSynced_ = false;
         }
         actionsAfterTimeStep();
     }
@@ -371,9 +411,9 @@ void DPMBase::solve()
     actionsAfterSolve();
 }
 
-void DPMBase::synchroniseHandlers()
+void DPMBase::synchroniseParticles()
 {
-    if (!ctx_ || synced_) return;
+    if (!ctx_ || particlesSynced_) return;
     const size_t n = particleHandler.getNumberOfObjects();
     std::vector<double> x(3 * n), v(3 * n), w(3 * n), q(4 * n), f(3 * n), t(3 * n);
     check(dem_download_particles(ctx_, x.data(), v.data(), w.data(), q.data(), f.data(), t.data()),
@@ -387,6 +427,14 @@ void DPMBase::synchroniseHandlers()
         p->torque_ = Vec3D(t[3 * k], t[3 * k + 1], t[3 * k + 2]);
         for (int a = 0; a < 4; ++a) p->q_[a] = q[4 * k + a];
     }
+    particlesSynced_ = true;
+}
+
+void DPMBase::synchroniseHandlers()
+{
+    if (!ctx_) return;
+    synchroniseParticles();
+    if (interactionsSynced_) return;
     int64_t nc = 0;
     check(dem_contact_count(ctx_, &nc), "dem_contact_count");
     std::vector<uint32_t> pid(nc), iid(nc), ts(nc);
@@ -407,5 +455,5 @@ void DPMBase::synchroniseHandlers()
         c.timeStamp = ts[k];
         objs.push_back(c);
     }
-    synced_ = true;
+    interactionsSynced_ = true;
 }

@@ -177,6 +177,11 @@ public:
     void setPosition(const Vec3D& x) { position_ = x; }
     void setVelocity(const Vec3D& v) { velocity_ = v; }
     void setAngularVelocity(const Vec3D& w) { angularVelocity_ = w; }
+    // BaseInteractable::setOrientation: the quaternion (q0, q1, q2, q3)
+    void setOrientation(const double q[4])
+    {
+        for (int a = 0; a < 4; ++a) q_[a] = q[a];
+    }
     void setRadius(double r) { radius_ = r; }
     void setSpecies(const ParticleSpecies* s) { species_ = s; }
     // BaseParticle::fixParticle (BaseParticle.cc:171-183)
@@ -417,7 +422,13 @@ public:
     void setVerletSkin(double s) { skin_ = s; }
 
     virtual void setupInitialConditions() {}
-    virtual void actionsBeforeTimeStep() {}
+    // Called where DPMBase::computeOneTimeStep calls it (DPMBase.cc:4347):
+    // after the drift of the step and before its force pass.  The particle
+    // handler holds the drifted state; positions, velocities, angular
+    // velocities and orientations set here are sent to the device.
+    virtual void actionsBeforeTimeStep() { beforeHookIsBase_ = true; }
+    // Called after each step (after the closing half-kick; the reference
+    // calls it just before that half-kick, DPMBase.cc:4361).
     virtual void actionsAfterTimeStep() {}
     virtual void actionsAfterSolve() {}
     // return false to stop the time loop (DPMBase::continueSolve)
@@ -436,6 +447,8 @@ public:
     void setHookStride(unsigned s) { hookStride_ = s ? s : 1; }
     // Copy device state (particles, forces, interactions) into the handlers.
     void synchroniseHandlers();
+    // Copy the particles only (positions, velocities, orientations, forces).
+    void synchroniseParticles();
     dem_stats getEngineStats();
 
 protected:
@@ -444,6 +457,8 @@ protected:
 private:
     void initialiseSolve();
     void check(int rc, const char* what);
+    void runBeforeHook();
+    void pushKinematics();
     std::string name_ = "out";
     double dt_ = 0, tMax_ = 0, time_ = 0, drag_ = 0;
     int64_t nt_ = 0;
@@ -455,7 +470,10 @@ private:
     int device_ = 0;
     double skin_ = 0;
     dem_ctx* ctx_ = nullptr;
-    bool synced_ = true;
+    bool particlesSynced_ = true, interactionsSynced_ = true;
+    // the base actionsBeforeTimeStep marks itself; once it has run, no host
+    // round trip is made for the hook again
+    bool beforeHookIsBase_ = false, beforeHookMaybeOverridden_ = true;
 };
 
 // MercuryBase / Mercury3D: the HGrid problem classes.  Their broad phase is

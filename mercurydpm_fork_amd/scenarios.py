@@ -296,6 +296,21 @@ def load_fstat(path):
     return np.loadtxt(path, comments="#", ndmin=2)
 
 
+def cg_basic_self_test():
+    """Drivers/SelfTests/CG/CGBasicSelfTest.cpp:70-108: two spheres stacked on
+    a fixed one under gravity (-0.1, 0, -1), linear viscoelastic, until
+    t = 0.21; its CG<Z, Gauss> output (n = 1000, width 0.5 / 3.01, over the
+    domain below) is SelfTestData/CGBasicSelfTest.1.stat.  Returns the scenario
+    and the CG settings (n, width, min, max)."""
+    s = Species(density=6.0 / PI)
+    s.set_collision_time_and_restitution(.01, .1, 1.)
+    r = 0.5 * math.sqrt(1.01)
+    st = make_state([(0.5, 0.5, 0.5), (0.6, 0.5, 1.5), (0.4, 0.5, -0.5)], [(0, 0, 0)] * 3, [r, r, r], [0, 0, 0], [s],
+                    fixed=[False, False, True])
+    sc = Scenario("CGBasicSelfTest", [s], [], [], (-0.1, 0.0, -1.0), s.collision_time(1.0) / 50.0, st, t_max=0.21)
+    return sc, dict(n=1000, width=0.5 / 3.01, lo=(-0.1, 0.0, -0.6), hi=(1.1, 1.0, 2.1))
+
+
 # --------------------------------------------------------------------------
 # Seeded workloads
 # --------------------------------------------------------------------------

@@ -1162,3 +1162,169 @@ int oracle_dem_force_scale(oracle_ctx* c, double* fs, double* ts)
     }
     return DEM_OK;
 }
+
+/* ---- CG<Z, Gauss, StandardFields> ---------------------------------------- */
+
+static const double kSqrtPi = 1.7724538509055160272981674833411; /* ExtendedMath.h:45 */
+static const double kSqrt2 = 1.4142135623730950488016887242097;  /* ExtendedMath.h:47 */
+
+enum { CG_NONE, CG_I_TO_P, CG_I_TO_CONTACT, CG_CONTACT_TO_P };
+
+/* Base_X_Y_Z::getGaussPrefactor (Base_X_Y_Z.cc:32-36) */
+static double cgGaussPrefactor(double width, double cutoff)
+{
+    double prefactor = 1.0 / (kSqrt2 * kSqrtPi * width);
+    return prefactor / erf(cutoff / (kSqrt2 * width));
+}
+
+/* Base_X_Y_Z::getGaussIntegralPrefactor (Base_X_Y_Z.cc:38-49) */
+static double cgGaussIntegralPrefactor(double distance, double width, double cutoff)
+{
+    double widthSqrt2 = width * kSqrt2;
+    double a = -cutoff;
+    double b = cutoff + distance;
+    return 0.5 / (+erf(b / widthSqrt2) * b + widthSqrt2 / kSqrtPi * exp(-((b / widthSqrt2) * (b / widthSqrt2))) -
+                  erf(a / widthSqrt2) * a - widthSqrt2 / kSqrtPi * exp(-((a / widthSqrt2) * (a / widthSqrt2))));
+}
+
+/* Gauss::evaluateCGFunction (Gauss.hcc:79-85) with Z::getDistanceSquared */
+static double cgPhi(double pz, double z, double prefactor, double width, double cutoff)
+{
+    double d2 = (pz - z) * (pz - z);
+    return d2 > cutoff * cutoff ? 0.0 : prefactor * exp(-d2 / (2.0 * width * width));
+}
+
+/* static_cast<std::size_t> of a non-negative double (negative arguments,
+ * undefined in the reference, are taken as 0) */
+static size_t cgIndex(double x) { return x <= 0.0 ? 0 : (size_t)x; }
+
+int oracle_dem_cg_z(oracle_ctx* c, const dem_cg_config* g, double* out)
+{
+    if (!g || !out || g->n <= 0 || !(g->width > 0)) return fail(c, DEM_ERR_ARG, "cg: bad configuration");
+    const size_t n = (size_t)g->n;
+    const int dims = g->particle_dimensions ? g->particle_dimensions : 3;
+    const double w = g->width, cutoff = 3.0 * w; /* Gauss::setWidth (Gauss.hcc:41-44) */
+    const double prefactor = cgGaussPrefactor(w, cutoff);
+    const V3 mn = v3(g->min[0], g->min[1], g->min[2]);
+    const double maxZ = g->max[2];
+    memset(out, 0, sizeof(double) * n * DEM_CG_FIELDS);
+    /* spaceEvenly (Coordinates/Z.h:126-138) */
+    const double delta = (maxZ - mn.Z) / (double)n;
+    const double start = mn.Z + 0.5 * delta;
+    const double invDelta = (double)n / (maxZ - mn.Z);
+    double f[DEM_CG_FIELDS];
+
+    /* CG::evaluateParticle (CG.hcc:303-376); StandardFields::setFields(p) */
+    for (int64_t i = 0; i < c->nReal; ++i) {
+        const Particle* p = &c->p[i];
+        if (isFixed(p)) continue; /* StandardFields::evaluateFixedParticles is false */
+        const V3 pos = vsub(p->pos, mn);
+        const double idPos = pos.Z * invDelta - 0.5;
+        const double idCutoff = cutoff * invDelta;
+        const size_t k0 = cgIndex(fmax(0.0, idPos - idCutoff + 1));
+        size_t k1 = cgIndex(idPos + idCutoff + 1);
+        if (k1 > n) k1 = n;
+        const double r = p->radius, m = 1.0 / p->invMass;
+        memset(f, 0, sizeof f);
+        f[0] = dims == 3 ? 4.0 / 3.0 * 3.1415926535897932384626433832795 * r * r * r
+                         : 3.1415926535897932384626433832795 * r * r;
+        f[1] = m;
+        f[2] = p->vel.X * m;
+        f[3] = p->vel.Y * m;
+        f[4] = p->vel.Z * m;
+        const double sd[6] = {p->vel.X * p->vel.X, p->vel.X * p->vel.Y, p->vel.X * p->vel.Z,
+                              p->vel.Y * p->vel.Y, p->vel.Y * p->vel.Z, p->vel.Z * p->vel.Z};
+        for (int a = 0; a < 6; ++a) f[5 + a] = sd[a] * m;
+        double size = 1.0;
+        for (int a = 0; a < 6; ++a) {
+            f[23 + a] = size;
+            size *= r;
+        }
+        for (size_t k = k0; k < k1; ++k) {
+            const double phi = cgPhi(p->pos.Z, start + delta * (double)k, prefactor, w, cutoff);
+            double* o = out + k * DEM_CG_FIELDS;
+            /* addParticleStatistics (StandardFields.cc:188-196) */
+            o[0] += f[0] * phi;
+            o[1] += f[1] * phi;
+            for (int a = 2; a < 11; ++a) o[a] += f[a] * phi;
+            for (int a = 23; a < 29; ++a) o[a] += f[a] * phi;
+        }
+    }
+
+    /* CG::evaluateContact (CG.hcc:445-527) */
+    for (int64_t k = 0; k < c->nIn; ++k) {
+        const Inter* x = &c->in[k];
+        if (!x->alive) continue;
+        const Particle* P = &c->p[x->P];
+        const Particle* I = x->IisWall ? NULL : &c->p[x->I];
+        const int pFixed = isFixed(P), iFixed = I ? isFixed(I) : 1; /* BaseWall::isFixed is true */
+        int type;
+        if (pFixed) type = iFixed ? CG_NONE : CG_I_TO_CONTACT;
+        else if (iFixed) type = CG_CONTACT_TO_P;
+        else type = CG_I_TO_P;
+        if (type == CG_NONE) continue;
+        /* StandardFields::setFields(c, type) (StandardFields.cc:232-249) */
+        V3 branch, ifd = v3(0, 0, 0);
+        if (type == CG_I_TO_P) {
+            branch = vmul(x->normal, x->distance); /* getIP (BaseInteraction.cc:459-464) */
+        } else if (type == CG_I_TO_CONTACT) {
+            ifd = vneg(x->force);
+            branch = vsub(vadd(vmul(x->normal, x->distance), x->contact), P->pos); /* getIC (466-470) */
+        } else {
+            ifd = x->force;
+            branch = vsub(P->pos, x->contact); /* getCP (472-476) */
+        }
+        const V3 F = x->force;
+        const double cs[9] = {F.X * branch.X, F.X * branch.Y, F.X * branch.Z, F.Y * branch.X, F.Y * branch.Y,
+                              F.Y * branch.Z, F.Z * branch.X, F.Z * branch.Y, F.Z * branch.Z};
+        const V3 pp = P->pos;
+        const V3 ip = iFixed ? x->contact : I->pos;
+        const double pZ = (pp.Z - mn.Z) * invDelta - 0.5;
+        const double iZ = (ip.Z - mn.Z) * invDelta - 0.5;
+        const double cZ = cutoff * invDelta;
+        const size_t k0 = n == 1 ? 0 : cgIndex(fmax(0.0, fmin(pZ, iZ) - cZ + 1));
+        size_t k1 = n == 1 ? 1 : cgIndex(fmax(pZ, iZ) + cZ + 1);
+        if (k1 > n) k1 = n;
+        /* Gauss::evaluateCGIntegral1D (Gauss.hcc:171-212), for the
+         * Z coordinate (Gauss.hcc:232-237, Z.cc:47-70) */
+        const double normalLength = fabs(x->normal.Z);
+        const double nz = x->normal.Z / normalLength;
+        int havePrefactor = 0;
+        double integralPrefactor = 0.0;
+        const double wSqrt2 = kSqrt2 * w;
+        for (size_t kk = k0; kk < k1; ++kk) {
+            const double z = start + delta * (double)kk;
+            double psi;
+            if (fabs(normalLength) < 1e-10) {
+                psi = cgPhi(x->contact.Z, z, prefactor, w, cutoff);
+            } else {
+                const double iNormal = type != CG_CONTACT_TO_P ? (I->pos.Z - z) * nz : (x->contact.Z - z) * nz;
+                const double pNormal = type != CG_I_TO_CONTACT ? (pp.Z - z) * nz : (x->contact.Z - z) * nz;
+                if (iNormal > cutoff || pNormal < -cutoff) {
+                    psi = 0.0;
+                } else {
+                    if (!havePrefactor) {
+                        integralPrefactor = cgGaussIntegralPrefactor(pNormal - iNormal, w, cutoff);
+                        havePrefactor = 1;
+                    }
+                    psi = integralPrefactor * (erf(pNormal / wSqrt2) - erf(iNormal / wSqrt2));
+                }
+            }
+            double* o = out + kk * DEM_CG_FIELDS;
+            /* addInteractionStatistics (StandardFields.cc:204-207) */
+            for (int a = 0; a < 9; ++a) o[11 + a] += cs[a] * psi;
+            if (type != CG_I_TO_P) {
+                /* addContactPointStatistics (StandardFields.cc:214-217) */
+                const double phi = cgPhi(x->contact.Z, z, prefactor, w, cutoff);
+                o[20] += ifd.X * phi;
+                o[21] += ifd.Y * phi;
+                o[22] += ifd.Z * phi;
+            }
+        }
+    }
+
+    /* CG::volumeAverageVariables (CG.hcc:199-207), Z::getVolumeOfAveragedDimensions */
+    const double volume = (g->max[0] - g->min[0]) * (g->max[1] - g->min[1]);
+    for (size_t k = 0; k < n * DEM_CG_FIELDS; ++k) out[k] /= volume;
+    return DEM_OK;
+}

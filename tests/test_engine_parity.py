@@ -239,3 +239,36 @@ def test_engine_split_step_equals_step():
     for k in ("x", "v", "w", "q", "f", "t"):
         assert np.array_equal(da[k], db[k]), k
     assert_contacts_equal(a, b)
+
+
+def test_engine_cg_basic_self_test_gold():
+    """dem_cg_z on CGBasicSelfTest against the reference's
+    SelfTestData/CGBasicSelfTest.1.stat (CG<Z, Gauss>, 1000 points)."""
+    from test_oracle_golden import check_cg_basic_gold, run_cg_basic
+    check_cg_basic_gold(run_cg_basic(dem.Engine))
+
+
+@pytest.mark.parametrize("laws,periodic,walls", [("hvm", (True, True), ("bottom",)),
+                                                 ("lvf", (False, False), ("bottom", "cylinder"))])
+def test_engine_cg_vs_oracle(laws, periodic, walls):
+    """CG<Z, Gauss> of the engine and the oracle one step after identical
+    inputs with fixed particles (all integral types: I_TO_P, CONTACT_TO_P,
+    I_TO_CONTACT, walls): within 1e-10 of each column's largest value (the
+    contact forces agree to round-off of their summation)."""
+    sc = S.mixture(n=400, seed=17, laws=laws, periodic=periodic, walls=walls, fixed_frac=0.15, inflate=1.1)
+    g, o = make_pair(sc)
+    g.compute_forces()
+    o.compute_forces()
+    g.step(1)
+    o.step(1)
+    x = sc.state.x
+    lo = (float(x[:, 0].min()) - 1, float(x[:, 1].min()) - 1, float(x[:, 2].min()) - 1)
+    hi = (float(x[:, 0].max()) + 1, float(x[:, 1].max()) + 1, float(x[:, 2].max()) + 1)
+    a = g.cg_z(64, 0.3, lo, hi)
+    b = o.cg_z(64, 0.3, lo, hi)
+    scale = np.max(np.abs(b), axis=0) + 1e-300
+    assert np.max(np.abs(b[:, 11:20])) > 0, "no contact stress"
+    assert np.max(np.abs(b[:, 20:23])) > 0, "no boundary interaction force"
+    err = np.abs(a - b) / scale
+    k, col = np.unravel_index(err.argmax(), err.shape)
+    assert err.max() < 1e-10, (int(k), dem.CG_COLUMNS[col], a[k, col], b[k, col])

@@ -15,7 +15,8 @@ HOST = os.path.join(ROOT, "mercurydpm_fork_amd", "host")
 LIB = os.path.join(ROOT, "mercurydpm_fork_amd", "lib")
 BIN = os.path.join(ROOT, "tests", "drivers", "bin")
 DRIVERS = ["TangentialSpringUnitTest", "InclinedPlaneUnitTest", "HertzContactRestitutionUnitTest",
-           "PeriodicWallsUnitTest"]
+           "PeriodicWallsUnitTest", "MindlinSelfTest"]
+GOLDEN = os.path.join(ROOT, "tests", "golden")
 
 SETTERS_SRC = r'''
 #include <cstdio>
@@ -68,7 +69,7 @@ def test_driver_sources_compile(tmp_path):
 def test_driver_reproduces_reference_answer(driver):
     exe = os.path.join(BIN, driver)
     assert os.path.exists(exe), "run __graft_entry__.build() first"
-    r = subprocess.run([exe], capture_output=True, text=True, timeout=150)
+    r = subprocess.run([exe, GOLDEN], capture_output=True, text=True, timeout=150)
     print(r.stdout, r.stderr, file=sys.stderr)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "ok" in r.stdout

@@ -204,3 +204,64 @@ def test_mindlin_self_test_fstat_gold(which):
     the motion prescribed between the drift and the force pass."""
     from fstat_check import check_mindlin_self_test
     check_mindlin_self_test(oracle_lib.engine, which)
+
+
+def run_cg_basic(make_engine):
+    """CGBasicSelfTest run to t = 0.21, then CG<Z, Gauss> (the reference
+    writes its CG output in finaliseSolve, after the last step)."""
+    sc, cg = S.cg_basic_self_test()
+    e = make_engine()
+    S.apply(e, sc)
+    e.compute_forces()
+    e.step(sc.n_steps())
+    return e.cg_z(cg["n"], cg["width"], cg["lo"], cg["hi"])
+
+
+def check_cg_basic_gold(out):
+    """Against SelfTestData/CGBasicSelfTest.1.stat (6 printed digits); the
+    absolute floor is 1e-5 of each column's largest value."""
+    import os
+    gold = np.loadtxt(os.path.join(os.path.dirname(__file__), "golden", "cg_basic_self_test_gauss_z.stat"))
+    want = gold[:, 2:]
+    assert out.shape == want.shape
+    scale = np.max(np.abs(want), axis=0)
+    err = np.abs(out - want) / (5e-6 * np.abs(want) + 1e-5 * scale + 1e-300)
+    k, col = np.unravel_index(err.argmax(), err.shape)
+    assert err.max() <= 1.0, (int(k), dem.CG_COLUMNS[col], out[k, col], want[k, col])
+
+
+def test_cg_basic_self_test_gauss_z_gold():
+    """CG<Z, Gauss, StandardFields> of CGBasicSelfTest (particle fields,
+    contact stress with I_TO_P and CONTACT_TO_P line integrals, interaction
+    force density) matches the reference's .stat file."""
+    check_cg_basic_gold(run_cg_basic(oracle_lib.engine))
+
+
+def test_cg_mass_and_virial_closed_forms():
+    """Closed forms the CG must honour: the density integrates over z to the
+    bulk mass per unit area, and the contact stress zz integrates to the
+    virial sum F_z b_z of the contacts per unit area."""
+    sc, cg = S.cg_basic_self_test()
+    e = oracle_lib.engine()
+    S.apply(e, sc)
+    e.compute_forces()
+    e.step(sc.n_steps())
+    n, lo, hi = 4000, (-0.1, 0.0, -3.0), (1.1, 1.0, 4.0)
+    out = e.cg_z(n, cg["width"], lo, hi)
+    dz = (hi[2] - lo[2]) / n
+    area = (hi[0] - lo[0]) * (hi[1] - lo[1])
+    st = e.download()
+    mass = float(np.sum(1.0 / sc.state.inv_mass[:2]))
+    # the midpoint sum over the points is exact up to the jumps of the
+    # truncated Gauss function at +-3 w: dz * phi(3 w) / phi(0) ~ 1e-4 * 0.01
+    assert abs(out[:, 1].sum() * dz * area - mass) < 1e-5 * mass
+    # the two contacts: particle 1 on 0 (I_TO_P), particle 0 on the fixed 2 (CONTACT_TO_P)
+    x = st["x"]
+    f1 = st["f"][1] - np.asarray(sc.gravity) / sc.state.inv_mass[1]
+    virial = f1[2] * (x[1][2] - x[0][2])
+    r = sc.state.radius[0]
+    d02 = np.linalg.norm(x[0] - x[2])
+    c02 = x[0] - (x[0] - x[2]) / d02 * (r - 0.5 * (2 * r - d02))
+    f0 = st["f"][0] - np.asarray(sc.gravity) / sc.state.inv_mass[0] + f1  # contact force from particle 2 on 0
+    virial += f0[2] * (x[0][2] - c02[2])
+    assert abs(out[:, 19].sum() * dz * area - virial) < 1e-5 * abs(virial)
