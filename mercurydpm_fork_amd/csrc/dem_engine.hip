@@ -96,7 +96,7 @@ struct Params {
     uint32_t nb[kMaxLevels];    // buckets per level (nbx * nby * nbz)
     int nbd[kMaxLevels][3];     // bucket box dimensions per level
     uint32_t bstride[kMaxLevels][3]; // level-local bucket index = sum of wrapped coordinate x stride
-    int bax0;                   // slab axis (the longest)
+    int bax0, bax1, bax2;       // bucket axes: slab axis (the longest, slowest), middle, fastest
     int nslab;                  // slabs along it
     int32_t sdelta[kMaxSlabs][kMaxLevels]; // bucket = level-local index + sdelta[slab][level]
 };
@@ -179,6 +179,9 @@ __device__ inline uint32_t cell_key(const Params& p, int l, int cx, int cy, int 
     const int slab = (int)((uint64_t)ws * (uint32_t)p.nslab / (uint32_t)p.nbd[l][p.bax0]);
     return wx * p.bstride[l][0] + wy * p.bstride[l][1] + wz * p.bstride[l][2] + (uint32_t)p.sdelta[slab][l];
 }
+
+// cell coordinate along axis a of a stored cell (level | flags, x, y, z)
+__device__ inline int cell_coord(int4 c, int a) { return a == 0 ? c.y : (a == 1 ? c.z : c.w); }
 
 // Blocks are dealt round-robin to the 8 XCDs.  Remap them so that each XCD
 // (and its L2) works on one contiguous range of the sorted particles, i.e. on
@@ -618,21 +621,35 @@ __global__ void __launch_bounds__(kBlock) k_build_lists(Dev<NS> d, Params p)
                     hi[a] = (int)floor((xa + R) / p.h[m]);
                 }
             }
-            for (int tz = lo[2]; tz <= hi[2]; ++tz) {
-                int cz = tz;
-                if (nc[2]) cz = ((tz % nc[2]) + nc[2]) % nc[2];
-                for (int ty = lo[1]; ty <= hi[1]; ++ty) {
-                    int cy = ty;
-                    if (nc[1]) cy = ((ty % nc[1]) + nc[1]) % nc[1];
-                    for (int tx = lo[0]; tx <= hi[0]; ++tx) {
-                        int cx = tx;
-                        if (nc[0]) cx = ((tx % nc[0]) + nc[0]) % nc[0];
-                        const uint32_t b = cell_key(p, m, cx, cy, cz);
-                        const uint32_t s = d.start[b];
-                        const uint32_t e = d.start[b + 1];
-                        for (uint32_t q = s; q < e; ++q) {
+            // Buckets of one level are numbered with the fastest axis
+            // varying fastest, so the cells of a row along that axis lie in
+            // consecutive buckets and their particles in one range of the
+            // sorted arrays: walk each row as a single range (split where the
+            // bucket box wraps).  Cell coordinates are stored wrapped along
+            // periodic axes and as they are along open ones.
+            const int as = p.bax0, am = p.bax1, af = p.bax2;
+            const int nbf = p.nbd[m][af];
+            for (int ts = lo[as]; ts <= hi[as]; ++ts) {
+                const int cs = nc[as] ? wrap_mod(ts, nc[as]) : ts;
+                for (int tm = lo[am]; tm <= hi[am]; ++tm) {
+                    const int cm = nc[am] ? wrap_mod(tm, nc[am]) : tm;
+                    for (int t = lo[af]; t <= hi[af];) {
+                        const int bw = wrap_mod(t, nbf);
+                        const int len = min(hi[af] - t + 1, nbf - bw);
+                        const int s0 = nc[af] ? bw : t;
+                        const int s1 = s0 + len - 1;
+                        int cc[3];
+                        cc[as] = cs;
+                        cc[am] = cm;
+                        cc[af] = s0;
+                        const uint32_t b = cell_key(p, m, cc[0], cc[1], cc[2]);
+                        const uint32_t qs = d.start[b];
+                        const uint32_t qe = d.start[b + (uint32_t)len];
+                        for (uint32_t q = qs; q < qe; ++q) {
                             const int4 cq = d.t_cellc[q];
-                            if ((cq.x & 0xFF) != m || cq.y != cx || cq.z != cy || cq.w != cz) continue;
+                            if ((cq.x & 0xFF) != m || cell_coord(cq, as) != cs || cell_coord(cq, am) != cm) continue;
+                            const int cf = cell_coord(cq, af);
+                            if (cf < s0 || cf > s1) continue;
                             if (q == (uint32_t)i) continue;
                             if (fixed_i && (cq.x & 0x100)) continue;
                             const double4 pq = d.t_posr[q];
@@ -643,6 +660,7 @@ __global__ void __launch_bounds__(kBlock) k_build_lists(Dev<NS> d, Params p)
                                 ++cnt;
                             }
                         }
+                        t += len;
                     }
                 }
             }
@@ -2210,6 +2228,8 @@ int dem_upload_particles(dem_ctx* ctx, int64_t n, const uint32_t* id, const int3
         std::stable_sort(ax, ax + 3, [&](int u, int v) { return ext[u] > ext[v]; });
         if (ext[ax[0]] == ext[ax[1]] && ax[1] > ax[0]) std::swap(ax[0], ax[1]);  // ties: y before x
         p.bax0 = ax[0];
+        p.bax1 = ax[1];
+        p.bax2 = ax[2];
         p.nslab = std::max(1, std::min(kMaxSlabs, p.nbd[lc][ax[0]]));
         int64_t total = 0;
         for (int l = 0; l < L; ++l) {
