@@ -189,7 +189,7 @@ __device__ inline int cell_coord(int4 c, int a) { return a == 0 ? c.y : (a == 1 
 // (kXcdChunk = 0), or on runs of kXcdChunk consecutive blocks interleaved
 // over the whole range.  gridDim.x must be a multiple of 8 * max(kXcdChunk, 1).
 #ifndef DEM_XCD_CHUNK
-#define DEM_XCD_CHUNK 0
+#define DEM_XCD_CHUNK 1
 #endif
 constexpr int kXcdChunk = DEM_XCD_CHUNK;
 constexpr unsigned kXcdQuantum = 8u * (kXcdChunk > 0 ? kXcdChunk : 1);
