@@ -114,6 +114,11 @@ struct Dev {
     uint32_t* nbr;
     int32_t* ncount;
     uint8_t* rev;
+    // the same lists packed back to back per wavefront of 64 particles: the
+    // lists of particles 64w .. 64w+63 fill, in order, the region starting at
+    // 64w * stride -- what the force kernel walks
+    uint32_t* nbrc;
+    uint8_t* revc;
     // contact records
     Rec<NS>* rec[2];
     unsigned long long* mask[2];
@@ -471,8 +476,10 @@ __device__ inline uint32_t block_exclusive_scan(uint32_t t, uint32_t* s, uint32_
     return incl - t;
 }
 
+// exclusive scan of tsum[0..ntiles) into toff
 template <int NS>
-__global__ void __launch_bounds__(kBlock) k_scan_tile_sums(Dev<NS> d, uint32_t ntiles)
+__global__ void __launch_bounds__(kBlock) k_scan_tile_sums(Dev<NS> d, const uint32_t* tsum, uint32_t* toff,
+                                                           uint32_t ntiles)
 {
     if (!rebuild_pending(d.flags)) return;
     __shared__ uint32_t s[kBlock];
@@ -481,36 +488,40 @@ __global__ void __launch_bounds__(kBlock) k_scan_tile_sums(Dev<NS> d, uint32_t n
         const uint32_t b0 = base + threadIdx.x * kScanPer;
         uint32_t v[kScanPer], t = 0;
         for (int k = 0; k < kScanPer; ++k) {
-            v[k] = b0 + k < ntiles ? d.tsum[b0 + k] : 0u;
+            v[k] = b0 + k < ntiles ? tsum[b0 + k] : 0u;
             t += v[k];
         }
         uint32_t total;
         uint32_t run = carry + block_exclusive_scan(t, s, total);
         for (int k = 0; k < kScanPer; ++k) {
-            if (b0 + k < ntiles) d.toff[b0 + k] = run;
+            if (b0 + k < ntiles) toff[b0 + k] = run;
             run += v[k];
         }
         carry += total;
     }
 }
 
+// exclusive scan of count[0..n) into out, tile by tile from toff; the tile
+// totals tsum are spent and left at zero
 template <int NS>
-__global__ void __launch_bounds__(kBlock) k_scan_tiles(Dev<NS> d, uint32_t nbuckets)
+__global__ void __launch_bounds__(kBlock) k_scan_tiles(Dev<NS> d, const uint32_t* count, const uint32_t* toff,
+                                                       uint32_t* tsum, uint32_t* out, uint32_t n)
 {
     if (!rebuild_pending(d.flags)) return;
     __shared__ uint32_t s[kBlock];
     const uint32_t b0 = blockIdx.x * kScanTile + threadIdx.x * kScanPer;
     uint32_t v[kScanPer], t = 0;
     for (int k = 0; k < kScanPer; ++k) {
-        v[k] = b0 + k < nbuckets ? d.count[b0 + k] : 0u;
+        v[k] = b0 + k < n ? count[b0 + k] : 0u;
         t += v[k];
     }
     uint32_t total;
-    uint32_t run = d.toff[blockIdx.x] + block_exclusive_scan(t, s, total);
+    uint32_t run = toff[blockIdx.x] + block_exclusive_scan(t, s, total);
     for (int k = 0; k < kScanPer; ++k) {
-        if (b0 + k < nbuckets) d.start[b0 + k] = run;
+        if (b0 + k < n) out[b0 + k] = run;
         run += v[k];
     }
+    if (tsum && threadIdx.x == 0) tsum[blockIdx.x] = 0;
 }
 
 template <int NS>
@@ -529,7 +540,6 @@ template <int NS>
 __global__ void __launch_bounds__(kBlock) k_sort_buckets(Dev<NS> d, uint32_t nbuckets, uint32_t n)
 {
     if (!rebuild_pending(d.flags)) return;
-    GRID_STRIDE(t, (nbuckets + kScanTile - 1) / kScanTile) d.tsum[t] = 0;
     GRID_STRIDE(b, nbuckets)
     {
         // the counters are spent: leave them at zero for the next rebuild
@@ -677,15 +687,26 @@ template <int NS>
 __global__ void __launch_bounds__(kBlock) k_rematch(Dev<NS> d, Params p, int cur)
 {
     if (!rebuild_pending(d.flags)) return;
-    GRID_STRIDE(ii, p.n)
-    {
-        const int i = (int)ii;
+    // block-uniform stride loop: every lane of a wavefront takes part in the
+    // prefix of the list lengths that places its packed list
+    for (uint64_t i0 = (uint64_t)blockIdx.x * blockDim.x; i0 < (uint64_t)p.n; i0 += (uint64_t)gridDim.x * blockDim.x) {
+        const int i = (int)(i0 + threadIdx.x);
+        const int lane = threadIdx.x & 63;
+        const bool valid = i < p.n;
+        const int cnt = valid ? d.ncount[i] : 0;
+        int incl = cnt;
+        for (int sft = 1; sft < 64; sft <<= 1) {
+            const int t = __shfl_up(incl, sft, 64);
+            if (lane >= sft) incl += t;
+        }
+        const size_t off = (size_t)(i - lane) * p.maxn + (size_t)(incl - cnt);
+        if (!valid) continue;
         const uint32_t idi = d.t_id[i];
-        const int cnt = d.ncount[i];
         unsigned long long m = 0;
         for (int k = 0; k < cnt; ++k) {
             const size_t sl = lslot(k, p.maxn, i);
             const uint32_t e = d.nbr[sl];
+            d.nbrc[off + k] = e;
             const uint32_t j = e & ~kWriterBit;
             if (e & kWriterBit) {
                 Rec<NS> r;
@@ -694,6 +715,7 @@ __global__ void __launch_bounds__(kBlock) k_rematch(Dev<NS> d, Params p, int cur
                     m |= 1ull << k;
                 }
                 d.rev[sl] = 0xFF;
+                d.revc[off + k] = 0xFF;
             } else {
                 const int cj = d.ncount[j];
                 int kk = -1;
@@ -704,6 +726,7 @@ __global__ void __launch_bounds__(kBlock) k_rematch(Dev<NS> d, Params p, int cur
                     }
                 if (kk < 0) atomicOr(&d.flags[F_LIST_ASYM], 1);
                 d.rev[sl] = (uint8_t)(kk < 0 ? 0xFF : kk);
+                d.revc[off + k] = (uint8_t)(kk < 0 ? 0xFF : kk);
             }
         }
         d.mask[cur][i] = m;
@@ -1033,6 +1056,8 @@ __global__ void __launch_bounds__(kBlock) FORCE_WAVES_ATTR k_force(Dev<NS> d, Pa
     }
     const int total = __shfl(incl, 63, 64);
     L.pre[lane] = incl - cnt;
+    // the wavefront's packed lists start at wbase * stride
+    const size_t lbase = (size_t)wbase * p.maxn;
     unsigned long long touched = 0;
     V3 F = zero3(), T = zero3();
     int qn = 0;
@@ -1088,9 +1113,8 @@ __global__ void __launch_bounds__(kBlock) FORCE_WAVES_ATTR k_force(Dev<NS> d, Pa
             ej[u] = 0u;
             kr[u] = 0xFFu;
             if (e < total) {
-                const size_t sl = lslot(ek[u], p.maxn, wbase + o);
-                ej[u] = d.nbr[sl];
-                kr[u] = d.rev[sl];
+                ej[u] = d.nbrc[lbase + (size_t)e];
+                kr[u] = d.revc[lbase + (size_t)e];
             }
         }
 #pragma unroll
@@ -1719,7 +1743,8 @@ int alloc_particles(dem_ctx* ctx, int64_t n)
     A(d.t_posr, n) A(d.t_vel, n) A(d.t_omg, n) A(d.t_quat, n) A(d.t_frc, n) A(d.t_trq, n);
     A(d.t_id, n) A(d.t_orig, n) A(d.t_spc, n) A(d.t_cellc, n);
     const size_t slots = (size_t)n * ctx->maxn_cfg;
-    A(d.nbr, slots) A(d.ncount, n) A(d.rev, slots);
+    const size_t wslots_packed = ((size_t)n + 63) / 64 * 64 * ctx->maxn_cfg;
+    A(d.nbr, slots) A(d.ncount, n) A(d.rev, slots) A(d.nbrc, wslots_packed) A(d.revc, wslots_packed);
     A(d.rec[0], slots) A(d.rec[1], slots) A(d.mask[0], n) A(d.mask[1], n) A(d.fc, slots);
     const size_t wslots = (size_t)n * std::max<size_t>(ctx->walls.size(), 1);
     A(d.wrec[0], wslots) A(d.wrec[1], wslots) A(d.wmask[0], n) A(d.wmask[1], n) A(d.fw, wslots);
@@ -1749,8 +1774,10 @@ void launch_scan(dem_ctx* ctx)
 {
     Dev<NS>& d = dev<NS>(ctx);
     const uint32_t ntiles = (ctx->nbuckets + kScanTile - 1) / kScanTile;
-    hipLaunchKernelGGL(k_scan_tile_sums<NS>, dim3(1), dim3(kBlock), 0, ctx->stream, d, ntiles);
-    hipLaunchKernelGGL(k_scan_tiles<NS>, dim3(ntiles), dim3(kBlock), 0, ctx->stream, d, ctx->nbuckets);
+    hipLaunchKernelGGL(k_scan_tile_sums<NS>, dim3(1), dim3(kBlock), 0, ctx->stream, d, (const uint32_t*)d.tsum, d.toff,
+                       ntiles);
+    hipLaunchKernelGGL(k_scan_tiles<NS>, dim3(ntiles), dim3(kBlock), 0, ctx->stream, d, (const uint32_t*)d.count,
+                       (const uint32_t*)d.toff, d.tsum, d.start, ctx->nbuckets);
 }
 
 template <int NS>
