@@ -25,6 +25,21 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per force-kernel launch on this workload, measured with rocprofv3
+# FETCH_SIZE / WRITE_SIZE passes (calibrated against a plain copy); see the
+# file for the commands and the scaling
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_force_chute2M_traffic.md")
+
+
+def measured_traffic():
+    """traffic_bytes_per_launch from the committed counter summary, or None."""
+    try:
+        for line in open(TRAFFIC_PROFILE):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return int(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
 
 
 def parse():
@@ -190,7 +205,9 @@ def main():
                    "rebuilds_in_window": int(rebuilds_window), "step_hbm_gbs_algorithmic": round(step_gbs, 2),
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 5),
+                     "traffic": measured_traffic() if a.n == 2_000_000 else None,
+                     "traffic_source": "profiles/k_force_chute2M_traffic.md (rocprofv3 FETCH_SIZE x2.000 + WRITE_SIZE)",
                      "kernel": "k_force", "kernel_ms": round(force_ms, 4),
                      "bytes_per_particle": round(b_force, 2)},
         "cpu_baseline": cpu,
