@@ -904,7 +904,7 @@ struct OwnerLds {
 
 __device__ inline int q_owner(unsigned long long e) { return (int)((e >> 38) & 63); }
 
-template <int NS>
+template <int NS, int LAW>
 __device__ inline void process_batch(Dev<NS>& d, const Params& p, int cur, OwnerLds& L, int wbase, int base, int nb,
                                      int lane)
 {
@@ -991,7 +991,7 @@ __device__ inline void process_batch(Dev<NS>& d, const Params& p, int cur, Owner
         }
         const dem_pair_params& sp = d.sp[L.sp[o] * p.ns + sj];
         V3 f, tc;
-        contact_force<NS>(sp, P, I, g, spring, p.dt, f, tc);
+        contact_force<NS, LAW>(sp, P, I, g, spring, p.dt, f, tc);
         V3 cF, cT = zero3();
         if (i_is_P) {
             cF = f;
@@ -1023,7 +1023,9 @@ __device__ inline void process_batch(Dev<NS>& d, const Params& p, int cur, Owner
     }
 }
 
-template <int NS>
+// LAW: a force law shared by every species pair (LAW_HERTZ_MINDLIN), or
+// LAW_ANY (the law is read from each pair's parameters)
+template <int NS, int LAW>
 __global__ void __launch_bounds__(kBlock) FORCE_WAVES_ATTR k_force(Dev<NS> d, Params p, int cur)
 {
     __shared__ OwnerLds lds[kWaves];
@@ -1071,7 +1073,7 @@ __global__ void __launch_bounds__(kBlock) FORCE_WAVES_ATTR k_force(Dev<NS> d, Pa
         while (qn - head >= 64 || (all && qn > head)) {
             const int nb = qn - head < 64 ? qn - head : 64;
             wave_sync();
-            process_batch<NS>(d, p, cur, L, wbase, head, nb, lane);
+            process_batch<NS, LAW>(d, p, cur, L, wbase, head, nb, lane);
             L.first[lane] = -1;
             wave_sync();
             if (lane < nb) {
@@ -1226,7 +1228,7 @@ __global__ void __launch_bounds__(kBlock) FORCE_WAVES_ATTR k_force(Dev<NS> d, Pa
                     for (int a = 0; a < NS; ++a) spring[a] = d.wrec[cur][wbase + w].s[a];
                 const dem_pair_params sp = d.sp[si * p.ns + wl.species];
                 V3 f, tc;
-                contact_force<NS>(sp, bi, bw, g, spring, p.dt, f, tc);
+                contact_force<NS, LAW>(sp, bi, bw, g, spring, p.dt, f, tc);
                 if (p.store_fc) d.fw[wbase + w] = make_double4(f.x, f.y, f.z, p.fc_stamp);
                 F = vadd(F, f);
                 if (p.rotation) T = vadd(T, vsub(tc, vcross(vsub(bi.pos, g.contact), f)));
@@ -1659,6 +1661,7 @@ struct dem_ctx {
     int rotation = 1;
     int64_t nt = 0;
     int ns_springs = 3; // 3 or 9
+    bool all_hertz_mindlin = false; // every species pair: HertzianViscoelastic + Mindlin
     // particles
     int64_t n = 0;
     bool uploaded = false;
@@ -1808,7 +1811,10 @@ void launch_force(dem_ctx* ctx, int erase, int kick)
     ctx->prm.store_fc = kick != KICK_BOTH;
     if (ctx->prm.store_fc) ctx->prm.fc_stamp = (double)++ctx->fc_pass;
     const unsigned g = xcd_grid(grid_for(ctx->n));
-    hipLaunchKernelGGL(k_force<NS>, dim3(g), dim3(kBlock), 0, ctx->stream, d, ctx->prm, ctx->cur);
+    if (NS == 3 && ctx->all_hertz_mindlin)
+        hipLaunchKernelGGL((k_force<NS, LAW_HERTZ_MINDLIN>), dim3(g), dim3(kBlock), 0, ctx->stream, d, ctx->prm, ctx->cur);
+    else
+        hipLaunchKernelGGL((k_force<NS, LAW_ANY>), dim3(g), dim3(kBlock), 0, ctx->stream, d, ctx->prm, ctx->cur);
     ctx->cur ^= 1;
     if (kick != KICK_NONE) {
         // the integrated state went to the gather buffers: make it current
@@ -2074,8 +2080,12 @@ int dem_set_species(dem_ctx* ctx, int32_t n, const dem_pair_params* table)
     ctx->species.assign(table, table + (size_t)n * n);
     ctx->ns = n;
     ctx->ns_springs = 3;
-    for (const auto& s : ctx->species)
+    ctx->all_hertz_mindlin = true;
+    for (const auto& s : ctx->species) {
         if (s.friction_law == DEM_FRICTION_FULL && (s.rolling_mu != 0.0 || s.torsion_mu != 0.0)) ctx->ns_springs = 9;
+        if (s.normal_law != DEM_NORMAL_HERTZIAN_VISCOELASTIC || s.friction_law != DEM_FRICTION_MINDLIN)
+            ctx->all_hertz_mindlin = false;
+    }
     return DEM_OK;
 }
 

@@ -27,9 +27,13 @@ struct ContactGeom {
     bool i_is_wall;
 };
 
+// Force laws a kernel can be specialised for when every species pair uses
+// them (the operations are the same as on the general path).
+enum { LAW_ANY = 0, LAW_HERTZ_MINDLIN = 1 };
+
 // Returns the contact force on P and the contact torque (the part added with
 // addTorque; the lever-arm torque is applied by the caller).
-template <int NS>
+template <int NS, int LAW = LAW_ANY>
 __device__ inline void contact_force(const dem_pair_params& s, const Body& P, const Body& I, const ContactGeom& g,
                                      double* spring, double dt, V3& force, V3& torque)
 {
@@ -43,7 +47,7 @@ __device__ inline void contact_force(const dem_pair_params& s, const Body& P, co
     const double eff_m = 1.0 / (P.inv_mass + (g.i_is_wall ? 0.0 : I.inv_mass));
     if (g.overlap > 0) {
         double fn;
-        if (s.normal_law == DEM_NORMAL_LINEAR_VISCOELASTIC) {
+        if (LAW != LAW_HERTZ_MINDLIN && s.normal_law == DEM_NORMAL_LINEAR_VISCOELASTIC) {
             fn = s.stiffness * g.overlap - s.dissipation * nrv;
             if (s.constant_restitution) fn *= 2.0 * eff_m;
         } else {
@@ -61,7 +65,7 @@ __device__ inline void contact_force(const dem_pair_params& s, const Body& P, co
     }
     if (fn_abs == 0.0) return;
 
-    if (s.friction_law == DEM_FRICTION_MINDLIN) {
+    if (LAW == LAW_HERTZ_MINDLIN || s.friction_law == DEM_FRICTION_MINDLIN) {
         if (s.sliding_mu != 0.0) {
             const V3 vt = vsub(rel_vel, vmul(g.normal, nrv));
             V3 xi = mk(spring[0], spring[1], spring[2]);
