@@ -274,6 +274,27 @@ int dem_download_contacts(dem_ctx* ctx, int64_t capacity, int64_t* n_out, uint32
 
 int dem_get_stats(dem_ctx* ctx, dem_stats* stats);
 
+/* A stored interaction with the state BaseInteraction::writeToFStat
+ * (BaseInteraction.cc:487-531) and getElasticEnergy read: the force on P and
+ * the geometry of the last force pass.  `current` is 0 for an interaction
+ * that pass did not evaluate (kept only by the time-stamp rules of
+ * eraseOldInteractions); its force is then reported as zero. */
+typedef struct dem_interaction {
+    uint32_t p_id, i_id;  /* as dem_download_contacts (i_id: DEM_WALL_KEY | wall index for a wall) */
+    uint32_t time_stamp;
+    uint32_t current;
+    double force[3];      /* BaseInteraction::getForce, on P */
+    double tangential[3]; /* getTangentialForce: the friction part of force */
+    double contact[3];    /* getContactPoint */
+    double normal[3];     /* getNormal: from I towards P */
+    double overlap, distance;
+    double spring[9];     /* sliding, rolling, torsion springs */
+} dem_interaction;
+
+/* The stored interactions sorted by (p_id, i_id); capacity as in
+ * dem_download_contacts. */
+int dem_download_interactions(dem_ctx* ctx, int64_t capacity, int64_t* n_out, dem_interaction* out);
+
 /* ---- coarse graining ----------------------------------------------------
  * CG<CGCoordinates::Z, CGFunctions::Gauss, CGFields::StandardFields>
  * evaluated on the current state, as cgHandler.evaluate() does in

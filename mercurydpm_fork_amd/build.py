@@ -50,10 +50,14 @@ HOST_FLAGS = ["-std=c++17", "-O2", "-fPIC", "-Wall", "-Wextra", "-ffp-contract=o
 DRIVER_DIR = os.path.join(ROOT, "tests", "drivers")
 
 
+HOST_SOURCES = [os.path.join(HERE, "host", f) for f in ("Mercury3D.cc", "DPMBaseOutput.cc", "DPMBaseRestart.cc")]
+
+
 def build_host():
-    """C++ host API (host/Mercury3D.cc) linked against the engine."""
+    """C++ host API (host/Mercury3D.cc, host/DPMBaseOutput.cc) linked against
+    the engine."""
     out = os.path.join(LIB, "libmercurydpm_host.so")
-    _run([CXX, *HOST_FLAGS, "-shared", "-o", out, os.path.join(HERE, "host", "Mercury3D.cc"),
+    _run([CXX, *HOST_FLAGS, "-shared", "-o", out, *HOST_SOURCES,
           "-L" + LIB, "-lmercurydpm_dem", "-Wl,-rpath,$ORIGIN"])
     return out
 

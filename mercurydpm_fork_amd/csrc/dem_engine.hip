@@ -128,6 +128,10 @@ struct Dev {
     // stamp), per list slot of the writer and per particle-wall slot
     double4* fc;
     double4* fw;
+    // tangential part of those forces (the interaction's tangentialForce_,
+    // on P; w = pass stamp)
+    double4* ftc;
+    double4* ftw;
     // grid
     uint32_t *key, *count, *start, *cursor, *perm;
     uint32_t *tsum, *toff; // per-scan-tile bucket counts and their exclusive scan
@@ -990,8 +994,8 @@ __device__ inline void process_batch(Dev<NS>& d, const Params& p, int cur, Owner
             }
         }
         const dem_pair_params& sp = d.sp[L.sp[o] * p.ns + sj];
-        V3 f, tc;
-        contact_force<NS, LAW>(sp, P, I, g, spring, p.dt, f, tc);
+        V3 f, tc, ft;
+        contact_force<NS, LAW>(sp, P, I, g, spring, p.dt, f, tc, ft);
         V3 cF, cT = zero3();
         if (i_is_P) {
             cF = f;
@@ -1018,7 +1022,11 @@ __device__ inline void process_batch(Dev<NS>& d, const Params& p, int cur, Owner
             // previous record survives unchanged.
             if (img && had && hist.ts >= r.ts) r = hist;
             d.rec[cur ^ 1][slot] = r;
-            if (p.store_fc) d.fc[slot] = make_double4(cF.x, cF.y, cF.z, p.fc_stamp);
+            if (p.store_fc) {
+                d.fc[slot] = make_double4(cF.x, cF.y, cF.z, p.fc_stamp);
+                if (!i_is_P) ft = vneg(ft);
+                d.ftc[slot] = make_double4(ft.x, ft.y, ft.z, p.fc_stamp);
+            }
         }
     }
 }
@@ -1227,9 +1235,12 @@ __global__ void __launch_bounds__(kBlock) FORCE_WAVES_ATTR k_force(Dev<NS> d, Pa
                 if ((wm_in >> w) & 1u)
                     for (int a = 0; a < NS; ++a) spring[a] = d.wrec[cur][wbase + w].s[a];
                 const dem_pair_params sp = d.sp[si * p.ns + wl.species];
-                V3 f, tc;
-                contact_force<NS, LAW>(sp, bi, bw, g, spring, p.dt, f, tc);
-                if (p.store_fc) d.fw[wbase + w] = make_double4(f.x, f.y, f.z, p.fc_stamp);
+                V3 f, tc, ft;
+                contact_force<NS, LAW>(sp, bi, bw, g, spring, p.dt, f, tc, ft);
+                if (p.store_fc) {
+                    d.fw[wbase + w] = make_double4(f.x, f.y, f.z, p.fc_stamp);
+                    d.ftw[wbase + w] = make_double4(ft.x, ft.y, ft.z, p.fc_stamp);
+                }
                 F = vadd(F, f);
                 if (p.rotation) T = vadd(T, vsub(tc, vcross(vsub(bi.pos, g.contact), f)));
                 Rec<NS> r;
@@ -1277,6 +1288,24 @@ __global__ void __launch_bounds__(kBlock) FORCE_WAVES_ATTR k_force(Dev<NS> d, Pa
             d.disp[i] = ds;
         }
     }
+}
+
+// The geometry of a stored pair as the force pass evaluated it, P (the
+// writer) at xp and its partner at the minimum image (BaseParticle::
+// getInteractionWith, BaseParticle.cc:688-710; periodic images as in k_force).
+__device__ inline void pair_geometry(const Params& p, V3 xp, double rp, V3 xj, double rj, ContactGeom& g)
+{
+    for (int b = 0; b < p.nper; ++b) {
+        const int a = p.paxis[b];
+        const double dc = comp(xj, a) - comp(xp, a);
+        if (dc > 0.5 * p.pshift[b] || dc < -0.5 * p.pshift[b]) xj = periodic_shift(p, b, xj);
+    }
+    const V3 branch = vsub(xp, xj);
+    g.distance = sqrt(vlen2(branch));
+    g.normal = vdiv(branch, g.distance);
+    g.overlap = rp + rj - g.distance;
+    g.contact = vsub(xp, vmul(g.normal, rp - 0.5 * g.overlap));
+    g.i_is_wall = false;
 }
 
 // ------------------------------------------------------------------------
@@ -1447,17 +1476,10 @@ __global__ void __launch_bounds__(kBlock) k_cg_contacts(Dev<NS> d, Params p, CgP
             if (f4.w != c.stamp) continue; // not a contact of that pass
             const uint32_t j = d.nbr[slot] & ~kWriterBit;
             const double4 pq = d.posr[j];
-            V3 xj = xyz(pq);
-            for (int b = 0; b < p.nper; ++b) {
-                const int a = p.paxis[b];
-                const double dc = comp(xj, a) - comp(xp, a);
-                if (dc > 0.5 * p.pshift[b] || dc < -0.5 * p.pshift[b]) xj = periodic_shift(p, b, xj);
-            }
-            const V3 branch = vsub(xp, xj);
-            const double distance = sqrt(vlen2(branch));
-            const V3 n = vdiv(branch, distance);
-            const double overlap = pr.w + pq.w - distance;
-            const V3 contact = vsub(xp, vmul(n, pr.w - 0.5 * overlap));
+            ContactGeom pg;
+            pair_geometry(p, xp, pr.w, xyz(pq), pq.w, pg);
+            const V3 n = pg.normal, contact = pg.contact;
+            const double distance = pg.distance;
             const bool i_fixed = d.vel[j].w == 0.0;
             const int type = p_fixed ? (i_fixed ? CG_NONE : CG_I_TO_CONTACT) : (i_fixed ? CG_CONTACT_TO_P : CG_I_TO_P);
             if (type == CG_NONE) continue;
@@ -1561,6 +1583,96 @@ __global__ void __launch_bounds__(kBlock) k_collect_contacts(Dev<NS> d, Params p
             iid[o] = DEM_WALL_KEY | (uint32_t)w;
             for (int a = 0; a < 9; ++a) spring[9 * (size_t)o + a] = a < NS ? d.wrec[cur][s].s[a] : 0.0;
             ts[o] = d.wrec[cur][s].ts;
+        }
+    }
+}
+
+// The stored interactions with the geometry and force of the last force pass
+// (what BaseInteraction::writeToFStat writes, BaseInteraction.cc:487-531).
+// An interaction the last pass did not evaluate (kept only by the time-stamp
+// rules) carries a zero force.
+template <int NS>
+__global__ void __launch_bounds__(kBlock) k_collect_interactions(Dev<NS> d, Params p, int cur, double stamp,
+                                                                 dem_interaction* out, int* counter, int cap)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= p.n) return;
+    const uint32_t idi = d.id[i];
+    const double4 pr = d.posr[i];
+    const V3 xp = xyz(pr);
+    unsigned long long m = d.mask[cur][i];
+    while (m) {
+        const int k = __ffsll((long long)m) - 1;
+        m &= m - 1;
+        const int o = atomicAdd(counter, 1);
+        if (o >= cap) continue;
+        const size_t s = lslot(k, p.maxn, i);
+        const uint32_t j = d.nbr[s] & ~kWriterBit;
+        const double4 pq = d.posr[j];
+        ContactGeom g;
+        pair_geometry(p, xp, pr.w, xyz(pq), pq.w, g);
+        const double4 f4 = d.fc[s];
+        const bool current = f4.w == stamp;
+        dem_interaction& r = out[o];
+        r.p_id = idi;
+        r.i_id = d.id[j];
+        r.time_stamp = d.rec[cur][s].ts;
+        r.current = current ? 1 : 0;
+        r.force[0] = current ? f4.x : 0.0;
+        r.force[1] = current ? f4.y : 0.0;
+        r.force[2] = current ? f4.z : 0.0;
+        const double4 t4 = d.ftc[s];
+        r.tangential[0] = current ? t4.x : 0.0;
+        r.tangential[1] = current ? t4.y : 0.0;
+        r.tangential[2] = current ? t4.z : 0.0;
+        r.contact[0] = g.contact.x;
+        r.contact[1] = g.contact.y;
+        r.contact[2] = g.contact.z;
+        r.normal[0] = g.normal.x;
+        r.normal[1] = g.normal.y;
+        r.normal[2] = g.normal.z;
+        r.overlap = g.overlap;
+        r.distance = g.distance;
+        for (int a = 0; a < 9; ++a) r.spring[a] = a < NS ? d.rec[cur][s].s[a] : 0.0;
+    }
+    if (p.nwalls > 0) {
+        uint32_t wm = d.wmask[cur][i];
+        const int si = (int)(d.spc[i] & 0xFFFF);
+        while (wm) {
+            const int w = __ffs(wm) - 1;
+            wm &= wm - 1;
+            const int o = atomicAdd(counter, 1);
+            if (o >= cap) continue;
+            const size_t s = (size_t)i * p.nwalls + w;
+            double distance = 0.0;
+            V3 wn = zero3();
+            (void)wall_contact(d.walls[w], xp, pr.w, d.sp[si * p.ns + d.walls[w].species].interaction_distance, distance,
+                               wn);
+            ContactGeom g;
+            wall_geometry(xp, pr.w, distance, wn, g);
+            const double4 f4 = d.fw[s];
+            const bool current = f4.w == stamp;
+            dem_interaction& r = out[o];
+            r.p_id = idi;
+            r.i_id = DEM_WALL_KEY | (uint32_t)w;
+            r.time_stamp = d.wrec[cur][s].ts;
+            r.current = current ? 1 : 0;
+            r.force[0] = current ? f4.x : 0.0;
+            r.force[1] = current ? f4.y : 0.0;
+            r.force[2] = current ? f4.z : 0.0;
+            const double4 t4 = d.ftw[s];
+            r.tangential[0] = current ? t4.x : 0.0;
+            r.tangential[1] = current ? t4.y : 0.0;
+            r.tangential[2] = current ? t4.z : 0.0;
+            r.contact[0] = g.contact.x;
+            r.contact[1] = g.contact.y;
+            r.contact[2] = g.contact.z;
+            r.normal[0] = g.normal.x;
+            r.normal[1] = g.normal.y;
+            r.normal[2] = g.normal.z;
+            r.overlap = g.overlap;
+            r.distance = g.distance;
+            for (int a = 0; a < 9; ++a) r.spring[a] = a < NS ? d.wrec[cur][s].s[a] : 0.0;
         }
     }
 }
@@ -1748,9 +1860,9 @@ int alloc_particles(dem_ctx* ctx, int64_t n)
     const size_t slots = (size_t)n * ctx->maxn_cfg;
     const size_t wslots_packed = ((size_t)n + 63) / 64 * 64 * ctx->maxn_cfg;
     A(d.nbr, slots) A(d.ncount, n) A(d.rev, slots) A(d.nbrc, wslots_packed) A(d.revc, wslots_packed);
-    A(d.rec[0], slots) A(d.rec[1], slots) A(d.mask[0], n) A(d.mask[1], n) A(d.fc, slots);
+    A(d.rec[0], slots) A(d.rec[1], slots) A(d.mask[0], n) A(d.mask[1], n) A(d.fc, slots) A(d.ftc, slots);
     const size_t wslots = (size_t)n * std::max<size_t>(ctx->walls.size(), 1);
-    A(d.wrec[0], wslots) A(d.wrec[1], wslots) A(d.wmask[0], n) A(d.wmask[1], n) A(d.fw, wslots);
+    A(d.wrec[0], wslots) A(d.wrec[1], wslots) A(d.wmask[0], n) A(d.wmask[1], n) A(d.fw, wslots) A(d.ftw, wslots);
     A(d.key, n) A(d.perm, n);
     A(d.count, ctx->nbuckets) A(d.cursor, ctx->nbuckets) A(d.start, (size_t)ctx->nbuckets + 1);
     const size_t ntiles = ((size_t)ctx->nbuckets + kScanTile - 1) / kScanTile;
@@ -2580,6 +2692,50 @@ int dem_download_contacts(dem_ctx* ctx, int64_t capacity, int64_t* n_out, uint32
         if (spring)
             for (int a = 0; a < 9; ++a) spring[9 * k + a] = hs[9 * o + a];
     }
+    return DEM_OK;
+}
+
+int dem_download_interactions(dem_ctx* ctx, int64_t capacity, int64_t* n_out, dem_interaction* out)
+{
+    if (!ctx || !n_out || capacity < 0) return DEM_ERR_ARG;
+    if (ctx->in_step) return set_err(ctx, DEM_ERR_STATE, "dem_download_interactions inside a split step");
+    int64_t total = 0;
+    int r = dem_contact_count(ctx, &total);
+    if (r) return r;
+    if (total > capacity) return set_err(ctx, DEM_ERR_CAPACITY, "download_interactions: capacity too small");
+    *n_out = total;
+    if (total == 0) return DEM_OK;
+    dem_interaction* dout = nullptr;
+    int* dc = nullptr;
+    HIPCHK(hipMalloc(&dout, sizeof(dem_interaction) * total));
+    if (hipMalloc(&dc, sizeof(int)) != hipSuccess) {
+        (void)hipFree(dout);
+        return set_err(ctx, DEM_ERR_DEVICE, "hipMalloc failed");
+    }
+    HIPCHK(hipMemsetAsync(dc, 0, sizeof(int), ctx->stream));
+    const double stamp = (double)ctx->fc_pass;
+    if (ctx->ns_springs == 9)
+        hipLaunchKernelGGL(k_collect_interactions<9>, dim3(grid_for(ctx->n)), dim3(kBlock), 0, ctx->stream, ctx->d9,
+                           ctx->prm, ctx->cur, stamp, dout, dc, (int)total);
+    else
+        hipLaunchKernelGGL(k_collect_interactions<3>, dim3(grid_for(ctx->n)), dim3(kBlock), 0, ctx->stream, ctx->d3,
+                           ctx->prm, ctx->cur, stamp, dout, dc, (int)total);
+    std::vector<dem_interaction> h(total);
+    int got = 0;
+    hipError_t e = hipMemcpyAsync(h.data(), dout, sizeof(dem_interaction) * total, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(&got, dc, sizeof(int), hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    (void)hipFree(dout);
+    (void)hipFree(dc);
+    if (e != hipSuccess) return set_err(ctx, DEM_ERR_DEVICE, std::string("download_interactions: ") + hipGetErrorString(e));
+    // orphans (records whose pair left the lists) are stored interactions too,
+    // without a geometry: they are listed by dem_download_contacts only
+    const int64_t n = std::min<int64_t>(got, total);
+    std::sort(h.begin(), h.begin() + n, [](const dem_interaction& a, const dem_interaction& b) {
+        return a.p_id != b.p_id ? a.p_id < b.p_id : a.i_id < b.i_id;
+    });
+    std::copy(h.begin(), h.begin() + n, out);
+    *n_out = n;
     return DEM_OK;
 }
 

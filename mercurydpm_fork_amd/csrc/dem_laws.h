@@ -31,12 +31,15 @@ struct ContactGeom {
 // them (the operations are the same as on the general path).
 enum { LAW_ANY = 0, LAW_HERTZ_MINDLIN = 1 };
 
-// Returns the contact force on P and the contact torque (the part added with
-// addTorque; the lever-arm torque is applied by the caller).
+// Returns the contact force on P, the contact torque (the part added with
+// addTorque; the lever-arm torque is applied by the caller) and the
+// tangential (friction) part of the force, the interaction's tangentialForce_
+// (zero where the friction law adds none).
 template <int NS, int LAW = LAW_ANY>
 __device__ inline void contact_force(const dem_pair_params& s, const Body& P, const Body& I, const ContactGeom& g,
-                                     double* spring, double dt, V3& force, V3& torque)
+                                     double* spring, double dt, V3& force, V3& torque, V3& tangential)
 {
+    tangential = zero3();
     // --- normal force
     const V3 rel_vel = vsub(vel_at_contact(P.vel, P.pos, P.omega, g.contact),
                             vel_at_contact(I.vel, I.pos, I.omega, g.contact));
@@ -88,6 +91,7 @@ __device__ inline void contact_force(const dem_pair_params& s, const Body& P, co
                 force = vadd(force, ft);
                 xi = vneg(vdiv(ft, 2.0 / 3.0 * kt0));
             }
+            tangential = ft;
             spring[0] = xi.x;
             spring[1] = xi.y;
             spring[2] = xi.z;
@@ -120,6 +124,7 @@ __device__ inline void contact_force(const dem_pair_params& s, const Body& P, co
                 force = vadd(force, ft);
                 xi = vdiv(vneg(vadd(ft, vmul(vt, s.sliding_dissipation))), s.sliding_stiffness);
             }
+            tangential = ft;
             spring[0] = xi.x;
             spring[1] = xi.y;
             spring[2] = xi.z;
@@ -132,6 +137,7 @@ __device__ inline void contact_force(const dem_pair_params& s, const Body& P, co
             else
                 ft = vmul(vt, -(s.sliding_mu * fn_abs / sqrt(vtsq)));
             force = vadd(force, ft);
+            tangential = ft;
         }
     }
     if constexpr (NS == 9) {

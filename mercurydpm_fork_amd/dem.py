@@ -92,6 +92,15 @@ class CGConfig(C.Structure):
                 ("min", C.c_double * 3), ("max", C.c_double * 3)]
 
 
+# dem_interaction (include/mercurydpm_dem.h): one stored interaction with the
+# force on P and the geometry of the last force pass
+INTERACTION_DTYPE = np.dtype([("p_id", np.uint32), ("i_id", np.uint32), ("time_stamp", np.uint32),
+                              ("current", np.uint32), ("force", np.float64, 3), ("tangential", np.float64, 3),
+                              ("contact", np.float64, 3),
+                              ("normal", np.float64, 3), ("overlap", np.float64), ("distance", np.float64),
+                              ("spring", np.float64, 9)], align=True)
+assert INTERACTION_DTYPE.itemsize == 200
+
 CG_FIELDS = 29
 CG_COLUMNS = (["VolumeFraction", "Density"] + [f"Momentum{a}" for a in "XYZ"] +
               [f"MomentumFlux{a}" for a in ("XX", "XY", "XZ", "YY", "YZ", "ZZ")] +
@@ -128,6 +137,7 @@ def _declare(lib, prefix: str, has_config: bool):
     f("download_particles", _I32, _PTR, *([_PTR] * 6))
     f("contact_count", _I32, _PTR, C.POINTER(_I64))
     f("download_contacts", _I32, _PTR, _I64, C.POINTER(_I64), _PTR, _PTR, _PTR, _PTR)
+    f("download_interactions", _I32, _PTR, _I64, C.POINTER(_I64), _PTR)
 
 
 _hip_lib = None
@@ -341,6 +351,15 @@ class Engine:
                                                   _ptr(sp), _ptr(ts)))
         k = int(n.value)
         return Contacts(p[:k].copy(), i[:k].copy(), sp[:k].copy(), ts[:k].copy())
+
+    def interactions(self) -> np.ndarray:
+        """The stored interactions as an INTERACTION_DTYPE array sorted by
+        (p_id, i_id): what the .fstat writer reads."""
+        cap = max(1, self.contact_count())
+        out = np.zeros(cap, INTERACTION_DTYPE)
+        n = _I64()
+        self._check(self._fn("download_interactions")(self.h, cap, C.byref(n), _ptr(out)))
+        return out[:int(n.value)].copy()
 
     def set_profiling(self, on: bool):
         self._check(self.lib.dem_set_profiling(self.h, 1 if on else 0))

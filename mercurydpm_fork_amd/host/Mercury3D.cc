@@ -178,30 +178,38 @@ BaseParticle* ParticleHandler::copyAndAddObject(const BaseParticle& p)
 
 // Quaternion::setOrientationViaNormal (Quaternion.cc:543-566) followed by
 // the coefficients of Quaternion::getDistance (658-669) and getAxis (501-512)
-dem_wall InfiniteWall::toEngine() const
+void InfiniteWall::set(const Vec3D& normal, const Vec3D& point)
 {
-    double nx = normal_.X, ny = normal_.Y, nz = normal_.Z;
+    double nx = normal.X, ny = normal.Y, nz = normal.Z;
     const double l = std::sqrt(nx * nx + ny * ny + nz * nz);
     nx /= l;
     ny /= l;
     nz /= l;
-    double q0, q1, q2, q3;
     if (nx <= -1 + 1e-14) {
-        q0 = 0;
-        q1 = 0;
-        q2 = 1;
-        q3 = 0;
+        q_[0] = 0;
+        q_[1] = 0;
+        q_[2] = 1;
+        q_[3] = 0;
     } else {
-        q0 = 1.0 + nx;
-        q1 = 0;
-        q2 = -(0.0 + nz);
-        q3 = 0.0 + ny;
-        const double ql = std::sqrt(q0 * q0 + q1 * q1 + q2 * q2 + q3 * q3);
-        q0 /= ql;
-        q1 /= ql;
-        q2 /= ql;
-        q3 /= ql;
+        q_[0] = 1.0 + nx;
+        q_[1] = 0;
+        q_[2] = -(0.0 + nz);
+        q_[3] = 0.0 + ny;
+        const double ql = std::sqrt(q_[0] * q_[0] + q_[1] * q_[1] + q_[2] * q_[2] + q_[3] * q_[3]);
+        for (double& c : q_) c /= ql;
     }
+    position_ = point;
+}
+
+void InfiniteWall::setPositionAndOrientation(const Vec3D& point, const double q[4])
+{
+    position_ = point;
+    for (int a = 0; a < 4; ++a) q_[a] = q[a];
+}
+
+dem_wall InfiniteWall::toEngine() const
+{
+    const double q0 = q_[0], q1 = q_[1], q2 = q_[2], q3 = q_[3];
     dem_wall w{};
     w.type = DEM_WALL_INFINITE;
     w.species = species_ ? (int32_t)species_->getIndex() : 0;
@@ -219,12 +227,6 @@ dem_wall InfiniteWall::toEngine() const
     w.axis[1] = 2.0 * (a12 + a30);
     w.axis[2] = 2.0 * (a13 - a20);
     return w;
-}
-
-void InfiniteWall::set(const Vec3D& normal, const Vec3D& point)
-{
-    normal_ = normal;
-    position_ = point;
 }
 
 dem_wall CylindricalWall::toEngine() const
@@ -252,11 +254,42 @@ void PeriodicBoundary::set(const Vec3D& normal, double left, double right)
     b_.right = right * scale;
     if (!(b_.right > b_.left)) throw std::invalid_argument("PeriodicBoundary: left must be smaller than right");
     b_.shift = (nn * scale) * (b_.right - b_.left);
+    scaleFactor_ = scale;
+    normal_ = normal * scale;
+    distanceLeft_ = b_.left;
+    distanceRight_ = b_.right;
+    shift_ = normal_ * (distanceRight_ - distanceLeft_);
+}
+
+void PeriodicBoundary::setFromRestart(const Vec3D& normal, double scale, double left, double right,
+                                      const Vec3D& shift)
+{
+    int axis = -1;
+    if (normal.Y == 0 && normal.Z == 0 && normal.X > 0) axis = 0;
+    if (normal.X == 0 && normal.Z == 0 && normal.Y > 0) axis = 1;
+    if (normal.X == 0 && normal.Y == 0 && normal.Z > 0) axis = 2;
+    if (axis < 0) throw std::invalid_argument("PeriodicBoundary: the normal must point along +axis");
+    normal_ = normal;
+    scaleFactor_ = scale;
+    distanceLeft_ = left;
+    distanceRight_ = right;
+    shift_ = shift;
+    b_.axis = axis;
+    b_.left = left;
+    b_.right = right;
+    b_.shift = axis == 0 ? shift.X : (axis == 1 ? shift.Y : shift.Z);
 }
 
 // ------------------------------------------------------------------ DPMBase
 
-DPMBase::DPMBase() = default;
+// DPMBase::constructor (DPMBase.cc:209-262): file names, save count 20,
+// restart files with 15 significant digits
+DPMBase::DPMBase()
+{
+    setName(name_);
+    setSaveCount(20);
+    restartFile.getFstream().precision(std::numeric_limits<double>::digits10);
+}
 
 DPMBase::~DPMBase()
 {
@@ -284,9 +317,17 @@ dem_stats DPMBase::getEngineStats()
 // DPMBase::initialiseSolve (DPMBase.cc:4180-4277): set up, upload, force pass
 void DPMBase::initialiseSolve()
 {
-    nt_ = 0;
-    time_ = 0.0;
-    setupInitialConditions();
+    if (!restarted_) {
+        nt_ = 0;
+        time_ = 0.0;
+        // DPMBase::resetFileCounter (DPMBase.cc:469-483)
+        for (File* f : {&dataFile, &fStatFile, &eneFile, &restartFile, &statFile}) {
+            f->setCounter(0);
+            f->setLastSavedTimeStep(NEVER);
+        }
+        interactionHandler.objects().clear();
+        setupInitialConditions();
+    }
     if (dt_ <= 0) throw std::runtime_error("DPMBase::solve: time step not set");
     if (ctx_) dem_destroy(ctx_);
     ctx_ = nullptr;
@@ -308,7 +349,7 @@ void DPMBase::initialiseSolve()
     check(dem_set_background_drag(ctx_, drag_), "dem_set_background_drag");
     check(dem_set_timestep(ctx_, dt_), "dem_set_timestep");
     check(dem_set_rotation(ctx_, getRotation() ? 1 : 0), "dem_set_rotation");
-    check(dem_set_time(ctx_, 0.0, 0), "dem_set_time");
+    check(dem_set_time(ctx_, time_, nt_), "dem_set_time");
     const size_t n = particleHandler.getNumberOfObjects();
     std::vector<uint32_t> id(n);
     std::vector<int32_t> sp(n);
@@ -332,21 +373,26 @@ void DPMBase::initialiseSolve()
     check(dem_upload_particles(ctx_, (int64_t)n, id.data(), sp.data(), r.data(), im.data(), ii.data(), x.data(),
                                v.data(), w.data(), q.data(), f.data(), t.data()),
           "dem_upload_particles");
+    if (restarted_) uploadInteractions();
     check(dem_compute_forces(ctx_), "dem_compute_forces");
     particlesSynced_ = interactionsSynced_ = false;
     beforeHookMaybeOverridden_ = true;
 }
 
-// DPMBase::computeOneTimeStep (DPMBase.cc:4315-4389) with
-// actionsBeforeTimeStep between the drift and the force pass.
+// DPMBase::computeOneTimeStep (DPMBase.cc:4315-4389): output files first,
+// actionsBeforeTimeStep between the drift and the force pass,
+// actionsAfterTimeStep before the clock advances.
 void DPMBase::computeOneTimeStep()
 {
+    writeOutputFiles();
     check(dem_step_begin(ctx_), "dem_step_begin");
     runBeforeHook();
     check(dem_step_end(ctx_), "dem_step_end");
+    particlesSynced_ = interactionsSynced_ = false;
+    afterHookIsBase_ = false;
+    actionsAfterTimeStep();
     time_ += dt_;
     ++nt_;
-    particlesSynced_ = interactionsSynced_ = false;
 }
 
 void DPMBase::runBeforeHook()
@@ -382,31 +428,45 @@ void DPMBase::pushKinematics()
     check(dem_set_kinematics(ctx_, (int64_t)n, x.data(), v.data(), w.data(), q.data()), "dem_set_kinematics");
 }
 
+// DPMBase::solve (DPMBase.cc:4279-4310): the time loop, then the forced
+// output of the last step (finaliseSolve).  Steps run one at a time while a
+// hook or continueSolve may be overridden; otherwise (or with a hook stride
+// above 1) the steps up to the next output step run as one device batch.
 void DPMBase::solve()
 {
     initialiseSolve();
+    continueIsBase_ = false;
     // the number of steps DPMBase::solve takes: time_ accumulates dt_ while
     // time_ < timeMax (DPMBase.cc:4287-4290)
     while (time_ < tMax_ && continueSolve()) {
+        const bool hooksAreBase = !beforeHookMaybeOverridden_ && afterHookIsBase_ && continueIsBase_;
+        if (!hooksAreBase && hookStride_ == 1) {
+            computeOneTimeStep();
+            continue;
+        }
+        writeOutputFiles();
+        int64_t limit = hooksAreBase ? std::numeric_limits<int64_t>::max() : (int64_t)hookStride_;
+        const unsigned next = nextOutputStep((unsigned)nt_ + 1);
+        if (next != NEVER) limit = std::min<int64_t>(limit, (int64_t)next - nt_);
         int64_t batch = 0;
         double t = time_;
-        while (t < tMax_ && batch < (int64_t)hookStride_) {
+        while (t < tMax_ && batch < limit) {
             t += dt_;
             ++batch;
         }
-        if (batch == 1) {
-            computeOneTimeStep();
-        } else {
-            // hooks every `batch` steps: the before-hook sees the state at
-            // the start of the batch
-            runBeforeHook();
-            check(dem_step(ctx_, batch), "dem_step");
-            for (int64_t k = 0; k < batch; ++k) time_ += dt_;
-            nt_ += batch;
-            particlesSynced_ = interactionsSynced_ = false;
-        }
+        // hooks every `batch` steps: the before-hook sees the state at the
+        // start of the batch, the after-hook the state at its end
+        runBeforeHook();
+        check(dem_step(ctx_, batch), "dem_step");
+        particlesSynced_ = interactionsSynced_ = false;
+        for (int64_t k = 0; k + 1 < batch; ++k) time_ += dt_;
+        nt_ += batch - 1;
+        afterHookIsBase_ = false;
         actionsAfterTimeStep();
+        time_ += dt_;
+        ++nt_;
     }
+    forceWriteOutputFiles();
     synchroniseHandlers();
     actionsAfterSolve();
 }
@@ -442,8 +502,17 @@ void DPMBase::synchroniseHandlers()
     int64_t got = 0;
     check(dem_download_contacts(ctx_, nc, &got, pid.data(), iid.data(), spring.data(), ts.data()),
           "dem_download_contacts");
+    // the force and geometry of the last force pass, for the interactions
+    // whose pair is in the neighbour lists (all but those kept only by the
+    // time-stamp rules after their pair separated); both lists are sorted
+    // by (P id, I id)
+    std::vector<dem_interaction> geo(std::max<int64_t>(got, 1));
+    int64_t ng = 0;
+    check(dem_download_interactions(ctx_, got, &ng, geo.data()), "dem_download_interactions");
     auto& objs = interactionHandler.objects();
     objs.clear();
+    auto v3 = [](const double* a) { return Vec3D(a[0], a[1], a[2]); };
+    int64_t g = 0;
     for (int64_t k = 0; k < got; ++k) {
         Interaction c;
         c.P = pid[k];
@@ -453,6 +522,17 @@ void DPMBase::synchroniseHandlers()
         c.rollingSpring = Vec3D(spring[9 * k + 3], spring[9 * k + 4], spring[9 * k + 5]);
         c.torsionSpring = Vec3D(spring[9 * k + 6], spring[9 * k + 7], spring[9 * k + 8]);
         c.timeStamp = ts[k];
+        while (g < ng && (geo[g].p_id < c.P || (geo[g].p_id == c.P && geo[g].i_id < c.I))) ++g;
+        if (g < ng && geo[g].p_id == c.P && geo[g].i_id == c.I) {
+            const dem_interaction& d = geo[g];
+            c.current = d.current != 0;
+            c.force = v3(d.force);
+            c.tangentialForce = v3(d.tangential);
+            c.contactPoint = v3(d.contact);
+            c.normal = v3(d.normal);
+            c.overlap = d.overlap;
+            c.distance = d.distance;
+        }
         objs.push_back(c);
     }
     interactionsSynced_ = true;

@@ -8,7 +8,8 @@
 // LinearViscoelasticSlidingFriction, LinearViscoelasticFriction (rolling and
 // torsion), HertzianViscoelasticMindlin.  Walls: InfiniteWall,
 // CylindricalWall.  Boundaries: axis-aligned PeriodicBoundary.  Particles:
-// SphericalParticle.  Output files, restart and CG are not part of it.
+// SphericalParticle.  Output: the .data, .fstat and .ene files of
+// DPMBase::writeOutputFiles; restart files and CG are not written here.
 //
 // The handlers are the set-up and output source of truth, as in the
 // reference; inside DPMBase::solve() the device state is authoritative and is
@@ -18,7 +19,9 @@
 
 #include <cmath>
 #include <cstdint>
+#include <fstream>
 #include <functional>
+#include <limits>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -50,10 +53,64 @@ struct Vec3D {
     Vec3D operator-() const { return Vec3D(-X, -Y, -Z); }
     friend Vec3D operator*(double a, const Vec3D& b) { return Vec3D(b.X * a, b.Y * a, b.Z * a); }
     static double dot(const Vec3D& a, const Vec3D& b) { return a.X * b.X + a.Y * b.Y + a.Z * b.Z; }
+    // Vec3D::cross (Vector.cc)
+    static Vec3D cross(const Vec3D& a, const Vec3D& b)
+    {
+        return Vec3D(a.Y * b.Z - a.Z * b.Y, a.Z * b.X - a.X * b.Z, a.X * b.Y - a.Y * b.X);
+    }
     double getLengthSquared() const { return X * X + Y * Y + Z * Z; }
     double getLength() const { return std::sqrt(getLengthSquared()); }
     // Vec3D::isEqualTo (Vector.cc): squared distance within tol^2
     bool isEqualTo(const Vec3D& o, double tol) const { return (*this - o).getLengthSquared() <= tol * tol; }
+};
+
+// --------------------------------------------------------------- Files ---
+// FileType (File.h:40-60)
+enum class FileType : unsigned char { NO_FILE = 0, ONE_FILE = 1, MULTIPLE_FILES = 2, MULTIPLE_FILES_PADDED = 3 };
+
+// File::lastSavedTimeStep_ of a file never written (File.h:35)
+const unsigned NEVER = static_cast<unsigned>(-1);
+
+// File (File.cc): an output stream with its save count, counter and the
+// time step it was last written at.
+class File {
+public:
+    File() { fstream_.precision(10); }
+    File(const File&) = delete;
+    File& operator=(const File&) = delete;
+    void setName(const std::string& n) { name_ = n; }
+    const std::string& getName() const { return name_; }
+    // File::getFullName (File.cc:170-193): name, then .counter for multiple
+    // files (padded to four digits for MULTIPLE_FILES_PADDED)
+    std::string getFullName() const;
+    void setFileType(FileType t) { fileType_ = t; }
+    FileType getFileType() const { return fileType_; }
+    void setSaveCount(unsigned c) { saveCount_ = c; }
+    unsigned getSaveCount() const { return saveCount_; }
+    unsigned getCounter() const { return counter_; }
+    void setCounter(unsigned c) { counter_ = c; }
+    unsigned getLastSavedTimeStep() const { return lastSavedTimeStep_; }
+    void setLastSavedTimeStep(unsigned t) { lastSavedTimeStep_ = t; }
+    // File::saveCurrentTimeStep (File.cc:312-340)
+    bool saveCurrentTimeStep(unsigned nt) const
+    {
+        return fileType_ != FileType::NO_FILE && saveCurrentTimeStepNoFileTypeCheck(nt);
+    }
+    bool saveCurrentTimeStepNoFileTypeCheck(unsigned nt) const
+    {
+        return lastSavedTimeStep_ == NEVER || nt >= lastSavedTimeStep_ + saveCount_;
+    }
+    // File::openWrite (File.cc:381-393): appends to a single file after its
+    // first write
+    bool openWrite(unsigned nt);
+    void close() { fstream_.close(); }
+    std::fstream& getFstream() { return fstream_; }
+
+private:
+    std::string name_ = "out";
+    FileType fileType_ = FileType::ONE_FILE;
+    unsigned saveCount_ = 0, counter_ = 0, lastSavedTimeStep_ = NEVER;
+    std::fstream fstream_;
 };
 
 // ------------------------------------------------------------- Species ---
@@ -194,6 +251,7 @@ public:
     const double* getOrientation() const { return q_; }
     double getRadius() const { return radius_; }
     double getInvMass() const { return invMass_; }
+    double getInvInertia() const { return invInertia_; }
     double getMass() const { return 1.0 / invMass_; }
     bool isFixed() const { return invMass_ == 0.0; }
     unsigned getId() const { return id_; }
@@ -222,15 +280,25 @@ public:
     void setSpecies(const ParticleSpecies* s) { species_ = s; }
     const ParticleSpecies* getSpecies() const { return species_; }
     unsigned getId() const { return id_; }
+    // the force and torque on the wall in the last force pass (the reactions
+    // of its interactions, DPMBase.cc:3199-3207), after synchroniseHandlers
     Vec3D getForce() const { return force_; }
+    Vec3D getTorque() const { return torque_; }
+    const Vec3D& getPosition() const { return position_; }
+    const double* getOrientation() const { return q_; }
     virtual dem_wall toEngine() const = 0;
+    virtual std::string getName() const = 0;
+    // BaseWall::write (BaseWall.cc:100-116): BaseInteractable::write
+    // (BaseInteractable.cc:274-284) of a static wall
+    virtual void write(std::ostream& os) const;
 
 protected:
     friend class WallHandler;
     friend class DPMBase;
     const ParticleSpecies* species_ = nullptr;
     unsigned id_ = 0;
-    Vec3D force_;
+    Vec3D position_, force_, torque_;
+    double q_[4] = {1, 0, 0, 0};
 };
 
 // InfiniteWall (InfiniteWall.cc): orientation via normal, position p0
@@ -242,11 +310,13 @@ public:
         set(normal, point);
         setSpecies(s);
     }
+    // InfiniteWall::set: the orientation from the normal
+    // (Quaternion::setOrientationViaNormal, Quaternion.cc:543-566)
     void set(const Vec3D& normal, const Vec3D& point);
+    // the state a restart file holds: position and orientation
+    void setPositionAndOrientation(const Vec3D& point, const double q[4]);
     dem_wall toEngine() const override;
-
-private:
-    Vec3D normal_ = Vec3D(1, 0, 0), position_;
+    std::string getName() const override { return "InfiniteWall"; }
 };
 
 // CylindricalWall (CylindricalWall.cc): axis = z through the origin
@@ -255,6 +325,9 @@ public:
     void set(double radius) { radius_ = radius; }
     double getRadius() const { return radius_; }
     dem_wall toEngine() const override;
+    std::string getName() const override { return "CylindricalWall"; }
+    // CylindricalWall::write (CylindricalWall.cc:102-106)
+    void write(std::ostream& os) const override;
 
 private:
     double radius_ = 1.0;
@@ -267,9 +340,19 @@ public:
     // coordinate axis
     void set(const Vec3D& normal, double left, double right);
     dem_periodic toEngine() const { return b_; }
+    unsigned getId() const { return id_; }
+    // PeriodicBoundary::write (PeriodicBoundary.cc:426-434)
+    void write(std::ostream& os) const;
+    // the fields of a restart line: normal (unit), scaleFactor,
+    // distanceLeft, distanceRight, shift
+    void setFromRestart(const Vec3D& normal, double scale, double left, double right, const Vec3D& shift);
 
 private:
+    friend class BoundaryHandler;
     dem_periodic b_{};
+    Vec3D normal_, shift_;
+    double scaleFactor_ = 1.0, distanceLeft_ = 0.0, distanceRight_ = 0.0;
+    unsigned id_ = 0;
 };
 
 // ------------------------------------------------------------ Handlers ----
@@ -320,7 +403,10 @@ public:
     BaseParticle* copyAndAddObject(const BaseParticle& p);
     void setParticleDimensions(unsigned d) { dims_ = d; }
 
+    unsigned getParticleDimensions() const { return dims_; }
+
 private:
+    friend class DPMBase;
     unsigned nextId_ = 0;
     unsigned dims_ = 3;
 };
@@ -346,16 +432,30 @@ public:
     PeriodicBoundary* copyAndAddObject(const PeriodicBoundary& b)
     {
         objects_.push_back(std::make_unique<PeriodicBoundary>(b));
+        objects_.back()->id_ = getNumberOfObjects() - 1;
         return objects_.back().get();
     }
 };
 
-// One stored interaction (BaseInteraction): P/I ids, springs, time stamp.
+// One stored interaction (BaseInteraction): P/I ids, springs, time stamp,
+// and the force and geometry of the last force pass.
 struct Interaction {
-    unsigned P, I;  // I = DEM_WALL_KEY | wall index for a wall
+    unsigned P, I;  // particle ids; I = DEM_WALL_KEY | wall index for a wall
     bool isWall;
     Vec3D slidingSpring, rollingSpring, torsionSpring;
     unsigned timeStamp;
+    // false for an interaction the last force pass did not evaluate (kept
+    // by the time-stamp rules only): its force is zero
+    bool current = false;
+    Vec3D force, tangentialForce, contactPoint, normal;
+    double overlap = 0, distance = 0;
+    const Vec3D& getForce() const { return force; }
+    const Vec3D& getTangentialForce() const { return tangentialForce; }
+    const Vec3D& getContactPoint() const { return contactPoint; }
+    const Vec3D& getNormal() const { return normal; }
+    double getOverlap() const { return overlap; }
+    double getDistance() const { return distance; }
+    unsigned getTimeStamp() const { return timeStamp; }
 };
 
 class InteractionHandler {
@@ -382,7 +482,8 @@ public:
     BoundaryHandler boundaryHandler;
     InteractionHandler interactionHandler;
 
-    void setName(const std::string& n) { name_ = n; }
+    // DPMBase::setName (DPMBase.cc:420-455): also names the output files
+    void setName(const std::string& n);
     const std::string& getName() const { return name_; }
     void setTimeStep(double dt) { dt_ = dt; }
     double getTimeStep() const { return dt_; }
@@ -413,10 +514,49 @@ public:
     double getYMax() const { return max_.Y; }
     double getZMax() const { return max_.Z; }
     void setSystemDimensions(unsigned d) { systemDims_ = d; }
+    unsigned getSystemDimensions() const { return systemDims_; }
     void setParticleDimensions(unsigned d) { particleHandler.setParticleDimensions(d); }
-    // accepted for source compatibility; output files are not written
-    void setSaveCount(unsigned) {}
-    void setFileType(int) {}
+    // DPMBase::setDimension (DPMBase.cc:1401-1405)
+    void setDimension(unsigned d)
+    {
+        setSystemDimensions(d);
+        setParticleDimensions(d);
+    }
+    // Output files (DPMBase.cc:406-464).  Defaults as in the reference: one
+    // file each, every 20 steps, precision 10 (restart: 15).  statFile is
+    // kept for the restart format (this API writes no .stat file).
+    File dataFile, fStatFile, eneFile, restartFile, statFile;
+    void setSaveCount(unsigned c);
+    void setFileType(FileType t);
+    void setLastSavedTimeStep(unsigned t);
+    // DPMBase::writeOutputFiles (DPMBase.cc:4058-4096): .fstat, .ene and
+    // .data for the current step when their save counts say so;
+    // forceWriteOutputFiles writes all of them (DPMBase.cc:4037-4041)
+    void writeOutputFiles();
+    void forceWriteOutputFiles();
+    void writeFstatHeader(std::ostream& os);
+    void writeEneHeader(std::ostream& os) const;
+    void writeEneTimeStep(std::ostream& os);
+    void outputXBallsData(std::ostream& os);
+    // Restart files (DPMBase.cc:2951-3110, 3550-3690): write() gives the
+    // reference's format for the classes this API has; readRestartFile()
+    // reads it back (also files the reference wrote for those classes), and
+    // a following solve() continues from the stored time without calling
+    // setupInitialConditions, as the reference does.
+    void writeRestartFile();
+    void write(std::ostream& os);
+    bool readRestartFile(const std::string& fileName);
+    void read(std::istream& is);
+    bool getRestarted() const { return restarted_; }
+    void setRestarted(bool r) { restarted_ = r; }
+    // sums over the particle and interaction handlers (DPMBase.cc:1528-1560,
+    // ParticleHandler.cc:560-668); they synchronise the handlers first
+    double getElasticEnergy();
+    double getKineticEnergy();
+    double getRotationalEnergy();
+    double getGravitationalEnergy();
+    double getTotalMass();
+    Vec3D getCentreOfMass();
     // engine options
     void setDevice(int d) { device_ = d; }
     void setVerletSkin(double s) { skin_ = s; }
@@ -427,17 +567,28 @@ public:
     // handler holds the drifted state; positions, velocities, angular
     // velocities and orientations set here are sent to the device.
     virtual void actionsBeforeTimeStep() { beforeHookIsBase_ = true; }
-    // Called after each step (after the closing half-kick; the reference
-    // calls it just before that half-kick, DPMBase.cc:4361).
-    virtual void actionsAfterTimeStep() {}
+    // Called after each step's force pass, before the clock advances
+    // (DPMBase.cc:4361); the particles hold the state after the closing
+    // half-kick, which the reference applies just after this hook.
+    virtual void actionsAfterTimeStep() { afterHookIsBase_ = true; }
     virtual void actionsAfterSolve() {}
     // return false to stop the time loop (DPMBase::continueSolve)
-    virtual bool continueSolve() const { return true; }
+    virtual bool continueSolve() const
+    {
+        continueIsBase_ = true;
+        return true;
+    }
 
     // DPMBase::solve (DPMBase.cc:4279-4295): initialiseSolve (set-up, upload,
     // first force pass), then computeOneTimeStep until time >= timeMax.
     void solve();
-    void solve(int, char**) { solve(); }
+    // DPMBase::solve(argc, argv) (DPMBase.cc:797-801): command-line
+    // arguments first (readArguments / readNextArgument, DPMBase.cc:4500-4700:
+    // -name, -xmin ... -zmax, -dt, -tmax, -saveCount[Data|FStat|Stat|Ene|
+    // Restart], -dim, -gravity gx gy gz, -fileType[FStat|Restart|Data|Stat|
+    // Ene] (numbers), -restart / -r [file])
+    void solve(int argc, char** argv);
+    bool readArguments(int argc, char** argv);
     // One time step on the device (DPMBase::computeOneTimeStep,
     // DPMBase.cc:4315-4389).  Steps are issued in batches when the
     // before/after hooks are not overridden (see setHookStride).
@@ -453,6 +604,8 @@ public:
 
 protected:
     int grid_levels_ = 0;
+    // the hGrid line MercuryBase appends to restart files
+    virtual void hGridInfo(std::ostream&) {}
 
 private:
     void initialiseSolve();
@@ -472,8 +625,19 @@ private:
     dem_ctx* ctx_ = nullptr;
     bool particlesSynced_ = true, interactionsSynced_ = true;
     // the base actionsBeforeTimeStep marks itself; once it has run, no host
-    // round trip is made for the hook again
+    // round trip is made for the hook again.  The same for the after hook and
+    // continueSolve: when none of the three is overridden, solve() runs the
+    // steps up to the next output step as one device batch.
     bool beforeHookIsBase_ = false, beforeHookMaybeOverridden_ = true;
+    bool afterHookIsBase_ = false;
+    mutable bool continueIsBase_ = false;
+    // the first step at or after `nt` that writes an output file
+    unsigned nextOutputStep(unsigned nt) const;
+    double interactionEnergy(const Interaction& c, const std::vector<dem_pair_params>& tab,
+                             const std::vector<int>& index) const;
+    std::vector<int> indexOfIds() const;
+    void uploadInteractions();
+    bool restarted_ = false;
 };
 
 // MercuryBase / Mercury3D: the HGrid problem classes.  Their broad phase is
@@ -481,6 +645,11 @@ private:
 class MercuryBase : public DPMBase {
 public:
     void setHGridMaxLevels(unsigned n) { grid_levels_ = (int)n; }
+
+protected:
+    // MercuryBase::hGridInfo (MercuryBase.cc:655-679) with this engine's
+    // cell grid: its level sizes stand in for the HGrid cell sizes
+    void hGridInfo(std::ostream& os) override;
 };
 
 class Mercury3D : public MercuryBase {};

@@ -25,7 +25,12 @@
  * ObliqueImpactSelfTest.fstat (Hertz-Mindlin sliding on a wall, 10 digits),
  * free-flight closed form, two-body collision time / restitution closed forms,
  * brute-force O(N^2) pair sets, shifted-copy invariance of periodic images,
- * cylinder-vs-plane equivalence.
+ * cylinder-vs-plane equivalence.  Through the dem_* ABI it serves
+ * (dem_abi_on_oracle.c) and host-API drivers (tests/test_output_files.py):
+ * the three MindlinSelfTest .fstat files, the .ene files of
+ * TwoParticleElasticCollisionSelfTest and FreeFallSelfTest and the .data file
+ * of ExtremeOverlapUnitTest.  oracle_dem_download_interactions: its forces sum
+ * to the particle forces and its geometry follows from the radii.
  */
 #include "dem_oracle.h"
 
@@ -70,8 +75,10 @@ typedef struct {
     int32_t P, I;   /* particle indices; I is a wall index when IisWall */
     int32_t IisWall;
     uint32_t ts;
+    int64_t pass;   /* the force pass that last evaluated this interaction */
     const dem_pair_params* sp;
     V3 normal, contact, force, torque, relVel;
+    V3 tangentialForce; /* SlidingFrictionInteraction / MindlinInteraction::tangentialForce_ */
     double overlap, distance, absNormalForce, normalRelVel;
     V3 slidingSpring, rollingSpring, torsionSpring;
 } Inter;
@@ -88,6 +95,7 @@ struct oracle_ctx {
     double drag, dt, time;
     int rotation;
     int64_t nt;
+    int64_t pass; /* force passes done */
     /* particle handler: nReal real particles followed by ghosts */
     Particle* p;
     int64_t nReal, nP, capP;
@@ -334,6 +342,10 @@ static void normalForce(const oracle_ctx* c, Inter* x)
     partnerKinematics(c, x, &iv, &ip, &iw);
     x->relVel = vsub(velAtContact(P->vel, P->pos, P->omega, x->contact), velAtContact(iv, ip, iw, x->contact));
     x->normalRelVel = vdot(x->relVel, x->normal);
+    /* the friction laws set tangentialForce_ where they add it; the reference
+     * keeps the previous value through a pass with zero normal force, here
+     * (as on the device) it is zero then */
+    x->tangentialForce = v3(0.0, 0.0, 0.0);
     if (x->overlap > 0) {
         double fn;
         if (x->sp->normal_law == DEM_NORMAL_LINEAR_VISCOELASTIC) {
@@ -383,6 +395,7 @@ static void slidingFriction(oracle_ctx* c, Inter* x)
                 x->slidingSpring =
                     vdiv(vneg(vadd(ft, vmul(vt, s->sliding_dissipation))), s->sliding_stiffness);
             }
+            x->tangentialForce = ft;
         } else {
             double vtsq = vlen2(vt);
             double lim = s->sliding_mu_static * x->absNormalForce;
@@ -392,6 +405,7 @@ static void slidingFriction(oracle_ctx* c, Inter* x)
             else
                 ft = vmul(vt, -(s->sliding_mu * x->absNormalForce / sqrt(vtsq)));
             x->force = vadd(x->force, ft);
+            x->tangentialForce = ft;
         }
     }
 }
@@ -469,6 +483,7 @@ static void mindlinFriction(oracle_ctx* c, Inter* x)
             x->force = vadd(x->force, ft);
             x->slidingSpring = vneg(vdiv(ft, 2.0 / 3.0 * kt0));
         }
+        x->tangentialForce = ft;
     }
 }
 
@@ -517,6 +532,7 @@ static void computeInternalForce(oracle_ctx* c, int64_t a, int64_t b)
     PI = &c->p[iI];
     PJ = &c->p[iJ];
     Inter* x = &c->in[k];
+    x->pass = c->pass;
     double distance = sqrt(d2);
     x->normal = vdiv(branch, distance);
     x->overlap = PI->radius + PJ->radius - distance;
@@ -574,6 +590,7 @@ static void computeForcesDueToWall(oracle_ctx* c, int64_t pi, int32_t wi)
     int32_t k = getInteraction(c, (int32_t)pi, wi, 1, (uint32_t)(c->nt + 1));
     p = &c->p[pi];
     Inter* x = &c->in[k];
+    x->pass = c->pass;
     x->normal = vneg(normal);
     x->distance = distance;
     x->overlap = overlap;
@@ -765,6 +782,7 @@ static void removeGhosts(oracle_ctx* c)
 /* DPMBase::computeAllForces (DPMBase.cc:3397-3526) */
 static void computeAllForces(oracle_ctx* c)
 {
+    c->pass++;
     for (int64_t i = 0; i < c->nP; ++i) {
         c->p[i].force = v3(0.0, 0.0, 0.0);
         c->p[i].torque = v3(0.0, 0.0, 0.0);
@@ -993,6 +1011,7 @@ int oracle_dem_upload_contacts(oracle_ctx* c, int64_t n, const uint32_t* p_id, c
         int32_t x = interNew(c, (int32_t)P, (int32_t)I, isWall);
         Inter* in = &c->in[x];
         in->ts = ts[k];
+        in->pass = -1;
         in->sp = isWall ? mixed(c, c->p[P].species, c->walls[I].species) : mixed(c, c->p[P].species, c->p[I].species);
         in->slidingSpring = v3(spring[9 * k], spring[9 * k + 1], spring[9 * k + 2]);
         in->rollingSpring = v3(spring[9 * k + 3], spring[9 * k + 4], spring[9 * k + 5]);
@@ -1138,6 +1157,55 @@ int oracle_dem_download_contacts(oracle_ctx* c, int64_t cap, int64_t* nOut, uint
             s[6] = x->torsionSpring.X; s[7] = x->torsionSpring.Y; s[8] = x->torsionSpring.Z;
         }
         if (ts) ts[a] = x->ts;
+    }
+    *nOut = n;
+    free(keys);
+    return DEM_OK;
+}
+
+/* The stored interactions with the state BaseInteraction::writeToFStat
+ * (BaseInteraction.cc:487-531) writes: the force on P and the geometry of the
+ * last evaluation.  An interaction the last force pass did not evaluate
+ * (kept by the time-stamp rules only) is flagged not current and, as the
+ * engine reports it, with a zero force. */
+int oracle_dem_download_interactions(oracle_ctx* c, int64_t cap, int64_t* nOut, dem_interaction* out)
+{
+    Key* keys = (Key*)malloc(sizeof(Key) * (size_t)(c->nAlive > 0 ? c->nAlive : 1));
+    int64_t n = 0;
+    for (int64_t k = 0; k < c->nIn; ++k) {
+        const Inter* x = &c->in[k];
+        if (!x->alive) continue;
+        keys[n].p = c->p[x->P].id;
+        keys[n].i = x->IisWall ? (DEM_WALL_KEY | (uint32_t)x->I) : c->p[x->I].id;
+        keys[n].k = (int32_t)k;
+        n++;
+    }
+    qsort(keys, (size_t)n, sizeof(Key), keyCmp);
+    if (n > cap) {
+        free(keys);
+        return fail(c, DEM_ERR_CAPACITY, "download_interactions: capacity too small");
+    }
+    for (int64_t a = 0; a < n; ++a) {
+        const Inter* x = &c->in[keys[a].k];
+        dem_interaction* r = &out[a];
+        const int current = x->pass == c->pass;
+        r->p_id = keys[a].p;
+        r->i_id = keys[a].i;
+        r->time_stamp = x->ts;
+        r->current = (uint32_t)current;
+        r->force[0] = current ? x->force.X : 0.0;
+        r->force[1] = current ? x->force.Y : 0.0;
+        r->force[2] = current ? x->force.Z : 0.0;
+        r->tangential[0] = current ? x->tangentialForce.X : 0.0;
+        r->tangential[1] = current ? x->tangentialForce.Y : 0.0;
+        r->tangential[2] = current ? x->tangentialForce.Z : 0.0;
+        r->contact[0] = x->contact.X; r->contact[1] = x->contact.Y; r->contact[2] = x->contact.Z;
+        r->normal[0] = x->normal.X; r->normal[1] = x->normal.Y; r->normal[2] = x->normal.Z;
+        r->overlap = x->overlap;
+        r->distance = x->distance;
+        r->spring[0] = x->slidingSpring.X; r->spring[1] = x->slidingSpring.Y; r->spring[2] = x->slidingSpring.Z;
+        r->spring[3] = x->rollingSpring.X; r->spring[4] = x->rollingSpring.Y; r->spring[5] = x->rollingSpring.Z;
+        r->spring[6] = x->torsionSpring.X; r->spring[7] = x->torsionSpring.Y; r->spring[8] = x->torsionSpring.Z;
     }
     *nOut = n;
     free(keys);

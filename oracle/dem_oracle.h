@@ -42,6 +42,7 @@ int oracle_dem_download_particles(oracle_ctx* c, double* x, double* v, double* w
 int oracle_dem_contact_count(oracle_ctx* c, int64_t* n);
 int oracle_dem_download_contacts(oracle_ctx* c, int64_t cap, int64_t* n_out, uint32_t* p_id, uint32_t* i_id,
                                  double* spring, uint32_t* ts);
+int oracle_dem_download_interactions(oracle_ctx* c, int64_t cap, int64_t* n_out, dem_interaction* out);
 int oracle_dem_get_time(oracle_ctx* c, double* time, int64_t* nt);
 int oracle_dem_force_scale(oracle_ctx* c, double* fs, double* ts);
 int oracle_dem_cg_z(oracle_ctx* c, const dem_cg_config* cfg, double* out);

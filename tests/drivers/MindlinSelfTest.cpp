@@ -1,19 +1,14 @@
-// Drop-in check with prescribed motion: the three Hertz-Mindlin contact tests
-// of Drivers/UnitTests/MindlinSelfTest.cpp (set-ups of
-// Drivers/UnitTests/ContactLawTestHelpers.h), written against the host API
-// with the particles driven from actionsBeforeTimeStep as the reference's
-// drivers do.  After every step the contact is compared with the reference's
-// gold .fstat rows (tests/golden/mindlin_self_test_*.fstat, directory given
-// as argv[1]): contact point, overlap, tangential overlap, normal force and
-// |tangential force| to 10 digits, with an absolute floor of 1e-9 of the
-// largest overlap / normal force; rows with an overlap below 1e-12 of the
-// largest are only required to exist.
+// Drop-in check with prescribed motion and .fstat output: the three
+// Hertz-Mindlin contact tests of Drivers/UnitTests/MindlinSelfTest.cpp, with
+// the set-ups of Drivers/UnitTests/ContactLawTestHelpers.h, written against
+// the host API.  The particles are driven from actionsBeforeTimeStep, and the
+// save counts and file types are the reference's, so the program writes
+// MindlinSelfTestLoading.fstat, MindlinSelfTestNormalAndTangentialLoading.fstat
+// and MindlinSelfTestFrameIndependence.fstat (+ .data) into the working
+// directory, to be compared with the reference's gold files.
 #include <cmath>
 #include <cstdio>
-#include <fstream>
-#include <sstream>
 #include <string>
-#include <vector>
 
 #include "Mercury3D.h"
 
@@ -24,97 +19,22 @@ const double kTangentialDisplacement = 0.0025;
 const double kVelocity = 0.01;
 const double kRadius = 0.25;
 
-using Row = std::vector<double>;  // contact point (3), overlap, tangential overlap, fn, |ft|
-
-// The .fstat quantities of the single interaction on the P side
-// (BaseInteraction::writeToFStat, BaseInteraction.cc:487-531).
-bool contactRow(DPMBase& d, bool wall, Row& row)
-{
-    d.synchroniseHandlers();
-    if (d.interactionHandler.getNumberOfObjects() != 1) return false;
-    const BaseParticle* p = d.particleHandler.getObject(0);
-    Vec3D n;
-    double overlap;
-    if (wall) {
-        // InfiniteWall with normal (0,0,-1) through the origin
-        n = Vec3D(0, 0, 1);
-        overlap = p->getRadius() - p->getPosition().Z;
-    } else {
-        const BaseParticle* q = d.particleHandler.getObject(1);
-        const Vec3D b = p->getPosition() - q->getPosition();
-        const double dist = b.getLength();
-        n = b / dist;
-        overlap = p->getRadius() + q->getRadius() - dist;
-    }
-    const Vec3D c = p->getPosition() - n * (p->getRadius() - 0.5 * overlap);
-    const Vec3D s = d.interactionHandler.getObject(0).slidingSpring;
-    const double sl = s.getLength();
-    // MindlinInteraction::getTangentialOverlap (MindlinInteraction.cc:368-375)
-    const double tov = sl == 0.0 ? 0.0 : sl * Vec3D::dot(s / sl, Vec3D(1, 1, 1));
-    const double fn = Vec3D::dot(p->getForce(), n);
-    const double ft = (p->getForce() - n * fn).getLength();
-    row = {c.X, c.Y, c.Z, overlap, tov, fn, ft};
-    return true;
-}
-
-std::vector<Row> readGold(const std::string& path, bool pair)
-{
-    std::ifstream f(path);
-    std::vector<Row> rows;
-    std::string line;
-    int k = 0;
-    while (std::getline(f, line)) {
-        if (line.empty() || line[0] == '#') continue;
-        std::istringstream is(line);
-        std::vector<double> v;
-        double x;
-        while (is >> x) v.push_back(x);
-        if (v.size() < 10) continue;
-        if (!pair || k % 2 == 0) rows.push_back(Row(v.begin() + 3, v.begin() + 10));
-        ++k;
-    }
-    return rows;
-}
-
-class Recorder : public DPMBase {
-public:
-    unsigned saveCount = 1;
-    bool wall = true;
-    std::vector<Row> rows;  // rows 1.. (the state after each saved step)
-    bool missing = false;
-
-    void actionsAfterTimeStep() override
-    {
-        const double tEnd = getTimeMax();
-        if (getNumberOfTimeSteps() % saveCount == 0 || getTime() >= tEnd) {
-            Row r;
-            if (!contactRow(*this, wall, r)) missing = true;
-            rows.push_back(r);
-        }
-    }
-};
-
-HertzianViscoelasticMindlinSpecies mindlinSpecies()
-{
-    HertzianViscoelasticMindlinSpecies species;
-    species.setDensity(6. / constants::pi);
-    species.setEffectiveElasticModulusAndPoissonRatio(1e5, 0.3);
-    species.setDissipation(2.0);
-    species.setSlidingFrictionCoefficient(0.1);
-    return species;
-}
-
 // loadingTest (ContactLawTestHelpers.h:21-84)
-class LoadingTest : public Recorder {
+class LoadingTest : public DPMBase {
 public:
+    explicit LoadingTest(const HertzianViscoelasticMindlinSpecies& s) : species_(s) {}
     void setupInitialConditions() override
     {
         setTimeMax(2.0 * kDisplacement / kVelocity);
         setTimeStep(2e-3 * getTimeMax());
+        setSaveCount(1);
+        setFileType(FileType::NO_FILE);
+        fStatFile.setFileType(FileType::ONE_FILE);
         setMax({kRadius, kRadius, kRadius + kRadius});
         setMin({-kRadius, -kRadius, 0});
         setSystemDimensions(3);
         setParticleDimensions(3);
+        speciesHandler.copyAndAddObject(species_);
         SphericalParticle p;
         p.setSpecies(speciesHandler.getObject(0));
         p.setRadius(kRadius);
@@ -127,7 +47,7 @@ public:
     }
     void actionsBeforeTimeStep() override
     {
-        BaseParticle* p = particleHandler.getObject(particleHandler.getNumberOfObjects() - 1);
+        BaseParticle* p = particleHandler.getLastObject();
         p->setAngularVelocity({0, 0, 0});
         if (getTime() <= kDisplacement / kVelocity) {
             p->setVelocity({0, 0, kVelocity});
@@ -137,19 +57,27 @@ public:
             p->setPosition({0, 0, kRadius - kDisplacement - kDisplacement + kVelocity * getTime()});
         }
     }
+
+private:
+    const HertzianViscoelasticMindlinSpecies& species_;
 };
 
 // normalAndTangentialLoadingTest (ContactLawTestHelpers.h:92-162)
-class TangentialLoadingTest : public Recorder {
+class TangentialLoadingTest : public DPMBase {
 public:
+    explicit TangentialLoadingTest(const HertzianViscoelasticMindlinSpecies& s) : species_(s) {}
     void setupInitialConditions() override
     {
         setTimeMax(4.0 * kTangentialDisplacement / kVelocity);
         setTimeStep(4e-4 * getTimeMax());
+        setSaveCount(1);
+        setFileType(FileType::NO_FILE);
+        fStatFile.setFileType(FileType::ONE_FILE);
         setMax({kRadius, kRadius, kRadius + kRadius});
         setMin({-kRadius, -kRadius, 0});
         setSystemDimensions(3);
         setParticleDimensions(3);
+        speciesHandler.copyAndAddObject(species_);
         SphericalParticle p;
         p.setSpecies(speciesHandler.getObject(0));
         p.setRadius(kRadius);
@@ -162,7 +90,7 @@ public:
     }
     void actionsBeforeTimeStep() override
     {
-        BaseParticle* p = particleHandler.getObject(particleHandler.getNumberOfObjects() - 1);
+        BaseParticle* p = particleHandler.getLastObject();
         p->setAngularVelocity({0, 0, 0});
         const bool moveRight =
             static_cast<int>(getTime() / (2.0 * kTangentialDisplacement / kVelocity) + 0.5) % 2 == 0;
@@ -174,19 +102,28 @@ public:
             p->setPosition({-2 * kTangentialDisplacement + kVelocity * getTime(), 0, kRadius - kDisplacement});
         }
     }
+
+private:
+    const HertzianViscoelasticMindlinSpecies& species_;
 };
 
 // objectivenessTest (ContactLawTestHelpers.h:170-249)
-class FrameIndependenceTest : public Recorder {
+class FrameIndependenceTest : public DPMBase {
 public:
+    explicit FrameIndependenceTest(const HertzianViscoelasticMindlinSpecies& s) : species_(s) {}
     void setupInitialConditions() override
     {
         setTimeMax((kTangentialDisplacement + 0.5 * constants::pi * kRadius) / kVelocity);
         setTimeStep(1e-4 * getTimeMax());
+        setSaveCount(20);
+        setFileType(FileType::NO_FILE);
+        dataFile.setFileType(FileType::ONE_FILE);
+        fStatFile.setFileType(FileType::ONE_FILE);
         setMax(kRadius * Vec3D(2, 2, 2));
         setMin(kRadius * Vec3D(-2, -2, -2));
         setSystemDimensions(2);
         setParticleDimensions(3);
+        speciesHandler.copyAndAddObject(species_);
         SphericalParticle p;
         p.setSpecies(speciesHandler.getObject(0));
         p.setRadius(kRadius);
@@ -198,7 +135,7 @@ public:
     void actionsBeforeTimeStep() override
     {
         BaseParticle* p = particleHandler.getObject(0);
-        BaseParticle* q = particleHandler.getObject(particleHandler.getNumberOfObjects() - 1);
+        BaseParticle* q = particleHandler.getLastObject();
         if (getTime() <= kTangentialDisplacement / kVelocity) {
             p->setAngularVelocity({0, 0, 0});
             p->setVelocity({kVelocity, 0, 0});
@@ -223,72 +160,36 @@ public:
             q->setPosition(-p->getPosition());
         }
     }
-};
 
-bool compare(const char* name, Recorder& test, const std::string& goldPath, bool pair)
-{
-    auto species = mindlinSpecies();
-    test.speciesHandler.copyAndAddObject(species);
-    test.setName(name);
-    test.solve();
-    const std::vector<Row> gold = readGold(goldPath, pair);
-    // the gold holds the initial contact as its first row
-    if (test.missing || gold.size() != test.rows.size() + 1) {
-        std::printf("%s: %zu rows, gold %zu%s FAILED\n", name, test.rows.size(), gold.size(),
-                    test.missing ? ", missing contact" : "");
-        return false;
-    }
-    double length = 0, force = 0;
-    for (const Row& g : gold) {
-        length = std::fmax(length, std::fabs(g[3]));
-        force = std::fmax(force, std::fabs(g[5]));
-    }
-    double worst = 0;
-    size_t marginal = 0, at = 0;
-    for (size_t k = 0; k < test.rows.size(); ++k) {
-        const Row& g = gold[k + 1];
-        const Row& r = test.rows[k];
-        if (std::fabs(g[3]) < 1e-12 * length) {
-            ++marginal;
-            continue;
-        }
-        for (int a = 0; a < 7; ++a) {
-            const double floor = 1e-9 * (a < 5 ? length : force);
-            const double e = std::fabs(r[a] - g[a]) / (5e-9 * std::fabs(g[a]) + floor);
-            if (e > worst) {
-                worst = e;
-                at = k + 1;
-            }
-        }
-    }
-    const bool ok = worst <= 1.0 && marginal <= gold.size() / 100;
-    std::printf("%s: %zu rows, worst error %.3g of tolerance (row %zu), %zu marginal %s\n", name,
-                test.rows.size(), worst, at, marginal, ok ? "ok" : "FAILED");
-    return ok;
-}
+private:
+    const HertzianViscoelasticMindlinSpecies& species_;
+};
 
 }  // namespace
 
-int main(int argc, char** argv)
+// MindlinSelfTest.cpp main: the species and the three tests
+int main()
 {
-    const std::string dir = argc > 1 ? argv[1] : "tests/golden";
-    bool ok = true;
+    HertzianViscoelasticMindlinSpecies species;
+    species.setDensity(6. / constants::pi);
+    species.setEffectiveElasticModulusAndPoissonRatio(1e5, 0.3);
+    species.setDissipation(2.0);
+    species.setSlidingFrictionCoefficient(0.1);
     {
-        LoadingTest t;
-        ok &= compare("MindlinSelfTestLoading", t, dir + "/mindlin_self_test_loading.fstat", false);
+        LoadingTest t(species);
+        t.setName("MindlinSelfTestLoading");
+        t.solve();
     }
     {
-        TangentialLoadingTest t;
-        ok &= compare("MindlinSelfTestNormalAndTangentialLoading", t,
-                      dir + "/mindlin_self_test_normal_and_tangential_loading.fstat", false);
+        TangentialLoadingTest t(species);
+        t.setName("MindlinSelfTestNormalAndTangentialLoading");
+        t.solve();
     }
     {
-        FrameIndependenceTest t;
-        t.saveCount = 20;
-        t.wall = false;
-        ok &= compare("MindlinSelfTestFrameIndependence", t, dir + "/mindlin_self_test_frame_independence.fstat",
-                      true);
+        FrameIndependenceTest t(species);
+        t.setName("MindlinSelfTestFrameIndependence");
+        t.solve();
     }
-    std::printf("MindlinSelfTest: %s\n", ok ? "ok" : "FAILED");
-    return ok ? 0 : 1;
+    std::printf("MindlinSelfTest: done\n");
+    return 0;
 }

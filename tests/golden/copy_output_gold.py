@@ -1,0 +1,37 @@
+"""Copies gold output files of the reference's self tests verbatim into
+tests/golden/ (data: the expected .ene / .data output of a driver).  Run in a
+checkout that has the reference next to it:
+
+    python tests/golden/copy_output_gold.py /root/reference
+
+Files: the energy series of FreeFallSelfTest and
+TwoParticleElasticCollisionSelfTest (DPMBase::writeEneTimeStep,
+DPMBase.cc:2100-2126; these gold files predate the `collisions` column) and
+the XBalls data of ExtremeOverlapUnitTest (DPMBase::outputXBallsData,
+DPMBase.cc:2356-2417), and the final restart files of FreeFallSelfTest and
+TwoParticleElasticCollisionSelfTest (DPMBase::write, DPMBase.cc:3550-3690).
+"""
+import os
+import shutil
+import sys
+
+FILES = {
+    "Drivers/SelfTests/Interactions/SelfTestData/FreeFallSelfTest.ene": "free_fall_self_test.ene",
+    "Drivers/SelfTests/Interactions/SelfTestData/TwoParticleElasticCollisionSelfTest.ene":
+        "two_particle_elastic_collision_self_test.ene",
+    "Drivers/UnitTests/SelfTestData/ExtremeOverlapUnitTest.data": "extreme_overlap_unit_test.data",
+    "Drivers/SelfTests/Interactions/SelfTestData/FreeFallSelfTest.restart": "free_fall_self_test.restart",
+    "Drivers/SelfTests/Interactions/SelfTestData/TwoParticleElasticCollisionSelfTest.restart":
+        "two_particle_elastic_collision_self_test.restart",
+}
+
+
+def main(reference):
+    here = os.path.dirname(os.path.abspath(__file__))
+    for src, dst in FILES.items():
+        shutil.copyfile(os.path.join(reference, src), os.path.join(here, dst))
+        print(f"{src} -> tests/golden/{dst}")
+
+
+if __name__ == "__main__":
+    main(sys.argv[1])

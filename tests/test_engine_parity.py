@@ -311,3 +311,36 @@ def test_box_lvf_long_run_statistics():
     for col, k, name in ((1, 1, "density"), (19, 2, "contact stress zz")):
         err = np.max(np.abs(out[:, col] - cg[:, k])) / np.max(np.abs(cg[:, k]))
         assert err < 0.01, (name, err)
+
+
+@pytest.mark.parametrize("case", [CASES[0], CASES[3], CASES[4]], ids=["lvf-P", "hvm-P", "hvm-Px"])
+def test_engine_interactions_vs_oracle(case):
+    """dem_download_interactions (what the .fstat / .ene writers read) after
+    one step from the oracle's state: the same interactions, flags, springs,
+    contact forces and tangential forces, bit for bit; the geometry, which
+    both recompute, to 1e-12 of the largest radius."""
+    sc = S.mixture(n=400, seed=21, lx=7.0, ly=7.0, lz=7.0, inflate=1.12, **case)
+    ref = oracle_lib.engine()
+    S.apply(ref, sc)
+    ref.compute_forces()
+    ref.step(3)
+    # both from the oracle's state, so that only the last pass's summation
+    # order differs
+    g, o = make_pair(reseeded(sc, ref.download()), ref.contacts())
+    for e in (g, o):
+        e.set_time(3 * sc.dt, 3)
+        e.step(1)
+    a, b = g.interactions(), o.interactions()
+    assert len(a) == len(b) and len(a) > 0
+    for k in ("p_id", "i_id", "time_stamp", "current"):
+        np.testing.assert_array_equal(a[k], b[k])
+    np.testing.assert_array_equal(a["spring"], b["spring"])
+    assert np.all(b["current"] == 1)
+    # a contact's force depends on its pair's state only, which is
+    # bit-identical here
+    np.testing.assert_array_equal(a["force"], b["force"])
+    np.testing.assert_array_equal(a["tangential"], b["tangential"])
+    L = float(np.max(sc.state.radius))
+    for k in ("contact", "normal", "overlap", "distance"):
+        d = np.max(np.abs(a[k] - b[k]))
+        assert d <= 1e-12 * (1.0 if k == "normal" else L), (k, d)

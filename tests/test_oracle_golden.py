@@ -265,3 +265,41 @@ def test_cg_mass_and_virial_closed_forms():
     f0 = st["f"][0] - np.asarray(sc.gravity) / sc.state.inv_mass[0] + f1  # contact force from particle 2 on 0
     virial += f0[2] * (x[0][2] - c02[2])
     assert abs(out[:, 19].sum() * dz * area - virial) < 1e-5 * abs(virial)
+
+
+def test_oracle_interactions_sum_to_particle_forces():
+    """The interactions the output writers read are the terms of the particle
+    forces: for every particle, gravity plus the forces on it as P minus
+    those on it as I equals its total force; overlap, distance and contact
+    point follow from the radii as BaseInteraction defines them."""
+    sc = S.mixture(n=300, seed=5, laws="hvm", periodic=(True, True), walls=("bottom",))
+    o = oracle_lib.engine()
+    S.apply(o, sc)
+    o.compute_forces()
+    o.step(3)
+    it = o.interactions()
+    st = o.download()
+    assert len(it) > 20 and np.all(it["current"] == 1)
+    index = {int(i): k for k, i in enumerate(sc.state.id)}
+    # gravity on the particles that are not fixed (DPMBase::computeExternalForces)
+    inv = sc.state.inv_mass
+    m = np.divide(1.0, inv, out=np.zeros_like(inv), where=inv > 0)
+    total = np.outer(m, sc.gravity)
+    for r in it:
+        p = index[int(r["p_id"])]
+        total[p] += r["force"]
+        if not r["i_id"] & dem.DEM_WALL_KEY:
+            total[index[int(r["i_id"])]] -= r["force"]
+    fs, _ = oracle_lib.force_scale(o)
+    err = np.linalg.norm(total - st["f"], axis=1)
+    assert np.all(err <= 1e-12 * fs), float(np.max(err / fs))
+    rad = sc.state.radius
+    for r in it:
+        rp = rad[index[int(r["p_id"])]]
+        ri = 0.0 if r["i_id"] & dem.DEM_WALL_KEY else rad[index[int(r["i_id"])]]
+        assert abs(r["overlap"] - (rp + ri - r["distance"])) <= 1e-14 * rp
+        x = st["x"][index[int(r["p_id"])]]
+        c = x - r["normal"] * (rp - 0.5 * r["overlap"])
+        assert np.max(np.abs(c - r["contact"])) <= 1e-12 * rp
+        fn = float(np.dot(r["force"], r["normal"]))
+        assert np.linalg.norm(r["force"] - fn * r["normal"] - r["tangential"]) <= 1e-12 * np.linalg.norm(r["force"])
