@@ -283,12 +283,11 @@ void DPMBase::write(std::ostream& os)
        << " zMin " << getZMin() << " zMax " << getZMax() << '\n';
     os << "timeStep " << getTimeStep() << " time " << getTime() << " ntimeSteps " << nt_ << " timeMax "
        << getTimeMax() << '\n';
-    // RNG::write (RNG.cc:79-90) of a freshly constructed generator: this API
-    // draws no random numbers
     os << "systemDimensions " << getSystemDimensions() << " particleDimensions "
        << particleHandler.getParticleDimensions() << " gravity " << getGravity() << " backgroundDrag "
-       << getBackgroundDrag() << " writeVTK 0 NO_FILE NO_FILE 0 0 0 0 0 random  0 1103515245 12345 1073741824 607 273 "
-       << "784607267\n";
+       << getBackgroundDrag() << " writeVTK 0 NO_FILE NO_FILE 0 0 0 0 0 random ";
+    random.write(os);
+    os << '\n';
     // SpeciesHandler::write (SpeciesHandler.cc:1016-1029): each species,
     // then its mixed species with every species of lower index
     const unsigned ns = speciesHandler.getNumberOfObjects();
@@ -418,6 +417,11 @@ void DPMBase::read(std::istream& is)
         ls >> k >> sd >> k >> pd >> k >> gravity_.X >> gravity_.Y >> gravity_.Z >> k >> drag_;
         setSystemDimensions(sd);
         setParticleDimensions(pd);
+        while (ls >> k)
+            if (k == "random") {
+                random.read(ls);
+                break;
+            }
     }
     // species, each followed by its mixed species; the mixed parameters are
     // applied once every species exists

@@ -2,6 +2,8 @@
 #include "Mercury3D.h"
 
 #include <algorithm>
+#include <cmath>
+#include <limits>
 #include <cstdio>
 #include <cstring>
 
@@ -152,28 +154,134 @@ bool SpeciesHandler::useAngularDOFs() const
     return false;
 }
 
+void ParticleSpecies::computeMass(BaseParticle* q) const
+{
+    if (q->fixed_) {
+        q->invMass_ = 0.0;
+        q->invInertia_ = 0.0;
+        return;
+    }
+    // BaseParticle::computeMass (BaseParticle.cc:874-883)
+    const double r = q->radius_, rho = getDensity();
+    if (q->dims_ == 3) {
+        q->invMass_ = 1.0 / (4.0 / 3.0 * constants::pi * r * r * r * rho);
+        q->invInertia_ = 1.0 / (.4 * (1.0 / q->invMass_) * mathsFunc::square(r));
+    } else {
+        q->invMass_ = 1.0 / (constants::pi * r * r * rho);
+        q->invInertia_ = 1.0 / (.5 * (1.0 / q->invMass_) * mathsFunc::square(r));
+    }
+}
+
 BaseParticle* ParticleHandler::copyAndAddObject(const BaseParticle& p)
 {
     auto q = std::make_unique<BaseParticle>(p);
     q->id_ = nextId_++;
     q->index_ = getNumberOfObjects();
+    q->dims_ = dims_;
     if (!q->species_) throw std::invalid_argument("particle has no species");
-    if (q->fixed_) {
-        q->invMass_ = 0.0;
-        q->invInertia_ = 0.0;
-    } else {
-        // BaseParticle::computeMass (BaseParticle.cc:874-883)
-        const double r = q->radius_, rho = q->species_->getDensity();
-        if (dims_ == 3) {
-            q->invMass_ = 1.0 / (4.0 / 3.0 * constants::pi * r * r * r * rho);
-            q->invInertia_ = 1.0 / (.4 * (1.0 / q->invMass_) * mathsFunc::square(r));
-        } else {
-            q->invMass_ = 1.0 / (constants::pi * r * r * rho);
-            q->invInertia_ = 1.0 / (.5 * (1.0 / q->invMass_) * mathsFunc::square(r));
-        }
-    }
+    q->species_->computeMass(q.get());
     objects_.push_back(std::move(q));
     return objects_.back().get();
+}
+
+// ------------------------------------------------------------------ RNG
+
+RNG::RNG() { seedLaggedFibonacci(); }
+
+void RNG::setRandomSeed(unsigned long seed)
+{
+    seed_ = seed;
+    seedLaggedFibonacci();
+}
+
+void RNG::setLinearCongruentialGeneratorParmeters(unsigned a, unsigned c, unsigned m)
+{
+    a_ = a;
+    c_ = c;
+    m_ = m;
+}
+
+// RNG::getRandomNumberFromLinearCongruentialGenerator (RNG.cc:234-243)
+double RNG::fromLinearCongruential(double min, double max)
+{
+    seed_ = (a_ * seed_ + c_) % m_;
+    const double range = max - min;
+    return min + range * seed_ / (static_cast<double>(m_) + 1.0);
+}
+
+// RNG::seedLaggedFibonacciGenerator (RNG.cc:250-256)
+void RNG::seedLaggedFibonacci()
+{
+    lags_.assign(p_, 0.0);
+    head_ = 0;
+    for (unsigned long k = 0; k < p_; ++k) lags_[k] = fromLinearCongruential(0, 1.0);
+}
+
+// RNG::getRandomNumberFromLaggedFibonacciGenerator (RNG.cc:265-280): the
+// new value is (oldest + the (p-q)-th oldest) mod 1, and replaces the oldest
+// (a ring buffer instead of the reference's erase/emplace_back)
+double RNG::fromLaggedFibonacci(double min, double max)
+{
+    const double v = std::fmod(lags_[head_] + lags_[(head_ + p_ - q_) % p_], 1.0);
+    lags_[head_] = v;
+    head_ = (head_ + 1) % p_;
+    const double range = max - min;
+    return min + range * v;
+}
+
+double RNG::getRandomNumber(double min, double max)
+{
+    return type_ == Type::LINEAR_CONGRUENTIAL_GENERATOR ? fromLinearCongruential(min, max)
+                                                        : fromLaggedFibonacci(min, max);
+}
+
+double RNG::getNormalVariate()
+{
+    if (haveSaved_) {
+        haveSaved_ = false;
+        return saved_;
+    }
+    const double epsilon = std::numeric_limits<double>::min();
+    double radius, theta;
+    do {
+        radius = getRandomNumber(0, 1);
+        theta = getRandomNumber(0, 2 * constants::pi);
+    } while (radius <= epsilon);
+    saved_ = std::sqrt(-2.0 * std::log(radius)) * std::sin(theta);
+    haveSaved_ = true;
+    return std::sqrt(-2.0 * std::log(radius)) * std::cos(theta);
+}
+
+double RNG::getNormalVariate(double mean, double stdev)
+{
+    if (stdev == 0) return mean;
+    if (stdev < 0) throw std::invalid_argument("RNG::getNormalVariate: negative stdev");
+    return getNormalVariate() * stdev + mean;
+}
+
+unsigned RNG::getPoissonVariate(double lambda)
+{
+    unsigned k = 0;
+    double u;
+    do {
+        k++;
+        u = getRandomNumber(0, 1);
+    } while (u > std::exp(-lambda));
+    return k - 1;
+}
+
+void RNG::write(std::ostream& os) const
+{
+    os << ' ' << static_cast<unsigned>(type_) << ' ' << a_ << ' ' << c_ << ' ' << m_ << ' ' << p_ << ' ' << q_ << ' '
+       << seed_;
+}
+
+void RNG::read(std::istream& is)
+{
+    unsigned type = 0;
+    is >> type >> a_ >> c_ >> m_ >> p_ >> q_ >> seed_;
+    type_ = static_cast<Type>(type);
+    seedLaggedFibonacci();
 }
 
 // Quaternion::setOrientationViaNormal (Quaternion.cc:543-566) followed by

@@ -8,8 +8,9 @@
 // LinearViscoelasticSlidingFriction, LinearViscoelasticFriction (rolling and
 // torsion), HertzianViscoelasticMindlin.  Walls: InfiniteWall,
 // CylindricalWall.  Boundaries: axis-aligned PeriodicBoundary.  Particles:
-// SphericalParticle.  Output: the .data, .fstat and .ene files of
-// DPMBase::writeOutputFiles; restart files and CG are not written here.
+// SphericalParticle.  Output: the .data, .fstat, .ene and .restart files of
+// DPMBase::writeOutputFiles, and restart files read back; CG is a device call
+// (dem_cg_z), not a handler here.
 //
 // The handlers are the set-up and output source of truth, as in the
 // reference; inside DPMBase::solve() the device state is authoritative and is
@@ -62,6 +63,55 @@ struct Vec3D {
     double getLength() const { return std::sqrt(getLengthSquared()); }
     // Vec3D::isEqualTo (Vector.cc): squared distance within tol^2
     bool isEqualTo(const Vec3D& o, double tol) const { return (*this - o).getLengthSquared() <= tol * tol; }
+};
+
+// --------------------------------------------------------- Quaternion ---
+// The four components (q0, q1, q2, q3) of an orientation (Quaternion.h).
+struct Quaternion {
+    double q0 = 1, q1 = 0, q2 = 0, q3 = 0;
+    Quaternion() = default;
+    Quaternion(double a, double b, double c, double d) : q0(a), q1(b), q2(c), q3(d) {}
+};
+
+// ------------------------------------------------------------------ RNG ---
+// RNG (Kernel/Math/RNG.cc): a linear congruential generator (a, c, m) that
+// seeds a lagged Fibonacci generator of lags (p, q); the same sequence as the
+// reference for the same seed.
+class RNG {
+public:
+    // RNGType (RNG.h:38-42)
+    enum class Type : unsigned char { LAGGED_FIBONACCI_GENERATOR = 0, LINEAR_CONGRUENTIAL_GENERATOR = 1 };
+    RNG();
+    // RNG::setRandomSeed (RNG.cc:53-57)
+    void setRandomSeed(unsigned long seed);
+    // a uniform number in [min, max) (RNG.cc:148-156)
+    double getRandomNumber(double min, double max);
+    double getRandomNumber() { return getRandomNumber(0, 1); }
+    double operator()(double min, double max) { return getRandomNumber(min, max); }
+    double operator()() { return getRandomNumber(0, 1); }
+    // Box-Muller normal variates (RNG.cc:163-192) and Knuth's Poisson
+    // variates (RNG.cc:212-228)
+    double getNormalVariate();
+    double getNormalVariate(double mean, double stdev);
+    unsigned getPoissonVariate(double lambda);
+    void setLinearCongruentialGeneratorParmeters(unsigned a, unsigned c, unsigned m);
+    void setRandomNumberGenerator(Type t) { type_ = t; }
+    // RNG::write / RNG::read (RNG.cc:59-90): the generator parameters and
+    // the linear congruential state; reading re-seeds the Fibonacci lags
+    void write(std::ostream& os) const;
+    void read(std::istream& is);
+
+private:
+    double fromLinearCongruential(double min, double max);
+    double fromLaggedFibonacci(double min, double max);
+    void seedLaggedFibonacci();
+    unsigned long seed_ = 0, a_ = 1103515245, c_ = 12345, m_ = 1024 * 1024 * 1024;
+    unsigned long p_ = 607, q_ = 273;
+    Type type_ = Type::LAGGED_FIBONACCI_GENERATOR;
+    std::vector<double> lags_;
+    size_t head_ = 0;  // lags_[head_] is the oldest value
+    bool haveSaved_ = false;
+    double saved_ = 0;
 };
 
 // --------------------------------------------------------------- Files ---
@@ -118,9 +168,14 @@ class SpeciesHandler;
 
 // A ParticleSpecies with its normal and friction force laws
 // (Species<NormalForceSpecies, FrictionForceSpecies> in the reference).
+class BaseParticle;
+
 class ParticleSpecies {
 public:
     virtual ~ParticleSpecies() = default;
+    // BaseParticle::computeMass (BaseParticle.cc:874-883) with this
+    // species' density, for the particle's dimensions
+    void computeMass(BaseParticle* p) const;
     void setDensity(double rho) { density_ = rho; }
     double getDensity() const { return density_; }
     unsigned getIndex() const { return index_; }
@@ -239,6 +294,13 @@ public:
     {
         for (int a = 0; a < 4; ++a) q_[a] = q[a];
     }
+    void setOrientation(const Quaternion& q)
+    {
+        q_[0] = q.q0;
+        q_[1] = q.q1;
+        q_[2] = q.q2;
+        q_[3] = q.q3;
+    }
     void setRadius(double r) { radius_ = r; }
     void setSpecies(const ParticleSpecies* s) { species_ = s; }
     // BaseParticle::fixParticle (BaseParticle.cc:171-183)
@@ -268,7 +330,9 @@ protected:
     double invMass_ = 0.0, invInertia_ = 0.0;
     bool fixed_ = false;
     unsigned id_ = 0, index_ = 0;
+    unsigned dims_ = 3;  // the particle dimensions its mass was computed for
     const ParticleSpecies* species_ = nullptr;
+    friend class ParticleSpecies;
 };
 
 class SphericalParticle : public BaseParticle {};
@@ -476,6 +540,8 @@ public:
     DPMBase(const DPMBase&) = delete;
     DPMBase& operator=(const DPMBase&) = delete;
 
+    // DPMBase::random (DPMBase.h): the problem's random number generator
+    RNG random;
     SpeciesHandler speciesHandler;
     ParticleHandler particleHandler;
     WallHandler wallHandler;

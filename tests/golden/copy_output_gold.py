@@ -7,7 +7,8 @@ checkout that has the reference next to it:
 Files: the energy series of FreeFallSelfTest and
 TwoParticleElasticCollisionSelfTest (DPMBase::writeEneTimeStep,
 DPMBase.cc:2100-2126; these gold files predate the `collisions` column) and
-the XBalls data of ExtremeOverlapUnitTest (DPMBase::outputXBallsData,
+the energies and XBalls data of ForceSelfTest (cases drawn from the
+problem's random number generator), the XBalls data of ExtremeOverlapUnitTest (DPMBase::outputXBallsData,
 DPMBase.cc:2356-2417), and the final restart files of FreeFallSelfTest and
 TwoParticleElasticCollisionSelfTest (DPMBase::write, DPMBase.cc:3550-3690).
 """
@@ -24,6 +25,12 @@ FILES = {
     "Drivers/SelfTests/Interactions/SelfTestData/TwoParticleElasticCollisionSelfTest.restart":
         "two_particle_elastic_collision_self_test.restart",
 }
+# ForceSelfTest: the energies of all ten cases, the XBalls data of a wall
+# case (2) and a pair case (7)
+for k in range(10):
+    FILES[f"Drivers/SelfTests/Interactions/SelfTestData/ForceSelfTest{k}.ene"] = f"force_self_test_{k}.ene"
+for k in (2, 7):
+    FILES[f"Drivers/SelfTests/Interactions/SelfTestData/ForceSelfTest{k}.data"] = f"force_self_test_{k}.data"
 
 
 def main(reference):

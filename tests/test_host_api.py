@@ -93,3 +93,42 @@ def test_driver_reproduces_reference_answer(driver, tmp_path):
     print(r.stdout, r.stderr, file=sys.stderr)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "ok" in r.stdout
+
+
+RNG_SRC = r'''
+#include <cstdio>
+#include <sstream>
+#include "Mercury3D.h"
+int main() {
+    RNG fresh;
+    std::ostringstream s;
+    fresh.write(s);
+    RNG seeded;
+    seeded.setRandomSeed(5);
+    double a = seeded.getRandomNumber(0.0005, 0.001), b = seeded.getRandomNumber(0.0005, 0.001);
+    std::printf("%s|%.17g %.17g\n", s.str().c_str(), a, b);
+    return 0;
+}
+'''
+
+
+def test_rng_matches_reference_state(tmp_path):
+    """RNG (Kernel/Math/RNG.cc): a fresh generator writes the state the
+    reference's restart files hold (seed 0 after seeding the 607 Fibonacci
+    lags: 784607267, e.g. tests/golden/free_fall_self_test.restart); the
+    first radii of ForceSelfTest's pair case (seed 5) are the ones in the
+    gold ForceSelfTest7.data."""
+    src = tmp_path / "rng.cpp"
+    src.write_text(RNG_SRC)
+    exe = tmp_path / "rng"
+    subprocess.run(["g++", "-std=c++17", "-O2", "-ffp-contract=off", "-I" + HOST, "-o", str(exe), str(src),
+                    *[os.path.join(HOST, f) for f in oc.HOST_SOURCES], "-L" + LIB, "-lmercurydpm_dem",
+                    "-Wl,-rpath," + LIB], check=True)
+    state, radii = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split("|")
+    gold = open(os.path.join(GOLDEN, "free_fall_self_test.restart")).read()
+    assert gold.split(" random ")[1].split()[:7] == state.split()
+    # the radii column of the first frame of the pair case's .data file
+    frame = [ln.split() for ln in open(os.path.join(GOLDEN, "force_self_test_7.data"))][1:3]
+    want = [float(r[4]) for r in frame]
+    got = [float(x) for x in radii.split()]
+    assert [float(f"{g:.10g}") for g in got] == want

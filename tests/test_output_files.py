@@ -43,6 +43,10 @@ CASES = {
          "restart:1e-14")]),
     "ExtremeOverlapUnitTest": ("ExtremeOverlapUnitTest", [],
                                [("ExtremeOverlapUnitTest.data", "extreme_overlap_unit_test.data", "data")]),
+    # ten collisions whose parameters come from the random number generator
+    "ForceSelfTest": ("ForceSelfTest", [],
+                      [(f"ForceSelfTest{k}.ene", f"force_self_test_{k}.ene", "ene") for k in range(10)] +
+                      [(f"ForceSelfTest{k}.data", f"force_self_test_{k}.data", "data") for k in (2, 7)]),
     # stop at t = 1.05, restart from the .restart file (also with numbered
     # .data files), compare with the uninterrupted run (self-checking)
     "FreeFallRestartUnitTest": ("FreeFallRestartUnitTest", [], []),
