@@ -60,7 +60,8 @@ enum Flag {
     F_ORPH_COUNT1 = 10,
     F_ORPH_OVERFLOW = 11,
     F_DONE_BLOCKS = 12, // blocks of the last rebuild kernel that have finished
-    F_NUM = 13
+    F_NONFINITE = 13,   // a particle position is not finite: the run diverged
+    F_NUM = 14
 };
 
 // Integration fused into the force kernel: none (a force pass only), the
@@ -263,7 +264,23 @@ static_assert(kRebuildBlocks % kXcdQuantum == 0, "rebuild grid must suit the XCD
 constexpr uint32_t kWriterBit = 0x80000000u;
 __host__ __device__ inline size_t lslot(int k, int stride, int i) { return (size_t)i * stride + k; }
 
-__device__ inline bool rebuild_pending(const int* flags) { return __atomic_load_n(&flags[F_REBUILD], __ATOMIC_RELAXED) != 0; }
+// A rebuild is pending unless the state has diverged (a non-finite
+// position), in which case the rebuild is skipped and the host reports the
+// error: no grid work is done on such a state.
+__device__ inline bool rebuild_pending(const int* flags)
+{
+    return __atomic_load_n(&flags[F_REBUILD], __ATOMIC_RELAXED) != 0 &&
+           __atomic_load_n(&flags[F_NONFINITE], __ATOMIC_RELAXED) == 0;
+}
+
+// floor(v) as a cell coordinate, clamped to +-2^29 (NaN: 0), so that cell
+// ranges stay small and loops over them end for any position.
+__device__ inline int floor_to_cell(double v)
+{
+    const double f = floor(v);
+    if (!(f == f)) return 0;
+    return f > 536870912.0 ? 536870912 : (f < -536870912.0 ? -536870912 : (int)f);
+}
 
 // ------------------------------------------------------------------------
 // Velocity Verlet for one particle, applying in this order
@@ -318,7 +335,8 @@ __device__ inline bool integrate(const Params& p, bool after, bool before, doubl
         ds.x += step.x;
         ds.y += step.y;
         ds.z += step.z;
-        flag = ds.x * ds.x + ds.y * ds.y + ds.z * ds.z > p.thr2;
+        // (a non-finite displacement asks for a rebuild too, which detects it)
+        flag = !(ds.x * ds.x + ds.y * ds.y + ds.z * ds.z <= p.thr2);
     }
     vl = make_double4(v.x, v.y, v.z, inv_mass);
     om = make_double4(w.x, w.y, w.z, om.w);
@@ -366,12 +384,12 @@ __device__ inline int4 cell_of(const Params& p, int lvl, V3 x)
         const int pb = p.axis_pb[a];
         if (pb >= 0) {
             const int nc = p.ncell[lvl][a];
-            int t = (int)floor((xa - p.pleft[pb]) / p.hc[lvl][a]);
+            int t = floor_to_cell((xa - p.pleft[pb]) / p.hc[lvl][a]);
             t %= nc;
             if (t < 0) t += nc;
             c[a] = t;
         } else {
-            c[a] = (int)floor(xa / p.h[lvl]);
+            c[a] = floor_to_cell(xa / p.h[lvl]);
         }
     }
     return make_int4(lvl, c[0], c[1], c[2]);
@@ -426,7 +444,9 @@ __global__ void __launch_bounds__(kBlock) k_rb_begin(Dev<NS> d, Params p, int cu
         // at zero by the previous rebuild
         {
             const int lvl = (int)(d.spc[i] >> 16);
-            int4 c = cell_of(p, lvl, xyz(d.posr[i]));
+            const double4 x4 = d.posr[i];
+            if (!isfinite(x4.x) || !isfinite(x4.y) || !isfinite(x4.z)) atomicOr(&d.flags[F_NONFINITE], 1);
+            int4 c = cell_of(p, lvl, xyz(x4));
             const uint32_t key = cell_key(p, lvl, c.y, c.z, c.w);
             d.key[i] = key;
             if (d.vel[i].w == 0.0) c.x |= 0x100; // fixed particle
@@ -623,16 +643,16 @@ __global__ void __launch_bounds__(kBlock) k_build_lists(Dev<NS> d, Params p)
                 const int pb = p.axis_pb[a];
                 if (pb >= 0) {
                     nc[a] = p.ncell[m][a];
-                    lo[a] = (int)floor((xa - R - p.pleft[pb]) / p.hc[m][a]);
-                    hi[a] = (int)floor((xa + R - p.pleft[pb]) / p.hc[m][a]);
+                    lo[a] = floor_to_cell((xa - R - p.pleft[pb]) / p.hc[m][a]);
+                    hi[a] = floor_to_cell((xa + R - p.pleft[pb]) / p.hc[m][a]);
                     if (hi[a] - lo[a] + 1 >= nc[a]) {
                         lo[a] = 0;
                         hi[a] = nc[a] - 1;
                     }
                 } else {
                     nc[a] = 0;
-                    lo[a] = (int)floor((xa - R) / p.h[m]);
-                    hi[a] = (int)floor((xa + R) / p.h[m]);
+                    lo[a] = floor_to_cell((xa - R) / p.h[m]);
+                    hi[a] = floor_to_cell((xa + R) / p.h[m]);
                 }
             }
             // Buckets of one level are numbered with the fastest axis
@@ -1036,6 +1056,9 @@ __device__ inline void process_batch(Dev<NS>& d, const Params& p, int cur, Owner
 template <int NS, int LAW>
 __global__ void __launch_bounds__(kBlock) FORCE_WAVES_ATTR k_force(Dev<NS> d, Params p, int cur)
 {
+    // a diverged state (non-finite position, found by the rebuild) gets no
+    // force pass: its lists were not rebuilt
+    if (__atomic_load_n(&d.flags[F_NONFINITE], __ATOMIC_RELAXED)) return;
     __shared__ OwnerLds lds[kWaves];
     const int lane = threadIdx.x & 63;
     OwnerLds& L = lds[threadIdx.x >> 6];
@@ -1721,7 +1744,7 @@ __global__ void __launch_bounds__(kBlock) k_set_kinematics(Dev<NS> d, Params p, 
         ds.y += nx.y - pr.y;
         ds.z += nx.z - pr.z;
         d.disp[i] = ds;
-        if (ds.x * ds.x + ds.y * ds.y + ds.z * ds.z > p.thr2) atomicOr(&d.flags[F_REBUILD], 1);
+        if (!(ds.x * ds.x + ds.y * ds.y + ds.z * ds.z <= p.thr2)) atomicOr(&d.flags[F_REBUILD], 1);
         d.posr[i] = make_double4(nx.x, nx.y, nx.z, pr.w);
     }
     if (v) d.vel[i] = make_double4(v[3 * o], v[3 * o + 1], v[3 * o + 2], d.vel[i].w);
@@ -1778,6 +1801,7 @@ struct dem_ctx {
     int64_t n = 0;
     bool uploaded = false;
     bool in_step = false; // between dem_step_begin and dem_step_end
+    bool diverged = false; // a non-finite position was seen; cleared by an upload
     int64_t fc_pass = 0;  // force passes that stored their contact forces
     int cur = 0;
     Params prm{};
@@ -1958,6 +1982,11 @@ int check_flags(dem_ctx* ctx)
     if (f[F_HASH_OVERFLOW]) return set_err(ctx, DEM_ERR_CAPACITY, "contact hash table overflow");
     if (f[F_ORPH_OVERFLOW]) return set_err(ctx, DEM_ERR_CAPACITY, "orphaned-interaction buffer overflow");
     if (f[F_LIST_ASYM]) return set_err(ctx, DEM_ERR_STATE, "internal error: asymmetric Verlet list");
+    if (f[F_NONFINITE]) {
+        ctx->diverged = true;
+        return set_err(ctx, DEM_ERR_STATE,
+                       "a particle position is not finite: the simulation diverged (upload a new state)");
+    }
     return DEM_OK;
 }
 
@@ -2078,6 +2107,8 @@ int ready(dem_ctx* ctx)
     if (!ctx) return DEM_ERR_ARG;
     if (!ctx->uploaded) return set_err(ctx, DEM_ERR_STATE, "no particles uploaded");
     if (ctx->dt <= 0.0) return set_err(ctx, DEM_ERR_STATE, "time step not set");
+    if (ctx->diverged)
+        return set_err(ctx, DEM_ERR_STATE, "the simulation diverged (a non-finite position); upload a new state");
     return DEM_OK;
 }
 
@@ -2229,6 +2260,8 @@ int dem_set_periodic(dem_ctx* ctx, int32_t n, const dem_periodic* b)
 int dem_set_gravity(dem_ctx* ctx, const double g[3])
 {
     if (!ctx || !g) return DEM_ERR_ARG;
+    for (int a = 0; a < 3; ++a)
+        if (!std::isfinite(g[a])) return set_err(ctx, DEM_ERR_ARG, "gravity must be finite");
     for (int a = 0; a < 3; ++a) ctx->g[a] = g[a];
     return DEM_OK;
 }
@@ -2242,7 +2275,7 @@ int dem_set_background_drag(dem_ctx* ctx, double drag)
 
 int dem_set_timestep(dem_ctx* ctx, double dt)
 {
-    if (!ctx || !(dt > 0)) return set_err(ctx, DEM_ERR_ARG, "time step must be positive");
+    if (!ctx || !(dt > 0) || !std::isfinite(dt)) return set_err(ctx, DEM_ERR_ARG, "time step must be positive and finite");
     ctx->dt = dt;
     return DEM_OK;
 }
@@ -2276,13 +2309,15 @@ int dem_upload_particles(dem_ctx* ctx, int64_t n, const uint32_t* id, const int3
     double xmin[3] = {1e300, 1e300, 1e300}, xmax[3] = {-1e300, -1e300, -1e300};
     for (int64_t i = 0; i < n; ++i) {
         for (int a = 0; a < 3; ++a) {
+            if (!std::isfinite(x[3 * i + a]) || !std::isfinite(v[3 * i + a]) || (w && !std::isfinite(w[3 * i + a])))
+                return set_err(ctx, DEM_ERR_ARG, "positions and velocities must be finite");
             xmin[a] = std::min(xmin[a], x[3 * i + a]);
             xmax[a] = std::max(xmax[a], x[3 * i + a]);
         }
         if (i > 0 && id[i] <= id[i - 1]) return set_err(ctx, DEM_ERR_ARG, "ids must increase with the handler index");
         if (id[i] >= DEM_WALL_KEY) return set_err(ctx, DEM_ERR_ARG, "particle ids must be < 2^31");
         if (species[i] < 0 || species[i] >= ctx->ns) return set_err(ctx, DEM_ERR_ARG, "species index out of range");
-        if (!(radius[i] > 0)) return set_err(ctx, DEM_ERR_ARG, "radius must be positive");
+        if (!(radius[i] > 0) || !std::isfinite(radius[i])) return set_err(ctx, DEM_ERR_ARG, "radius must be positive and finite");
         const double* I = inv_inertia + 6 * i;
         if (I[1] != 0.0 || I[2] != 0.0 || I[4] != 0.0 || I[0] != I[3] || I[0] != I[5])
             return set_err(ctx, DEM_ERR_UNSUPPORTED, "inverse inertia must be isotropic (spheres)");
@@ -2461,6 +2496,7 @@ int dem_upload_particles(dem_ctx* ctx, int64_t n, const uint32_t* id, const int3
     HIPCHK(hipMemcpy(dspc, spc.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice));
     int flags[F_NUM] = {0};
     flags[F_REBUILD] = 1;
+    ctx->diverged = false;
     HIPCHK(hipMemcpy(ctx->d_flags, flags, sizeof flags, hipMemcpyHostToDevice));
     if (ctx->ns_springs == 9) {
         hipLaunchKernelGGL(k_zero_masks<9>, dim3(grid_for(n)), dim3(kBlock), 0, ctx->stream, ctx->d9, p);
@@ -2591,6 +2627,12 @@ int dem_set_kinematics(dem_ctx* ctx, int64_t n, const double* x, const double* v
     if (!ctx) return DEM_ERR_ARG;
     if (!ctx->uploaded) return set_err(ctx, DEM_ERR_STATE, "no particles uploaded");
     if (n != ctx->n) return set_err(ctx, DEM_ERR_ARG, "dem_set_kinematics: n differs from the uploaded count");
+    for (const double* a : {x, v, w, q}) {
+        if (!a) continue;
+        const int64_t m = (a == q ? 4 : 3) * n;
+        for (int64_t k = 0; k < m; ++k)
+            if (!std::isfinite(a[k])) return set_err(ctx, DEM_ERR_ARG, "dem_set_kinematics: values must be finite");
+    }
     HIPCHK(hipSetDevice(ctx->device));
     refresh_params(ctx);
     return ctx->ns_springs == 9 ? run_set_kinematics<9>(ctx, x, v, w, q) : run_set_kinematics<3>(ctx, x, v, w, q);

@@ -344,3 +344,27 @@ def test_engine_interactions_vs_oracle(case):
     for k in ("contact", "normal", "overlap", "distance"):
         d = np.max(np.abs(a[k] - b[k]))
         assert d <= 1e-12 * (1.0 if k == "normal" else L), (k, d)
+
+
+def test_engine_rejects_non_finite_input():
+    """Non-finite positions, velocities, gravity or prescribed kinematics are
+    refused at the ABI (DEM_ERR_ARG) before anything reaches the device; a
+    valid state uploaded afterwards steps as usual."""
+    sc = S.mixture(n=100, seed=3, laws="lvf", periodic=(True, True), walls=("bottom",))
+    bad = reseeded(sc, dict(x=sc.state.x.copy(), v=sc.state.v, w=sc.state.w, q=sc.state.q, f=None, t=None))
+    bad.state.x[7, 1] = np.nan
+    g = dem.Engine()
+    with pytest.raises(dem.DemError, match="finite"):
+        S.apply(g, bad)
+    S.apply(g, sc)
+    with pytest.raises(dem.DemError, match="finite"):
+        g.set_gravity((0.0, 0.0, np.inf))
+    g.compute_forces()
+    x = g.download()["x"]
+    x[3, 0] = np.nan
+    g.step_begin()
+    with pytest.raises(dem.DemError, match="finite"):
+        g.set_kinematics(x=x)
+    g.step_end()
+    g.step(5)
+    assert np.all(np.isfinite(g.download()["x"]))
