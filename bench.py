@@ -57,6 +57,10 @@ def parse():
     ap.add_argument("--cpu-steps", type=int, default=3, help="oracle steps in the CPU baseline (0: skip)")
     ap.add_argument("--skin", type=float, default=0.0)
     ap.add_argument("--seed", type=int, default=3)
+    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64",
+                    help="contact arithmetic: fp64 (the metric) or mixed (FP32 contact geometry and forces)")
+    ap.add_argument("--mixed-steps", type=int, default=500,
+                    help="after the FP64 timing, time this many steps of the same state in mixed precision (0: skip)")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only copy this many bytes device-to-device (calibrates HBM counters) and exit")
     return ap.parse_args()
@@ -88,7 +92,8 @@ def main():
     t_gen = time.time()
     sc = S.chute_hvm(n_flow=a.n, seed=a.seed)
     n_total = sc.state.n
-    eng = dem.Engine(device=0, skin=a.skin)
+    prec = dem.PRECISION_MIXED if a.precision == "mixed" else dem.PRECISION_FP64
+    eng = dem.Engine(device=0, skin=a.skin, precision=prec)
     chute_g = sc.gravity
     S.apply(eng, sc)
     eng.compute_forces()
@@ -101,7 +106,7 @@ def main():
                         dem.State(id=sc.state.id, species=sc.state.species, radius=sc.state.radius,
                                   inv_mass=sc.state.inv_mass, inv_inertia=sc.state.inv_inertia,
                                   x=z["x"], v=z["v"], w=z["w"], q=z["q"], f=z["f"], t=z["t"]))
-        eng = dem.Engine(device=0, skin=a.skin)
+        eng = dem.Engine(device=0, skin=a.skin, precision=prec)
         S.apply(eng, s2, dem.Contacts(z["p_id"], z["i_id"], z["spring"], z["ts"]))
         eng.set_time(float(z["time"]), int(z["nt"]))
         a.settle_steps = 0
@@ -163,6 +168,37 @@ def main():
           f"per step; {st.n_rebuilds} rebuilds total; C/N {cbar:.3f}; max list {st.max_list_length}",
           file=sys.stderr, flush=True)
 
+    # the mixed-precision variant on the same state: a second engine, started
+    # from the FP64 run's particles and stored contacts (not the metric)
+    mixed = None
+    if a.mixed_steps > 0 and a.precision == "fp64":
+        d = eng.download()
+        ct = eng.contacts()
+        s3 = S.Scenario(sc.name, sc.species, sc.walls, sc.periodic, chute_g, sc.dt,
+                        dem.State(id=sc.state.id, species=sc.state.species, radius=sc.state.radius,
+                                  inv_mass=sc.state.inv_mass, inv_inertia=sc.state.inv_inertia,
+                                  x=d["x"], v=d["v"], w=d["w"], q=d["q"], f=d["f"], t=d["t"]))
+        em = dem.Engine(device=0, skin=a.skin, precision=dem.PRECISION_MIXED)
+        S.apply(em, s3, ct)
+        em.set_time(st.time, int(st.n_steps))
+        em.compute_forces()
+        em.step(200)
+        torch.cuda.synchronize()
+        em.set_profiling(True)
+        tm0 = time.perf_counter()
+        em.step(a.mixed_steps)
+        torch.cuda.synchronize()
+        tm1 = time.perf_counter()
+        sm = em.stats()
+        mixed = {"value": n_flow * a.mixed_steps / (tm1 - tm0), "unit": "particle-steps/s",
+                 "ms_per_step": 1e3 * (tm1 - tm0) / a.mixed_steps, "steps": a.mixed_steps,
+                 "kernel_ms": round(sm.t_force_ms / max(sm.n_timed_steps, 1), 4),
+                 "dtype": "f32 contact geometry and forces, f64 state and springs",
+                 "speedup_vs_fp64": round((n_flow * a.mixed_steps / (tm1 - tm0)) / value, 4)}
+        em.close()
+        print(f"# mixed precision: {mixed['ms_per_step']:.3f} ms per step, force {mixed['kernel_ms']:.3f} ms",
+              file=sys.stderr, flush=True)
+
     cpu = None
     if a.cpu_steps > 0:
         sys.path.insert(0, os.path.join(ROOT, "tests"))
@@ -195,7 +231,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if a.precision == "fp64" else "f32 contacts / f64 state",
         "data": "synthetic (seeded rock-salt lattice packing settled under gravity)",
         "config": {"workload": "polydisperse periodic chute, HertzianViscoelasticMindlin e=0.8 mu=0.5, 24 deg",
                    "flowing_spheres": n_flow, "base_spheres": sc.info["n_base"],
@@ -211,6 +247,7 @@ def main():
                      "kernel": "k_force", "kernel_ms": round(force_ms, 4),
                      "bytes_per_particle": round(b_force, 2)},
         "cpu_baseline": cpu,
+        "mixed_precision": mixed,
     }
     print(json.dumps(out))
 

@@ -45,7 +45,7 @@
 extern "C" {
 #endif
 
-#define DEM_ABI_VERSION 2
+#define DEM_ABI_VERSION 3
 
 /* error codes */
 #define DEM_OK 0
@@ -146,8 +146,16 @@ typedef struct dem_config {
     int32_t max_neighbors;   /* Verlet-list capacity per particle; 0 = 64 */
     double skin;             /* Verlet skin distance; <= 0 = 0.2 * min diameter */
     int32_t grid_levels;     /* cell-grid levels; 0 = automatic (<= 3) */
-    int32_t reserved;
+    int32_t precision;       /* DEM_PRECISION_FP64 (0) or DEM_PRECISION_MIXED */
 } dem_config;
+
+/* Arithmetic of the contact forces.  FP64 follows the reference bit for bit
+ * where the summation order allows; MIXED computes the contact geometry
+ * (overlap, normal, lever arms, relative velocity) and the contact forces in
+ * FP32 and keeps positions, velocities, the integration and the tangential
+ * springs in FP64 (no reference counterpart: a precision option of this
+ * engine). */
+enum dem_precision { DEM_PRECISION_FP64 = 0, DEM_PRECISION_MIXED = 1 };
 
 typedef struct dem_stats {
     int64_t n_particles;

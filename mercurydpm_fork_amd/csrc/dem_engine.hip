@@ -33,6 +33,7 @@
 
 #include "../../include/mercurydpm_dem.h"
 #include "dem_laws.h"
+#include "dem_laws_mixed.h"
 #include "dem_vec.h"
 
 using namespace dem;
@@ -928,7 +929,7 @@ struct OwnerLds {
 
 __device__ inline int q_owner(unsigned long long e) { return (int)((e >> 38) & 63); }
 
-template <int NS, int LAW>
+template <int NS, int LAW, int PREC>
 __device__ inline void process_batch(Dev<NS>& d, const Params& p, int cur, OwnerLds& L, int wbase, int base, int nb,
                                      int lane)
 {
@@ -991,14 +992,6 @@ __device__ inline void process_batch(Dev<NS>& d, const Params& p, int cur, Owner
         const Body& P = i_is_P ? bi : bj;
         const Body& I = i_is_P ? bj : bi;
         const V3 branch = vsub(P.pos, I.pos);
-        const double d2 = vlen2(branch);
-        ContactGeom g;
-        const double distance = sqrt(d2);
-        g.normal = vdiv(branch, distance);
-        g.overlap = P.radius + I.radius - distance;
-        g.distance = distance;
-        g.contact = vsub(P.pos, vmul(g.normal, P.radius - 0.5 * g.overlap));
-        g.i_is_wall = false;
         double spring[NS];
         for (int a = 0; a < NS; ++a) spring[a] = has_hist ? hist.s[a] : 0.0;
         if (!writer && img) {
@@ -1014,15 +1007,40 @@ __device__ inline void process_batch(Dev<NS>& d, const Params& p, int cur, Owner
             }
         }
         const dem_pair_params& sp = d.sp[L.sp[o] * p.ns + sj];
-        V3 f, tc, ft;
-        contact_force<NS, LAW>(sp, P, I, g, spring, p.dt, f, tc, ft);
-        V3 cF, cT = zero3();
-        if (i_is_P) {
-            cF = f;
-            if (p.rotation) cT = vsub(tc, vcross(vsub(P.pos, g.contact), f));
+        V3 cF, cT = zero3(), ft;
+        if constexpr (PREC == DEM_PRECISION_MIXED) {
+            ContactM gm;
+            pair_contact_mixed(branch, P.radius, I.radius, P.vel, I.vel, P.omega, I.omega, gm);
+            F3 f, tc, ftm;
+            contact_force_mixed<NS, LAW>(sp, P.radius, I.radius, P.inv_mass, I.inv_mass, P.omega, I.omega, gm, spring,
+                                         p.dt, f, tc, ftm);
+            // torques as below, with the lever arms c - xP and c - xI
+            if (i_is_P) {
+                cF = fd(f);
+                if (p.rotation) cT = fd(fadd(tc, fcross(gm.armP, f)));
+            } else {
+                cF = fd(fneg(f));
+                if (p.rotation) cT = fd(fsub(fneg(tc), fcross(gm.armI, f)));
+            }
+            ft = fd(ftm);
         } else {
-            cF = vneg(f);
-            if (p.rotation) cT = vadd(vneg(tc), vcross(vsub(I.pos, g.contact), f));
+            const double d2 = vlen2(branch);
+            ContactGeom g;
+            const double distance = sqrt(d2);
+            g.normal = vdiv(branch, distance);
+            g.overlap = P.radius + I.radius - distance;
+            g.distance = distance;
+            g.contact = vsub(P.pos, vmul(g.normal, P.radius - 0.5 * g.overlap));
+            g.i_is_wall = false;
+            V3 f, tc;
+            contact_force<NS, LAW>(sp, P, I, g, spring, p.dt, f, tc, ft);
+            if (i_is_P) {
+                cF = f;
+                if (p.rotation) cT = vsub(tc, vcross(vsub(P.pos, g.contact), f));
+            } else {
+                cF = vneg(f);
+                if (p.rotation) cT = vadd(vneg(tc), vcross(vsub(I.pos, g.contact), f));
+            }
         }
         L.cf[0][lane] = cF.x;
         L.cf[1][lane] = cF.y;
@@ -1052,8 +1070,9 @@ __device__ inline void process_batch(Dev<NS>& d, const Params& p, int cur, Owner
 }
 
 // LAW: a force law shared by every species pair (LAW_HERTZ_MINDLIN), or
-// LAW_ANY (the law is read from each pair's parameters)
-template <int NS, int LAW>
+// LAW_ANY (the law is read from each pair's parameters).  PREC: FP64, or
+// mixed (contact geometry and forces in FP32, springs and integration FP64).
+template <int NS, int LAW, int PREC>
 __global__ void __launch_bounds__(kBlock) FORCE_WAVES_ATTR k_force(Dev<NS> d, Params p, int cur)
 {
     // a diverged state (non-finite position, found by the rebuild) gets no
@@ -1104,7 +1123,7 @@ __global__ void __launch_bounds__(kBlock) FORCE_WAVES_ATTR k_force(Dev<NS> d, Pa
         while (qn - head >= 64 || (all && qn > head)) {
             const int nb = qn - head < 64 ? qn - head : 64;
             wave_sync();
-            process_batch<NS, LAW>(d, p, cur, L, wbase, head, nb, lane);
+            process_batch<NS, LAW, PREC>(d, p, cur, L, wbase, head, nb, lane);
             L.first[lane] = -1;
             wave_sync();
             if (lane < nb) {
@@ -1245,27 +1264,40 @@ __global__ void __launch_bounds__(kBlock) FORCE_WAVES_ATTR k_force(Dev<NS> d, Pa
             if (!fixed_i)
                 contact = wall_contact(wl, bi.pos, pr.w, d.sp[si * p.ns + wl.species].interaction_distance, distance, n);
             if (contact) {
-                ContactGeom g;
-                wall_geometry(bi.pos, pr.w, distance, n, g);
-                Body bw;
-                bw.pos = wl.type == DEM_WALL_INFINITE ? mk(wl.position[0], wl.position[1], wl.position[2]) : zero3();
-                bw.vel = zero3();
-                bw.omega = zero3();
-                bw.radius = 0.0;
-                bw.inv_mass = 0.0;
                 double spring[NS];
                 for (int a = 0; a < NS; ++a) spring[a] = 0.0;
                 if ((wm_in >> w) & 1u)
                     for (int a = 0; a < NS; ++a) spring[a] = d.wrec[cur][wbase + w].s[a];
                 const dem_pair_params sp = d.sp[si * p.ns + wl.species];
-                V3 f, tc, ft;
-                contact_force<NS, LAW>(sp, bi, bw, g, spring, p.dt, f, tc, ft);
+                V3 f, ft, t_add;
+                if constexpr (PREC == DEM_PRECISION_MIXED) {
+                    ContactM gm;
+                    wall_contact_mixed(distance, n, pr.w, bi.vel, bi.omega, gm);
+                    F3 fm, tcm, ftm;
+                    contact_force_mixed<NS, LAW>(sp, pr.w, 0.0, bi.inv_mass, 0.0, bi.omega, zero3(), gm, spring, p.dt,
+                                                 fm, tcm, ftm);
+                    f = fd(fm);
+                    ft = fd(ftm);
+                    t_add = fd(fadd(tcm, fcross(gm.armP, fm)));
+                } else {
+                    ContactGeom g;
+                    wall_geometry(bi.pos, pr.w, distance, n, g);
+                    Body bw;
+                    bw.pos = wl.type == DEM_WALL_INFINITE ? mk(wl.position[0], wl.position[1], wl.position[2]) : zero3();
+                    bw.vel = zero3();
+                    bw.omega = zero3();
+                    bw.radius = 0.0;
+                    bw.inv_mass = 0.0;
+                    V3 tc;
+                    contact_force<NS, LAW>(sp, bi, bw, g, spring, p.dt, f, tc, ft);
+                    t_add = vsub(tc, vcross(vsub(bi.pos, g.contact), f));
+                }
                 if (p.store_fc) {
                     d.fw[wbase + w] = make_double4(f.x, f.y, f.z, p.fc_stamp);
                     d.ftw[wbase + w] = make_double4(ft.x, ft.y, ft.z, p.fc_stamp);
                 }
                 F = vadd(F, f);
-                if (p.rotation) T = vadd(T, vsub(tc, vcross(vsub(bi.pos, g.contact), f)));
+                if (p.rotation) T = vadd(T, t_add);
                 Rec<NS> r;
                 for (int a = 0; a < NS; ++a) r.s[a] = spring[a];
                 r.ts = ts_new;
@@ -1797,6 +1829,7 @@ struct dem_ctx {
     int64_t nt = 0;
     int ns_springs = 3; // 3 or 9
     bool all_hertz_mindlin = false; // every species pair: HertzianViscoelastic + Mindlin
+    int precision = DEM_PRECISION_FP64;
     // particles
     int64_t n = 0;
     bool uploaded = false;
@@ -1947,10 +1980,22 @@ void launch_force(dem_ctx* ctx, int erase, int kick)
     ctx->prm.store_fc = kick != KICK_BOTH;
     if (ctx->prm.store_fc) ctx->prm.fc_stamp = (double)++ctx->fc_pass;
     const unsigned g = xcd_grid(grid_for(ctx->n));
-    if (NS == 3 && ctx->all_hertz_mindlin)
-        hipLaunchKernelGGL((k_force<NS, LAW_HERTZ_MINDLIN>), dim3(g), dim3(kBlock), 0, ctx->stream, d, ctx->prm, ctx->cur);
-    else
-        hipLaunchKernelGGL((k_force<NS, LAW_ANY>), dim3(g), dim3(kBlock), 0, ctx->stream, d, ctx->prm, ctx->cur);
+    const bool hm = NS == 3 && ctx->all_hertz_mindlin;
+    if (ctx->precision == DEM_PRECISION_MIXED) {
+        if (hm)
+            hipLaunchKernelGGL((k_force<NS, LAW_HERTZ_MINDLIN, DEM_PRECISION_MIXED>), dim3(g), dim3(kBlock), 0,
+                               ctx->stream, d, ctx->prm, ctx->cur);
+        else
+            hipLaunchKernelGGL((k_force<NS, LAW_ANY, DEM_PRECISION_MIXED>), dim3(g), dim3(kBlock), 0, ctx->stream, d,
+                               ctx->prm, ctx->cur);
+    } else {
+        if (hm)
+            hipLaunchKernelGGL((k_force<NS, LAW_HERTZ_MINDLIN, DEM_PRECISION_FP64>), dim3(g), dim3(kBlock), 0,
+                               ctx->stream, d, ctx->prm, ctx->cur);
+        else
+            hipLaunchKernelGGL((k_force<NS, LAW_ANY, DEM_PRECISION_FP64>), dim3(g), dim3(kBlock), 0, ctx->stream, d,
+                               ctx->prm, ctx->cur);
+    }
     ctx->cur ^= 1;
     if (kick != KICK_NONE) {
         // the integrated state went to the gather buffers: make it current
@@ -2179,7 +2224,10 @@ int dem_create(dem_ctx** out, const dem_config* cfg)
         if (cfg->max_neighbors > 0) ctx->maxn_cfg = cfg->max_neighbors;
         ctx->skin_cfg = cfg->skin;
         ctx->levels_cfg = cfg->grid_levels;
+        ctx->precision = cfg->precision;
     }
+    if (ctx->precision != DEM_PRECISION_FP64 && ctx->precision != DEM_PRECISION_MIXED)
+        return set_err(ctx, DEM_ERR_ARG, "unknown precision mode");
     if (ctx->maxn_cfg > 64) return set_err(ctx, DEM_ERR_ARG, "max_neighbors must be <= 64");
     if (ctx->levels_cfg > kMaxLevels) return set_err(ctx, DEM_ERR_ARG, "grid_levels must be <= 4");
     int ndev = 0;

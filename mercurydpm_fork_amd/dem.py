@@ -69,7 +69,11 @@ class Periodic(C.Structure):
 
 class Config(C.Structure):
     _fields_ = [("device", C.c_int32), ("max_neighbors", C.c_int32), ("skin", C.c_double),
-                ("grid_levels", C.c_int32), ("reserved", C.c_int32)]
+                ("grid_levels", C.c_int32), ("precision", C.c_int32)]
+
+
+PRECISION_FP64 = 0
+PRECISION_MIXED = 1
 
 
 class Stats(C.Structure):
@@ -210,13 +214,13 @@ class Engine:
     """One simulation context behind the C ABI."""
 
     def __init__(self, lib=None, prefix: str = "dem_", device: int = 0, max_neighbors: int = 0,
-                 skin: float = 0.0, grid_levels: int = 0):
+                 skin: float = 0.0, grid_levels: int = 0, precision: int = 0):
         self.lib = lib if lib is not None else load_hip_library()
         self.prefix = prefix
         self._has_stats = hasattr(self.lib, prefix + "get_stats") and prefix == "dem_"
         h = _PTR()
         if prefix == "dem_":
-            cfg = Config(device, max_neighbors, skin, grid_levels, 0)
+            cfg = Config(device, max_neighbors, skin, grid_levels, precision)
             rc = self._fn("create")(C.byref(h), C.byref(cfg))
         else:
             rc = self._fn("create")(C.byref(h))

@@ -368,3 +368,50 @@ def test_engine_rejects_non_finite_input():
     g.step_end()
     g.step(5)
     assert np.all(np.isfinite(g.download()["x"]))
+
+
+def _contact_force_scale(e, sc):
+    """Per particle: the sum of |force| over its interactions."""
+    idx = {int(i): k for k, i in enumerate(sc.state.id)}
+    scale = np.zeros(sc.state.n)
+    for r in e.interactions():
+        m = np.linalg.norm(r["force"])
+        scale[idx[int(r["p_id"])]] += m
+        if not r["i_id"] & dem.DEM_WALL_KEY:
+            scale[idx[int(r["i_id"])]] += m
+    return scale
+
+
+@pytest.mark.parametrize("laws,seed", [("lvf", 31), ("hvm", 32), ("full", 33)])
+def test_mixed_precision_against_fp64(laws, seed):
+    """DEM_PRECISION_MIXED against FP64 on identical inputs (the state of an
+    FP64 run after 100 steps): the same stored contacts (the contact test is
+    FP64 in both); forces per particle within 1e-6 on average and 2e-4 at
+    most of the sum of the magnitudes of its contact forces (FP32 contact
+    geometry and forces; measured: mean 1e-7 to 3e-7, max 1e-5 to 9e-5);
+    torques likewise per radius; springs to 1e-5 of their scale."""
+    sc = S.mixture(n=800, seed=seed, laws=laws, inflate=1.2, lx=8.4, ly=8.4, lz=8.4)
+    ref = dem.Engine()
+    S.apply(ref, sc)
+    ref.compute_forces()
+    ref.step(100)
+    st, ct = ref.download(), ref.contacts()
+    runs = {}
+    for prec in (dem.PRECISION_FP64, dem.PRECISION_MIXED):
+        e = dem.Engine(precision=prec)
+        S.apply(e, reseeded(sc, st), ct)
+        e.set_time(100 * sc.dt, 100)
+        e.compute_forces()
+        runs[prec] = e
+    a, b = runs[dem.PRECISION_FP64], runs[dem.PRECISION_MIXED]
+    ca, cb = a.contacts(), b.contacts()
+    assert ca.keys() == cb.keys() and ca.n > 300
+    sscale = max(np.max(np.abs(ca.spring)), 1e-300)
+    assert np.max(np.abs(ca.spring - cb.spring)) <= 1e-5 * sscale
+    fa, fb = a.download(), b.download()
+    scale = _contact_force_scale(a, sc)
+    m = scale > 0
+    err = np.linalg.norm(fb["f"] - fa["f"], axis=1)[m] / scale[m]
+    terr = np.linalg.norm(fb["t"] - fa["t"], axis=1)[m] / (scale[m] * sc.state.radius[m])
+    assert err.mean() <= 1e-6 and err.max() <= 2e-4, (err.mean(), err.max())
+    assert terr.mean() <= 1e-6 and terr.max() <= 2e-4, (terr.mean(), terr.max())
