@@ -45,7 +45,7 @@
 extern "C" {
 #endif
 
-#define DEM_ABI_VERSION 3
+#define DEM_ABI_VERSION 4
 
 /* error codes */
 #define DEM_OK 0
@@ -215,7 +215,8 @@ int dem_set_time(dem_ctx* ctx, double time, int64_t n_time_steps);
  * handler order.  inv_inertia holds MatrixSymmetric3D (XX,XY,XZ,YY,YZ,ZZ) per
  * particle; spheres must be isotropic (BaseParticle::computeMass,
  * BaseParticle.cc:874-883).  f and t (force/torque of the previous force pass)
- * may be NULL (zero).  Replaces ParticleHandler::copyAndAddObject for the
+ * may be NULL (zero).  n may be 0: an empty system, to be filled by
+ * dem_add_particles.  Replaces ParticleHandler::copyAndAddObject for the
  * device copy; invalidates all stored contacts (upload them afterwards).
  */
 int dem_upload_particles(dem_ctx* ctx, int64_t n, const uint32_t* id, const int32_t* species,
@@ -266,6 +267,32 @@ int dem_step_end(dem_ctx* ctx);
  * Verlet-list rebuild like any other motion. */
 int dem_set_kinematics(dem_ctx* ctx, int64_t n, const double* x, const double* v, const double* w,
                        const double* q);
+
+/* ParticleHandler::copyAndAddObject during a run, as the insertion
+ * boundaries do it (InsertionBoundary::checkBoundaryBeforeTimeStep,
+ * InsertionBoundary.cc:184-310): append k particles after the resident ones
+ * (handler indices n .. n+k-1).  Ids must increase and exceed every resident
+ * id; arrays as dem_upload_particles (w and q may be NULL).  The resident
+ * state and the stored interactions are kept.  Allowed between
+ * dem_step_begin and dem_step_end, where the reference's boundaries insert
+ * (DPMBase.cc:4348-4351): the new particles take part in that step's force
+ * pass and closing half-kick.  An empty system (dem_upload_particles with
+ * n = 0) is allowed: its steps only advance the clock. */
+int dem_add_particles(dem_ctx* ctx, int64_t k, const uint32_t* id, const int32_t* species, const double* radius,
+                      const double* inv_mass, const double* inv_inertia, const double* x, const double* v,
+                      const double* w, const double* q);
+
+/* DPMBase::checkParticleForInteraction (DPMBase.cc:4819-4905 with
+ * MercuryBase::checkParticleForInteractionLocal, MercuryBase.cc:622-650) for
+ * k candidate spheres against the resident particles (at their current
+ * positions) and the walls: free_out[j] = 1 when candidate j (x: 3 doubles,
+ * radius, species index) touches neither, 0 otherwise.  Near a periodic
+ * boundary the candidate's images are tested as well; max_distance[j] is the
+ * reference's image range, the candidate's getMaxInteractionRadius plus that
+ * of the largest particle in the caller's handler, or a negative value when
+ * that handler is empty (no images).  At most 65535 candidates per call. */
+int dem_check_particles(dem_ctx* ctx, int64_t k, const double* x, const double* radius, const int32_t* species,
+                        const double* max_distance, int32_t* free_out);
 
 /* Download state in handler order (any pointer may be NULL). */
 int dem_download_particles(dem_ctx* ctx, double* x, double* v, double* w, double* q,

@@ -50,11 +50,12 @@ HOST_FLAGS = ["-std=c++17", "-O2", "-fPIC", "-Wall", "-Wextra", "-ffp-contract=o
 DRIVER_DIR = os.path.join(ROOT, "tests", "drivers")
 
 
-HOST_SOURCES = [os.path.join(HERE, "host", f) for f in ("Mercury3D.cc", "DPMBaseOutput.cc", "DPMBaseRestart.cc")]
+HOST_SOURCES = [os.path.join(HERE, "host", f) for f in ("Mercury3D.cc", "DPMBaseOutput.cc", "DPMBaseRestart.cc",
+                                                                "InsertionBoundary.cc")]
 
 
 def build_host():
-    """C++ host API (host/Mercury3D.cc, host/DPMBaseOutput.cc) linked against
+    """C++ host API (host/*.cc) linked against
     the engine."""
     out = os.path.join(LIB, "libmercurydpm_host.so")
     _run([CXX, *HOST_FLAGS, "-shared", "-o", out, *HOST_SOURCES,

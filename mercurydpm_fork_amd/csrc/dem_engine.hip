@@ -1574,6 +1574,69 @@ __global__ void __launch_bounds__(kBlock) k_cg_reduce(CgParams c, const double* 
 }
 
 // ------------------------------------------------------------------------
+// Insertion test: DPMBase::checkParticleForInteraction for candidate spheres.
+// ------------------------------------------------------------------------
+constexpr int kMaxImages = 8;  // a candidate and its images across <= 3 periodic boundaries
+
+// The candidate and its periodic images as checkParticleForInteraction-
+// LocalPeriodic builds them (DPMBase.cc:4868-4905): per boundary, first the
+// images already made (newest first), then the candidate itself, each shifted
+// when nearer to the boundary than max_distance.  No images when the caller's
+// particle handler is empty (max_distance < 0).
+__device__ inline int candidate_images(const Params& p, V3 x, double max_distance, V3* img)
+{
+    img[0] = x;
+    int m = 1;
+    if (max_distance < 0.0) return m;
+    for (int b = 0; b < p.nper; ++b) {
+        for (int i = m - 1; i >= 1; --i)
+            if (periodic_distance(p, b, img[i]) < max_distance) img[m++] = periodic_shift(p, b, img[i]);
+        if (periodic_distance(p, b, x) < max_distance) img[m++] = periodic_shift(p, b, x);
+    }
+    return m;
+}
+
+// busy[c] = 1 when candidate c (cand[c] = x, y, z, radius; species csp[c]) or
+// one of its images touches a wall (BaseWall::getDistanceAndNormal,
+// checkParticleForInteractionLocal, MercuryBase.cc:622-650) or a particle:
+// Vec3D::getDistanceSquared(candidate, particle) below the square of the
+// sum of the radii (BaseParticle::isInContactWith, BaseParticle.cc:863-871).
+// blockIdx.y is the candidate; blocks in x stride over the particles, and
+// block 0 also tests the walls.
+template <int NS>
+__global__ void __launch_bounds__(kBlock) k_check_particles(Dev<NS> d, Params p, const double4* cand, const int* csp,
+                                                            const double* max_distance, int* busy)
+{
+    const int c = blockIdx.y;
+    const double4 cr = cand[c];
+    const int sc = csp[c];
+    V3 img[kMaxImages];
+    const int m = candidate_images(p, mk(cr.x, cr.y, cr.z), max_distance[c], img);
+    if (blockIdx.x == 0 && (int)threadIdx.x < p.nwalls) {
+        const dem_wall& wl = d.walls[threadIdx.x];
+        const double idist = d.sp[sc * p.ns + wl.species].interaction_distance;
+        for (int k = 0; k < m; ++k) {
+            double distance;
+            V3 n;
+            if (wall_contact(wl, img[k], cr.w, idist, distance, n)) busy[c] = 1;
+        }
+    }
+    GRID_STRIDE(i, p.n)
+    {
+        const double4 q = d.posr[i];
+        const int sq = (int)(d.spc[i] & 0xFFFFu);
+        const double reach = cr.w + q.w + d.sp[sc * p.ns + sq].interaction_distance;
+        const double reach2 = reach * reach;
+        for (int k = 0; k < m; ++k) {
+            const double X = img[k].x - q.x;
+            const double Y = img[k].y - q.y;
+            const double Z = img[k].z - q.z;
+            if (X * X + Y * Y + Z * Z < reach2) busy[c] = 1;
+        }
+    }
+}
+
+// ------------------------------------------------------------------------
 // Diagnostics: stored interactions.
 // ------------------------------------------------------------------------
 template <int NS>
@@ -1848,6 +1911,12 @@ struct dem_ctx {
     uint32_t nbuckets = 0;
     int64_t last_rebuilds = 0;
     uint32_t alive_min = 0;
+    uint32_t max_id = 0;          // largest particle id (ids increase with the handler index)
+    bool hash_pending = false;    // uploaded interactions wait in the hash for the next rebuild
+    int64_t rebuilds_before = 0;  // rebuilds counted before the last re-upload by dem_add_particles
+    // candidate buffer of dem_check_particles (not in allocs: it outlives re-uploads)
+    void* d_check = nullptr;
+    int64_t check_cap = 0;
     // profiling
     bool profiling = false;
     std::vector<hipEvent_t> events;
@@ -1957,6 +2026,7 @@ void launch_rebuild(dem_ctx* ctx)
 {
     Dev<NS>& d = dev<NS>(ctx);
     hipStream_t s = ctx->stream;
+    ctx->hash_pending = false;
     const Params& p = ctx->prm;
     const dim3 g(kRebuildBlocks), b(kBlock);
     hipLaunchKernelGGL(k_rb_begin<NS>, g, b, 0, s, d, p, ctx->cur);
@@ -2018,7 +2088,7 @@ int check_flags(dem_ctx* ctx)
     int f[F_NUM];
     HIPCHK(hipMemcpyAsync(f, ctx->d_flags, sizeof f, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
-    ctx->last_rebuilds = f[F_REBUILDS];
+    ctx->last_rebuilds = ctx->rebuilds_before + f[F_REBUILDS];
     ctx->max_list = f[F_MAX_LIST];
     if (f[F_LIST_OVERFLOW])
         return set_err(ctx, DEM_ERR_CAPACITY,
@@ -2166,6 +2236,116 @@ void refresh_params(dem_ctx* ctx)
     p.rotation = ctx->rotation;
 }
 
+// An empty ParticleHandler (a run that starts before its insertion
+// boundaries have added anything): no particle arrays, only the tables that
+// dem_check_particles reads.  Steps then only advance the clock.
+int upload_empty(dem_ctx* ctx)
+{
+    HIPCHK(hipSetDevice(ctx->device));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    free_all(ctx);
+    for (const auto& wl : ctx->walls)
+        if (wl.species < 0 || wl.species >= ctx->ns) return set_err(ctx, DEM_ERR_ARG, "wall species out of range");
+    Params& p = ctx->prm;
+    p = Params{};
+    p.nwalls = (int)ctx->walls.size();
+    p.ns = ctx->ns;
+    p.levels = 1;
+    p.nper = (int)ctx->periodic.size();
+    for (int a = 0; a < 3; ++a) p.axis_pb[a] = -1;
+    for (int b = 0; b < p.nper; ++b) {
+        p.axis_pb[ctx->periodic[b].axis] = b;
+        p.paxis[b] = ctx->periodic[b].axis;
+        p.pleft[b] = ctx->periodic[b].left;
+        p.pright[b] = ctx->periodic[b].right;
+        p.pshift[b] = ctx->periodic[b].shift;
+    }
+    refresh_params(ctx);
+    int r;
+    if ((r = dalloc(ctx, &ctx->d_flags, F_NUM)) != 0) return r;
+    if ((r = dalloc(ctx, &ctx->d_sp, ctx->species.size())) != 0) return r;
+    if ((r = dalloc(ctx, &ctx->d_walls, std::max<size_t>(ctx->walls.size(), 1))) != 0) return r;
+    HIPCHK(hipMemcpy(ctx->d_sp, ctx->species.data(), sizeof(dem_pair_params) * ctx->species.size(),
+                     hipMemcpyHostToDevice));
+    if (!ctx->walls.empty())
+        HIPCHK(hipMemcpy(ctx->d_walls, ctx->walls.data(), sizeof(dem_wall) * ctx->walls.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemset(ctx->d_flags, 0, sizeof(int) * F_NUM));
+    ctx->d3 = Dev<3>{};
+    ctx->d9 = Dev<9>{};
+    ctx->d3.sp = ctx->d9.sp = ctx->d_sp;
+    ctx->d3.walls = ctx->d9.walls = ctx->d_walls;
+    ctx->d3.flags = ctx->d9.flags = ctx->d_flags;
+    ctx->n = 0;
+    ctx->nbuckets = 0;
+    ctx->diverged = false;
+    ctx->cur = 0;
+    ctx->alive_min = 0;
+    ctx->rebuilds_before = 0;
+    ctx->uploaded = true;
+    ctx->in_step = false;
+    return DEM_OK;
+}
+
+// The clock of one step of an empty system (run_step_end's bookkeeping).
+void empty_step(dem_ctx* ctx)
+{
+    ctx->time += ctx->dt;
+    ctx->nt += 1;
+    ctx->alive_min = (uint32_t)ctx->nt;
+    ctx->prm.alive_min = ctx->alive_min;
+}
+
+// Uploaded interaction records wait in the hash table until a rebuild
+// matches them to list slots; whoever reads the stored interactions before
+// the next force pass runs that rebuild first (no erase: an upload keeps
+// every record it was given).
+int match_uploaded(dem_ctx* ctx)
+{
+    if (!ctx->hash_pending || ctx->n == 0) return DEM_OK;
+    HIPCHK(hipSetDevice(ctx->device));
+    refresh_params(ctx);
+    ctx->prm.erase = 0;
+    ctx->prm.nt = (uint32_t)ctx->nt;
+    ctx->prm.alive_min = ctx->alive_min;
+    if (ctx->ns_springs == 9)
+        launch_rebuild<9>(ctx);
+    else
+        launch_rebuild<3>(ctx);
+    HIPCHK(hipGetLastError());
+    return check_flags(ctx);
+}
+
+// The static properties of the resident particles in handler order.
+int download_static(dem_ctx* ctx, std::vector<uint32_t>& id, std::vector<int32_t>& species, std::vector<double>& radius,
+                    std::vector<double>& inv_mass, std::vector<double>& inv_inertia)
+{
+    const int64_t n = ctx->n;
+    const bool nine = ctx->ns_springs == 9;
+    std::vector<uint32_t> orig(n), sid(n), spc(n);
+    std::vector<double4> posr(n), vel(n), omg(n);
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    HIPCHK(hipMemcpy(orig.data(), nine ? ctx->d9.orig : ctx->d3.orig, sizeof(uint32_t) * n, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(sid.data(), nine ? ctx->d9.id : ctx->d3.id, sizeof(uint32_t) * n, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(spc.data(), nine ? ctx->d9.spc : ctx->d3.spc, sizeof(uint32_t) * n, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(posr.data(), nine ? ctx->d9.posr : ctx->d3.posr, sizeof(double4) * n, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(vel.data(), nine ? ctx->d9.vel : ctx->d3.vel, sizeof(double4) * n, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(omg.data(), nine ? ctx->d9.omg : ctx->d3.omg, sizeof(double4) * n, hipMemcpyDeviceToHost));
+    id.resize(n);
+    species.resize(n);
+    radius.resize(n);
+    inv_mass.resize(n);
+    inv_inertia.assign(6 * n, 0.0);
+    for (int64_t i = 0; i < n; ++i) {
+        const uint32_t k = orig[i];
+        id[k] = sid[i];
+        species[k] = (int32_t)(spc[i] & 0xFFFFu);
+        radius[k] = posr[i].w;
+        inv_mass[k] = vel[i].w;
+        inv_inertia[6 * k] = inv_inertia[6 * k + 3] = inv_inertia[6 * k + 5] = omg[i].w;
+    }
+    return DEM_OK;
+}
+
 }  // namespace
 
 template <int NS>
@@ -2244,6 +2424,7 @@ int dem_destroy(dem_ctx* ctx)
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     free_all(ctx);
+    if (ctx->d_check) (void)hipFree(ctx->d_check);
     for (hipEvent_t e : ctx->events) (void)hipEventDestroy(e);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -2347,10 +2528,12 @@ int dem_upload_particles(dem_ctx* ctx, int64_t n, const uint32_t* id, const int3
                          const double* inv_mass, const double* inv_inertia, const double* x, const double* v,
                          const double* w, const double* q, const double* f, const double* t)
 {
-    if (!ctx || n <= 0 || !id || !species || !radius || !inv_mass || !inv_inertia || !x || !v)
+    if (!ctx || n < 0) return set_err(ctx, DEM_ERR_ARG, "upload_particles: bad arguments");
+    if (ctx->ns == 0) return set_err(ctx, DEM_ERR_STATE, "set_species before upload_particles");
+    if (n == 0) return upload_empty(ctx);
+    if (!id || !species || !radius || !inv_mass || !inv_inertia || !x || !v)
         return set_err(ctx, DEM_ERR_ARG, "upload_particles: bad arguments");
     if (n > (int64_t)0x7FFFFFFF) return set_err(ctx, DEM_ERR_ARG, "too many particles");
-    if (ctx->ns == 0) return set_err(ctx, DEM_ERR_STATE, "set_species before upload_particles");
     HIPCHK(hipSetDevice(ctx->device));
     // --- validate and derive the grid on the host
     double rmin = 1e300, rmax = 0.0;
@@ -2559,6 +2742,8 @@ int dem_upload_particles(dem_ctx* ctx, int64_t n, const uint32_t* id, const int3
     ctx->alive_min = 0;
     ctx->uploaded = true;
     ctx->in_step = false;
+    ctx->max_id = id[n - 1];
+    ctx->rebuilds_before = 0;
     return DEM_OK;
 }
 
@@ -2568,6 +2753,7 @@ int dem_upload_contacts(dem_ctx* ctx, int64_t n, const uint32_t* p_id, const uin
     if (!ctx || n < 0) return DEM_ERR_ARG;
     if (!ctx->uploaded) return set_err(ctx, DEM_ERR_STATE, "upload particles before contacts");
     if (n == 0) return DEM_OK;
+    if (ctx->n == 0) return set_err(ctx, DEM_ERR_ARG, "upload_contacts: no particles");
     if (!p_id || !i_id || !spring || !ts) return set_err(ctx, DEM_ERR_ARG, "upload_contacts: null array");
     for (int64_t k = 0; k < n; ++k) {
         if (i_id[k] & DEM_WALL_KEY) {
@@ -2606,6 +2792,7 @@ int dem_upload_contacts(dem_ctx* ctx, int64_t n, const uint32_t* p_id, const uin
     (void)hipFree(di);
     (void)hipFree(dt);
     (void)hipFree(ds);
+    ctx->hash_pending = true;
     return DEM_OK;
 }
 
@@ -2614,6 +2801,7 @@ int dem_compute_forces(dem_ctx* ctx)
     int r = ready(ctx);
     if (r) return r;
     if (ctx->in_step) return set_err(ctx, DEM_ERR_STATE, "dem_compute_forces inside dem_step_begin/dem_step_end");
+    if (ctx->n == 0) return DEM_OK;
     HIPCHK(hipSetDevice(ctx->device));
     refresh_params(ctx);
     return ctx->ns_springs == 9 ? run_forces<9>(ctx) : run_forces<3>(ctx);
@@ -2625,6 +2813,10 @@ int dem_step(dem_ctx* ctx, int64_t n_steps)
     if (r) return r;
     if (n_steps < 0) return set_err(ctx, DEM_ERR_ARG, "negative step count");
     if (ctx->in_step) return set_err(ctx, DEM_ERR_STATE, "dem_step inside dem_step_begin/dem_step_end");
+    if (ctx->n == 0) {
+        for (int64_t k = 0; k < n_steps; ++k) empty_step(ctx);
+        return DEM_OK;
+    }
     HIPCHK(hipSetDevice(ctx->device));
     refresh_params(ctx);
     return ctx->ns_springs == 9 ? run_steps<9>(ctx, n_steps) : run_steps<3>(ctx, n_steps);
@@ -2635,6 +2827,10 @@ int dem_step_begin(dem_ctx* ctx)
     int r = ready(ctx);
     if (r) return r;
     if (ctx->in_step) return set_err(ctx, DEM_ERR_STATE, "dem_step_begin: a step is already open");
+    if (ctx->n == 0) {
+        ctx->in_step = true;
+        return DEM_OK;
+    }
     HIPCHK(hipSetDevice(ctx->device));
     refresh_params(ctx);
     if (ctx->ns_springs == 9)
@@ -2651,8 +2847,12 @@ int dem_step_end(dem_ctx* ctx)
     int r = ready(ctx);
     if (r) return r;
     if (!ctx->in_step) return set_err(ctx, DEM_ERR_STATE, "dem_step_end without dem_step_begin");
-    HIPCHK(hipSetDevice(ctx->device));
     ctx->in_step = false;
+    if (ctx->n == 0) {
+        empty_step(ctx);
+        return DEM_OK;
+    }
+    HIPCHK(hipSetDevice(ctx->device));
     return ctx->ns_springs == 9 ? run_step_end<9>(ctx) : run_step_end<3>(ctx);
 }
 
@@ -2665,6 +2865,10 @@ int dem_cg_z(dem_ctx* ctx, const dem_cg_config* cfg, double* out)
         return set_err(ctx, DEM_ERR_ARG, "dem_cg_z: particle_dimensions must be 2 or 3");
     if (!ctx->uploaded) return set_err(ctx, DEM_ERR_STATE, "no particles uploaded");
     if (ctx->in_step) return set_err(ctx, DEM_ERR_STATE, "dem_cg_z inside dem_step_begin/dem_step_end");
+    if (ctx->n == 0) {
+        std::fill(out, out + (size_t)cfg->n * DEM_CG_FIELDS, 0.0);
+        return DEM_OK;
+    }
     if (ctx->fc_pass == 0) return set_err(ctx, DEM_ERR_STATE, "dem_cg_z before the first force pass");
     HIPCHK(hipSetDevice(ctx->device));
     return ctx->ns_springs == 9 ? run_cg_z<9>(ctx, cfg, out) : run_cg_z<3>(ctx, cfg, out);
@@ -2675,6 +2879,7 @@ int dem_set_kinematics(dem_ctx* ctx, int64_t n, const double* x, const double* v
     if (!ctx) return DEM_ERR_ARG;
     if (!ctx->uploaded) return set_err(ctx, DEM_ERR_STATE, "no particles uploaded");
     if (n != ctx->n) return set_err(ctx, DEM_ERR_ARG, "dem_set_kinematics: n differs from the uploaded count");
+    if (n == 0) return DEM_OK;
     for (const double* a : {x, v, w, q}) {
         if (!a) continue;
         const int64_t m = (a == q ? 4 : 3) * n;
@@ -2686,10 +2891,133 @@ int dem_set_kinematics(dem_ctx* ctx, int64_t n, const double* x, const double* v
     return ctx->ns_springs == 9 ? run_set_kinematics<9>(ctx, x, v, w, q) : run_set_kinematics<3>(ctx, x, v, w, q);
 }
 
+// The appended particles join the resident ones through a re-upload of the
+// whole state and history: the grid (levels, cell sizes, bucket box) is
+// derived from the full particle set again, and the stored interactions are
+// matched to the new sorted order by their id keys, as after any re-sort.
+int dem_add_particles(dem_ctx* ctx, int64_t k, const uint32_t* id, const int32_t* species, const double* radius,
+                      const double* inv_mass, const double* inv_inertia, const double* x, const double* v,
+                      const double* w, const double* q)
+{
+    int r = ready(ctx);
+    if (r) return r;
+    if (k < 0) return set_err(ctx, DEM_ERR_ARG, "add_particles: negative count");
+    if (k == 0) return DEM_OK;
+    if (!id || !species || !radius || !inv_mass || !inv_inertia || !x || !v)
+        return set_err(ctx, DEM_ERR_ARG, "add_particles: bad arguments");
+    if (ctx->n > 0 && id[0] <= ctx->max_id)
+        return set_err(ctx, DEM_ERR_ARG, "add_particles: ids must be above every resident id");
+    if (ctx->n + k > (int64_t)0x7FFFFFFF) return set_err(ctx, DEM_ERR_ARG, "too many particles");
+    HIPCHK(hipSetDevice(ctx->device));
+    const int64_t n = ctx->n, m = n + k;
+    std::vector<uint32_t> aid;
+    std::vector<int32_t> asp;
+    std::vector<double> ar, aim, aii;
+    std::vector<double> ax(3 * m), av(3 * m), aw(3 * m), aq(4 * m), af(3 * m, 0.0), at(3 * m, 0.0);
+    std::vector<uint32_t> cp, ci, cts;
+    std::vector<double> cs;
+    int64_t nc = 0;
+    int64_t rebuilds = 0;
+    if (n > 0) {
+        if ((r = check_flags(ctx)) != 0) return r;
+        rebuilds = ctx->last_rebuilds;
+        if ((r = download_static(ctx, aid, asp, ar, aim, aii)) != 0) return r;
+        if ((r = dem_download_particles(ctx, ax.data(), av.data(), aw.data(), aq.data(), af.data(), at.data())) != 0)
+            return r;
+        if ((r = dem_contact_count(ctx, &nc)) != 0) return r;
+        cp.resize(nc);
+        ci.resize(nc);
+        cts.resize(nc);
+        cs.resize(9 * nc);
+        int64_t got = 0;
+        if ((r = dem_download_contacts(ctx, nc, &got, cp.data(), ci.data(), cs.data(), cts.data())) != 0) return r;
+        nc = got;
+    }
+    for (int64_t j = 0; j < k; ++j) {
+        aid.push_back(id[j]);
+        asp.push_back(species[j]);
+        ar.push_back(radius[j]);
+        aim.push_back(inv_mass[j]);
+        for (int c = 0; c < 6; ++c) aii.push_back(inv_inertia[6 * j + c]);
+        for (int c = 0; c < 3; ++c) {
+            ax[3 * (n + j) + c] = x[3 * j + c];
+            av[3 * (n + j) + c] = v[3 * j + c];
+            aw[3 * (n + j) + c] = w ? w[3 * j + c] : 0.0;
+        }
+        for (int c = 0; c < 4; ++c) aq[4 * (n + j) + c] = q ? q[4 * j + c] : (c == 0 ? 1.0 : 0.0);
+    }
+    const bool in_step = ctx->in_step;
+    const uint32_t alive_min = ctx->alive_min;
+    if ((r = dem_upload_particles(ctx, m, aid.data(), asp.data(), ar.data(), aim.data(), aii.data(), ax.data(),
+                                  av.data(), aw.data(), aq.data(), af.data(), at.data())) != 0)
+        return r;
+    if (nc > 0 && (r = dem_upload_contacts(ctx, nc, cp.data(), ci.data(), cs.data(), cts.data())) != 0) return r;
+    ctx->in_step = in_step;
+    ctx->alive_min = alive_min;
+    ctx->prm.alive_min = alive_min;
+    ctx->rebuilds_before = rebuilds;
+    return DEM_OK;
+}
+
+int dem_check_particles(dem_ctx* ctx, int64_t k, const double* x, const double* radius, const int32_t* species,
+                        const double* max_distance, int32_t* free_out)
+{
+    if (!ctx) return DEM_ERR_ARG;
+    if (!ctx->uploaded) return set_err(ctx, DEM_ERR_STATE, "no particles uploaded");
+    if (k < 0) return set_err(ctx, DEM_ERR_ARG, "check_particles: negative count");
+    if (k == 0) return DEM_OK;
+    if (!x || !radius || !species || !max_distance || !free_out)
+        return set_err(ctx, DEM_ERR_ARG, "check_particles: null array");
+    if (k > 65535) return set_err(ctx, DEM_ERR_ARG, "check_particles: at most 65535 candidates per call");
+    for (int64_t j = 0; j < k; ++j) {
+        if (species[j] < 0 || species[j] >= ctx->ns) return set_err(ctx, DEM_ERR_ARG, "species index out of range");
+        if (!std::isfinite(x[3 * j]) || !std::isfinite(x[3 * j + 1]) || !std::isfinite(x[3 * j + 2]) ||
+            !(radius[j] > 0) || !std::isfinite(radius[j]) || std::isnan(max_distance[j]))
+            return set_err(ctx, DEM_ERR_ARG, "check_particles: candidates must be finite with a positive radius");
+    }
+    HIPCHK(hipSetDevice(ctx->device));
+    // candidate buffer: k double4, then k species, k distances and k flags
+    const size_t bytes = (size_t)k * (sizeof(double4) + sizeof(double) + 2 * sizeof(int));
+    if ((int64_t)bytes > ctx->check_cap) {
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        if (ctx->d_check) (void)hipFree(ctx->d_check);
+        ctx->d_check = nullptr;
+        ctx->check_cap = 0;
+        HIPCHK(hipMalloc(&ctx->d_check, bytes));
+        ctx->check_cap = (int64_t)bytes;
+    }
+    double4* dc = (double4*)ctx->d_check;
+    double* dmax = (double*)(dc + k);
+    int* dsp = (int*)(dmax + k);
+    int* dbusy = dsp + k;
+    std::vector<double4> hc(k);
+    for (int64_t j = 0; j < k; ++j) hc[j] = make_double4(x[3 * j], x[3 * j + 1], x[3 * j + 2], radius[j]);
+    std::vector<int> hbusy(k);
+    HIPCHK(hipMemcpyAsync(dc, hc.data(), sizeof(double4) * k, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(dmax, max_distance, sizeof(double) * k, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(dsp, species, sizeof(int) * k, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemsetAsync(dbusy, 0, sizeof(int) * k, ctx->stream));
+    refresh_params(ctx);
+    const unsigned gx = (unsigned)std::max<int64_t>(1, std::min<int64_t>(grid_for(ctx->n), 1024));
+    const dim3 grid(gx, (unsigned)k);
+    if (ctx->ns_springs == 9)
+        hipLaunchKernelGGL(k_check_particles<9>, grid, dim3(kBlock), 0, ctx->stream, ctx->d9, ctx->prm,
+                           (const double4*)dc, (const int*)dsp, (const double*)dmax, dbusy);
+    else
+        hipLaunchKernelGGL(k_check_particles<3>, grid, dim3(kBlock), 0, ctx->stream, ctx->d3, ctx->prm,
+                           (const double4*)dc, (const int*)dsp, (const double*)dmax, dbusy);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(hbusy.data(), dbusy, sizeof(int) * k, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    for (int64_t j = 0; j < k; ++j) free_out[j] = hbusy[j] ? 0 : 1;
+    return DEM_OK;
+}
+
 int dem_download_particles(dem_ctx* ctx, double* x, double* v, double* w, double* q, double* f, double* t)
 {
     if (!ctx) return DEM_ERR_ARG;
     if (!ctx->uploaded) return set_err(ctx, DEM_ERR_STATE, "no particles uploaded");
+    if (ctx->n == 0) return DEM_OK;
     HIPCHK(hipSetDevice(ctx->device));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     const int64_t n = ctx->n;
@@ -2714,10 +3042,12 @@ int dem_download_particles(dem_ctx* ctx, double* x, double* v, double* w, double
 int dem_contact_count(dem_ctx* ctx, int64_t* n)
 {
     if (!ctx || !n) return DEM_ERR_ARG;
-    if (!ctx->uploaded) {
+    if (!ctx->uploaded || ctx->n == 0) {
         *n = 0;
         return DEM_OK;
     }
+    int r = match_uploaded(ctx);
+    if (r) return r;
     HIPCHK(hipSetDevice(ctx->device));
     int zero[2] = {0, 0};
     HIPCHK(hipMemcpyAsync(ctx->d_flags + F_COUNT_A, zero, sizeof zero, hipMemcpyHostToDevice, ctx->stream));
@@ -2849,7 +3179,7 @@ int dem_get_stats(dem_ctx* ctx, dem_stats* s)
     s->t_force_ms = ctx->t_force;
     s->t_kick_ms = ctx->t_kick;
     s->t_rebuild_ms = ctx->t_rebuild;
-    if (!ctx->uploaded) return DEM_OK;
+    if (!ctx->uploaded || ctx->n == 0) return DEM_OK;
     int r = check_flags(ctx);
     if (r) return r;
     s->n_rebuilds = ctx->last_rebuilds;

@@ -137,6 +137,8 @@ def _declare(lib, prefix: str, has_config: bool):
     f("step_begin", _I32, _PTR)
     f("step_end", _I32, _PTR)
     f("set_kinematics", _I32, _PTR, _I64, _PTR, _PTR, _PTR, _PTR)
+    f("add_particles", _I32, _PTR, _I64, *([_PTR] * 9))
+    f("check_particles", _I32, _PTR, _I64, *([_PTR] * 5))
     f("cg_z", _I32, _PTR, C.POINTER(CGConfig), _PTR)
     f("download_particles", _I32, _PTR, *([_PTR] * 6))
     f("contact_count", _I32, _PTR, C.POINTER(_I64))
@@ -331,6 +333,32 @@ class Engine:
         for a, m in ((x, 3), (v, 3), (w, 3), (q, 4)):
             arrs.append(None if a is None else np.ascontiguousarray(np.asarray(a, np.float64).reshape(self.n, m)))
         self._check(self._fn("set_kinematics")(self.h, self.n, *[_ptr(a) for a in arrs]))
+
+    def add_particles(self, s: State):
+        """ParticleHandler::copyAndAddObject during a run (between step_begin
+        and step_end, as insertion boundaries do): append the particles of
+        `s` (ids above every resident id) after the resident ones."""
+        def c(a, dt):
+            return None if a is None else np.ascontiguousarray(a, dtype=dt)
+        arrs = [c(s.id, np.uint32), c(s.species, np.int32), c(s.radius, np.float64),
+                c(s.inv_mass, np.float64), c(s.inv_inertia, np.float64), c(s.x, np.float64),
+                c(s.v, np.float64), c(s.w, np.float64), c(s.q, np.float64)]
+        self._check(self._fn("add_particles")(self.h, s.n, *[_ptr(a) for a in arrs]))
+        self.n += s.n
+
+    def check_particles(self, x, radius, species, max_distance) -> np.ndarray:
+        """DPMBase::checkParticleForInteraction for candidate spheres against
+        the resident particles and the walls: True where a candidate (and its
+        periodic images within max_distance of a boundary; none where
+        max_distance < 0) touches nothing."""
+        x = np.ascontiguousarray(np.asarray(x, np.float64).reshape(-1, 3))
+        k = len(x)
+        r = np.ascontiguousarray(np.broadcast_to(np.asarray(radius, np.float64), (k,)))
+        sp = np.ascontiguousarray(np.broadcast_to(np.asarray(species, np.int32), (k,)))
+        md = np.ascontiguousarray(np.broadcast_to(np.asarray(max_distance, np.float64), (k,)))
+        out = np.zeros(k, np.int32)
+        self._check(self._fn("check_particles")(self.h, k, _ptr(x), _ptr(r), _ptr(sp), _ptr(md), _ptr(out)))
+        return out.astype(bool)
 
     # ------------------------------------------------------------- download
     def download(self):

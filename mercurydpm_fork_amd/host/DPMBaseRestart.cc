@@ -10,9 +10,9 @@
 
 #include "Mercury3D.h"
 
-namespace {
-
 std::ostream& operator<<(std::ostream& os, const Vec3D& v) { return os << v.X << ' ' << v.Y << ' ' << v.Z; }
+
+namespace {
 
 // operator<<(FileType) (File.cc:55-70)
 const char* fileTypeName(FileType t)
@@ -228,6 +228,14 @@ void CylindricalWall::write(std::ostream& os) const
     os << "  radius " << radius_;
 }
 
+void BaseParticle::write(std::ostream& os) const
+{
+    os << "BaseParticle id " << id_ << " indSpecies " << getIndSpecies() << " position " << position_
+       << " orientation " << q_[0] << ' ' << q_[1] << ' ' << q_[2] << ' ' << q_[3] << " velocity " << velocity_
+       << " angularVelocity " << angularVelocity_ << " force " << force_ << " torque " << torque_ << " radius "
+       << radius_ << " invMass " << invMass_ << " timeStamp " << timeStamp_;
+}
+
 void PeriodicBoundary::write(std::ostream& os) const
 {
     os << "PeriodicBoundary id " << id_ << " normal " << normal_ << " scaleFactor " << scaleFactor_
@@ -318,12 +326,8 @@ void DPMBase::write(std::ostream& os)
     // BaseParticle::write (BaseParticle.cc:340-346)
     os << "Particles " << particleHandler.getNumberOfObjects() << '\n';
     for (auto& p : particleHandler) {
-        const double* q = p->getOrientation();
-        os << "BaseParticle id " << p->getId() << " indSpecies " << p->getIndSpecies() << " position "
-           << p->getPosition() << " orientation " << q[0] << ' ' << q[1] << ' ' << q[2] << ' ' << q[3]
-           << " velocity " << p->getVelocity() << " angularVelocity " << p->getAngularVelocity() << " force "
-           << p->getForce() << " torque " << p->getTorque() << " radius " << p->getRadius() << " invMass "
-           << p->getInvMass() << " timeStamp 0\n";
+        p->write(os);
+        os << '\n';
     }
     // InteractionHandler::write (InteractionHandler.cc:433-447) with
     // BaseInteraction::write (BaseInteraction.cc:159-177) and the friction
@@ -523,6 +527,7 @@ void DPMBase::read(std::istream& is)
         q->velocity_ = f.vec("velocity");
         q->angularVelocity_ = f.vec("angularVelocity");
         q->id_ = (unsigned)f.get("id");
+        q->timeStamp_ = (unsigned)f.get("timeStamp", 0.0);
         maxId = std::max(maxId, q->id_);
     }
     particleHandler.nextId_ = np ? maxId + 1 : 0;

@@ -177,11 +177,19 @@ BaseParticle* ParticleHandler::copyAndAddObject(const BaseParticle& p)
     auto q = std::make_unique<BaseParticle>(p);
     q->id_ = nextId_++;
     q->index_ = getNumberOfObjects();
+    q->timeStamp_ = dpm_ ? dpm_->getNumberOfTimeSteps() : 0;
     q->dims_ = dims_;
     if (!q->species_) throw std::invalid_argument("particle has no species");
     q->species_->computeMass(q.get());
     objects_.push_back(std::move(q));
     return objects_.back().get();
+}
+
+double ParticleHandler::getLargestRadius() const
+{
+    double r = 0.0;
+    for (const auto& p : objects_) r = std::max(r, p->radius_);
+    return r;
 }
 
 // ------------------------------------------------------------------ RNG
@@ -394,6 +402,7 @@ void PeriodicBoundary::setFromRestart(const Vec3D& normal, double scale, double 
 // restart files with 15 significant digits
 DPMBase::DPMBase()
 {
+    particleHandler.dpm_ = this;
     setName(name_);
     setSaveCount(20);
     restartFile.getFstream().precision(std::numeric_limits<double>::digits10);
@@ -450,7 +459,8 @@ void DPMBase::initialiseSolve()
     for (auto& w : wallHandler) walls.push_back(w->toEngine());
     check(dem_set_walls(ctx_, (int32_t)walls.size(), walls.data()), "dem_set_walls");
     std::vector<dem_periodic> per;
-    for (auto& b : boundaryHandler) per.push_back(b->toEngine());
+    for (auto& b : boundaryHandler)
+        if (auto* pb = dynamic_cast<PeriodicBoundary*>(b.get())) per.push_back(pb->toEngine());
     check(dem_set_periodic(ctx_, (int32_t)per.size(), per.data()), "dem_set_periodic");
     const double g[3] = {gravity_.X, gravity_.Y, gravity_.Z};
     check(dem_set_gravity(ctx_, g), "dem_set_gravity");
@@ -481,6 +491,8 @@ void DPMBase::initialiseSolve()
     check(dem_upload_particles(ctx_, (int64_t)n, id.data(), sp.data(), r.data(), im.data(), ii.data(), x.data(),
                                v.data(), w.data(), q.data(), f.data(), t.data()),
           "dem_upload_particles");
+    devN_ = (int64_t)n;
+    solving_ = true;
     if (restarted_) uploadInteractions();
     check(dem_compute_forces(ctx_), "dem_compute_forces");
     particlesSynced_ = interactionsSynced_ = false;
@@ -494,6 +506,13 @@ void DPMBase::computeOneTimeStep()
 {
     writeOutputFiles();
     check(dem_step_begin(ctx_), "dem_step_begin");
+    if (boundariesActBeforeTimeStep()) {
+        // insertion boundaries see the drifted state; what they add joins
+        // this step's force pass
+        particlesSynced_ = false;
+        for (auto& b : boundaryHandler) b->checkBoundaryBeforeTimeStep(this);
+        flushAddedParticles();
+    }
     runBeforeHook();
     check(dem_step_end(ctx_), "dem_step_end");
     particlesSynced_ = interactionsSynced_ = false;
@@ -509,11 +528,161 @@ void DPMBase::runBeforeHook()
     synchroniseParticles();
     beforeHookIsBase_ = false;
     actionsBeforeTimeStep();
+    flushAddedParticles();
     if (beforeHookIsBase_) {
         beforeHookMaybeOverridden_ = false;
         return;
     }
     pushKinematics();
+}
+
+bool DPMBase::boundariesActBeforeTimeStep() const
+{
+    for (const auto& b : boundaryHandler)
+        if (b->actsBeforeTimeStep()) return true;
+    return false;
+}
+
+// The particles added to the handler during a run, appended on the device.
+void DPMBase::flushAddedParticles()
+{
+    const size_t n = particleHandler.getNumberOfObjects();
+    if (!solving_ || (int64_t)n <= devN_) return;
+    const size_t k = n - (size_t)devN_;
+    std::vector<uint32_t> id(k);
+    std::vector<int32_t> sp(k);
+    std::vector<double> r(k), im(k), ii(6 * k, 0.0), x(3 * k), v(3 * k), w(3 * k), q(4 * k);
+    for (size_t j = 0; j < k; ++j) {
+        const BaseParticle* p = particleHandler.getObject((unsigned)(devN_ + j));
+        id[j] = p->id_;
+        sp[j] = (int32_t)p->getIndSpecies();
+        r[j] = p->radius_;
+        im[j] = p->invMass_;
+        ii[6 * j] = ii[6 * j + 3] = ii[6 * j + 5] = p->invInertia_;
+        const Vec3D* vs[3] = {&p->position_, &p->velocity_, &p->angularVelocity_};
+        double* ds[3] = {&x[3 * j], &v[3 * j], &w[3 * j]};
+        for (int a = 0; a < 3; ++a) {
+            ds[a][0] = vs[a]->X;
+            ds[a][1] = vs[a]->Y;
+            ds[a][2] = vs[a]->Z;
+        }
+        for (int a = 0; a < 4; ++a) q[4 * j + a] = p->q_[a];
+    }
+    check(dem_add_particles(ctx_, (int64_t)k, id.data(), sp.data(), r.data(), im.data(), ii.data(), x.data(), v.data(),
+                            w.data(), q.data()),
+          "dem_add_particles");
+    devN_ = (int64_t)n;
+}
+
+// The candidate's images of checkParticleForInteractionLocalPeriodic
+// (DPMBase.cc:4868-4905): per periodic boundary, shifted copies of the images
+// made so far (newest first) and of the candidate, each when nearer to the
+// boundary than the candidate's radius plus the largest particle radius.
+std::vector<Vec3D> DPMBase::periodicImages(const BaseParticle& p) const
+{
+    std::vector<Vec3D> img;
+    if (particleHandler.getNumberOfObjects() == 0) return img;
+    const double maxDistance = p.getRadius() + particleHandler.getLargestRadius();
+    for (const auto& b : boundaryHandler) {
+        const auto* pb = dynamic_cast<const PeriodicBoundary*>(b.get());
+        if (!pb) continue;
+        const dem_periodic e = pb->toEngine();
+        auto distance = [&](const Vec3D& x) {
+            const double d = e.axis == 0 ? x.X : (e.axis == 1 ? x.Y : x.Z);
+            return std::min(d - e.left, e.right - d);
+        };
+        auto shifted = [&](Vec3D x) {
+            double& c = e.axis == 0 ? x.X : (e.axis == 1 ? x.Y : x.Z);
+            const double d = c;
+            c = (e.right - d > d - e.left) ? c + e.shift : c - e.shift;
+            return x;
+        };
+        for (int i = (int)img.size() - 1; i >= 0; --i)
+            if (distance(img[i]) < maxDistance) img.push_back(shifted(img[i]));
+        if (distance(p.getPosition()) < maxDistance) img.push_back(shifted(p.getPosition()));
+    }
+    return img;
+}
+
+// checkParticleForInteractionLocal on the host for the candidate and its
+// images: the walls (when `walls`) and the particles from handler index
+// `first` on, with Vec3D::getDistanceSquared against the square of the sum
+// of the radii (BaseParticle::isInContactWith, BaseParticle.cc:863-871).
+bool DPMBase::hostCheckParticle(const BaseParticle& p, size_t first, bool walls) const
+{
+    std::vector<Vec3D> pos = periodicImages(p);
+    pos.insert(pos.begin(), p.getPosition());
+    const double r = p.getRadius();
+    if (walls) {
+        for (const auto& w : wallHandler) {
+            const dem_wall e = w->toEngine();
+            for (const Vec3D& x : pos) {
+                if (e.type == DEM_WALL_INFINITE) {
+                    const double d = e.dist_coef[0] * (e.position[0] - x.X) +
+                                     (e.dist_coef[1] * (e.position[1] - x.Y) + e.dist_coef[2] * (e.position[2] - x.Z));
+                    if (d < r) return false;
+                } else if (e.radius - std::sqrt(x.X * x.X + x.Y * x.Y) < r) {
+                    return false;
+                }
+            }
+        }
+    }
+    const size_t n = particleHandler.getNumberOfObjects();
+    auto touches = [&](const Vec3D& x, const BaseParticle* q) {
+        const double X = x.X - q->position_.X, Y = x.Y - q->position_.Y, Z = x.Z - q->position_.Z;
+        const double s = r + q->radius_;
+        return X * X + Y * Y + Z * Z < s * s;
+    };
+    if (solving_ || n - first < 256) {
+        for (size_t k = first; k < n; ++k)
+            for (const Vec3D& x : pos)
+                if (touches(x, particleHandler.getObject((unsigned)k))) return false;
+        return true;
+    }
+    // set-up: a cell hash of side 2 x the largest radius over the handler,
+    // extended by the particles added since the last call
+    const double h = 2.0 * std::max(particleHandler.getLargestRadius(), r);
+    auto cellKey = [](int64_t a, int64_t b, int64_t c) {
+        return (uint64_t)(a & 0x1FFFFF) | ((uint64_t)(b & 0x1FFFFF) << 21) | ((uint64_t)(c & 0x1FFFFF) << 42);
+    };
+    auto cellOf = [&](double v) { return (int64_t)std::floor(v / setupCellSize_); };
+    if (h > setupCellSize_ || n < setupGridCount_) {
+        setupGrid_.clear();
+        setupGridCount_ = 0;
+        setupCellSize_ = h;
+    }
+    for (; setupGridCount_ < n; ++setupGridCount_) {
+        const Vec3D& x = particleHandler.getObject((unsigned)setupGridCount_)->position_;
+        setupGrid_[cellKey(cellOf(x.X), cellOf(x.Y), cellOf(x.Z))].push_back((unsigned)setupGridCount_);
+    }
+    for (const Vec3D& x : pos) {
+        const int64_t cx = cellOf(x.X), cy = cellOf(x.Y), cz = cellOf(x.Z);
+        for (int64_t a = cx - 1; a <= cx + 1; ++a)
+            for (int64_t b = cy - 1; b <= cy + 1; ++b)
+                for (int64_t c = cz - 1; c <= cz + 1; ++c) {
+                    auto it = setupGrid_.find(cellKey(a, b, c));
+                    if (it == setupGrid_.end()) continue;
+                    for (unsigned k : it->second)
+                        if (k >= first && touches(x, particleHandler.getObject(k))) return false;
+                }
+    }
+    return true;
+}
+
+bool DPMBase::checkParticleForInteraction(const BaseParticle& p)
+{
+    if (!solving_) return hostCheckParticle(p, 0, true);
+    // resident particles and walls on the device, at the drifted positions
+    const double x[3] = {p.getPosition().X, p.getPosition().Y, p.getPosition().Z};
+    const double r = p.getRadius();
+    const int32_t sp = (int32_t)p.getIndSpecies();
+    const double maxDistance =
+        particleHandler.getNumberOfObjects() > 0 ? r + particleHandler.getLargestRadius() : -1.0;
+    int32_t free = 0;
+    check(dem_check_particles(ctx_, 1, x, &r, &sp, &maxDistance, &free), "dem_check_particles");
+    if (!free) return false;
+    // particles added to the handler since the step began
+    return hostCheckParticle(p, (size_t)devN_, false);
 }
 
 // BaseInteractable::setPosition / setVelocity / setAngularVelocity /
@@ -547,8 +716,9 @@ void DPMBase::solve()
     // the number of steps DPMBase::solve takes: time_ accumulates dt_ while
     // time_ < timeMax (DPMBase.cc:4287-4290)
     while (time_ < tMax_ && continueSolve()) {
-        const bool hooksAreBase = !beforeHookMaybeOverridden_ && afterHookIsBase_ && continueIsBase_;
-        if (!hooksAreBase && hookStride_ == 1) {
+        const bool hooksAreBase = !beforeHookMaybeOverridden_ && afterHookIsBase_ && continueIsBase_ &&
+                                  !boundariesActBeforeTimeStep();
+        if ((!hooksAreBase && hookStride_ == 1) || boundariesActBeforeTimeStep()) {
             computeOneTimeStep();
             continue;
         }
@@ -576,13 +746,14 @@ void DPMBase::solve()
     }
     forceWriteOutputFiles();
     synchroniseHandlers();
+    solving_ = false;
     actionsAfterSolve();
 }
 
 void DPMBase::synchroniseParticles()
 {
     if (!ctx_ || particlesSynced_) return;
-    const size_t n = particleHandler.getNumberOfObjects();
+    const size_t n = std::min<size_t>(particleHandler.getNumberOfObjects(), (size_t)devN_);
     std::vector<double> x(3 * n), v(3 * n), w(3 * n), q(4 * n), f(3 * n), t(3 * n);
     check(dem_download_particles(ctx_, x.data(), v.data(), w.data(), q.data(), f.data(), t.data()),
           "dem_download_particles");

@@ -26,6 +26,7 @@
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/mercurydpm_dem.h"
@@ -64,6 +65,8 @@ struct Vec3D {
     // Vec3D::isEqualTo (Vector.cc): squared distance within tol^2
     bool isEqualTo(const Vec3D& o, double tol) const { return (*this - o).getLengthSquared() <= tol * tol; }
 };
+// Vec3D operator<< (Vector.cc): the three components separated by spaces
+std::ostream& operator<<(std::ostream& os, const Vec3D& v);
 
 // --------------------------------------------------------- Quaternion ---
 // The four components (q0, q1, q2, q3) of an orientation (Quaternion.h).
@@ -304,7 +307,14 @@ public:
     void setRadius(double r) { radius_ = r; }
     void setSpecies(const ParticleSpecies* s) { species_ = s; }
     // BaseParticle::fixParticle (BaseParticle.cc:171-183)
-    void fixParticle() { fixed_ = true; velocity_ = Vec3D(); angularVelocity_ = Vec3D(); }
+    void fixParticle()
+    {
+        fixed_ = true;
+        invMass_ = 0.0;
+        invInertia_ = 0.0;
+        velocity_ = Vec3D();
+        angularVelocity_ = Vec3D();
+    }
     const Vec3D& getPosition() const { return position_; }
     const Vec3D& getVelocity() const { return velocity_; }
     const Vec3D& getAngularVelocity() const { return angularVelocity_; }
@@ -315,21 +325,27 @@ public:
     double getInvMass() const { return invMass_; }
     double getInvInertia() const { return invInertia_; }
     double getMass() const { return 1.0 / invMass_; }
-    bool isFixed() const { return invMass_ == 0.0; }
+    bool isFixed() const { return fixed_ || invMass_ == 0.0; }
     unsigned getId() const { return id_; }
     unsigned getIndex() const { return index_; }
     const ParticleSpecies* getSpecies() const { return species_; }
     unsigned getIndSpecies() const { return species_ ? species_->getIndex() : 0; }
+    unsigned getTimeStamp() const { return timeStamp_; }
+    // BaseParticle::write (BaseParticle.cc:340-346)
+    void write(std::ostream& os) const;
 
 protected:
     friend class ParticleHandler;
     friend class DPMBase;
     Vec3D position_, velocity_, angularVelocity_, force_, torque_;
     double q_[4] = {1, 0, 0, 0};
-    double radius_ = 1.0;  // BaseParticle default radius (BaseParticle.cc:36)
-    double invMass_ = 0.0, invInertia_ = 0.0;
+    // BaseParticle defaults (BaseParticle.cc:36-37)
+    double radius_ = 1.0;
+    double invMass_ = 1.0, invInertia_ = 0.0;
     bool fixed_ = false;
     unsigned id_ = 0, index_ = 0;
+    // the step count at which the particle was added (ParticleHandler.cc:210)
+    unsigned timeStamp_ = 0;
     unsigned dims_ = 3;  // the particle dimensions its mass was computed for
     const ParticleSpecies* species_ = nullptr;
     friend class ParticleSpecies;
@@ -398,7 +414,28 @@ private:
 };
 
 // ---------------------------------------------------------- Boundaries ----
-class PeriodicBoundary {
+class DPMBase;
+
+// BaseBoundary (BaseBoundary.h): what the BoundaryHandler holds
+class BaseBoundary {
+public:
+    virtual ~BaseBoundary() = default;
+    unsigned getId() const { return id_; }
+    virtual std::string getName() const = 0;
+    virtual void write(std::ostream& os) const = 0;
+    // BaseBoundary::checkBoundaryBeforeTimeStep, called by every step after
+    // the drift and before actionsBeforeTimeStep (DPMBase.cc:4348-4351)
+    virtual void checkBoundaryBeforeTimeStep(DPMBase*) {}
+    // whether checkBoundaryBeforeTimeStep has work to do: the host then
+    // runs the steps one at a time
+    virtual bool actsBeforeTimeStep() const { return false; }
+
+protected:
+    friend class BoundaryHandler;
+    unsigned id_ = 0;
+};
+
+class PeriodicBoundary : public BaseBoundary {
 public:
     // PeriodicBoundary::set (PeriodicBoundary.cc:84-94); the normal must be a
     // coordinate axis
@@ -406,7 +443,8 @@ public:
     dem_periodic toEngine() const { return b_; }
     unsigned getId() const { return id_; }
     // PeriodicBoundary::write (PeriodicBoundary.cc:426-434)
-    void write(std::ostream& os) const;
+    void write(std::ostream& os) const override;
+    std::string getName() const override { return "PeriodicBoundary"; }
     // the fields of a restart line: normal (unit), scaleFactor,
     // distanceLeft, distanceRight, shift
     void setFromRestart(const Vec3D& normal, double scale, double left, double right, const Vec3D& shift);
@@ -416,7 +454,6 @@ private:
     dem_periodic b_{};
     Vec3D normal_, shift_;
     double scaleFactor_ = 1.0, distanceLeft_ = 0.0, distanceRight_ = 0.0;
-    unsigned id_ = 0;
 };
 
 // ------------------------------------------------------------ Handlers ----
@@ -430,6 +467,8 @@ public:
     T* getLastObject() { return objects_.back().get(); }
     typename std::vector<std::unique_ptr<T>>::iterator begin() { return objects_.begin(); }
     typename std::vector<std::unique_ptr<T>>::iterator end() { return objects_.end(); }
+    typename std::vector<std::unique_ptr<T>>::const_iterator begin() const { return objects_.begin(); }
+    typename std::vector<std::unique_ptr<T>>::const_iterator end() const { return objects_.end(); }
     void clear() { objects_.clear(); }
 
 protected:
@@ -468,9 +507,14 @@ public:
     void setParticleDimensions(unsigned d) { dims_ = d; }
 
     unsigned getParticleDimensions() const { return dims_; }
+    // ParticleHandler::getLargestParticle()->getRadius() (0 when empty); the
+    // particles' interaction distances are zero here, so this is also their
+    // largest getMaxInteractionRadius
+    double getLargestRadius() const;
 
 private:
     friend class DPMBase;
+    const DPMBase* dpm_ = nullptr;  // the problem owning this handler
     unsigned nextId_ = 0;
     unsigned dims_ = 3;
 };
@@ -491,13 +535,16 @@ private:
     unsigned nextId_ = 0;
 };
 
-class BoundaryHandler : public HandlerBase<PeriodicBoundary> {
+class BoundaryHandler : public HandlerBase<BaseBoundary> {
 public:
-    PeriodicBoundary* copyAndAddObject(const PeriodicBoundary& b)
+    template <class B>
+    B* copyAndAddObject(const B& b)
     {
-        objects_.push_back(std::make_unique<PeriodicBoundary>(b));
-        objects_.back()->id_ = getNumberOfObjects() - 1;
-        return objects_.back().get();
+        auto p = std::make_unique<B>(b);
+        B* raw = p.get();
+        raw->id_ = getNumberOfObjects();
+        objects_.push_back(std::move(p));
+        return raw;
     }
 };
 
@@ -623,6 +670,15 @@ public:
     double getGravitationalEnergy();
     double getTotalMass();
     Vec3D getCentreOfMass();
+    // DPMBase::getNextTime: the time at the end of the current step
+    double getNextTime() const { return time_ + dt_; }
+    // DPMBase::checkParticleForInteraction (DPMBase.cc:4819-4905, with
+    // MercuryBase::checkParticleForInteractionLocal, MercuryBase.cc:622-650):
+    // true when p, and its images near periodic boundaries, touch neither a
+    // wall nor a particle.  During a run the resident particles are tested on
+    // the device (dem_check_particles) and the ones added since the step
+    // began on the host; before a run (set-up) the handlers are tested here.
+    bool checkParticleForInteraction(const BaseParticle& p);
     // engine options
     void setDevice(int d) { device_ = d; }
     void setVerletSkin(double s) { skin_ = s; }
@@ -676,6 +732,18 @@ protected:
 private:
     void initialiseSolve();
     void check(int rc, const char* what);
+    // send the particles added to the handler since the last upload
+    void flushAddedParticles();
+    bool boundariesActBeforeTimeStep() const;
+    bool hostCheckParticle(const BaseParticle& p, size_t first, bool walls) const;
+    std::vector<Vec3D> periodicImages(const BaseParticle& p) const;
+    int64_t devN_ = 0;     // particles resident on the device
+    bool solving_ = false;  // between the upload of initialiseSolve and the end of solve
+    // set-up checks: a cell hash over the particle handler (particles are
+    // taken to stay where they were added until the run starts)
+    mutable std::unordered_map<uint64_t, std::vector<unsigned>> setupGrid_;
+    mutable size_t setupGridCount_ = 0;
+    mutable double setupCellSize_ = 0;
     void runBeforeHook();
     void pushKinematics();
     std::string name_ = "out";

@@ -41,6 +41,17 @@ int dem_set_kinematics(dem_ctx* ctx, int64_t n, const double* x, const double* v
 {
     return oracle_dem_set_kinematics(O(ctx), n, x, v, w, q);
 }
+int dem_add_particles(dem_ctx* ctx, int64_t k, const uint32_t* id, const int32_t* species, const double* radius,
+                      const double* inv_mass, const double* inv_inertia, const double* x, const double* v,
+                      const double* w, const double* q)
+{
+    return oracle_dem_add_particles(O(ctx), k, id, species, radius, inv_mass, inv_inertia, x, v, w, q);
+}
+int dem_check_particles(dem_ctx* ctx, int64_t k, const double* x, const double* radius, const int32_t* species,
+                        const double* max_distance, int32_t* free_out)
+{
+    return oracle_dem_check_particles(O(ctx), k, x, radius, species, max_distance, free_out);
+}
 int dem_download_particles(dem_ctx* ctx, double* x, double* v, double* w, double* q, double* f, double* t)
 {
     return oracle_dem_download_particles(O(ctx), x, v, w, q, f, t);

@@ -1097,6 +1097,104 @@ int oracle_dem_set_kinematics(oracle_ctx* c, int64_t n, const double* x, const d
     return DEM_OK;
 }
 
+/* ParticleHandler::copyAndAddObject between the two halves of a step, where
+ * InsertionBoundary::checkBoundaryBeforeTimeStep (InsertionBoundary.cc:
+ * 184-310) adds particles (DPMBase.cc:4348-4351): no ghosts exist there, so
+ * the new particles follow the real ones, with no interactions. */
+int oracle_dem_add_particles(oracle_ctx* c, int64_t k, const uint32_t* id, const int32_t* species,
+                             const double* radius, const double* inv_mass, const double* inv_inertia,
+                             const double* x, const double* v, const double* w, const double* q)
+{
+    if (!c || k < 0) return fail(c, DEM_ERR_ARG, "add_particles: bad arguments");
+    if (c->nP != c->nReal) return fail(c, DEM_ERR_STATE, "add_particles while periodic ghosts exist");
+    ensureParticleCapacity(c, c->nReal + k + 1);
+    for (int64_t j = 0; j < k; ++j) {
+        if ((c->nReal > 0 || j > 0) && id[j] < c->nextId) return fail(c, DEM_ERR_ARG, "ids must increase");
+        if (species[j] < 0 || species[j] >= c->nSpecies) return fail(c, DEM_ERR_ARG, "species index out of range");
+        Particle* p = &c->p[c->nReal];
+        memset(p, 0, sizeof *p);
+        p->id = id[j];
+        p->species = species[j];
+        p->radius = radius[j];
+        p->invMass = inv_mass[j];
+        for (int a = 0; a < 6; ++a) p->invI[a] = inv_inertia[6 * j + a];
+        p->pos = v3(x[3 * j], x[3 * j + 1], x[3 * j + 2]);
+        p->vel = v3(v[3 * j], v[3 * j + 1], v[3 * j + 2]);
+        p->omega = w ? v3(w[3 * j], w[3 * j + 1], w[3 * j + 2]) : v3(0, 0, 0);
+        if (q) {
+            for (int a = 0; a < 4; ++a) p->q[a] = q[4 * j + a];
+        } else {
+            p->q[0] = 1.0;
+        }
+        p->from = -1;
+        if (p->radius > c->largestRadius) c->largestRadius = p->radius;
+        c->nextId = id[j] + 1;
+        c->nP = ++c->nReal;
+    }
+    return DEM_OK;
+}
+
+/* MercuryBase::checkParticleForInteractionLocal (MercuryBase.cc:622-650):
+ * the walls, then every real particle (BaseParticle::isInContactWith,
+ * BaseParticle.cc:863-871, with Vec3D::getDistanceSquared, Vector.h:311-317). */
+static int insertableLocal(const oracle_ctx* c, V3 pos, double radius, int32_t species)
+{
+    Particle cand;
+    memset(&cand, 0, sizeof cand);
+    cand.pos = pos;
+    cand.radius = radius;
+    cand.species = species;
+    for (int32_t w = 0; w < c->nWalls; ++w) {
+        double distance;
+        V3 normal;
+        const double wallRadius = radius + mixed(c, species, c->walls[w].species)->interaction_distance;
+        if (wallDistanceAndNormal(&c->walls[w], &cand, wallRadius, &distance, &normal)) return 0;
+    }
+    for (int64_t i = 0; i < c->nReal; ++i) {
+        const Particle* q = &c->p[i];
+        const double X = pos.X - q->pos.X, Y = pos.Y - q->pos.Y, Z = pos.Z - q->pos.Z;
+        const double s = radius + q->radius + mixed(c, species, q->species)->interaction_distance;
+        if (X * X + Y * Y + Z * Z < s * s) return 0;
+    }
+    return 1;
+}
+
+/* DPMBase::checkParticleForInteractionLocalPeriodic (DPMBase.cc:4868-4905):
+ * the candidate and, per periodic boundary, shifted copies of the images made
+ * so far (newest first) and of the candidate when nearer to the boundary than
+ * maxDistance; the candidate is insertable when none of them touches
+ * anything.  maxDistance < 0 stands for an empty particle handler. */
+int oracle_dem_check_particles(oracle_ctx* c, int64_t k, const double* x, const double* radius,
+                               const int32_t* species, const double* max_distance, int32_t* free_out)
+{
+    if (!c || k < 0) return fail(c, DEM_ERR_ARG, "check_particles: bad arguments");
+    for (int64_t j = 0; j < k; ++j) {
+        const V3 pos = v3(x[3 * j], x[3 * j + 1], x[3 * j + 2]);
+        V3 img[8];
+        int m = 0;
+        if (max_distance[j] >= 0.0) {
+            for (int b = 0; b < c->nPeriodic; ++b) {
+                for (int i = m - 1; i >= 0; --i) {
+                    if (periodicDistance(&c->periodic[b], img[i]) < max_distance[j]) {
+                        img[m] = img[i];
+                        periodicShift(&c->periodic[b], &img[m]);
+                        ++m;
+                    }
+                }
+                if (periodicDistance(&c->periodic[b], pos) < max_distance[j]) {
+                    img[m] = pos;
+                    periodicShift(&c->periodic[b], &img[m]);
+                    ++m;
+                }
+            }
+        }
+        int ok = insertableLocal(c, pos, radius[j], species[j]);
+        for (int i = 0; i < m; ++i) ok &= insertableLocal(c, img[i], radius[j], species[j]);
+        free_out[j] = ok;
+    }
+    return DEM_OK;
+}
+
 int oracle_dem_download_particles(oracle_ctx* c, double* x, double* v, double* w, double* q, double* f,
                                   double* t)
 {

@@ -39,6 +39,11 @@ int oracle_dem_set_kinematics(oracle_ctx* c, int64_t n, const double* x, const d
                               const double* q);
 int oracle_dem_download_particles(oracle_ctx* c, double* x, double* v, double* w, double* q, double* f,
                                   double* t);
+int oracle_dem_add_particles(oracle_ctx* c, int64_t k, const uint32_t* id, const int32_t* species,
+                             const double* radius, const double* inv_mass, const double* inv_inertia,
+                             const double* x, const double* v, const double* w, const double* q);
+int oracle_dem_check_particles(oracle_ctx* c, int64_t k, const double* x, const double* radius,
+                               const int32_t* species, const double* max_distance, int32_t* free_out);
 int oracle_dem_contact_count(oracle_ctx* c, int64_t* n);
 int oracle_dem_download_contacts(oracle_ctx* c, int64_t cap, int64_t* n_out, uint32_t* p_id, uint32_t* i_id,
                                  double* spring, uint32_t* ts);

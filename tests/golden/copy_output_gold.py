@@ -10,7 +10,8 @@ DPMBase.cc:2100-2126; these gold files predate the `collisions` column) and
 the energies and XBalls data of ForceSelfTest (cases drawn from the
 problem's random number generator), the XBalls data of ExtremeOverlapUnitTest (DPMBase::outputXBallsData,
 DPMBase.cc:2356-2417), and the final restart files of FreeFallSelfTest and
-TwoParticleElasticCollisionSelfTest (DPMBase::write, DPMBase.cc:3550-3690).
+TwoParticleElasticCollisionSelfTest (DPMBase::write, DPMBase.cc:3550-3690),
+and all four output files of InsertionBoundarySelfTest.
 """
 import os
 import shutil
@@ -31,6 +32,11 @@ for k in range(10):
     FILES[f"Drivers/SelfTests/Interactions/SelfTestData/ForceSelfTest{k}.ene"] = f"force_self_test_{k}.ene"
 for k in (2, 7):
     FILES[f"Drivers/SelfTests/Interactions/SelfTestData/ForceSelfTest{k}.data"] = f"force_self_test_{k}.data"
+# InsertionBoundarySelfTest: cube, cylinder and sphere insertion boundaries
+# (InsertionBoundary::checkBoundaryBeforeTimeStep, InsertionBoundary.cc:184-310)
+for ext in ("data", "ene", "fstat", "restart"):
+    FILES[f"Drivers/SelfTests/Boundaries/SelfTestData/InsertionBoundarySelfTest.{ext}"] = \
+        f"insertion_boundary_self_test.{ext}"
 
 
 def main(reference):

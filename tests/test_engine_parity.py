@@ -415,3 +415,88 @@ def test_mixed_precision_against_fp64(laws, seed):
     terr = np.linalg.norm(fb["t"] - fa["t"], axis=1)[m] / (scale[m] * sc.state.radius[m])
     assert err.mean() <= 1e-6 and err.max() <= 2e-4, (err.mean(), err.max())
     assert terr.mean() <= 1e-6 and terr.max() <= 2e-4, (terr.mean(), terr.max())
+
+
+def _subset(sc, idx):
+    """The scenario restricted to the particles idx (the rest are added later)."""
+    st = sc.state
+    part = dem.State(id=st.id[idx], species=st.species[idx], radius=st.radius[idx], inv_mass=st.inv_mass[idx],
+                     inv_inertia=st.inv_inertia[idx], x=st.x[idx], v=st.v[idx], w=st.w[idx], q=st.q[idx],
+                     f=None if st.f is None else st.f[idx], t=None if st.t is None else st.t[idx])
+    return S.Scenario(sc.name, sc.species, sc.walls, sc.periodic, sc.gravity, sc.dt, part)
+
+
+@pytest.mark.parametrize("case", [CASES[0], CASES[1], CASES[4]], ids=["lvf-P", "lvf-np-3w", "hvm-Px"])
+def test_check_particles_against_oracle(case):
+    """DPMBase::checkParticleForInteraction on the device (dem_check_particles)
+    against the oracle: 6000 candidates spread over the box and across its
+    periodic faces and walls, with periodic images (the range of a handler
+    holding the largest particle) and without (an empty handler): the same
+    decision for every candidate."""
+    sc = S.mixture(n=400, seed=41, lx=7.0, ly=7.0, lz=7.0, inflate=1.12, **case)
+    g, o = make_pair(sc)
+    g.compute_forces()
+    o.compute_forces()
+    rng = np.random.default_rng(5)
+    k = 6000
+    lo, hi = sc.state.x.min(0) - 0.6, sc.state.x.max(0) + 0.6
+    x = rng.uniform(lo, hi, (k, 3))
+    r = rng.uniform(0.03, 0.3, k)
+    sp = rng.integers(0, sc.n_species, k)
+    for md in (r + sc.state.radius.max(), -np.ones(k)):
+        a, b = g.check_particles(x, r, sp, md), o.check_particles(x, r, sp, md)
+        np.testing.assert_array_equal(a, b)
+        assert 0.01 < a.mean() < 0.99, a.mean()
+
+
+@pytest.mark.parametrize("case", [CASES[0], CASES[4]], ids=["lvf-P", "hvm-Px"])
+def test_add_particles_mid_step(case):
+    """Particles appended between step_begin and step_end, where insertion
+    boundaries add them (DPMBase.cc:4348-4351), take part in that step's
+    force pass: after the step the contacts and positions equal the oracle's
+    with the same append; 20 steps later they still agree to round-off."""
+    sc = S.mixture(n=400, seed=43, lx=7.0, ly=7.0, lz=7.0, inflate=1.12, **case)
+    first, later = np.arange(300), np.arange(300, 400)
+    base = _subset(sc, first)
+    ref = oracle_lib.engine()
+    S.apply(ref, base)
+    ref.compute_forces()
+    g, o = make_pair(reseeded(base, ref.download()), ref.contacts())
+    added = _subset(sc, later).state
+    for e in (g, o):
+        e.step_begin()
+        e.add_particles(added)
+        e.step_end()
+    assert g.n == o.n == 400
+    assert_contacts_equal(g, o)
+    a, b = assert_forces_close(g, o)
+    np.testing.assert_array_equal(a["x"], b["x"])
+    g.step(20)
+    o.step(20)
+    assert_contacts_close(g, o)
+    a, b = g.download(), o.download()
+    np.testing.assert_allclose(a["x"], b["x"], rtol=0, atol=1e-9)
+
+
+def test_empty_system_then_insertion():
+    """A run that starts empty (its insertion boundaries fill it): steps of
+    an empty system only advance the clock; particles added later run as on
+    the oracle."""
+    sc = S.mixture(n=300, seed=45, lx=7.0, ly=7.0, lz=7.0, inflate=1.12, laws="lvf")
+    empty = _subset(sc, np.arange(0))
+    g, o = make_pair(empty)
+    for e in (g, o):
+        e.compute_forces()
+        e.step(3)
+        e.step_begin()
+        assert e.download()["x"].shape == (0, 3)
+        assert e.check_particles([[1.0, 1.0, 1.0]], 0.3, 0, -1.0).shape == (1,)
+        e.add_particles(sc.state)
+        e.step_end()
+    assert g.stats().n_steps == 4
+    a, b = g.download(), o.download()
+    np.testing.assert_array_equal(a["x"], b["x"])
+    g.step(10)
+    o.step(10)
+    assert_contacts_close(g, o)
+    np.testing.assert_allclose(g.download()["x"], o.download()["x"], rtol=0, atol=1e-9)

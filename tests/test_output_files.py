@@ -47,6 +47,12 @@ CASES = {
     "ForceSelfTest": ("ForceSelfTest", [],
                       [(f"ForceSelfTest{k}.ene", f"force_self_test_{k}.ene", "ene") for k in range(10)] +
                       [(f"ForceSelfTest{k}.data", f"force_self_test_{k}.data", "data") for k in (2, 7)]),
+    # three insertion boundaries fill an empty box for 201 steps: the RNG
+    # sequence, the PSD samples and every accept/reject decision of
+    # checkParticleForInteraction decide the 1942 particles of the files
+    "InsertionBoundarySelfTest": ("InsertionBoundarySelfTest", [], [
+        (f"InsertionBoundarySelfTest.{e}", f"insertion_boundary_self_test.{e}", k)
+        for e, k in (("data", "data"), ("ene", "ene"), ("fstat", "fstat"), ("restart", "restart:1e-14"))]),
     # stop at t = 1.05, restart from the .restart file (also with numbered
     # .data files), compare with the uninterrupted run (self-checking)
     "FreeFallRestartUnitTest": ("FreeFallRestartUnitTest", [], []),
