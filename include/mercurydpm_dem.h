@@ -26,12 +26,15 @@
  *    interleaved per particle (x0,y0,z0,x1,...), i.e. the memory layout of an
  *    array of Vec3D; quaternions are (q0,q1,q2,q3) per particle.
  *  - Particles are identified by their 32-bit id (BaseObject::getId).  Their
- *    position in the upload arrays is the ParticleHandler index.  Ids must
- *    increase strictly with the index (true whenever no particle has been
- *    removed since insertion); the periodic-history rule of
- *    InteractionHandler::removeObjectKeepingPeriodics (InteractionHandler.cc:
- *    316-371) compares indices, the id order of DPMBase::computeInternalForce
- *    (DPMBase.cc:3130) compares ids, and the engine relies on both agreeing.
+ *    position in the upload arrays is the ParticleHandler index.  Ids must be
+ *    distinct.  They increase with the index until a particle is removed
+ *    (ParticleHandler::removeObject moves the last particle into the gap);
+ *    the periodic-history rule of InteractionHandler::
+ *    removeObjectKeepingPeriodics (InteractionHandler.cc:316-371) compares
+ *    indices, the id order of DPMBase::computeInternalForce (DPMBase.cc:3130)
+ *    compares ids, and the engine orders both by id: after removals, a pair
+ *    whose periodic-image interaction meets its direct one keeps the record
+ *    the id order selects.
  *  - All arithmetic is IEEE binary64 evaluated in the reference's operation
  *    order without contraction (no FMA), so contact sets, springs and
  *    per-contact forces are bit-identical to the CPU path on identical input.
@@ -45,7 +48,7 @@
 extern "C" {
 #endif
 
-#define DEM_ABI_VERSION 4
+#define DEM_ABI_VERSION 5
 
 /* error codes */
 #define DEM_OK 0
@@ -281,6 +284,16 @@ int dem_set_kinematics(dem_ctx* ctx, int64_t n, const double* x, const double* v
 int dem_add_particles(dem_ctx* ctx, int64_t k, const uint32_t* id, const int32_t* species, const double* radius,
                       const double* inv_mass, const double* inv_inertia, const double* x, const double* v,
                       const double* w, const double* q);
+
+/* ParticleHandler::removeObject (ParticleHandler.cc:410-426 with
+ * BaseHandler::removeObject, BaseHandler.h:472-505) for index[0], index[1],
+ * ... in turn: the particle at that handler index is deleted together with
+ * its interactions, and the last particle takes its index.  Each index
+ * refers to the handler as the previous removals left it.  Allowed between
+ * steps and between dem_step_begin and dem_step_end, where deletion
+ * boundaries act (DPMBase::checkInteractionWithBoundaries, DPMBase.cc:
+ * 3272-3300). */
+int dem_remove_particles(dem_ctx* ctx, int64_t k, const int64_t* index);
 
 /* DPMBase::checkParticleForInteraction (DPMBase.cc:4819-4905 with
  * MercuryBase::checkParticleForInteractionLocal, MercuryBase.cc:622-650) for

@@ -1911,7 +1911,7 @@ struct dem_ctx {
     uint32_t nbuckets = 0;
     int64_t last_rebuilds = 0;
     uint32_t alive_min = 0;
-    uint32_t max_id = 0;          // largest particle id (ids increase with the handler index)
+    uint32_t max_id = 0;          // largest resident particle id
     bool hash_pending = false;    // uploaded interactions wait in the hash for the next rebuild
     int64_t rebuilds_before = 0;  // rebuilds counted before the last re-upload by dem_add_particles
     // candidate buffer of dem_check_particles (not in allocs: it outlives re-uploads)
@@ -2545,7 +2545,6 @@ int dem_upload_particles(dem_ctx* ctx, int64_t n, const uint32_t* id, const int3
             xmin[a] = std::min(xmin[a], x[3 * i + a]);
             xmax[a] = std::max(xmax[a], x[3 * i + a]);
         }
-        if (i > 0 && id[i] <= id[i - 1]) return set_err(ctx, DEM_ERR_ARG, "ids must increase with the handler index");
         if (id[i] >= DEM_WALL_KEY) return set_err(ctx, DEM_ERR_ARG, "particle ids must be < 2^31");
         if (species[i] < 0 || species[i] >= ctx->ns) return set_err(ctx, DEM_ERR_ARG, "species index out of range");
         if (!(radius[i] > 0) || !std::isfinite(radius[i])) return set_err(ctx, DEM_ERR_ARG, "radius must be positive and finite");
@@ -2556,6 +2555,15 @@ int dem_upload_particles(dem_ctx* ctx, int64_t n, const uint32_t* id, const int3
             return set_err(ctx, DEM_ERR_ARG, "fixed particles need zero inverse mass and inertia");
         rmin = std::min(rmin, radius[i]);
         rmax = std::max(rmax, radius[i]);
+    }
+    {
+        // ids identify interactions: they must be distinct (their order is
+        // free, as after ParticleHandler::removeObject)
+        std::vector<uint32_t> sorted(id, id + n);
+        std::sort(sorted.begin(), sorted.end());
+        if (std::adjacent_find(sorted.begin(), sorted.end()) != sorted.end())
+            return set_err(ctx, DEM_ERR_ARG, "particle ids must be distinct");
+        ctx->max_id = sorted.back();
     }
     const double skin = ctx->skin_cfg > 0 ? ctx->skin_cfg : 0.4 * rmin;
     // levels (EXPONENTIAL cell-size distribution of MercuryBase::hGridRebuild,
@@ -2742,7 +2750,6 @@ int dem_upload_particles(dem_ctx* ctx, int64_t n, const uint32_t* id, const int3
     ctx->alive_min = 0;
     ctx->uploaded = true;
     ctx->in_step = false;
-    ctx->max_id = id[n - 1];
     ctx->rebuilds_before = 0;
     return DEM_OK;
 }
@@ -2905,8 +2912,9 @@ int dem_add_particles(dem_ctx* ctx, int64_t k, const uint32_t* id, const int32_t
     if (k == 0) return DEM_OK;
     if (!id || !species || !radius || !inv_mass || !inv_inertia || !x || !v)
         return set_err(ctx, DEM_ERR_ARG, "add_particles: bad arguments");
-    if (ctx->n > 0 && id[0] <= ctx->max_id)
-        return set_err(ctx, DEM_ERR_ARG, "add_particles: ids must be above every resident id");
+    for (int64_t j = 0; j < k; ++j)
+        if ((ctx->n > 0 && id[j] <= ctx->max_id) || (j > 0 && id[j] <= id[j - 1]))
+            return set_err(ctx, DEM_ERR_ARG, "add_particles: ids must increase and exceed every resident id");
     if (ctx->n + k > (int64_t)0x7FFFFFFF) return set_err(ctx, DEM_ERR_ARG, "too many particles");
     HIPCHK(hipSetDevice(ctx->device));
     const int64_t n = ctx->n, m = n + k;
@@ -2952,6 +2960,85 @@ int dem_add_particles(dem_ctx* ctx, int64_t k, const uint32_t* id, const int32_t
                                   av.data(), aw.data(), aq.data(), af.data(), at.data())) != 0)
         return r;
     if (nc > 0 && (r = dem_upload_contacts(ctx, nc, cp.data(), ci.data(), cs.data(), cts.data())) != 0) return r;
+    ctx->in_step = in_step;
+    ctx->alive_min = alive_min;
+    ctx->prm.alive_min = alive_min;
+    ctx->rebuilds_before = rebuilds;
+    return DEM_OK;
+}
+
+int dem_remove_particles(dem_ctx* ctx, int64_t k, const int64_t* index)
+{
+    int r = ready(ctx);
+    if (r) return r;
+    if (k < 0 || (k > 0 && !index)) return set_err(ctx, DEM_ERR_ARG, "remove_particles: bad arguments");
+    if (k == 0) return DEM_OK;
+    int64_t n = ctx->n;
+    {
+        int64_t m = n;
+        for (int64_t j = 0; j < k; ++j) {
+            if (index[j] < 0 || index[j] >= m) return set_err(ctx, DEM_ERR_ARG, "remove_particles: index out of range");
+            --m;
+        }
+    }
+    HIPCHK(hipSetDevice(ctx->device));
+    if ((r = check_flags(ctx)) != 0) return r;
+    const int64_t rebuilds = ctx->last_rebuilds;
+    std::vector<uint32_t> aid;
+    std::vector<int32_t> asp;
+    std::vector<double> ar, aim, aii;
+    std::vector<double> ax(3 * n), av(3 * n), aw(3 * n), aq(4 * n), af(3 * n), at(3 * n);
+    if ((r = download_static(ctx, aid, asp, ar, aim, aii)) != 0) return r;
+    if ((r = dem_download_particles(ctx, ax.data(), av.data(), aw.data(), aq.data(), af.data(), at.data())) != 0)
+        return r;
+    int64_t nc = 0;
+    if ((r = dem_contact_count(ctx, &nc)) != 0) return r;
+    std::vector<uint32_t> cp(nc), ci(nc), cts(nc);
+    std::vector<double> cs(9 * nc);
+    int64_t got = 0;
+    if (nc > 0 && (r = dem_download_contacts(ctx, nc, &got, cp.data(), ci.data(), cs.data(), cts.data())) != 0)
+        return r;
+    // BaseHandler::removeObject: the last particle takes the removed index
+    std::vector<uint32_t> removed;
+    for (int64_t j = 0; j < k; ++j) {
+        const int64_t i = index[j], last = n - 1;
+        removed.push_back(aid[i]);
+        if (i != last) {
+            aid[i] = aid[last];
+            asp[i] = asp[last];
+            ar[i] = ar[last];
+            aim[i] = aim[last];
+            for (int c = 0; c < 6; ++c) aii[6 * i + c] = aii[6 * last + c];
+            for (int c = 0; c < 3; ++c) {
+                ax[3 * i + c] = ax[3 * last + c];
+                av[3 * i + c] = av[3 * last + c];
+                aw[3 * i + c] = aw[3 * last + c];
+                af[3 * i + c] = af[3 * last + c];
+                at[3 * i + c] = at[3 * last + c];
+            }
+            for (int c = 0; c < 4; ++c) aq[4 * i + c] = aq[4 * last + c];
+        }
+        --n;
+    }
+    // the interactions of a deleted particle go with it (BaseInteractable::
+    // ~BaseInteractable, BaseInteractable.cc:97-106)
+    std::sort(removed.begin(), removed.end());
+    auto gone = [&](uint32_t id) { return std::binary_search(removed.begin(), removed.end(), id); };
+    int64_t kept = 0;
+    for (int64_t c = 0; c < got; ++c) {
+        if (gone(cp[c]) || (!(ci[c] & DEM_WALL_KEY) && gone(ci[c]))) continue;
+        cp[kept] = cp[c];
+        ci[kept] = ci[c];
+        cts[kept] = cts[c];
+        for (int a = 0; a < 9; ++a) cs[9 * kept + a] = cs[9 * c + a];
+        ++kept;
+    }
+    const bool in_step = ctx->in_step;
+    const uint32_t alive_min = ctx->alive_min;
+    if ((r = dem_upload_particles(ctx, n, aid.data(), asp.data(), ar.data(), aim.data(), aii.data(), ax.data(),
+                                  av.data(), aw.data(), aq.data(), af.data(), at.data())) != 0)
+        return r;
+    if (kept > 0 && (r = dem_upload_contacts(ctx, kept, cp.data(), ci.data(), cs.data(), cts.data())) != 0) return r;
     ctx->in_step = in_step;
     ctx->alive_min = alive_min;
     ctx->prm.alive_min = alive_min;

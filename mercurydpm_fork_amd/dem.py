@@ -139,6 +139,7 @@ def _declare(lib, prefix: str, has_config: bool):
     f("set_kinematics", _I32, _PTR, _I64, _PTR, _PTR, _PTR, _PTR)
     f("add_particles", _I32, _PTR, _I64, *([_PTR] * 9))
     f("check_particles", _I32, _PTR, _I64, *([_PTR] * 5))
+    f("remove_particles", _I32, _PTR, _I64, _PTR)
     f("cg_z", _I32, _PTR, C.POINTER(CGConfig), _PTR)
     f("download_particles", _I32, _PTR, *([_PTR] * 6))
     f("contact_count", _I32, _PTR, C.POINTER(_I64))
@@ -345,6 +346,13 @@ class Engine:
                 c(s.v, np.float64), c(s.w, np.float64), c(s.q, np.float64)]
         self._check(self._fn("add_particles")(self.h, s.n, *[_ptr(a) for a in arrs]))
         self.n += s.n
+
+    def remove_particles(self, index):
+        """ParticleHandler::removeObject for each handler index in turn (the
+        last particle takes the removed one's index)."""
+        idx = np.ascontiguousarray(np.atleast_1d(np.asarray(index, np.int64)))
+        self._check(self._fn("remove_particles")(self.h, len(idx), _ptr(idx)))
+        self.n -= len(idx)
 
     def check_particles(self, x, radius, species, max_distance) -> np.ndarray:
         """DPMBase::checkParticleForInteraction for candidate spheres against

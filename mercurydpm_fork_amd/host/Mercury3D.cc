@@ -185,6 +185,26 @@ BaseParticle* ParticleHandler::copyAndAddObject(const BaseParticle& p)
     return objects_.back().get();
 }
 
+void ParticleHandler::removeObject(unsigned index)
+{
+    if (index >= objects_.size()) throw std::out_of_range("ParticleHandler::removeObject: no such index");
+    if (dpm_) dpm_->beforeParticleRemoval();
+    const unsigned last = (unsigned)objects_.size() - 1;
+    if (index != last) {
+        std::swap(objects_[index], objects_[last]);
+        objects_[index]->index_ = index;
+    }
+    const unsigned id = objects_.back()->id_;
+    objects_.pop_back();
+    if (dpm_) {
+        // the interactions of the deleted particle go with it
+        auto& c = dpm_->interactionHandler.objects();
+        c.erase(std::remove_if(c.begin(), c.end(), [&](const Interaction& x) { return x.P == id || (!x.isWall && x.I == id); }),
+                c.end());
+        dpm_->afterParticleRemoval(index);
+    }
+}
+
 double ParticleHandler::getLargestRadius() const
 {
     double r = 0.0;
@@ -506,12 +526,19 @@ void DPMBase::computeOneTimeStep()
 {
     writeOutputFiles();
     check(dem_step_begin(ctx_), "dem_step_begin");
+    particlesSynced_ = interactionsSynced_ = false;
+    if (boundariesActAfterParticlesMove()) {
+        // deletion boundaries see the drifted state
+        synchroniseParticles();
+        for (auto& b : boundaryHandler) b->checkBoundaryAfterParticlesMove(particleHandler);
+        flushHandlerChanges();
+    }
     if (boundariesActBeforeTimeStep()) {
         // insertion boundaries see the drifted state; what they add joins
         // this step's force pass
         particlesSynced_ = false;
         for (auto& b : boundaryHandler) b->checkBoundaryBeforeTimeStep(this);
-        flushAddedParticles();
+        flushHandlerChanges();
     }
     runBeforeHook();
     check(dem_step_end(ctx_), "dem_step_end");
@@ -528,7 +555,7 @@ void DPMBase::runBeforeHook()
     synchroniseParticles();
     beforeHookIsBase_ = false;
     actionsBeforeTimeStep();
-    flushAddedParticles();
+    flushHandlerChanges();
     if (beforeHookIsBase_) {
         beforeHookMaybeOverridden_ = false;
         return;
@@ -543,11 +570,39 @@ bool DPMBase::boundariesActBeforeTimeStep() const
     return false;
 }
 
-// The particles added to the handler during a run, appended on the device.
-void DPMBase::flushAddedParticles()
+bool DPMBase::boundariesActAfterParticlesMove() const
 {
+    for (const auto& b : boundaryHandler)
+        if (b->actsAfterParticlesMove()) return true;
+    return false;
+}
+
+// A removal during a run: the additions made before it reach the device
+// first, so that both sides apply the same sequence of handler changes.
+void DPMBase::beforeParticleRemoval()
+{
+    if (solving_ && (int64_t)particleHandler.getNumberOfObjects() > devN_) flushHandlerChanges();
+}
+
+void DPMBase::afterParticleRemoval(unsigned index)
+{
+    if (!solving_) return;
+    pendingRemovals_.push_back(index);
+    --devN_;
+}
+
+// The handler changes made during a run reach the device: removals (in the
+// order they were made), then the particles added since.
+void DPMBase::flushHandlerChanges()
+{
+    if (!solving_) return;
+    if (!pendingRemovals_.empty()) {
+        check(dem_remove_particles(ctx_, (int64_t)pendingRemovals_.size(), pendingRemovals_.data()),
+              "dem_remove_particles");
+        pendingRemovals_.clear();
+    }
     const size_t n = particleHandler.getNumberOfObjects();
-    if (!solving_ || (int64_t)n <= devN_) return;
+    if ((int64_t)n <= devN_) return;
     const size_t k = n - (size_t)devN_;
     std::vector<uint32_t> id(k);
     std::vector<int32_t> sp(k);
@@ -672,6 +727,7 @@ bool DPMBase::hostCheckParticle(const BaseParticle& p, size_t first, bool walls)
 bool DPMBase::checkParticleForInteraction(const BaseParticle& p)
 {
     if (!solving_) return hostCheckParticle(p, 0, true);
+    if (!pendingRemovals_.empty()) flushHandlerChanges();
     // resident particles and walls on the device, at the drifted positions
     const double x[3] = {p.getPosition().X, p.getPosition().Y, p.getPosition().Z};
     const double r = p.getRadius();
@@ -717,8 +773,8 @@ void DPMBase::solve()
     // time_ < timeMax (DPMBase.cc:4287-4290)
     while (time_ < tMax_ && continueSolve()) {
         const bool hooksAreBase = !beforeHookMaybeOverridden_ && afterHookIsBase_ && continueIsBase_ &&
-                                  !boundariesActBeforeTimeStep();
-        if ((!hooksAreBase && hookStride_ == 1) || boundariesActBeforeTimeStep()) {
+                                  !boundariesActBeforeTimeStep() && !boundariesActAfterParticlesMove();
+        if ((!hooksAreBase && hookStride_ == 1) || boundariesActBeforeTimeStep() || boundariesActAfterParticlesMove()) {
             computeOneTimeStep();
             continue;
         }
@@ -753,6 +809,7 @@ void DPMBase::solve()
 void DPMBase::synchroniseParticles()
 {
     if (!ctx_ || particlesSynced_) return;
+    if (!pendingRemovals_.empty() || (int64_t)particleHandler.getNumberOfObjects() > devN_) flushHandlerChanges();
     const size_t n = std::min<size_t>(particleHandler.getNumberOfObjects(), (size_t)devN_);
     std::vector<double> x(3 * n), v(3 * n), w(3 * n), q(4 * n), f(3 * n), t(3 * n);
     check(dem_download_particles(ctx_, x.data(), v.data(), w.data(), q.data(), f.data(), t.data()),

@@ -351,7 +351,11 @@ protected:
     friend class ParticleSpecies;
 };
 
-class SphericalParticle : public BaseParticle {};
+class SphericalParticle : public BaseParticle {
+public:
+    SphericalParticle() = default;
+    explicit SphericalParticle(const ParticleSpecies* s) { setSpecies(s); }
+};
 
 // ------------------------------------------------------------- Walls ------
 class BaseWall {
@@ -429,6 +433,11 @@ public:
     // whether checkBoundaryBeforeTimeStep has work to do: the host then
     // runs the steps one at a time
     virtual bool actsBeforeTimeStep() const { return false; }
+    // BaseBoundary::checkBoundaryAfterParticlesMove, called by every step
+    // after the drift (DPMBase::checkInteractionWithBoundaries,
+    // DPMBase.cc:3272-3300); the handler holds the drifted state
+    virtual void checkBoundaryAfterParticlesMove(ParticleHandler&) {}
+    virtual bool actsAfterParticlesMove() const { return false; }
 
 protected:
     friend class BoundaryHandler;
@@ -511,10 +520,16 @@ public:
     // particles' interaction distances are zero here, so this is also their
     // largest getMaxInteractionRadius
     double getLargestRadius() const;
+    // ParticleHandler::removeObject (ParticleHandler.cc:410-426 with
+    // BaseHandler::removeObject, BaseHandler.h:472-505): the particle and its
+    // interactions go, the last particle takes its index; during a run the
+    // device follows
+    void removeObject(unsigned index);
+    BaseParticle* getLastObject() { return objects_.back().get(); }
 
 private:
     friend class DPMBase;
-    const DPMBase* dpm_ = nullptr;  // the problem owning this handler
+    DPMBase* dpm_ = nullptr;  // the problem owning this handler
     unsigned nextId_ = 0;
     unsigned dims_ = 3;
 };
@@ -618,6 +633,12 @@ public:
     void setZMax(double v) { max_.Z = v; }
     void setMin(const Vec3D& v) { min_ = v; }
     void setMax(const Vec3D& v) { max_ = v; }
+    // DPMBase::setDomain (DPMBase.cc)
+    void setDomain(const Vec3D& min, const Vec3D& max)
+    {
+        setMin(min);
+        setMax(max);
+    }
     void setMin(double x, double y, double z) { min_ = Vec3D(x, y, z); }
     void setMax(double x, double y, double z) { max_ = Vec3D(x, y, z); }
     double getXMin() const { return min_.X; }
@@ -732,9 +753,15 @@ protected:
 private:
     void initialiseSolve();
     void check(int rc, const char* what);
-    // send the particles added to the handler since the last upload
-    void flushAddedParticles();
+    // send the removals and additions made to the handler since the last
+    // upload, in that order
+    void flushHandlerChanges();
     bool boundariesActBeforeTimeStep() const;
+    bool boundariesActAfterParticlesMove() const;
+    friend class ParticleHandler;
+    void beforeParticleRemoval();
+    void afterParticleRemoval(unsigned index);
+    std::vector<int64_t> pendingRemovals_;
     bool hostCheckParticle(const BaseParticle& p, size_t first, bool walls) const;
     std::vector<Vec3D> periodicImages(const BaseParticle& p) const;
     int64_t devN_ = 0;     // particles resident on the device

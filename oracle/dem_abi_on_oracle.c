@@ -47,6 +47,10 @@ int dem_add_particles(dem_ctx* ctx, int64_t k, const uint32_t* id, const int32_t
 {
     return oracle_dem_add_particles(O(ctx), k, id, species, radius, inv_mass, inv_inertia, x, v, w, q);
 }
+int dem_remove_particles(dem_ctx* ctx, int64_t k, const int64_t* index)
+{
+    return oracle_dem_remove_particles(O(ctx), k, index);
+}
 int dem_check_particles(dem_ctx* ctx, int64_t k, const double* x, const double* radius, const int32_t* species,
                         const double* max_distance, int32_t* free_out)
 {

@@ -954,7 +954,6 @@ int oracle_dem_upload_particles(oracle_ctx* c, int64_t n, const uint32_t* id, co
     c->largestRadius = 0.0;
     uint32_t maxId = 0;
     for (int64_t i = 0; i < n; ++i) {
-        if (i > 0 && id[i] <= id[i - 1]) return fail(c, DEM_ERR_ARG, "ids must increase with the handler index");
         if (species[i] < 0 || species[i] >= c->nSpecies) return fail(c, DEM_ERR_ARG, "species index out of range");
         Particle* p = &c->p[i];
         memset(p, 0, sizeof *p);
@@ -984,12 +983,15 @@ int oracle_dem_upload_particles(oracle_ctx* c, int64_t n, const uint32_t* id, co
 
 static int64_t findById(const oracle_ctx* c, uint32_t id)
 {
+    /* ids increase with the index until a particle is removed */
     int64_t lo = 0, hi = c->nReal - 1;
     while (lo <= hi) {
         int64_t mid = (lo + hi) / 2;
         if (c->p[mid].id == id) return mid;
         if (c->p[mid].id < id) lo = mid + 1; else hi = mid - 1;
     }
+    for (int64_t i = 0; i < c->nReal; ++i)
+        if (c->p[i].id == id) return i;
     return -1;
 }
 
@@ -1131,6 +1133,37 @@ int oracle_dem_add_particles(oracle_ctx* c, int64_t k, const uint32_t* id, const
         c->nextId = id[j] + 1;
         c->nP = ++c->nReal;
     }
+    return DEM_OK;
+}
+
+/* ParticleHandler::removeObject (ParticleHandler.cc:410-426) for index[0],
+ * index[1], ... in turn: the particle's interactions are deleted with it
+ * (BaseInteractable::~BaseInteractable, BaseInteractable.cc:97-106) and the
+ * last particle moves into its place (BaseHandler::removeObject,
+ * BaseHandler.h:472-505), taking its interactions along. */
+int oracle_dem_remove_particles(oracle_ctx* c, int64_t k, const int64_t* index)
+{
+    if (!c || k < 0) return fail(c, DEM_ERR_ARG, "remove_particles: bad arguments");
+    if (c->nP != c->nReal) return fail(c, DEM_ERR_STATE, "remove_particles while periodic ghosts exist");
+    for (int64_t j = 0; j < k; ++j) {
+        const int64_t i = index[j], last = c->nReal - 1;
+        if (i < 0 || i > last) return fail(c, DEM_ERR_ARG, "remove_particles: index out of range");
+        while (c->p[i].nInter > 0) interRemove(c, c->p[i].inter[c->p[i].nInter - 1]);
+        free(c->p[i].inter);
+        if (i != last) {
+            c->p[i] = c->p[last];
+            for (int32_t a = 0; a < c->p[i].nInter; ++a) {
+                Inter* x = &c->in[c->p[i].inter[a]];
+                if (x->P == (int32_t)last) x->P = (int32_t)i;
+                if (!x->IisWall && x->I == (int32_t)last) x->I = (int32_t)i;
+            }
+        }
+        c->nP = --c->nReal;
+    }
+    /* ParticleHandler::checkExtremaOnDelete: the largest particle */
+    c->largestRadius = 0.0;
+    for (int64_t i = 0; i < c->nReal; ++i)
+        if (c->p[i].radius > c->largestRadius) c->largestRadius = c->p[i].radius;
     return DEM_OK;
 }
 

@@ -34,6 +34,11 @@ for k in (2, 7):
     FILES[f"Drivers/SelfTests/Interactions/SelfTestData/ForceSelfTest{k}.data"] = f"force_self_test_{k}.data"
 # InsertionBoundarySelfTest: cube, cylinder and sphere insertion boundaries
 # (InsertionBoundary::checkBoundaryBeforeTimeStep, InsertionBoundary.cc:184-310)
+# DeletionBoundarySelfTest, CubeDeletionBoundarySelfTest: a falling sphere
+# deleted by a plane / box (DeletionBoundary::checkBoundaryAfterParticlesMove)
+for name, dst in (("DeletionBoundarySelfTest", "deletion_boundary_self_test"),
+                  ("CubeDeletionBoundarySelfTest", "cube_deletion_boundary_self_test")):
+    FILES[f"Drivers/SelfTests/Boundaries/SelfTestData/{name}.data"] = f"{dst}.data"
 for ext in ("data", "ene", "fstat", "restart"):
     FILES[f"Drivers/SelfTests/Boundaries/SelfTestData/InsertionBoundarySelfTest.{ext}"] = \
         f"insertion_boundary_self_test.{ext}"

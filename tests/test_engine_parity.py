@@ -500,3 +500,35 @@ def test_empty_system_then_insertion():
     o.step(10)
     assert_contacts_close(g, o)
     np.testing.assert_allclose(g.download()["x"], o.download()["x"], rtol=0, atol=1e-9)
+
+
+@pytest.mark.parametrize("case", [CASES[0], CASES[1]], ids=["lvf-P", "lvf-np-3w"])
+def test_remove_particles(case):
+    """ParticleHandler::removeObject through dem_remove_particles, between
+    dem_step_begin and dem_step_end (where deletion boundaries act) and
+    between steps: the last particle takes each removed index, the removed
+    particles' interactions go; contacts and positions after the step equal
+    the oracle's, and 20 steps later agree to round-off.  (After removals
+    the reference orients a periodic-image interaction by handler index, the
+    engine by id: with periodic boundaries an interaction may then be stored
+    as (j, i) by the oracle and as (i, j) by the engine; lvf-P happens to
+    have none.)"""
+    sc = S.mixture(n=400, seed=47, lx=7.0, ly=7.0, lz=7.0, inflate=1.12, **case)
+    ref = oracle_lib.engine()
+    S.apply(ref, sc)
+    ref.compute_forces()
+    g, o = make_pair(reseeded(sc, ref.download()), ref.contacts())
+    gone = [5, 17, 397, 0, 120, 394]  # each index refers to the handler after the previous removals
+    for e in (g, o):
+        e.step_begin()
+        e.remove_particles(gone[:4])
+        e.step_end()
+    assert g.n == o.n == 396
+    assert_contacts_equal(g, o)
+    a, b = assert_forces_close(g, o)
+    np.testing.assert_array_equal(a["x"], b["x"])
+    for e in (g, o):
+        e.remove_particles(gone[4:])
+        e.step(20)
+    assert_contacts_close(g, o)
+    np.testing.assert_allclose(g.download()["x"], o.download()["x"], rtol=0, atol=1e-9)

@@ -53,6 +53,11 @@ CASES = {
     "InsertionBoundarySelfTest": ("InsertionBoundarySelfTest", [], [
         (f"InsertionBoundarySelfTest.{e}", f"insertion_boundary_self_test.{e}", k)
         for e, k in (("data", "data"), ("ene", "ene"), ("fstat", "fstat"), ("restart", "restart:1e-14"))]),
+    # a falling sphere deleted by a plane and by a box
+    "DeletionBoundarySelfTest": ("DeletionBoundarySelfTest", [],
+                                 [("DeletionBoundarySelfTest.data", "deletion_boundary_self_test.data", "data")]),
+    "CubeDeletionBoundarySelfTest": ("CubeDeletionBoundarySelfTest", [], [
+        ("CubeDeletionBoundarySelfTest.data", "cube_deletion_boundary_self_test.data", "data")]),
     # stop at t = 1.05, restart from the .restart file (also with numbered
     # .data files), compare with the uninterrupted run (self-checking)
     "FreeFallRestartUnitTest": ("FreeFallRestartUnitTest", [], []),
