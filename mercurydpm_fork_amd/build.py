@@ -51,7 +51,7 @@ DRIVER_DIR = os.path.join(ROOT, "tests", "drivers")
 
 
 HOST_SOURCES = [os.path.join(HERE, "host", f) for f in ("Mercury3D.cc", "DPMBaseOutput.cc", "DPMBaseRestart.cc",
-                                                                "InsertionBoundary.cc")]
+                                                                "InsertionBoundary.cc", "Chute.cc")]
 
 
 def build_host():
