@@ -112,7 +112,12 @@ void Chute::createBottom()
         w0.set(Vec3D(0.0, 0.0, -1.0), Vec3D(0, 0, -F0.getRadius()));
         wallHandler.copyAndAddObject(w0);
     } else {
-        throw std::runtime_error("Chute::createBottom: MULTILAYER bottoms (ChuteBottom) are not provided");
+        // MULTILAYER: a layer cut from a settled packing (ChuteBottom)
+        ChuteBottom bottom(*this);
+        bottom.setInflowParticleRadius(getFixedParticleRadius());
+        bottom.makeRoughBottom(*this);
+        w0.set(Vec3D(0.0, 0.0, -1.0), Vec3D(0, 0, getZMin() - bottom.getThickness()));
+        wallHandler.copyAndAddObject(w0);
     }
     // the bottom particles are fixed
     for (auto& p : particleHandler) p->fixParticle();
@@ -133,4 +138,137 @@ void Chute::cleanChute()
     } else {
         cleanCount_++;
     }
+}
+
+ChuteBottom::ChuteBottom(const Chute& chute)
+{
+    setTimeStep(chute.getTimeStep());
+    setTimeMax(chute.getTimeMax());
+    setGravity(chute.getGravity());
+    setMin(Vec3D(chute.getXMin(), chute.getYMin(), chute.getZMin()));
+    setMax(Vec3D(chute.getXMax(), chute.getYMax(), chute.getZMax()));
+    setSystemDimensions(chute.getSystemDimensions());
+    setParticleDimensions(chute.particleHandler.getParticleDimensions());
+    random = chute.random;
+    speciesHandler.copySpeciesFrom(chute.speciesHandler);
+    for (const auto& w : chute.wallHandler) wallHandler.addCopy(*w);
+    for (const auto& b : chute.boundaryHandler)
+        if (const auto* pb = dynamic_cast<const PeriodicBoundary*>(b.get())) boundaryHandler.copyAndAddObject(*pb);
+    fixedParticleRadius_ = chute.getFixedParticleRadius();
+    fixedParticleSpacing_ = chute.getFixedParticleSpacing();
+    roughBottomType_ = chute.getRoughBottomType();
+    chuteAngle_ = chute.getChuteAngle();
+    maxFailed_ = chute.getMaxFailed();
+    minInflowParticleRadius_ = chute.getMinInflowParticleRadius();
+    maxInflowParticleRadius_ = chute.getMaxInflowParticleRadius();
+    inflowHeight_ = chute.getInflowHeight();
+    inflowVelocity_ = chute.getInflowVelocity();
+    inflowVelocityVariance_ = chute.getInflowVelocityVariance();
+    isChutePeriodic_ = chute.getIsPeriodic();
+    // ChuteBottom::constructor (ChuteBottom.cc:103-113)
+    setName("roughbottom");
+    fStatFile.setFileType(FileType::NO_FILE);
+    dataFile.setFileType(FileType::ONE_FILE);
+    restartFile.setFileType(FileType::ONE_FILE);
+    eneFile.setFileType(FileType::NO_FILE);
+}
+
+void ChuteBottom::makeRoughBottom(Chute& chute)
+{
+    setChuteAngle(0.0);
+    setInflowHeight(25. * getInflowParticleRadius());
+    setRoughBottomType(MONOLAYER_DISORDERED);
+    setFixedParticleRadius(getInflowParticleRadius());
+    // a softer, strongly damped species: 13.2863 times the collision time,
+    // restitution 0.343008, time step 2% of 9.41823 collision times
+    ParticleSpecies* s0 = speciesHandler.getObject(0);
+    if (s0->params.normal_law != DEM_NORMAL_LINEAR_VISCOELASTIC)
+        throw std::runtime_error("ChuteBottom: species 0 must have a linear viscoelastic normal force");
+    LinearViscoelasticSpecies lin;
+    lin.params = s0->params;
+    lin.setDensity(s0->getDensity());
+    const double mass = s0->getMassFromRadius(0.5 * (getMinInflowParticleRadius() + getMaxInflowParticleRadius()));
+    const double collisionTime = 13.2863 * lin.getCollisionTime(mass);
+    lin.setCollisionTimeAndRestitutionCoefficient(collisionTime, 0.343008, mass);
+    s0->params.stiffness = lin.params.stiffness;
+    s0->params.dissipation = lin.params.dissipation;
+    setTimeStep(0.02 * 9.41823 * collisionTime);
+    if (s0->params.friction_law != DEM_FRICTION_NONE) {
+        s0->params.sliding_mu = 0.0;
+        s0->params.sliding_mu_static = 0.0;
+    }
+    setTimeMax(getTimeStep() * 2000);
+    setSaveCount(100);
+    solve();
+    // the layer [height - (1 + thickness) r, height - r] below the top
+    double height = 0;
+    for (auto& p : particleHandler) height = std::max(height, p->getPosition().Z);
+    const double r = getMaxInflowParticleRadius();
+    for (unsigned i = 0; i < particleHandler.getNumberOfObjects();) {
+        const double z = particleHandler.getObject(i)->getPosition().Z;
+        if (z < height - (1.0 + thickness_) * r || z > height - r)
+            particleHandler.removeObject(i);
+        else
+            i++;
+    }
+    // lowered to the chute's origin, fixed, with the chute's species; the
+    // chute's handler numbers them afresh (ParticleHandler::operator=)
+    for (auto& p : particleHandler) {
+        SphericalParticle q;
+        q.setSpecies(chute.speciesHandler.getObject(0));
+        q.setRadius(p->getRadius());
+        q.setPosition(p->getPosition() + Vec3D(0.0, 0.0, r - height));
+        q.setOrientation(p->getOrientation());
+        q.fixParticle();
+        chute.particleHandler.copyAndAddObject(q);
+    }
+}
+
+void ChuteBottom::setupInitialConditions()
+{
+    createBottom();
+    wallHandler.clear();
+    boundaryHandler.clear();
+    InfiniteWall w0;
+    w0.setSpecies(speciesHandler.getObject(0));
+    w0.set(Vec3D(0.0, 0.0, -1.0), Vec3D(0, 0, getZMin() - getInflowParticleRadius()));
+    wallHandler.copyAndAddObject(w0);
+    if (isBottomPeriodic_) {
+        PeriodicBoundary b0;
+        b0.set(Vec3D(1.0, 0.0, 0.0), getXMin(), getXMax());
+        boundaryHandler.copyAndAddObject(b0);
+        b0.set(Vec3D(0.0, 1.0, 0.0), getYMin(), getYMax());
+        boundaryHandler.copyAndAddObject(b0);
+    } else {
+        w0.set(Vec3D(-1.0, 0.0, 0.0), Vec3D(getXMin(), 0, 0));
+        wallHandler.copyAndAddObject(w0);
+        w0.set(Vec3D(1.0, 0.0, 0.0), Vec3D(getXMax(), 0, 0));
+        wallHandler.copyAndAddObject(w0);
+        w0.set(Vec3D(0.0, -1.0, 0.0), Vec3D(0, getYMin(), 0));
+        wallHandler.copyAndAddObject(w0);
+        w0.set(Vec3D(0.0, 1.0, 0.0), Vec3D(0, getYMax(), 0));
+        wallHandler.copyAndAddObject(w0);
+    }
+    // fill the box at random until 501 candidates in a row fail
+    unsigned failed = 0;
+    const unsigned max_failed = 500;
+    SphericalParticle p(speciesHandler.getObject(0));
+    while (failed <= max_failed) {
+        p.setRadius(getFixedParticleRadius());
+        Vec3D position;
+        position.X = random.getRandomNumber(p.getRadius(), getXMax() - p.getRadius());
+        position.Y = random.getRandomNumber(p.getRadius(), getYMax() - p.getRadius());
+        position.Z = random.getRandomNumber(2 * p.getRadius(), getZMax() - p.getRadius());
+        p.setPosition(position);
+        p.setVelocity(Vec3D(0.0, 0.0, 0.0));
+        if (checkParticleForInteraction(p)) {
+            particleHandler.copyAndAddObject(p);
+            failed = 0;
+        } else {
+            failed++;
+        }
+    }
+    const double minCell = 2. * std::min(getFixedParticleRadius(), getMinInflowParticleRadius());
+    const double maxCell = 2. * std::max(getFixedParticleRadius(), getMaxInflowParticleRadius());
+    setHGridMaxLevels((minCell == maxCell) || (minCell == 0.) ? 1 : 2);
 }

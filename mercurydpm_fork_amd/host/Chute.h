@@ -78,8 +78,7 @@ public:
     void setupSideWalls();
     void makeChutePeriodic() { isChutePeriodic_ = true; }
     bool getIsPeriodic() const { return isChutePeriodic_; }
-    // Chute::createBottom (Chute.cc:323-500); MULTILAYER (ChuteBottom, a
-    // DEM run of its own) is not provided
+    // Chute::createBottom (Chute.cc:323-500); MULTILAYER runs a ChuteBottom
     virtual void createBottom();
     // Chute::cleanChute (Chute.cc:505-530): every 102nd call, particles
     // outside [xMin, xMax] are removed
@@ -92,13 +91,8 @@ public:
     void setRoughBottomType(RoughBottomType t) { roughBottomType_ = t; }
     RoughBottomType getRoughBottomType() const { return roughBottomType_; }
     // Chute::setChuteAngle (Chute.cc:768-801): gravity keeps its magnitude
-    // (1 when unset) and turns by the angle, in degrees
-    void setChuteAngle(double degrees)
-    {
-        double g = getGravity().getLength();
-        if (g == 0) g = 1.0;
-        setChuteAngleAndMagnitudeOfGravity(degrees, g);
-    }
+    // (zero stays zero) and turns by the angle, in degrees
+    void setChuteAngle(double degrees) { setChuteAngleAndMagnitudeOfGravity(degrees, getGravity().getLength()); }
     void setChuteAngleAndMagnitudeOfGravity(double degrees, double g)
     {
         if (!(degrees >= -90.0 && degrees <= 90.0)) throw std::invalid_argument("Chute: angle must be within [-90,90]");
@@ -139,6 +133,16 @@ public:
     InsertionBoundary* getInsertionBoundary() const { return insertionBoundary_; }
 
 protected:
+    // Chute::write (Chute.cc:205-219): the chute settings after the hGrid line
+    void writeProblemSettings(std::ostream& os) const override
+    {
+        os << "FixedParticleRadius " << fixedParticleRadius_ << " MinInflowParticleRadius "
+           << minInflowParticleRadius_ << " MaxInflowParticleRadius " << maxInflowParticleRadius_
+           << " RoughBottomType " << roughBottomType_ << " ChuteAngle " << chuteAngle_ << " MaxFailed " << maxFailed_
+           << " NumCreated " << (insertionBoundary_ ? insertionBoundary_->getNumberOfParticlesInserted() : 0)
+           << " InflowVelocity " << inflowVelocity_ << " InflowVelocityVariance " << inflowVelocityVariance_
+           << " InflowHeight " << inflowHeight_ << std::endl;
+    }
     double fixedParticleRadius_ = 0, fixedParticleSpacing_ = 0;
     RoughBottomType roughBottomType_ = MONOLAYER_DISORDERED;
     double chuteAngle_ = 0;
@@ -148,4 +152,32 @@ protected:
     bool isChutePeriodic_ = false;
     InsertionBoundary* insertionBoundary_ = nullptr;
     int cleanCount_ = 0;
+};
+
+// ChuteBottom (ChuteBottom.cc): the MULTILAYER rough bottom, made by a DEM
+// run of its own on the device: a box the size of the chute is filled with
+// fixed-radius spheres at random, settles for 2000 steps with a softer,
+// frictionless, strongly damped copy of species 0, and the layer of
+// thickness `thickness` below the top of the packing becomes the chute's
+// fixed bottom.
+class ChuteBottom : public Chute {
+public:
+    // ChuteBottom(const DPMBase&) followed by ChuteBottom::constructor
+    // (ChuteBottom.cc:50-118): the chute's domain, time step, gravity,
+    // species, walls, periodic boundaries, generator state and chute settings
+    explicit ChuteBottom(const Chute& chute);
+    // ChuteBottom::makeRoughBottom (ChuteBottom.cc:139-214)
+    void makeRoughBottom(Chute& chute);
+    // ChuteBottom::setupInitialConditions (ChuteBottom.cc:222-332)
+    void setupInitialConditions() override;
+    // ChuteBottom::actionsBeforeTimeStep (ChuteBottom.cc:334-336): nothing
+    void actionsBeforeTimeStep() override { DPMBase::actionsBeforeTimeStep(); }
+    double getThickness() const { return thickness_; }
+    void setThickness(double t) { thickness_ = t; }
+    bool getIsBottomPeriodic() const { return isBottomPeriodic_; }
+    void setIsBottomPeriodic(bool p) { isBottomPeriodic_ = p; }
+
+private:
+    double thickness_ = 2.4;
+    bool isBottomPeriodic_ = true;
 };

@@ -357,6 +357,7 @@ void DPMBase::write(std::ostream& os)
         os << '\n';
     }
     hGridInfo(os);
+    writeProblemSettings(os);
 }
 
 // ------------------------------------------------------------------- read

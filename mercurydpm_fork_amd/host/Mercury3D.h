@@ -372,6 +372,7 @@ public:
     const double* getOrientation() const { return q_; }
     virtual dem_wall toEngine() const = 0;
     virtual std::string getName() const = 0;
+    virtual std::unique_ptr<BaseWall> clone() const = 0;
     // BaseWall::write (BaseWall.cc:100-116): BaseInteractable::write
     // (BaseInteractable.cc:274-284) of a static wall
     virtual void write(std::ostream& os) const;
@@ -401,6 +402,7 @@ public:
     void setPositionAndOrientation(const Vec3D& point, const double q[4]);
     dem_wall toEngine() const override;
     std::string getName() const override { return "InfiniteWall"; }
+    std::unique_ptr<BaseWall> clone() const override { return std::make_unique<InfiniteWall>(*this); }
 };
 
 // CylindricalWall (CylindricalWall.cc): axis = z through the origin
@@ -410,6 +412,7 @@ public:
     double getRadius() const { return radius_; }
     dem_wall toEngine() const override;
     std::string getName() const override { return "CylindricalWall"; }
+    std::unique_ptr<BaseWall> clone() const override { return std::make_unique<CylindricalWall>(*this); }
     // CylindricalWall::write (CylindricalWall.cc:102-106)
     void write(std::ostream& os) const override;
 
@@ -503,6 +506,16 @@ public:
     dem_pair_params* getMixedObject(unsigned a, unsigned b);
     std::vector<dem_pair_params> table();
     bool useAngularDOFs() const;
+    // the species of another problem, as plain parameter sets
+    void copySpeciesFrom(const SpeciesHandler& o)
+    {
+        objects_.clear();
+        mixed_ = o.mixed_;
+        for (const auto& s : o.objects_) {
+            objects_.push_back(std::make_unique<ParticleSpecies>(*s));
+            objects_.back()->handler_ = this;
+        }
+    }
 
 private:
     std::vector<std::pair<bool, dem_pair_params>> mixed_;
@@ -536,6 +549,13 @@ private:
 
 class WallHandler : public HandlerBase<BaseWall> {
 public:
+    // a copy of a wall of another problem (its species index is kept)
+    BaseWall* addCopy(const BaseWall& w)
+    {
+        objects_.push_back(w.clone());
+        objects_.back()->id_ = nextId_++;
+        return objects_.back().get();
+    }
     template <class W>
     W* copyAndAddObject(const W& w)
     {
@@ -749,6 +769,8 @@ protected:
     int grid_levels_ = 0;
     // the hGrid line MercuryBase appends to restart files
     virtual void hGridInfo(std::ostream&) {}
+    // what a problem class appends after it (e.g. Chute::write, Chute.cc:205-219)
+    virtual void writeProblemSettings(std::ostream&) const {}
 
 private:
     void initialiseSolve();

@@ -39,6 +39,8 @@ for k in (2, 7):
 for name, dst in (("DeletionBoundarySelfTest", "deletion_boundary_self_test"),
                   ("CubeDeletionBoundarySelfTest", "cube_deletion_boundary_self_test")):
     FILES[f"Drivers/SelfTests/Boundaries/SelfTestData/{name}.data"] = f"{dst}.data"
+# roughBottomSelfTest: a chute on a MULTILAYER bottom (ChuteBottom::makeRoughBottom)
+FILES["Drivers/ChuteDemos/SelfTestData/roughBottomMultiLayer.restart"] = "rough_bottom_multi_layer.restart"
 for ext in ("data", "ene", "fstat", "restart"):
     FILES[f"Drivers/SelfTests/Boundaries/SelfTestData/InsertionBoundarySelfTest.{ext}"] = \
         f"insertion_boundary_self_test.{ext}"

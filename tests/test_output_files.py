@@ -58,6 +58,13 @@ CASES = {
                                  [("DeletionBoundarySelfTest.data", "deletion_boundary_self_test.data", "data")]),
     "CubeDeletionBoundarySelfTest": ("CubeDeletionBoundarySelfTest", [], [
         ("CubeDeletionBoundarySelfTest.data", "cube_deletion_boundary_self_test.data", "data")]),
+    # a chute on a MULTILAYER bottom: ChuteBottom fills a box at random and
+    # lets it settle for 2000 steps before a layer is cut from the top; the
+    # same 13 base and 2 inflow particles, the inflow ones to 15 digits, the
+    # base positions to 1e-3 relative (the settling run amplifies the
+    # round-off of force sums taken in another order to about 2e-5)
+    "roughBottomSelfTest": ("roughBottomSelfTest", [], [
+        ("roughBottomMultiLayer.restart", "rough_bottom_multi_layer.restart", "restart:1e-3")]),
     # stop at t = 1.05, restart from the .restart file (also with numbered
     # .data files), compare with the uninterrupted run (self-checking)
     "FreeFallRestartUnitTest": ("FreeFallRestartUnitTest", [], []),
