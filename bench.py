@@ -61,9 +61,65 @@ def parse():
                     help="contact arithmetic: fp64 (the metric) or mixed (FP32 contact geometry and forces)")
     ap.add_argument("--mixed-steps", type=int, default=500,
                     help="after the FP64 timing, time this many steps of the same state in mixed precision (0: skip)")
+    ap.add_argument("--ensemble", type=int, default=0,
+                    help="instead of the chute: this many calibration heaps (sizes log-uniform in [500, 20000]) "
+                         "stepped as one packed ensemble, against the same heaps run one at a time")
+    ap.add_argument("--ensemble-steps", type=int, default=20000, help="steps per simulation for sims/hour")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only copy this many bytes device-to-device (calibrates HBM counters) and exit")
     return ap.parse_args()
+
+
+def run_ensemble(a):
+    """Batched ensemble: a.ensemble heaps (log-uniform sizes in [500, 20000],
+    rolling friction) packed side by side in one context and stepped
+    together, against the same kind of heaps stepped one at a time.  The
+    sequential cost is measured on 16 sizes spread over the range and
+    interpolated (log-log) to every member.  One JSON line: simulations per
+    hour at a.ensemble_steps steps per simulation, both ways."""
+    import numpy as np
+    import torch
+
+    from mercurydpm_fork_amd import dem
+    from mercurydpm_fork_amd import ensemble as E
+    from mercurydpm_fork_amd import scenarios as S
+
+    rng = np.random.default_rng(a.seed)
+    sizes = np.exp(rng.uniform(np.log(500), np.log(20000), a.ensemble)).astype(int)
+    sp = E.calibration_species()
+
+    def time_steps(sc, warm, steps):
+        e = dem.Engine(device=0)
+        S.apply(e, sc)
+        e.compute_forces()
+        e.step(warm)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        e.step(steps)
+        e.stats()  # waits for the device
+        dt = (time.perf_counter() - t0) / steps
+        e.close()
+        return dt
+
+    members = [E.heap(int(n), 1000 + k, sp) for k, n in enumerate(sizes)]
+    packed = E.pack(members)
+    t_packed = time_steps(packed.scenario, 200, a.steps)
+    sample = np.unique(np.geomspace(500, 20000, 16).astype(int))
+    t_sample = np.array([time_steps(E.heap(int(n), 7 + k, sp), 200, a.steps) for k, n in enumerate(sample)])
+    t_seq = float(np.sum(np.exp(np.interp(np.log(sizes), np.log(sample), np.log(t_sample)))))
+    per_hour = lambda t: 3600.0 / (t * a.ensemble_steps)  # noqa: E731
+    print(json.dumps({
+        "metric": "calibration heaps per hour (batched ensemble)",
+        "value": per_hour(t_packed) * a.ensemble, "unit": "simulations/hour", "n_gpus": 1,
+        "steps": a.steps, "ms_per_step": t_packed * 1e3, "higher_is_better": True, "dtype": "f64",
+        "data": "synthetic (jittered lattice columns above a floor)",
+        "config": {"workload": "batched calibration heaps, LinearViscoelasticFriction with rolling friction",
+                   "simulations": int(a.ensemble), "particles": int(sizes.sum()),
+                   "size_range": [500, 20000], "steps_per_simulation": a.ensemble_steps},
+        "sequential": {"value": per_hour(t_seq) * a.ensemble, "unit": "simulations/hour",
+                       "ms_per_step_all_members": t_seq * 1e3,
+                       "sample": {int(n): round(t * 1e3, 4) for n, t in zip(sample, t_sample)}},
+        "speedup_vs_sequential": t_seq / t_packed}))
 
 
 def main():
@@ -88,6 +144,9 @@ def main():
 
     from mercurydpm_fork_amd import dem
     from mercurydpm_fork_amd import scenarios as S
+
+    if a.ensemble > 0:
+        return run_ensemble(a)
 
     t_gen = time.time()
     sc = S.chute_hvm(n_flow=a.n, seed=a.seed)
