@@ -37,10 +37,10 @@ def _run(cmd):
 
 def build_engine():
     os.makedirs(LIB, exist_ok=True)
-    src = os.path.join(HERE, "csrc", "dem_engine.hip")
+    srcs = [os.path.join(HERE, "csrc", f) for f in ("dem_engine.hip", "closure_mlp.hip")]
     out = os.path.join(LIB, "libmercurydpm_dem.so")
     tmp = out + ".tmp"
-    _run([HIPCC, *HIP_FLAGS, "-shared", "-o", tmp, src])
+    _run([HIPCC, *HIP_FLAGS, "-shared", "-o", tmp, *srcs])
     os.replace(tmp, out)
     return out
 

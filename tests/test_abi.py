@@ -1,5 +1,4 @@
-"""The C-ABI library loads and exports every function include/mercurydpm_dem.h
-declares; argument checks that need no device.  CPU only."""
+"""The C-ABI library loads and exports every function include/*.h declares; argument checks that need no device.  CPU only."""
 import ctypes
 import os
 import re
@@ -11,10 +10,12 @@ from mercurydpm_fork_amd import dem
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HEADER = os.path.join(ROOT, "include", "mercurydpm_dem.h")
+HEADERS = [os.path.join(ROOT, "include", f) for f in sorted(os.listdir(os.path.join(ROOT, "include")))
+           if f.endswith(".h")]
 
 
 def declared_functions():
-    src = open(HEADER).read()
+    src = "".join(open(h).read() for h in HEADERS)
     return sorted(set(re.findall(r"^(?:int|const char\*)\s+(dem_\w+)\s*\(", src, re.M)))
 
 
