@@ -72,3 +72,41 @@ def test_trained_closure_beats_constant():
     err_net = np.mean((net(xte) - tte) ** 2, axis=0)
     err_const = np.mean((ttr.mean(0) - tte) ** 2, axis=0)
     assert np.all(err_net < 0.05 * err_const), (err_net, err_const)
+
+
+def chute_data():
+    """Bulk CG states of the engine's own chute flows at 21..27 degrees
+    (tests/golden/make_closure_data.py)."""
+    from mercurydpm_fork_amd import closure_data as CD
+    d = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "closure_chute_cg.npz"))
+    return {float(k.split("_")[1]): CD.standardise(d[k]) for k in d.files}
+
+
+def test_chute_data_follow_a_friction_law():
+    """The profiles' effective friction grows with the inertial number, as
+    the local rheology says: the bin medians per angle are ordered."""
+    d = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "closure_chute_cg.npz"))
+    med = np.array([np.median(d[k], axis=0) for k in sorted(d.files, key=lambda k: float(k.split("_")[1]))])
+    assert np.all(np.diff(med[:, 0]) > -0.01) and med[-1, 0] > 2 * med[0, 0]  # I grows with the angle
+    assert np.corrcoef(np.log(med[:, 0]), med[:, 4])[0, 1] > 0.8  # and mu with I
+
+
+@pytest.mark.gpu
+def test_closure_trained_on_dem_profiles_predicts_held_out_angles():
+    """Trained on the CG states of five chute angles (50 full-batch epochs:
+    the per-bin states are noisy, and longer training fits the noise), the
+    network predicts mu at the two held-out angles (23 and 26 degrees) with
+    at most 0.8 of the squared error of the constant closure (the mean
+    training mu); the numpy oracle with the same updates gives 0.67."""
+    data = chute_data()
+    train = [a for a in data if a not in (23.0, 26.0)]
+    xtr = np.concatenate([data[a][0] for a in train])
+    ttr = np.concatenate([data[a][1] for a in train])
+    xte = np.concatenate([data[a][0] for a in (23.0, 26.0)])
+    tte = np.concatenate([data[a][1] for a in (23.0, 26.0)])
+    net = closure.MLP(seed=11)
+    net.train(xtr, ttr, epochs=50, lr=2e-3)
+    err_net = np.mean((net(xte)[:, 0] - tte[:, 0]) ** 2)
+    err_const = np.mean((ttr[:, 0].mean() - tte[:, 0]) ** 2)
+    print(f"held-out mu error: network {err_net:.4g}, constant {err_const:.4g}")
+    assert err_net < 0.8 * err_const
