@@ -74,6 +74,8 @@ public:
     void setupInitialConditions() override;
     // Chute::actionsBeforeTimeStep (Chute.cc:225-228)
     void actionsBeforeTimeStep() override { cleanChute(); }
+    // cleanChute looks at the particles on every 102nd call only
+    bool actionsBeforeTimeStepUsesParticles() const override { return cleanCount_ > 100; }
     // Chute::setupSideWalls (Chute.cc:288-309)
     void setupSideWalls();
     void makeChutePeriodic() { isChutePeriodic_ = true; }
@@ -172,6 +174,7 @@ public:
     void setupInitialConditions() override;
     // ChuteBottom::actionsBeforeTimeStep (ChuteBottom.cc:334-336): nothing
     void actionsBeforeTimeStep() override { DPMBase::actionsBeforeTimeStep(); }
+    bool actionsBeforeTimeStepUsesParticles() const override { return true; }
     double getThickness() const { return thickness_; }
     void setThickness(double t) { thickness_ = t; }
     bool getIsBottomPeriodic() const { return isBottomPeriodic_; }

@@ -552,6 +552,11 @@ void DPMBase::computeOneTimeStep()
 void DPMBase::runBeforeHook()
 {
     if (!beforeHookMaybeOverridden_) return;
+    if (!actionsBeforeTimeStepUsesParticles()) {
+        actionsBeforeTimeStep();
+        flushHandlerChanges();
+        return;
+    }
     synchroniseParticles();
     beforeHookIsBase_ = false;
     actionsBeforeTimeStep();

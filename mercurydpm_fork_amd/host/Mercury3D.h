@@ -730,6 +730,11 @@ public:
     // handler holds the drifted state; positions, velocities, angular
     // velocities and orientations set here are sent to the device.
     virtual void actionsBeforeTimeStep() { beforeHookIsBase_ = true; }
+    // Whether the next actionsBeforeTimeStep call reads or changes the
+    // particles: when it returns false the hook runs without the state
+    // exchange with the device (a hook that acts only every so many steps
+    // says so here).
+    virtual bool actionsBeforeTimeStepUsesParticles() const { return true; }
     // Called after each step's force pass, before the clock advances
     // (DPMBase.cc:4361); the particles hold the state after the closing
     // half-kick, which the reference applies just after this hook.
