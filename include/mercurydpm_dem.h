@@ -48,7 +48,7 @@
 extern "C" {
 #endif
 
-#define DEM_ABI_VERSION 5
+#define DEM_ABI_VERSION 6
 
 /* error codes */
 #define DEM_OK 0
@@ -294,6 +294,18 @@ int dem_add_particles(dem_ctx* ctx, int64_t k, const uint32_t* id, const int32_t
  * boundaries act (DPMBase::checkInteractionWithBoundaries, DPMBase.cc:
  * 3272-3300). */
 int dem_remove_particles(dem_ctx* ctx, int64_t k, const int64_t* index);
+
+/* The particles a deletion boundary removes, found on the device: the
+ * handler indices (increasing; n_out = how many, of which at most capacity
+ * are written) of the particles
+ *   DEM_REGION_HALF_SPACE:  params = normal (3), distance: distance -
+ *     dot(x, normal) < 0 (DeletionBoundary::getDistance, DeletionBoundary.cc:
+ *     109-112);
+ *   DEM_REGION_BOX_INTERIOR: params = min (3), max (3): strictly inside
+ *     (CubeDeletionBoundary::getDistance, CubeDeletionBoundary.cc:60-69). */
+enum dem_region { DEM_REGION_HALF_SPACE = 0, DEM_REGION_BOX_INTERIOR = 1 };
+int dem_find_in_region(dem_ctx* ctx, int32_t kind, const double* params, int64_t capacity, int64_t* n_out,
+                       int64_t* index_out);
 
 /* DPMBase::checkParticleForInteraction (DPMBase.cc:4819-4905 with
  * MercuryBase::checkParticleForInteractionLocal, MercuryBase.cc:622-650) for

@@ -1636,6 +1636,29 @@ __global__ void __launch_bounds__(kBlock) k_check_particles(Dev<NS> d, Params p,
     }
 }
 
+// Deletion regions: the handler index of every particle in the region.
+struct Region {
+    int kind;
+    double a[3], b[3], c;
+};
+
+template <int NS>
+__global__ void __launch_bounds__(kBlock) k_find_region(Dev<NS> d, Params p, Region g, int* counter, int cap,
+                                                        uint32_t* out)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= p.n) return;
+    const double4 x = d.posr[i];
+    bool in;
+    if (g.kind == DEM_REGION_HALF_SPACE)
+        in = g.c - (x.x * g.a[0] + x.y * g.a[1] + x.z * g.a[2]) < 0;
+    else
+        in = g.a[0] < x.x && x.x < g.b[0] && g.a[1] < x.y && x.y < g.b[1] && g.a[2] < x.z && x.z < g.b[2];
+    if (!in) return;
+    const int o = atomicAdd(counter, 1);
+    if (o < cap) out[o] = d.orig[i];
+}
+
 // ------------------------------------------------------------------------
 // Diagnostics: stored interactions.
 // ------------------------------------------------------------------------
@@ -3043,6 +3066,50 @@ int dem_remove_particles(dem_ctx* ctx, int64_t k, const int64_t* index)
     ctx->alive_min = alive_min;
     ctx->prm.alive_min = alive_min;
     ctx->rebuilds_before = rebuilds;
+    return DEM_OK;
+}
+
+int dem_find_in_region(dem_ctx* ctx, int32_t kind, const double* params, int64_t capacity, int64_t* n_out,
+                       int64_t* index_out)
+{
+    if (!ctx || !params || !n_out || capacity < 0 || (capacity > 0 && !index_out)) return DEM_ERR_ARG;
+    if (kind != DEM_REGION_HALF_SPACE && kind != DEM_REGION_BOX_INTERIOR)
+        return set_err(ctx, DEM_ERR_ARG, "find_in_region: unknown region kind");
+    if (!ctx->uploaded) return set_err(ctx, DEM_ERR_STATE, "no particles uploaded");
+    *n_out = 0;
+    if (ctx->n == 0) return DEM_OK;
+    HIPCHK(hipSetDevice(ctx->device));
+    Region g{};
+    g.kind = kind;
+    for (int a = 0; a < 3; ++a) {
+        g.a[a] = params[a];
+        g.b[a] = kind == DEM_REGION_BOX_INTERIOR ? params[3 + a] : 0.0;
+    }
+    g.c = kind == DEM_REGION_HALF_SPACE ? params[3] : 0.0;
+    // counter and indices in one buffer: at most n indices
+    const size_t bytes = sizeof(int) + sizeof(uint32_t) * (size_t)ctx->n;
+    void* buf = nullptr;
+    HIPCHK(hipMalloc(&buf, bytes));
+    int* counter = (int*)buf;
+    uint32_t* idx = (uint32_t*)(counter + 1);
+    HIPCHK(hipMemsetAsync(counter, 0, sizeof(int), ctx->stream));
+    if (ctx->ns_springs == 9)
+        hipLaunchKernelGGL(k_find_region<9>, dim3(grid_for(ctx->n)), dim3(kBlock), 0, ctx->stream, ctx->d9, ctx->prm, g,
+                           counter, (int)ctx->n, idx);
+    else
+        hipLaunchKernelGGL(k_find_region<3>, dim3(grid_for(ctx->n)), dim3(kBlock), 0, ctx->stream, ctx->d3, ctx->prm, g,
+                           counter, (int)ctx->n, idx);
+    int found = 0;
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(&found, counter, sizeof(int), hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    std::vector<uint32_t> h(found > 0 ? found : 0);
+    if (e == hipSuccess && found > 0) e = hipMemcpy(h.data(), idx, sizeof(uint32_t) * found, hipMemcpyDeviceToHost);
+    (void)hipFree(buf);
+    if (e != hipSuccess) return set_err(ctx, DEM_ERR_DEVICE, std::string("find_in_region: ") + hipGetErrorString(e));
+    std::sort(h.begin(), h.end());
+    *n_out = found;
+    for (int64_t k = 0; k < std::min<int64_t>(capacity, found); ++k) index_out[k] = h[k];
     return DEM_OK;
 }
 

@@ -6,6 +6,7 @@
 #pragma once
 
 #include <ostream>
+#include <typeinfo>
 
 #include "Mercury3D.h"
 
@@ -45,6 +46,37 @@ public:
             if (checkBoundaryAfterParticleMoved(pH.getObject(i), pH)) i--;
     }
     bool actsAfterParticlesMove() const override { return isActivated_; }
+    // the plane as a device region (not for classes that redefine getDistance)
+    bool deviceRegion(int32_t& kind, double* params) const override
+    {
+        if (typeid(*this) != typeid(DeletionBoundary)) return false;
+        kind = DEM_REGION_HALF_SPACE;
+        params[0] = normal_.X;
+        params[1] = normal_.Y;
+        params[2] = normal_.Z;
+        params[3] = distance_;
+        return true;
+    }
+    // checkBoundaryAfterParticlesMove with the deleting side known: the same
+    // removals in the same order, the flags following the particles' moves
+    void removeFlagged(ParticleHandler& pH, std::vector<char>& flagged) override
+    {
+        if (!isActivated_) return;
+        for (unsigned i = 0; i < pH.getSize(); i++) {
+            if (!flagged[i]) continue;
+            BaseParticle* p = pH.getObject(i);
+            ++numberOfParticlesDeleted_;
+            massDeleted_ += p->getMass();
+            const double r = p->getRadius();
+            const unsigned dims = pH.getParticleDimensions();
+            volumeDeleted_ += dims == 3 ? 4.0 / 3.0 * constants::pi * r * r * r
+                                        : (dims == 2 ? constants::pi * r * r : 2.0 * r);
+            flagged[i] = flagged.back();
+            flagged.pop_back();
+            pH.removeObject(i);
+            i--;
+        }
+    }
     void activate() { isActivated_ = true; }
     void deactivate() { isActivated_ = false; }
     unsigned getNumberOfParticlesDeleted() const { return numberOfParticlesDeleted_; }
@@ -90,6 +122,18 @@ public:
                    : 1;
     }
     std::string getName() const override { return "CubeDeletionBoundary"; }
+    bool deviceRegion(int32_t& kind, double* params) const override
+    {
+        if (typeid(*this) != typeid(CubeDeletionBoundary)) return false;
+        kind = DEM_REGION_BOX_INTERIOR;
+        params[0] = posMin_.X;
+        params[1] = posMin_.Y;
+        params[2] = posMin_.Z;
+        params[3] = posMax_.X;
+        params[4] = posMax_.Y;
+        params[5] = posMax_.Z;
+        return true;
+    }
     // CubeDeletionBoundary::write (CubeDeletionBoundary.cc:79-83)
     void write(std::ostream& os) const override
     {

@@ -528,9 +528,30 @@ void DPMBase::computeOneTimeStep()
     check(dem_step_begin(ctx_), "dem_step_begin");
     particlesSynced_ = interactionsSynced_ = false;
     if (boundariesActAfterParticlesMove()) {
-        // deletion boundaries see the drifted state
-        synchroniseParticles();
-        for (auto& b : boundaryHandler) b->checkBoundaryAfterParticlesMove(particleHandler);
+        // deletion boundaries see the drifted state: on the device when they
+        // are regions it can test, otherwise on the host
+        bool onDevice = true;
+        int32_t kind;
+        double prm[6];
+        for (const auto& b : boundaryHandler)
+            if (b->actsAfterParticlesMove() && !b->deviceRegion(kind, prm)) onDevice = false;
+        if (onDevice) {
+            for (auto& b : boundaryHandler) {
+                if (!b->actsAfterParticlesMove() || !b->deviceRegion(kind, prm)) continue;
+                flushHandlerChanges();
+                int64_t nf = 0;
+                check(dem_find_in_region(ctx_, kind, prm, 0, &nf, nullptr), "dem_find_in_region");
+                if (nf == 0) continue;
+                std::vector<int64_t> idx(nf);
+                check(dem_find_in_region(ctx_, kind, prm, nf, &nf, idx.data()), "dem_find_in_region");
+                std::vector<char> flagged(particleHandler.getNumberOfObjects(), 0);
+                for (int64_t i : idx) flagged[(size_t)i] = 1;
+                b->removeFlagged(particleHandler, flagged);
+            }
+        } else {
+            synchroniseParticles();
+            for (auto& b : boundaryHandler) b->checkBoundaryAfterParticlesMove(particleHandler);
+        }
         flushHandlerChanges();
     }
     if (boundariesActBeforeTimeStep()) {

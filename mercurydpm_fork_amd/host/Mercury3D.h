@@ -441,6 +441,12 @@ public:
     // DPMBase.cc:3272-3300); the handler holds the drifted state
     virtual void checkBoundaryAfterParticlesMove(ParticleHandler&) {}
     virtual bool actsAfterParticlesMove() const { return false; }
+    // A boundary whose particles to delete form a region the device can test
+    // (dem_find_in_region) gives it here, and removes the flagged particles
+    // (flags in handler order) in removeFlagged as its
+    // checkBoundaryAfterParticlesMove would.
+    virtual bool deviceRegion(int32_t& /*kind*/, double* /*params*/) const { return false; }
+    virtual void removeFlagged(ParticleHandler&, std::vector<char>&) {}
 
 protected:
     friend class BoundaryHandler;

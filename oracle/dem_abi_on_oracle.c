@@ -47,6 +47,11 @@ int dem_add_particles(dem_ctx* ctx, int64_t k, const uint32_t* id, const int32_t
 {
     return oracle_dem_add_particles(O(ctx), k, id, species, radius, inv_mass, inv_inertia, x, v, w, q);
 }
+int dem_find_in_region(dem_ctx* ctx, int32_t kind, const double* params, int64_t cap, int64_t* n_out,
+                       int64_t* index_out)
+{
+    return oracle_dem_find_in_region(O(ctx), kind, params, cap, n_out, index_out);
+}
 int dem_remove_particles(dem_ctx* ctx, int64_t k, const int64_t* index)
 {
     return oracle_dem_remove_particles(O(ctx), k, index);

@@ -1167,6 +1167,30 @@ int oracle_dem_remove_particles(oracle_ctx* c, int64_t k, const int64_t* index)
     return DEM_OK;
 }
 
+/* DeletionBoundary::getDistance (DeletionBoundary.cc:109-112) and
+ * CubeDeletionBoundary::getDistance (CubeDeletionBoundary.cc:60-69): the
+ * handler indices of the particles on the deleting side, increasing. */
+int oracle_dem_find_in_region(oracle_ctx* c, int32_t kind, const double* g, int64_t cap, int64_t* nOut,
+                              int64_t* out)
+{
+    if (!c || !g || !nOut) return fail(c, DEM_ERR_ARG, "find_in_region: bad arguments");
+    int64_t n = 0;
+    for (int64_t i = 0; i < c->nReal; ++i) {
+        const V3 x = c->p[i].pos;
+        int in;
+        if (kind == DEM_REGION_HALF_SPACE)
+            in = g[3] - vdot(x, v3(g[0], g[1], g[2])) < 0;
+        else
+            in = g[0] < x.X && x.X < g[3] && g[1] < x.Y && x.Y < g[4] && g[2] < x.Z && x.Z < g[5];
+        if (in) {
+            if (n < cap) out[n] = i;
+            ++n;
+        }
+    }
+    *nOut = n;
+    return DEM_OK;
+}
+
 /* MercuryBase::checkParticleForInteractionLocal (MercuryBase.cc:622-650):
  * the walls, then every real particle (BaseParticle::isInContactWith,
  * BaseParticle.cc:863-871, with Vec3D::getDistanceSquared, Vector.h:311-317). */

@@ -42,6 +42,8 @@ int oracle_dem_download_particles(oracle_ctx* c, double* x, double* v, double* w
 int oracle_dem_add_particles(oracle_ctx* c, int64_t k, const uint32_t* id, const int32_t* species,
                              const double* radius, const double* inv_mass, const double* inv_inertia,
                              const double* x, const double* v, const double* w, const double* q);
+int oracle_dem_find_in_region(oracle_ctx* c, int32_t kind, const double* params, int64_t cap, int64_t* n_out,
+                              int64_t* index_out);
 int oracle_dem_remove_particles(oracle_ctx* c, int64_t k, const int64_t* index);
 int oracle_dem_check_particles(oracle_ctx* c, int64_t k, const double* x, const double* radius,
                                const int32_t* species, const double* max_distance, int32_t* free_out);
