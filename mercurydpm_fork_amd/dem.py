@@ -140,6 +140,7 @@ def _declare(lib, prefix: str, has_config: bool):
     f("add_particles", _I32, _PTR, _I64, *([_PTR] * 9))
     f("check_particles", _I32, _PTR, _I64, *([_PTR] * 5))
     f("remove_particles", _I32, _PTR, _I64, _PTR)
+    f("find_in_region", _I32, _PTR, _I32, _PTR, _I64, C.POINTER(_I64), _PTR)
     f("cg_z", _I32, _PTR, C.POINTER(CGConfig), _PTR)
     f("download_particles", _I32, _PTR, *([_PTR] * 6))
     f("contact_count", _I32, _PTR, C.POINTER(_I64))
@@ -353,6 +354,16 @@ class Engine:
         idx = np.ascontiguousarray(np.atleast_1d(np.asarray(index, np.int64)))
         self._check(self._fn("remove_particles")(self.h, len(idx), _ptr(idx)))
         self.n -= len(idx)
+
+    def find_in_region(self, kind: int, params) -> np.ndarray:
+        """Handler indices of the particles a deletion boundary removes
+        (kind 0: half-space normal, distance; kind 1: open box min, max)."""
+        prm = np.ascontiguousarray(params, np.float64)
+        n = _I64()
+        self._check(self._fn("find_in_region")(self.h, int(kind), _ptr(prm), 0, C.byref(n), None))
+        out = np.zeros(max(n.value, 1), np.int64)
+        self._check(self._fn("find_in_region")(self.h, int(kind), _ptr(prm), n.value, C.byref(n), _ptr(out)))
+        return out[:n.value]
 
     def check_particles(self, x, radius, species, max_distance) -> np.ndarray:
         """DPMBase::checkParticleForInteraction for candidate spheres against

@@ -532,3 +532,28 @@ def test_remove_particles(case):
         e.step(20)
     assert_contacts_close(g, o)
     np.testing.assert_allclose(g.download()["x"], o.download()["x"], rtol=0, atol=1e-9)
+
+
+def test_find_in_region_against_oracle():
+    """dem_find_in_region (the deletion boundaries' test on the device)
+    against the oracle: half-spaces of several orientations and open boxes,
+    mid-step; the same handler indices."""
+    sc = S.mixture(n=400, seed=49, lx=7.0, ly=7.0, lz=7.0, inflate=1.12, **CASES[0])
+    g, o = make_pair(sc)
+    for e in (g, o):
+        e.compute_forces()
+        e.step(5)
+        e.step_begin()
+    rng = np.random.default_rng(9)
+    for _ in range(6):
+        nrm = rng.normal(size=3)
+        nrm /= np.linalg.norm(nrm)
+        prm = np.append(nrm, rng.uniform(-2.0, 6.0))
+        lo = rng.uniform(0.0, 4.0, 3)
+        box = np.concatenate([lo, lo + rng.uniform(1.0, 4.0, 3)])
+        for kind, params in ((0, prm), (1, box)):
+            a, b = g.find_in_region(kind, params), o.find_in_region(kind, params)
+            np.testing.assert_array_equal(a, b)
+    assert any(len(g.find_in_region(0, np.array([0.0, 0.0, -1.0, -3.5]))) > 0 for _ in range(1))
+    for e in (g, o):
+        e.step_end()
