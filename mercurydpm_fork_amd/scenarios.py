@@ -129,6 +129,49 @@ def periodic_walls_unit_test() -> Scenario:
     return Scenario("PeriodicWallsUnitTest", [s], [], per, (0.0, 0.0, 0.0), 1.1e-05, st, t_max=0.11)
 
 
+def periodic_walls_with_friction_unit_test() -> Scenario:
+    """Drivers/UnitTests/PeriodicWallsWithSlidingFrictionUnitTest.cpp:38-123
+    (Mercury2D, so discs of particle dimension 2): eight discs of radius 0.1
+    and density 2500 in a unit box periodic in x and y, no gravity.
+    LinearViscoelasticFrictionSpecies with k = 1e5, no dissipation, sliding
+    stiffness 2/7 k, rolling and torsion stiffness 2/5 k, all three friction
+    coefficients 0.5.  Three colliding pairs: (4, 5) meets directly, (2, 3)
+    starts direct and ends across y = 0, (6, 7) starts across x = 0, meets
+    directly and parts across the boundary again; 2, 4 and 6 spin at
+    (0.3, 0.6, 0.9).  Particles 0 and 1 sit apart and are removed mid-run
+    (`removals`: handler index removed in actionsBeforeTimeStep once the
+    time exceeds the given value, cpp:127-181), which moves 7 to index 1 and
+    then 6 to index 0 with their springs.  The reference then requires the
+    three pairs to end in the same state up to their shifts
+    (`PERIODIC_FRICTION_SHIFTS`, cpp:195-249, tolerance 1e-10)."""
+    k = 1e5
+    s = Species(density=2500, friction_law=dem.FRICTION_FULL)
+    s.stiffness = k
+    s.sliding_stiffness = 2.0 / 7.0 * k
+    s.set_sliding_friction(0.5)
+    s.rolling_stiffness = 2.0 / 5.0 * k
+    s.set_rolling_friction(0.5)
+    s.torsion_stiffness = 2.0 / 5.0 * k
+    s.set_torsion_friction(0.5)
+    pos = [(0.1, 0.1, 0.0), (0.3, 0.3, 0.0), (0.6, 0.041, 0.0), (0.4, 0.001, 0.0), (0.6, 0.82, 0.0),
+           (0.4, 0.78, 0.0), (0.02, 0.42, 0.0), (0.82, 0.38, 0.0)]
+    vel = [(0, 0, 0), (0, 0, 0), (-1, 0, 0), (1, 0, 0), (-1, 0, 0), (1, 0, 0), (-1, 0, 0), (1, 0, 0)]
+    spin = (0.3, 0.6, 0.9)
+    omega = [(0, 0, 0), (0, 0, 0), spin, (0, 0, 0), spin, (0, 0, 0), spin, (0, 0, 0)]
+    st = make_state(pos, vel, [0.1] * 8, [0] * 8, [s], dims=2, omega=omega)
+    per = [periodic_boundary(0, 0.0, 1.0), periodic_boundary(1, 0.0, 1.0)]
+    sc = Scenario("PeriodicWallsWithSlidingFrictionUnitTest", [s], [], per, (0.0, 0.0, 0.0), 2.0e-4, st,
+                  t_max=8e-2)
+    sc.info["removals"] = [(3.5e-2, 1), (4.5e-2, 0)]
+    return sc
+
+
+# (reference index, compared index, shift) after the run:
+# PeriodicWallsWithSlidingFrictionUnitTest.cpp:195-249
+PERIODIC_FRICTION_SHIFTS = [(4, 0, (-0.58, -0.4, 0.0)), (4, 2, (0.0, -0.779, 0.0)),
+                            (5, 1, (1.0 - 0.58, -0.4, 0.0)), (5, 3, (0.0, 1.0 - 0.779, 0.0))]
+
+
 PERIODIC_WALLS_GOLD = [  # PeriodicWallsUnitTest.cpp:158-361, tolerance 1e-10
     ((0.00950076062215577, 0.002, 0), (-0.005560409816604, 0, 0)),
     ((0.000725163257251641, 0.002, 0), (-0.000808302139899, 0, 0)),
@@ -196,6 +239,43 @@ def oblique_impact_self_test() -> Scenario:
 
 
 OBLIQUE_IMPACT_SAVE_COUNT = 500
+
+
+def squeezed_sphere_self_test(kind: str) -> Scenario:
+    """The rolling and torsion friction self tests
+    (Drivers/SelfTests/Interactions/RollingFrictionSelfTest.cpp:39-81 and
+    TorsionFrictionSelfTest.cpp:39-81): a free unit-mass sphere (r = 0.5,
+    density 6/pi) squeezed between two fixed spheres at z = -+0.995 spins at
+    omega = e_y (kind "rolling") or e_z ("torsion").  LinearViscoelastic-
+    FrictionSpecies with t_c = 0.02 and normal and tangential restitution
+    0.7 (mass argument 1), rolling (torsion) stiffness and dissipation 0.4 of
+    the normal ones and friction coefficient 0.1; no sliding friction, no
+    gravity, dt = 0.02 t_c, 500 steps.  The reference prints, after each
+    step's force pass, the torque (y for rolling, z for torsion) and the
+    force z of the upper fixed sphere (SelfTestData/*FrictionSelfTest.torque);
+    `torque_axis` names the component."""
+    if kind not in ("rolling", "torsion"):
+        raise ValueError("kind is 'rolling' or 'torsion'")
+    tc = 2e-2
+    s = Species(density=6.0 / PI, friction_law=dem.FRICTION_FULL)
+    s.set_collision_time_and_normal_and_tangential_restitution(tc, 0.7, 0.7, 1)
+    if kind == "rolling":
+        s.rolling_stiffness = 0.4 * s.stiffness
+        s.rolling_dissipation = 0.4 * s.dissipation
+        s.set_rolling_friction(0.1)
+        spin = (0.0, 1.0, 0.0)
+    else:
+        s.torsion_stiffness = 0.4 * s.stiffness
+        s.torsion_dissipation = 0.4 * s.dissipation
+        s.set_torsion_friction(0.1)
+        spin = (0.0, 0.0, 1.0)
+    r = 0.5
+    st = make_state([(0, 0, 0), (0, 0, -1.99 * r), (0, 0, 1.99 * r)], [(0, 0, 0)] * 3, [r] * 3, [0] * 3, [s],
+                    fixed=[False, True, True], omega=[spin, (0, 0, 0), (0, 0, 0)])
+    sc = Scenario(f"{kind.capitalize()}FrictionSelfTest", [s], [], [], (0.0, 0.0, 0.0), 0.02 * tc, st,
+                  t_max=10 * tc)
+    sc.info["torque_axis"] = 1 if kind == "rolling" else 2
+    return sc
 
 
 # Drivers/UnitTests/MindlinSelfTest.cpp:36-58 with the set-ups of

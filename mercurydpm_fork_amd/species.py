@@ -112,6 +112,25 @@ class Species:
             self.stiffness = .5 * mass * ((PI / tc) * (PI / tc) + (self.dissipation / mass) * (self.dissipation / mass))
         return self
 
+    def set_collision_time_and_normal_and_tangential_restitution(self, tc: float, eps: float, beta: float,
+                                                                   mass: float):
+        """SlidingFrictionSpecies::setCollisionTimeAndNormalAndTangential-
+        RestitutionCoefficient (SlidingFrictionSpecies.cc:207-246) for a
+        linear viscoelastic normal law: k and gamma from (tc, eps), then the
+        tangential stiffness and dissipation of Deen et al. (2007), eqs. 30
+        and 43.  The friction coefficient is left as it is."""
+        self.set_collision_time_and_restitution(tc, eps, mass)
+        stiffness = self.stiffness
+        lb = math.log(beta) if beta != 0.0 else -math.inf
+        le = math.log(eps)
+        self.sliding_stiffness = 2.0 / 7.0 * stiffness * (PI * PI + lb * lb) / (PI * PI + le * le)
+        if beta != 0.0:
+            self.sliding_dissipation = -2 * lb * math.sqrt(1.0 / 7.0 * mass * self.sliding_stiffness /
+                                                           (PI * PI + lb * lb))
+        else:
+            self.sliding_dissipation = 2. * math.sqrt(1.0 / 7.0 * mass * self.sliding_stiffness)
+        return self
+
     def set_stiffness_and_restitution(self, k: float, eps: float, mass: float):
         """setStiffnessAndRestitutionCoefficient / setRestitutionCoefficient (cc:207-222)."""
         if self.constant_restitution:

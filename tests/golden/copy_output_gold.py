@@ -11,7 +11,8 @@ the energies and XBalls data of ForceSelfTest (cases drawn from the
 problem's random number generator), the XBalls data of ExtremeOverlapUnitTest (DPMBase::outputXBallsData,
 DPMBase.cc:2356-2417), and the final restart files of FreeFallSelfTest and
 TwoParticleElasticCollisionSelfTest (DPMBase::write, DPMBase.cc:3550-3690),
-and all four output files of InsertionBoundarySelfTest.
+and all four output files of InsertionBoundarySelfTest, and the torque
+series of RollingFrictionSelfTest and TorsionFrictionSelfTest.
 """
 import os
 import shutil
@@ -41,6 +42,12 @@ for name, dst in (("DeletionBoundarySelfTest", "deletion_boundary_self_test"),
     FILES[f"Drivers/SelfTests/Boundaries/SelfTestData/{name}.data"] = f"{dst}.data"
 # roughBottomSelfTest: a chute on a MULTILAYER bottom (ChuteBottom::makeRoughBottom)
 FILES["Drivers/ChuteDemos/SelfTestData/roughBottomMultiLayer.restart"] = "rough_bottom_multi_layer.restart"
+# RollingFrictionSelfTest, TorsionFrictionSelfTest: time, torque and normal
+# force of a fixed sphere pressed on a spinning one, after every step
+# (FrictionInteraction rolling and torsion springs)
+for name in ("RollingFrictionSelfTest", "TorsionFrictionSelfTest"):
+    FILES[f"Drivers/SelfTests/Interactions/SelfTestData/{name}.torque"] = \
+        name.replace("FrictionSelfTest", "_friction_self_test").lower() + ".torque"
 for ext in ("data", "ene", "fstat", "restart"):
     FILES[f"Drivers/SelfTests/Boundaries/SelfTestData/InsertionBoundarySelfTest.{ext}"] = \
         f"insertion_boundary_self_test.{ext}"

@@ -188,6 +188,27 @@ def test_engine_inclined_plane_and_hertz():
     assert abs(st["v"][1][0] - 0.8) < 1e-2 and abs(st["v"][0][0] + 0.8) < 1e-2
 
 
+@pytest.mark.parametrize("kind", ["rolling", "torsion"])
+def test_engine_rolling_and_torsion_friction_self_test_gold(kind):
+    """The reference's RollingFrictionSelfTest / TorsionFrictionSelfTest gold
+    .torque on the engine: every row to its 6 printed digits."""
+    from friction_check import check_torque_rows, squeezed_sphere_rows
+    check_torque_rows(squeezed_sphere_rows(dem.Engine(), kind), kind)
+
+
+def test_engine_periodic_walls_with_friction_unit_test():
+    """PeriodicWallsWithSlidingFrictionUnitTest on the engine (full friction
+    across periodic boundaries, two removals mid-run): the reference's
+    shift invariance within 1e-10, and the oracle's final state to
+    round-off."""
+    from friction_check import check_periodic_friction, run_periodic_friction
+    d = run_periodic_friction(dem.Engine())
+    check_periodic_friction(d)
+    o = run_periodic_friction(oracle_lib.engine())
+    np.testing.assert_allclose(d["x"], o["x"], atol=1e-13, rtol=0)
+    np.testing.assert_allclose(d["w"], o["w"], atol=1e-12, rtol=0)
+
+
 def test_box_lvf_vs_oracle():
     """5000-sphere box (periodic x/y, floor, gravity), 400 steps: same stored
     contacts and state as the oracle."""

@@ -303,3 +303,21 @@ def test_oracle_interactions_sum_to_particle_forces():
         assert np.max(np.abs(c - r["contact"])) <= 1e-12 * rp
         fn = float(np.dot(r["force"], r["normal"]))
         assert np.linalg.norm(r["force"] - fn * r["normal"] - r["tangential"]) <= 1e-12 * np.linalg.norm(r["force"])
+
+
+@pytest.mark.parametrize("kind", ["rolling", "torsion"])
+def test_rolling_and_torsion_friction_self_test_gold(kind):
+    """RollingFrictionSelfTest / TorsionFrictionSelfTest gold .torque (500
+    rows, 6 printed digits): the FrictionInteraction rolling and torsion
+    springs (FrictionInteraction.cc:108-222) -- the torque is capped at
+    mu * r * f_n while the spring slips, then oscillates and decays."""
+    from friction_check import check_torque_rows, squeezed_sphere_rows
+    check_torque_rows(squeezed_sphere_rows(oracle_lib.engine(), kind), kind)
+
+
+def test_periodic_walls_with_friction_unit_test():
+    """PeriodicWallsWithSlidingFrictionUnitTest (sliding, rolling and torsion
+    springs across periodic boundaries, with particle removals moving pairs
+    to other handler indices): the shifted pairs end alike within 1e-10."""
+    from friction_check import check_periodic_friction, run_periodic_friction
+    check_periodic_friction(run_periodic_friction(oracle_lib.engine()))
