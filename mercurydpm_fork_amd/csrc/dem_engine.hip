@@ -1940,6 +1940,9 @@ struct dem_ctx {
     // candidate buffer of dem_check_particles (not in allocs: it outlives re-uploads)
     void* d_check = nullptr;
     int64_t check_cap = 0;
+    // counter and index buffer of dem_find_in_region (grow-only, outlives re-uploads)
+    void* d_region = nullptr;
+    int64_t region_cap = 0;
     // profiling
     bool profiling = false;
     std::vector<hipEvent_t> events;
@@ -2448,6 +2451,7 @@ int dem_destroy(dem_ctx* ctx)
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     free_all(ctx);
     if (ctx->d_check) (void)hipFree(ctx->d_check);
+    if (ctx->d_region) (void)hipFree(ctx->d_region);
     for (hipEvent_t e : ctx->events) (void)hipEventDestroy(e);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -3072,10 +3076,15 @@ int dem_remove_particles(dem_ctx* ctx, int64_t k, const int64_t* index)
 int dem_find_in_region(dem_ctx* ctx, int32_t kind, const double* params, int64_t capacity, int64_t* n_out,
                        int64_t* index_out)
 {
-    if (!ctx || !params || !n_out || capacity < 0 || (capacity > 0 && !index_out)) return DEM_ERR_ARG;
+    int r = ready(ctx);
+    if (r) return r;
+    if (!params || !n_out || capacity < 0 || (capacity > 0 && !index_out))
+        return set_err(ctx, DEM_ERR_ARG, "find_in_region: bad arguments");
     if (kind != DEM_REGION_HALF_SPACE && kind != DEM_REGION_BOX_INTERIOR)
         return set_err(ctx, DEM_ERR_ARG, "find_in_region: unknown region kind");
-    if (!ctx->uploaded) return set_err(ctx, DEM_ERR_STATE, "no particles uploaded");
+    const int np = kind == DEM_REGION_BOX_INTERIOR ? 6 : 4;
+    for (int a = 0; a < np; ++a)
+        if (!std::isfinite(params[a])) return set_err(ctx, DEM_ERR_ARG, "find_in_region: parameters must be finite");
     *n_out = 0;
     if (ctx->n == 0) return DEM_OK;
     HIPCHK(hipSetDevice(ctx->device));
@@ -3086,11 +3095,18 @@ int dem_find_in_region(dem_ctx* ctx, int32_t kind, const double* params, int64_t
         g.b[a] = kind == DEM_REGION_BOX_INTERIOR ? params[3 + a] : 0.0;
     }
     g.c = kind == DEM_REGION_HALF_SPACE ? params[3] : 0.0;
-    // counter and indices in one buffer: at most n indices
+    // counter and at most n indices, in a buffer of the context that only
+    // grows (deletion boundaries call this twice per step)
     const size_t bytes = sizeof(int) + sizeof(uint32_t) * (size_t)ctx->n;
-    void* buf = nullptr;
-    HIPCHK(hipMalloc(&buf, bytes));
-    int* counter = (int*)buf;
+    if ((int64_t)bytes > ctx->region_cap) {
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        if (ctx->d_region) (void)hipFree(ctx->d_region);
+        ctx->d_region = nullptr;
+        ctx->region_cap = 0;
+        HIPCHK(hipMalloc(&ctx->d_region, bytes));
+        ctx->region_cap = (int64_t)bytes;
+    }
+    int* counter = (int*)ctx->d_region;
     uint32_t* idx = (uint32_t*)(counter + 1);
     HIPCHK(hipMemsetAsync(counter, 0, sizeof(int), ctx->stream));
     if (ctx->ns_springs == 9)
@@ -3099,14 +3115,12 @@ int dem_find_in_region(dem_ctx* ctx, int32_t kind, const double* params, int64_t
     else
         hipLaunchKernelGGL(k_find_region<3>, dim3(grid_for(ctx->n)), dim3(kBlock), 0, ctx->stream, ctx->d3, ctx->prm, g,
                            counter, (int)ctx->n, idx);
+    HIPCHK(hipGetLastError());
     int found = 0;
-    hipError_t e = hipGetLastError();
-    if (e == hipSuccess) e = hipMemcpyAsync(&found, counter, sizeof(int), hipMemcpyDeviceToHost, ctx->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    HIPCHK(hipMemcpyAsync(&found, counter, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
     std::vector<uint32_t> h(found > 0 ? found : 0);
-    if (e == hipSuccess && found > 0) e = hipMemcpy(h.data(), idx, sizeof(uint32_t) * found, hipMemcpyDeviceToHost);
-    (void)hipFree(buf);
-    if (e != hipSuccess) return set_err(ctx, DEM_ERR_DEVICE, std::string("find_in_region: ") + hipGetErrorString(e));
+    if (found > 0) HIPCHK(hipMemcpy(h.data(), idx, sizeof(uint32_t) * found, hipMemcpyDeviceToHost));
     std::sort(h.begin(), h.end());
     *n_out = found;
     for (int64_t k = 0; k < std::min<int64_t>(capacity, found); ++k) index_out[k] = h[k];

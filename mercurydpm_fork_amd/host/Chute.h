@@ -5,6 +5,8 @@
 // in x.  Set-up runs on the host; the time step runs on the device.
 #pragma once
 
+#include <typeinfo>
+
 #include "DeletionBoundary.h"
 #include "InsertionBoundary.h"
 
@@ -74,8 +76,15 @@ public:
     void setupInitialConditions() override;
     // Chute::actionsBeforeTimeStep (Chute.cc:225-228)
     void actionsBeforeTimeStep() override { cleanChute(); }
-    // cleanChute looks at the particles on every 102nd call only
-    bool actionsBeforeTimeStepUsesParticles() const override { return cleanCount_ > 100; }
+    // cleanChute looks at the particles on every 102nd call only.  That
+    // holds while the hook that runs is this one: a class derived from Chute
+    // may override actionsBeforeTimeStep with code that reads or sets the
+    // particles every step (a driver that drives the bed, say), so it gets
+    // the exchange on every step unless it overrides this method as well.
+    bool actionsBeforeTimeStepUsesParticles() const override
+    {
+        return typeid(*this) != typeid(Chute) || cleanCount_ > 100;
+    }
     // Chute::setupSideWalls (Chute.cc:288-309)
     void setupSideWalls();
     void makeChutePeriodic() { isChutePeriodic_ = true; }

@@ -105,3 +105,32 @@ def test_chute_engine_vs_oracle(bottom, oracle_driver, tmp_path):
     a, b = np.nan_to_num(a), np.nan_to_num(b)
     scale = np.max(np.abs(b), axis=0)
     assert np.all(np.abs(a - b) <= 1e-6 * np.abs(b) + 1e-9 * scale)
+
+
+def check_hook_run(summary):
+    """ChuteHookTest: the driver's own actionsBeforeTimeStep holds the bed at
+    rest on every one of its 500 steps (the speed left is one closing
+    half-kick, g dt / 2 = 5e-5), and cleanChute removed the three spheres
+    beyond the chute's ends."""
+    assert summary["steps"] == 500
+    assert summary["particles"] == 10
+    assert summary["vmax"] < 1e-4
+
+
+def run_hook(exe, cwd):
+    r = subprocess.run([exe], cwd=cwd, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    w = r.stdout.split()
+    return {w[k]: float(w[k + 1]) for k in range(0, len(w) - 1, 2)}
+
+
+def test_chute_subclass_hook_runs_every_step_on_oracle(tmp_path):
+    exe = oc.build_on_oracle(["ChuteHookTest"], str(tmp_path))["ChuteHookTest"]
+    check_hook_run(run_hook(exe, str(tmp_path)))
+
+
+@pytest.mark.gpu
+def test_chute_subclass_hook_runs_every_step(tmp_path):
+    exe = os.path.join(oc.DRIVERS, "bin", "ChuteHookTest")
+    assert os.path.exists(exe), "run __graft_entry__.build() first"
+    check_hook_run(run_hook(exe, str(tmp_path)))
