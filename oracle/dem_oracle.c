@@ -31,6 +31,17 @@
  * TwoParticleElasticCollisionSelfTest and FreeFallSelfTest and the .data file
  * of ExtremeOverlapUnitTest.  oracle_dem_download_interactions: its forces sum
  * to the particle forces and its geometry follows from the radii.
+ *
+ * Built with -fopenmp (oracle/liboracle_dem_omp.so, the multi-core CPU
+ * baseline of bench.py; never a parity build) the pair, wall and integration
+ * loops run on OpenMP threads the way the reference's OpenMP build runs them
+ * (DPMBase::computeAllForces, DPMBase.cc:3397-3526): threads evaluate
+ * disjoint pairs, interactions new to the handler are created after the loop
+ * from per-thread lists (InteractionHandler::getInteraction with
+ * newObjects_, InteractionHandler.cc:146-176, addNewObjectsOMP), and each
+ * particle then sums its forces (sumForceTorqueOMP, BaseInteractable.cc:
+ * 116-175), here in the order of its interaction list.  Contact sets and
+ * springs are those of the serial build; force sums round differently.
  */
 #include "dem_oracle.h"
 
@@ -38,6 +49,9 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 typedef struct { double X, Y, Z; } V3;
 
@@ -670,6 +684,200 @@ static void pairLoop(oracle_ctx* c)
     }
 }
 
+#ifdef _OPENMP
+/* ---------------------------------------------- OpenMP timing build only */
+
+typedef struct {
+    Inter* v;
+    int64_t n, cap;
+} NewList;
+
+static NewList* ompLists;
+static int ompNLists;
+
+static void newPush(NewList* l, const Inter* x)
+{
+    if (l->n == l->cap) {
+        l->cap = l->cap ? 2 * l->cap : 256;
+        l->v = (Inter*)realloc(l->v, sizeof(Inter) * (size_t)l->cap);
+    }
+    l->v[l->n++] = *x;
+}
+
+/* computeInternalForce without the handler writes: an interaction not found
+ * in P's list is evaluated in a private copy that joins the handler after
+ * the parallel loop; the forces stay in the interaction until the per-
+ * particle sum. */
+static void evalPair(oracle_ctx* c, int64_t a, int64_t b, NewList* nl)
+{
+    const Particle* P1 = &c->p[a];
+    const Particle* P2 = &c->p[b];
+    if (isFixed(P1) && isFixed(P2)) return;
+    if (P1->from >= 0 && P2->from >= 0) return;
+    int64_t iI, iJ;
+    if (P1->id > P2->id) { iI = b; iJ = a; } else { iI = a; iJ = b; }
+    const Particle* PI = &c->p[iI];
+    const Particle* PJ = &c->p[iJ];
+    V3 branch = vsub(PI->pos, PJ->pos);
+    double d2 = vlen2(branch);
+    const dem_pair_params* sp = mixed(c, PJ->species, PI->species);
+    double sumR = PJ->radius + PI->radius + sp->interaction_distance;
+    if (d2 >= sumR * sumR) return;
+    int32_t k = getExisting(c, (int32_t)iI, (int32_t)iJ, 0);
+    Inter tmp;
+    Inter* x;
+    if (k >= 0) {
+        x = &c->in[k];
+    } else {
+        memset(&tmp, 0, sizeof tmp);
+        tmp.alive = 1;
+        tmp.P = (int32_t)iI;
+        tmp.I = (int32_t)iJ;
+        x = &tmp;
+    }
+    x->ts = (uint32_t)(c->nt + 1);
+    x->sp = mixed(c, PI->species, PJ->species);
+    x->pass = c->pass;
+    double distance = sqrt(d2);
+    x->normal = vdiv(branch, distance);
+    x->overlap = PI->radius + PJ->radius - distance;
+    x->distance = distance;
+    x->contact = vsub(PI->pos, vmul(x->normal, PI->radius - 0.5 * x->overlap));
+    computeForce(c, x);
+    if (k < 0) newPush(nl, &tmp);
+}
+
+static void evalWall(oracle_ctx* c, int64_t pi, int32_t wi, NewList* nl)
+{
+    const Particle* p = &c->p[pi];
+    if (p->from >= 0) return;
+    const dem_wall* w = &c->walls[wi];
+    double wallRadius = p->radius + mixed(c, p->species, w->species)->interaction_distance;
+    double distance;
+    V3 normal;
+    if (!wallDistanceAndNormal(w, p, wallRadius, &distance, &normal)) return;
+    int32_t k = getExisting(c, (int32_t)pi, wi, 1);
+    Inter tmp;
+    Inter* x;
+    if (k >= 0) {
+        x = &c->in[k];
+    } else {
+        memset(&tmp, 0, sizeof tmp);
+        tmp.alive = 1;
+        tmp.P = (int32_t)pi;
+        tmp.I = wi;
+        tmp.IisWall = 1;
+        x = &tmp;
+    }
+    x->ts = (uint32_t)(c->nt + 1);
+    x->sp = mixed(c, p->species, w->species);
+    x->pass = c->pass;
+    x->normal = vneg(normal);
+    x->distance = distance;
+    x->overlap = p->radius - distance;
+    x->contact = vsub(p->pos, vmul(x->normal, p->radius - 0.5 * x->overlap));
+    computeForce(c, x);
+    if (k < 0) newPush(nl, &tmp);
+}
+
+/* The pair loop, the wall loop and the force sums of computeAllForces on
+ * OpenMP threads (the serial pairLoop's grid; each pair once). */
+static void computeAllForcesOmp(oracle_ctx* c)
+{
+    double h = 2.0 * c->largestRadius;
+    if (!(h > 0)) h = 1.0;
+    int64_t nb = 1;
+    while (nb < 2 * c->nP) nb <<= 1;
+    if (nb != c->nBuckets) {
+        free(c->head);
+        c->head = (int64_t*)malloc(sizeof(int64_t) * (size_t)nb);
+        c->nBuckets = nb;
+    }
+    c->next = (int64_t*)realloc(c->next, sizeof(int64_t) * (size_t)(c->nP > 0 ? c->nP : 1));
+    c->cell = (int64_t*)realloc(c->cell, sizeof(int64_t) * 3 * (size_t)(c->nP > 0 ? c->nP : 1));
+    const int nth = omp_get_max_threads();
+    if (nth > ompNLists) {
+        ompLists = (NewList*)realloc(ompLists, sizeof(NewList) * (size_t)nth);
+        for (int t = ompNLists; t < nth; ++t) ompLists[t].v = NULL, ompLists[t].n = ompLists[t].cap = 0;
+        ompNLists = nth;
+    }
+    const int64_t nP = c->nP;
+#pragma omp parallel
+    {
+#pragma omp for schedule(static)
+        for (int64_t b = 0; b < nb; ++b) c->head[b] = -1;
+#pragma omp for schedule(static)
+        for (int64_t i = 0; i < nP; ++i) {
+            c->cell[3 * i] = (int64_t)floor(c->p[i].pos.X / h);
+            c->cell[3 * i + 1] = (int64_t)floor(c->p[i].pos.Y / h);
+            c->cell[3 * i + 2] = (int64_t)floor(c->p[i].pos.Z / h);
+        }
+#pragma omp single
+        for (int64_t i = nP - 1; i >= 0; --i) {
+            uint64_t b = hashCell(c->cell[3 * i], c->cell[3 * i + 1], c->cell[3 * i + 2]) & (uint64_t)(nb - 1);
+            c->next[i] = c->head[b];
+            c->head[b] = i;
+        }
+        NewList* nl = &ompLists[omp_get_thread_num()];
+        nl->n = 0;
+#pragma omp for schedule(dynamic, 256)
+        for (int64_t i = 0; i < nP; ++i) {
+            int64_t cx = c->cell[3 * i], cy = c->cell[3 * i + 1], cz = c->cell[3 * i + 2];
+            for (int dz = -1; dz <= 1; ++dz)
+                for (int dy = -1; dy <= 1; ++dy)
+                    for (int dx = -1; dx <= 1; ++dx) {
+                        int64_t tx = cx + dx, ty = cy + dy, tz = cz + dz;
+                        uint64_t b = hashCell(tx, ty, tz) & (uint64_t)(nb - 1);
+                        for (int64_t j = c->head[b]; j >= 0; j = c->next[j]) {
+                            if (j <= i) continue;
+                            if (c->cell[3 * j] != tx || c->cell[3 * j + 1] != ty || c->cell[3 * j + 2] != tz)
+                                continue;
+                            evalPair(c, i, j, nl);
+                        }
+                    }
+        }
+        for (int32_t w = 0; w < c->nWalls; ++w) {
+#pragma omp for schedule(dynamic, 1024)
+            for (int64_t i = 0; i < nP; ++i)
+                if (!isFixed(&c->p[i])) evalWall(c, i, w, nl);
+        }
+    }
+    /* InteractionHandler::addNewObjectsOMP: the new interactions join the
+     * handler, thread by thread */
+    for (int t = 0; t < nth; ++t)
+        for (int64_t a = 0; a < ompLists[t].n; ++a) {
+            const Inter* x = &ompLists[t].v[a];
+            int32_t k = interNew(c, x->P, x->I, x->IisWall);
+            c->in[k] = *x;
+        }
+    /* sumForceTorqueOMP: each particle adds the forces of the interactions
+     * this pass evaluated, then the external forces */
+#pragma omp parallel for schedule(dynamic, 1024)
+    for (int64_t i = 0; i < nP; ++i) {
+        Particle* p = &c->p[i];
+        for (int32_t a = 0; a < p->nInter; ++a) {
+            const Inter* x = &c->in[p->inter[a]];
+            if (x->pass != c->pass) continue;
+            V3 f, t;
+            if (x->P == (int32_t)i) {
+                f = x->force;
+                t = vsub(x->torque, vcross(vsub(p->pos, x->contact), x->force));
+            } else {
+                f = vneg(x->force);
+                t = vadd(vneg(x->torque), vcross(vsub(p->pos, x->contact), x->force));
+            }
+            p->force = vadd(p->force, f);
+            p->fscale += vlen(x->force);
+            if (c->rotation) {
+                p->torque = vadd(p->torque, t);
+                p->tscale += vlen(t);
+            }
+        }
+        computeExternalForces(c, p);
+    }
+}
+#endif
+
 /* -------------------------------------------------------- periodic images */
 
 static void ensureParticleCapacity(oracle_ctx* c, int64_t n)
@@ -789,10 +997,14 @@ static void computeAllForces(oracle_ctx* c)
         c->p[i].fscale = 0.0;
         c->p[i].tscale = 0.0;
     }
+#ifdef _OPENMP
+    computeAllForcesOmp(c);
+#else
     pairLoop(c);
     for (int32_t w = 0; w < c->nWalls; ++w)
         for (int64_t i = 0; i < c->nP; ++i)
             if (!isFixed(&c->p[i]) && c->p[i].from < 0) computeForcesDueToWall(c, i, w);
+#endif
 }
 
 /* BaseParticle::integrateBeforeForceComputation (BaseParticle.cc:718-752);
@@ -801,6 +1013,9 @@ static void computeAllForces(oracle_ctx* c)
 static void integrateBefore(oracle_ctx* c)
 {
     const double dt = c->dt;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static)
+#endif
     for (int64_t i = 0; i < c->nP; ++i) {
         Particle* p = &c->p[i];
         if (p->invMass == 0.0) {
@@ -823,6 +1038,9 @@ static void integrateBefore(oracle_ctx* c)
 static void integrateAfter(oracle_ctx* c)
 {
     const double dt = c->dt;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static)
+#endif
     for (int64_t i = 0; i < c->nP; ++i) {
         Particle* p = &c->p[i];
         if (p->invMass == 0.0) continue;
@@ -839,9 +1057,13 @@ static void integrateAfter(oracle_ctx* c)
  * PeriodicBoundary::checkBoundaryAfterParticlesMove (PeriodicBoundary.cc:374-391) */
 static void wrapPeriodic(oracle_ctx* c)
 {
-    for (int bi = 0; bi < c->nPeriodic; ++bi)
+    for (int bi = 0; bi < c->nPeriodic; ++bi) {
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static)
+#endif
         for (int64_t i = 0; i < c->nP; ++i)
             if (periodicDistance(&c->periodic[bi], c->p[i].pos) < 0) periodicShift(&c->periodic[bi], &c->p[i].pos);
+    }
 }
 
 /* InteractionHandler::eraseOldInteractions (InteractionHandler.cc:377-393) */
@@ -1550,4 +1772,17 @@ int oracle_dem_cg_z(oracle_ctx* c, const dem_cg_config* g, double* out)
     const double volume = (g->max[0] - g->min[0]) * (g->max[1] - g->min[1]);
     for (size_t k = 0; k < n * DEM_CG_FIELDS; ++k) out[k] /= volume;
     return DEM_OK;
+}
+
+/* Threads of the OpenMP timing build (0 in the serial build, which ignores
+ * the request). */
+int oracle_dem_set_threads(int32_t n)
+{
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+    return omp_get_max_threads();
+#else
+    (void)n;
+    return 0;
+#endif
 }

@@ -54,6 +54,7 @@ int oracle_dem_download_interactions(oracle_ctx* c, int64_t cap, int64_t* n_out,
 int oracle_dem_get_time(oracle_ctx* c, double* time, int64_t* nt);
 int oracle_dem_force_scale(oracle_ctx* c, double* fs, double* ts);
 int oracle_dem_cg_z(oracle_ctx* c, const dem_cg_config* cfg, double* out);
+int oracle_dem_set_threads(int32_t n);
 
 #ifdef __cplusplus
 }

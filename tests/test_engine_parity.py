@@ -65,6 +65,8 @@ CASES = [
     dict(laws="full", periodic=(True, True), walls=("bottom",)),
     dict(laws="hvm", periodic=(True, True), walls=("bottom", "top")),
     dict(laws="hvm", periodic=(True, False), walls=("bottom", "cylinder")),
+    # radius ratio above 2.5: three grid levels, pairs across all of them
+    dict(laws="hvm", periodic=(True, True), walls=("bottom",), radii=(0.15, 0.5)),
 ]
 
 
@@ -76,7 +78,8 @@ def reseeded(sc, st):
                                 x=st["x"], v=st["v"], w=st["w"], q=st["q"], f=st["f"], t=st["t"]))
 
 
-@pytest.mark.parametrize("case", CASES, ids=lambda c: f"{c['laws']}-{'P' if c['periodic'][0] else 'np'}-{len(c['walls'])}w")
+@pytest.mark.parametrize("case", CASES, ids=lambda c: f"{c['laws']}-{'P' if c['periodic'][0] else 'np'}-{len(c['walls'])}w"
+                         + ("-3lv" if "radii" in c else ""))
 def test_first_pass_and_one_step(case):
     """The force pass of initialiseSolve, then one computeOneTimeStep from
     the oracle's state after that pass (so both start from identical forces)."""

@@ -321,3 +321,24 @@ def test_periodic_walls_with_friction_unit_test():
     to other handler indices): the shifted pairs end alike within 1e-10."""
     from friction_check import check_periodic_friction, run_periodic_friction
     check_periodic_friction(run_periodic_friction(oracle_lib.engine()))
+
+
+def test_openmp_timing_build_matches_serial_oracle():
+    """The OpenMP build of the oracle (bench.py's multi-core CPU baseline)
+    computes the serial oracle's step: after 100 steps of a periodic mixture
+    with walls on 4 threads the stored contacts and time stamps are the
+    same, springs and positions agree to round-off (its force sums are
+    taken in another order)."""
+    sc = S.mixture(n=400, seed=17, lx=7.0, ly=7.0, lz=7.0, inflate=1.12, laws="full", periodic=(True, True),
+                   walls=("bottom", "top"))
+    a = run_oracle(sc, n_steps=100)
+    b, threads = oracle_lib.engine_omp(4)
+    assert threads == 4
+    S.apply(b, sc)
+    b.compute_forces()
+    b.step(100)
+    ca, cb = a.contacts(), b.contacts()
+    assert ca.keys() == cb.keys() and ca.n > 40
+    np.testing.assert_array_equal(ca.ts, cb.ts)
+    np.testing.assert_allclose(cb.spring, ca.spring, rtol=1e-6, atol=1e-9 * np.abs(ca.spring).max())
+    np.testing.assert_allclose(b.download()["x"], a.download()["x"], rtol=0, atol=1e-10)
