@@ -4,15 +4,18 @@
 Workload: the polydisperse periodic chute -- 2,000,000 flowing spheres with
 d ~ U[0.5, 1.5] (three grid levels) over a rough base of fixed spheres,
 tilted 24 degrees, periodic in x and y, HertzianViscoelasticMindlin with
-e = 0.8 and mu = 0.5, dt = t_c / 50, FP64.  The spheres start on a
-non-overlapping lattice and are first settled into a dense bed (untimed:
-strong gravity straight down, then the same with background drag), then the
-chute gravity is switched on and, after the warm-up steps, K steps are timed.
+e = 0.8 and mu = 0.5, dt = t_c / 50, FP64.  The spheres start as a loose,
+non-overlapping seeded packing (a rock-salt lattice) and the configured run
+follows: the chute gravity (|g| = 1) from the first step, 50,000 steps,
+timed chunk by chunk (run_average, and steady_window over the run's last
+chunks, rebuilds included).  Then, after W warm-up steps, K steps are timed:
+`value`.
 
 Prints one JSON line: particle-steps/s (flowing spheres x steps / s), the
 roofline of the dominant kernel (the force kernel, which also integrates the
-step; HIP-event timed on the engine's stream) and a CPU baseline (the oracle, one thread, on a bounded
-sample of the same state).
+step; HIP-event timed on the engine's stream) with the whole-step byte model
+264 + 64 C per particle, and a CPU baseline (the oracle's OpenMP build on the
+host's cores, on a bounded sample of the same state).
 """
 import argparse
 import json
@@ -32,29 +35,34 @@ TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_force_chute2M_traffic.md")
 
 
 def measured_traffic():
-    """traffic_bytes_per_launch from the committed counter summary, or None."""
+    """(traffic_bytes_per_launch, contacts_per_particle of the profiled state)
+    from the committed counter summary, or (None, None)."""
+    traffic, cbar = None, None
     try:
         for line in open(TRAFFIC_PROFILE):
             if line.startswith("traffic_bytes_per_launch:"):
-                return int(line.split(":", 1)[1])
+                traffic = int(line.split(":", 1)[1])
+            if line.startswith("contacts_per_particle:"):
+                cbar = float(line.split(":", 1)[1])
     except OSError:
         pass
-    return None
+    return traffic, cbar
 
 
 def parse():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=500)
-    ap.add_argument("--warmup", type=int, default=1000)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--n", type=int, default=2_000_000, help="flowing spheres")
-    ap.add_argument("--settle-steps", type=int, default=8000, help="untimed settling under strong gravity")
-    ap.add_argument("--settle-gravity", type=float, default=100.0)
-    ap.add_argument("--damp-steps", type=int, default=3000, help="untimed settling with background drag")
-    ap.add_argument("--damp-drag", type=float, default=15.0)
-    ap.add_argument("--save-state", default=None, help="write the settled state (npz) before timing")
-    ap.add_argument("--load-state", default=None, help="start from a state written by --save-state")
-    ap.add_argument("--cpu-steps", type=int, default=3, help="oracle steps in the CPU baseline (0: skip)")
+    ap.add_argument("--run-steps", type=int, default=50000,
+                    help="the configured run before the timed window: steps from the loose packing under the "
+                         "chute gravity, timed chunk by chunk (run_average, steady_window)")
+    ap.add_argument("--chunk", type=int, default=500, help="steps per timed chunk of the configured run")
+    ap.add_argument("--steady-steps", type=int, default=2000,
+                    help="the last chunks of the configured run reported as steady_window")
+    ap.add_argument("--cpu-steps", type=int, default=10,
+                    help="steps of the OpenMP oracle in the CPU baseline (0: skip)")
     ap.add_argument("--skin", type=float, default=0.0)
     ap.add_argument("--seed", type=int, default=3)
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64",
@@ -122,6 +130,38 @@ def run_ensemble(a):
         "speedup_vs_sequential": t_seq / t_packed}))
 
 
+def b_alg(cbar, mixed=False):
+    """Algorithmic HBM bytes per particle-step of the step (plain FP64,
+    sliding-spring laws): state read x, v, omega (72) + radius, inverse mass
+    (16) + id, species, flags (8); state written x, v, omega (72); force and
+    torque written by the force phase and read by the closing half-kick (96;
+    48 in mixed precision, where they are FP32); per active contact (pairs
+    once, plus wall contacts) an 8-byte key and a 24-byte spring, read and
+    written (64)."""
+    return (216.0 if mixed else 264.0) + 64.0 * cbar
+
+
+def fused_bytes(cbar, lbar):
+    """The force kernel's own model (it also integrates): own state in
+    (posr, vel, omg, quat, disp: 160; mask, list length, species, wall mask:
+    20) and out (posr, vel, omg, quat, disp: 160; masks: 12), 5 B per list
+    entry (partner index, reverse slot), 64 B per pair in contact."""
+    return 352.0 + 5.0 * lbar + 64.0 * cbar
+
+
+def cpu_threads():
+    """Host threads for the CPU baseline: the cores this process may use,
+    capped by OMP_NUM_THREADS when set."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit() and int(env) > 0:
+        n = min(n, int(env))
+    return max(1, n)
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -148,53 +188,67 @@ def main():
     if a.ensemble > 0:
         return run_ensemble(a)
 
+    def log(msg):
+        print("# " + msg, file=sys.stderr, flush=True)
+
     t_gen = time.time()
     sc = S.chute_hvm(n_flow=a.n, seed=a.seed)
     n_total = sc.state.n
+    n_flow = sc.info["n_flowing"]
     prec = dem.PRECISION_MIXED if a.precision == "mixed" else dem.PRECISION_FP64
     eng = dem.Engine(device=0, skin=a.skin, precision=prec)
-    chute_g = sc.gravity
     S.apply(eng, sc)
     eng.compute_forces()
-    print(f"# generated {n_total} spheres ({sc.info['n_flowing']} flowing, {sc.info['n_base']} base, "
-          f"footprint {sc.info['footprint']:.1f}) in {time.time() - t_gen:.1f}s", file=sys.stderr, flush=True)
+    log(f"generated {n_total} spheres ({n_flow} flowing, {sc.info['n_base']} base, footprint "
+        f"{sc.info['footprint']:.1f}) in {time.time() - t_gen:.1f}s")
 
-    if a.load_state:
-        z = np.load(a.load_state)
-        s2 = S.Scenario(sc.name, sc.species, sc.walls, sc.periodic, chute_g, sc.dt,
-                        dem.State(id=sc.state.id, species=sc.state.species, radius=sc.state.radius,
-                                  inv_mass=sc.state.inv_mass, inv_inertia=sc.state.inv_inertia,
-                                  x=z["x"], v=z["v"], w=z["w"], q=z["q"], f=z["f"], t=z["t"]))
-        eng = dem.Engine(device=0, skin=a.skin, precision=prec)
-        S.apply(eng, s2, dem.Contacts(z["p_id"], z["i_id"], z["spring"], z["ts"]))
-        eng.set_time(float(z["time"]), int(z["nt"]))
-        a.settle_steps = 0
-    # settle into a dense bed (untimed): strong gravity straight down, then
-    # background drag while gravity is lowered step-wise to |g| = 1, so the
-    # bed is at rest and not over-compressed when the chute gravity starts
-    if a.settle_steps > 0:
-        t0 = time.time()
-        stages = [(a.settle_gravity, 0.0, a.settle_steps)]
-        ramp = [a.settle_gravity * (1.0 / a.settle_gravity) ** ((s + 1) / 5.0) for s in range(5)]
-        stages += [(g, a.damp_drag, a.damp_steps // 5) for g in ramp]
-        done = 0
-        for g, drag, n in stages:
-            eng.set_gravity((0.0, 0.0, -g))
-            eng.set_background_drag(drag)
-            eng.step(n)
-            done += n
-            print(f"# settle {done} steps (g {g:.3g}, drag {drag:g}): {eng.contact_count()} contacts, "
-                  f"{time.time() - t0:.1f}s", file=sys.stderr, flush=True)
-        eng.set_gravity(chute_g)
-        eng.set_background_drag(0.0)
+    # the configuration's run: the loose packing under the chute gravity
+    # (|g| = 1, 24 degrees) for run_steps steps, timed chunk by chunk
+    chunks = []
+    done = 0
+    t_run = 0.0
+    rb_prev = 0
+    while done < a.run_steps:
+        k = min(a.chunk, a.run_steps - done)
+        eng.set_profiling(True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        eng.step(k)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        st = eng.stats()
+        done += k
+        t_run += dt
+        chunks.append(dict(steps=k, seconds=dt, contacts=st.n_contacts, rebuilds=st.n_rebuilds - rb_prev,
+                           force_ms=st.t_force_ms, kick_ms=st.t_kick_ms, rebuild_ms=st.t_rebuild_ms))
+        rb_prev = st.n_rebuilds
+        if done % (10 * a.chunk) == 0 or done == a.run_steps:
+            log(f"run {done}/{a.run_steps} steps, {t_run:.1f}s: {dt / k * 1e3:.3f} ms/step, "
+                f"C/N {st.n_contacts / n_total:.3f}, {st.n_rebuilds} rebuilds")
+    run_avg = None
+    if a.run_steps > 0:
+        run_avg = {"value": n_flow * a.run_steps / t_run, "unit": "particle-steps/s", "steps": a.run_steps,
+                   "seconds": round(t_run, 3), "ms_per_step": round(1e3 * t_run / a.run_steps, 4),
+                   "rebuilds": int(sum(c["rebuilds"] for c in chunks)),
+                   "note": "the whole configured run from the loose packing, rebuilds included"}
+    # the late window: the last chunks of the run covering >= steady_steps
+    steady = None
+    if chunks:
+        win, s_win = [], 0
+        for c in reversed(chunks):
+            win.append(c)
+            s_win += c["steps"]
+            if s_win >= a.steady_steps:
+                break
+        secs = sum(c["seconds"] for c in win)
+        reb_ms = sum(c["rebuild_ms"] for c in win)
+        steady = {"steps": s_win, "from_step": a.run_steps - s_win, "ms_per_step": round(1e3 * secs / s_win, 4),
+                  "value": n_flow * s_win / secs, "rebuilds": int(sum(c["rebuilds"] for c in win)),
+                  "rebuild_share": round(reb_ms / max(1e3 * secs, 1e-12), 4),
+                  "force_ms_per_step": round(sum(c["force_ms"] for c in win) / s_win, 4),
+                  "contacts_per_particle": round(float(np.mean([c["contacts"] for c in win])) / n_total, 4)}
 
-    if a.save_state:
-        d = eng.download()
-        ct = eng.contacts()
-        st0 = eng.stats()
-        np.savez(a.save_state, x=d["x"], v=d["v"], w=d["w"], q=d["q"], f=d["f"], t=d["t"], p_id=ct.p_id,
-                 i_id=ct.i_id, spring=ct.spring, ts=ct.ts, time=st0.time, nt=st0.n_steps)
-        print(f"# saved state to {a.save_state}", file=sys.stderr, flush=True)
+    # the driver's window: warm-up, then K steps timed at the run's end
     eng.step(a.warmup)
     c0 = eng.contact_count()
     rb0 = eng.stats().n_rebuilds
@@ -206,26 +260,20 @@ def main():
     t1 = time.perf_counter()
     st = eng.stats()
     c1 = st.n_contacts
-    rebuilds_window = st.n_rebuilds - rb0
     elapsed = t1 - t0
-    n_flow = sc.info["n_flowing"]
     value = n_flow * a.steps / elapsed
-    # contacts per particle over the window (pairs counted once + wall contacts)
+    # contacts per particle over the window (pairs once + wall contacts)
     cbar = 0.5 * (c0 + c1) / n_total
-    lbar = st.n_list_entries / n_total  # Verlet-list entries per particle
+    lbar = st.n_list_entries / n_total
     force_ms = st.t_force_ms / max(st.n_timed_steps, 1)
-    # algorithmic HBM bytes per particle of the force kernel (which also
-    # integrates): own state in (posr, vel, omg, quat, disp: 160; mask, list
-    # length, species, wall mask: 20) and out (posr, vel, omg, quat, disp:
-    # 160; masks: 12), 5 B per list entry (partner index, reverse slot), and
-    # 64 B per pair in contact (history record in and out)
-    b_force = 352.0 + 5.0 * lbar + 64.0 * cbar
-    achieved = b_force * n_total / (force_ms * 1e-3) / 1e9
-    step_gbs = b_force * n_total * a.steps / elapsed / 1e9
-    print(f"# timed {a.steps} steps in {elapsed:.3f}s: force+integration {force_ms:.3f} ms, kick "
-          f"{st.t_kick_ms / max(st.n_timed_steps, 1):.3f} ms, rebuild {st.t_rebuild_ms / max(st.n_timed_steps, 1):.3f} ms "
-          f"per step; {st.n_rebuilds} rebuilds total; C/N {cbar:.3f}; max list {st.max_list_length}",
-          file=sys.stderr, flush=True)
+    mixed_run = a.precision == "mixed"
+    balg = b_alg(cbar, mixed_run)
+    achieved = balg * n_total / (force_ms * 1e-3) / 1e9
+    bfused = fused_bytes(cbar, lbar)
+    log(f"timed {a.steps} steps in {elapsed:.3f}s: force+integration {force_ms:.3f} ms, kick "
+        f"{st.t_kick_ms / max(st.n_timed_steps, 1):.3f} ms, rebuild "
+        f"{st.t_rebuild_ms / max(st.n_timed_steps, 1):.3f} ms per step; {st.n_rebuilds - rb0} rebuilds in the "
+        f"window; C/N {cbar:.3f}; max list {st.max_list_length}")
 
     # the mixed-precision variant on the same state: a second engine, started
     # from the FP64 run's particles and stored contacts (not the metric)
@@ -233,7 +281,7 @@ def main():
     if a.mixed_steps > 0 and a.precision == "fp64":
         d = eng.download()
         ct = eng.contacts()
-        s3 = S.Scenario(sc.name, sc.species, sc.walls, sc.periodic, chute_g, sc.dt,
+        s3 = S.Scenario(sc.name, sc.species, sc.walls, sc.periodic, sc.gravity, sc.dt,
                         dem.State(id=sc.state.id, species=sc.state.species, radius=sc.state.radius,
                                   inv_mass=sc.state.inv_mass, inv_inertia=sc.state.inv_inertia,
                                   x=d["x"], v=d["v"], w=d["w"], q=d["q"], f=d["f"], t=d["t"]))
@@ -249,14 +297,15 @@ def main():
         torch.cuda.synchronize()
         tm1 = time.perf_counter()
         sm = em.stats()
+        mk = sm.t_force_ms / max(sm.n_timed_steps, 1)
         mixed = {"value": n_flow * a.mixed_steps / (tm1 - tm0), "unit": "particle-steps/s",
                  "ms_per_step": 1e3 * (tm1 - tm0) / a.mixed_steps, "steps": a.mixed_steps,
-                 "kernel_ms": round(sm.t_force_ms / max(sm.n_timed_steps, 1), 4),
+                 "kernel_ms": round(mk, 4),
+                 "roofline_frac": round(b_alg(cbar, True) * n_total / (mk * 1e-3) / 1e9 / HBM_PEAK_GBS, 5),
                  "dtype": "f32 contact geometry and forces, f64 state and springs",
                  "speedup_vs_fp64": round((n_flow * a.mixed_steps / (tm1 - tm0)) / value, 4)}
         em.close()
-        print(f"# mixed precision: {mixed['ms_per_step']:.3f} ms per step, force {mixed['kernel_ms']:.3f} ms",
-              file=sys.stderr, flush=True)
+        log(f"mixed precision: {mixed['ms_per_step']:.3f} ms per step, force {mixed['kernel_ms']:.3f} ms")
 
     cpu = None
     if a.cpu_steps > 0:
@@ -264,21 +313,27 @@ def main():
         import oracle_lib
         d = eng.download()
         ct = eng.contacts()
-        s2 = S.Scenario(sc.name, sc.species, sc.walls, sc.periodic, chute_g, sc.dt,
+        s2 = S.Scenario(sc.name, sc.species, sc.walls, sc.periodic, sc.gravity, sc.dt,
                         dem.State(id=sc.state.id, species=sc.state.species, radius=sc.state.radius,
                                   inv_mass=sc.state.inv_mass, inv_inertia=sc.state.inv_inertia,
                                   x=d["x"], v=d["v"], w=d["w"], q=d["q"], f=d["f"], t=d["t"]))
-        o = oracle_lib.engine()
+        threads = cpu_threads()
+        o, got = oracle_lib.engine_omp(threads)
         S.apply(o, s2, ct)
-        o.set_time(0.0, int(st.n_steps))
+        o.set_time(st.time, int(st.n_steps))
+        o.step(1)  # first touch of the oracle's pools, untimed
         tc0 = time.perf_counter()
         o.step(a.cpu_steps)
         tc1 = time.perf_counter()
-        cpu = {"value": n_flow * a.cpu_steps / (tc1 - tc0), "unit": "particle-steps/s", "cores": 1,
-               "kind": "port",
-               "sample": f"oracle/dem_oracle.c (serial restatement of the reference step), {a.cpu_steps} steps "
-                         f"of the same {n_total}-sphere chute state the GPU timed, on 1 host core"}
+        cpu = {"value": n_flow * a.cpu_steps / (tc1 - tc0), "unit": "particle-steps/s", "cores": int(got),
+               "kind": "port", "seconds": round(tc1 - tc0, 2),
+               "sample": f"oracle/dem_oracle.c built with OpenMP (the restated reference step: pair, wall and "
+                         f"integration loops on {got} threads, as the reference's OpenMP build runs them), "
+                         f"{a.cpu_steps} steps of the same {n_total}-sphere chute state the GPU timed"}
+        o.close()
+        log(f"cpu baseline: {cpu['value']:.3g} particle-steps/s on {got} threads ({cpu['seconds']} s)")
 
+    traffic, traffic_cbar = measured_traffic()
     out = {
         "metric": "particle-steps/sec at 2M polydisperse spheres",
         "value": value,
@@ -291,20 +346,32 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64" if a.precision == "fp64" else "f32 contacts / f64 state",
-        "data": "synthetic (seeded rock-salt lattice packing settled under gravity)",
+        "data": "synthetic (seeded loose rock-salt lattice packing, then the configured run under the chute "
+                "gravity |g| = 1)",
         "config": {"workload": "polydisperse periodic chute, HertzianViscoelasticMindlin e=0.8 mu=0.5, 24 deg",
                    "flowing_spheres": n_flow, "base_spheres": sc.info["n_base"],
                    "diameter": "U[0.5,1.5]", "footprint": round(sc.info["footprint"], 3),
-                   "dt": sc.dt, "contacts_per_particle": round(cbar, 4),
+                   "dt": sc.dt, "steps_before_window": a.run_steps + a.warmup,
+                   "contacts_per_particle": round(cbar, 4), "contacts_per_particle_survey": "2-3 (dense bed)",
                    "list_entries_per_particle": round(lbar, 3),
-                   "rebuilds_in_window": int(rebuilds_window), "step_hbm_gbs_algorithmic": round(step_gbs, 2),
+                   "rebuilds_in_window": int(st.n_rebuilds - rb0),
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 5),
-                     "traffic": measured_traffic() if a.n == 2_000_000 else None,
-                     "traffic_source": "profiles/k_force_chute2M_traffic.md (rocprofv3 FETCH_SIZE x2.000 + WRITE_SIZE)",
+                     "traffic": traffic if a.n == 2_000_000 else None,
+                     "traffic_contacts_per_particle": traffic_cbar,
+                     "traffic_source": "profiles/k_force_chute2M_traffic.md (rocprofv3 FETCH_SIZE, WRITE_SIZE, "
+                                       "copy-calibrated)",
                      "kernel": "k_force", "kernel_ms": round(force_ms, 4),
-                     "bytes_per_particle": round(b_force, 2)},
+                     "bytes_per_particle": round(balg, 2),
+                     "bytes_model": "264 + 64 C (whole step, per particle of the launch)",
+                     "particles_per_launch": n_total,
+                     "fused_kernel_model": {"bytes_per_particle": round(bfused, 2),
+                                            "achieved": round(bfused * n_total / (force_ms * 1e-3) / 1e9, 2),
+                                            "frac": round(bfused * n_total / (force_ms * 1e-3) / 1e9 /
+                                                          HBM_PEAK_GBS, 5)}},
+        "run_average": run_avg,
+        "steady_window": steady,
         "cpu_baseline": cpu,
         "mixed_precision": mixed,
     }

@@ -48,7 +48,7 @@
 extern "C" {
 #endif
 
-#define DEM_ABI_VERSION 6
+#define DEM_ABI_VERSION 7
 
 /* error codes */
 #define DEM_OK 0
@@ -105,9 +105,13 @@ typedef struct dem_pair_params {
 } dem_pair_params;
 
 enum dem_wall_type {
-    DEM_WALL_INFINITE = 0, /* InfiniteWall.cc:151-175 */
-    DEM_WALL_CYLINDER = 1  /* CylindricalWall.cc:72-83 (axis = z through origin) */
+    DEM_WALL_INFINITE = 0,     /* InfiniteWall.cc:151-175 */
+    DEM_WALL_CYLINDER = 1,     /* CylindricalWall.cc:72-83 (axis = z through origin) */
+    DEM_WALL_AXISYMMETRIC = 2  /* AxisymmetricIntersectionOfWalls.cc:93-128 */
 };
+
+/* Sub-walls of an AxisymmetricIntersectionOfWalls (at most this many). */
+#define DEM_AXI_MAX_SUB 4
 
 /*
  * One wall (WallHandler object).  For an InfiniteWall the host passes the
@@ -116,16 +120,30 @@ enum dem_wall_type {
  * q13-q02, with q12=2q1q2 etc.) and the normal Quaternion::getAxis
  * (Quaternion.cc:501-512) of the wall orientation, so the engine evaluates the
  * distance and normal with the reference's own rounding.
+ *
+ * An AxisymmetricIntersectionOfWalls (AxisymmetricIntersectionOfWalls.cc:
+ * 93-128) is the body of revolution of an IntersectionOfWalls drawn in the
+ * (radial, 0, axial) plane: position is its origin (getPosition), axis its
+ * symmetry axis (getOrientation().getAxis()), and n_sub sub-walls (the
+ * IntersectionOfWalls::addObject(normal, point) calls, in order,
+ * IntersectionOfWalls.cc:138-148) are given like InfiniteWalls in that plane:
+ * sub_position, sub_coef and sub_axis as position, dist_coef and axis above.
+ * The engine derives the intersection lines and corners of the sub-walls
+ * itself (IntersectionOfWalls::setPointsAndLines, IntersectionOfWalls.cc:
+ * 351-443).
  */
 typedef struct dem_wall {
     int32_t type;          /* enum dem_wall_type */
     int32_t species;       /* index into the species table */
     uint32_t id;           /* BaseObject id, reported in contact keys */
-    int32_t reserved;
-    double position[3];    /* InfiniteWall p0 */
+    int32_t n_sub;         /* AXISYMMETRIC: sub-walls (1..DEM_AXI_MAX_SUB); else 0 */
+    double position[3];    /* InfiniteWall p0; AXISYMMETRIC: origin */
     double dist_coef[3];   /* InfiniteWall distance coefficients */
-    double axis[3];        /* InfiniteWall normal (getAxis) */
+    double axis[3];        /* InfiniteWall normal (getAxis); AXISYMMETRIC: symmetry axis */
     double radius;         /* CylindricalWall radius */
+    double sub_position[DEM_AXI_MAX_SUB][3];
+    double sub_coef[DEM_AXI_MAX_SUB][3];
+    double sub_axis[DEM_AXI_MAX_SUB][3];
 } dem_wall;
 
 /*
@@ -196,7 +214,12 @@ int dem_set_species(dem_ctx* ctx, int32_t n_species, const dem_pair_params* tabl
 
 /* WallHandler contents (static walls).  Replaces the wall loop of
  * DPMBase::computeAllForces (DPMBase.cc:3434-3439) / computeWallForces
- * (DPMBase.cc:5451-5462). */
+ * (DPMBase.cc:5451-5462).  Before dem_upload_particles it sets the walls;
+ * afterwards it replaces them between steps (WallHandler::removeObject /
+ * removeLastObject during a run, BaseHandler.h:472-505): at most as many
+ * walls as at the upload; a wall that keeps its index and id keeps its
+ * stored interactions, the interactions of every other index are deleted
+ * (BaseInteractable::~BaseInteractable, BaseInteractable.cc:97-106). */
 int dem_set_walls(dem_ctx* ctx, int32_t n_walls, const dem_wall* walls);
 
 /* BoundaryHandler periodic boundaries.  Replaces checkInteractionWithBoundaries

@@ -35,6 +35,7 @@
 #include "dem_laws.h"
 #include "dem_laws_mixed.h"
 #include "dem_vec.h"
+#include "dem_walls.h"
 
 using namespace dem;
 
@@ -79,6 +80,7 @@ struct Rec {
 
 struct Params {
     int n, maxn, nwalls, ns, levels, rotation, nper;
+    int wstride; // particle-wall slots per particle (the wall count at upload, at least 1)
     int erase;
     int paxis[3];
     int axis_pb[3]; // periodic boundary index acting on each axis, -1 if none
@@ -149,7 +151,7 @@ struct Dev {
     int ocap;
     // tables
     const dem_pair_params* sp;
-    const dem_wall* walls;
+    const WallDev* walls;
     int* flags;
 };
 
@@ -469,7 +471,7 @@ __global__ void __launch_bounds__(kBlock) k_rb_begin(Dev<NS> d, Params p, int cu
             const int w = __ffs(wm) - 1;
             wm &= wm - 1;
             const uint64_t key = ((uint64_t)idi << 32) | (DEM_WALL_KEY | (uint32_t)w);
-            hash_insert(d, key, d.wrec[cur][i * p.nwalls + w]);
+            hash_insert(d, key, d.wrec[cur][(size_t)i * p.wstride + w]);
         }
     }
     // orphans still alive at this step (time stamp >= alive_min) go back in
@@ -759,7 +761,7 @@ __global__ void __launch_bounds__(kBlock) k_rematch(Dev<NS> d, Params p, int cur
         for (int w = 0; w < p.nwalls; ++w) {
             Rec<NS> r;
             if (hash_find(d, ((uint64_t)idi << 32) | (DEM_WALL_KEY | (uint32_t)w), r)) {
-                d.wrec[cur][(size_t)i * p.nwalls + w] = r;
+                d.wrec[cur][(size_t)i * p.wstride + w] = r;
                 wm |= 1u << w;
             }
         }
@@ -796,7 +798,8 @@ __global__ void __launch_bounds__(kBlock) k_orphans(Dev<NS> d, Params p)
     {
         const unsigned long long key = d.hkey[h];
         if (key == kEmptyKey) continue;
-        if (!d.hmatched[h]) {
+        const bool gone_wall = (key & DEM_WALL_KEY) && (int)(key & ~(unsigned long long)DEM_WALL_KEY & 0xFFFFFFFFull) >= p.nwalls;
+        if (!d.hmatched[h] && !gone_wall) {
             const Rec<NS> r = d.hval[h];
             if (!p.erase || r.ts > p.nt) {
                 const int o = atomicAdd(&d.flags[F_ORPH_COUNT0 + par], 1);
@@ -825,32 +828,6 @@ __global__ void __launch_bounds__(kBlock) k_orphans(Dev<NS> d, Params p)
             d.flags[F_ORPH_PARITY] = par;
         }
     }
-}
-
-// BaseWall::getDistanceNormalOverlap for the two wall types: InfiniteWall
-// (distance from Quaternion::getDistance coefficients, InfiniteWall.cc:
-// 151-175; normal Quaternion::getAxis) and CylindricalWall about the z axis
-// (CylindricalWall.cc:72-83).  n is the wall's normal.
-__device__ inline bool wall_contact(const dem_wall& wl, V3 x, double r, double interaction_distance, double& distance,
-                                    V3& n)
-{
-    if (wl.type == DEM_WALL_INFINITE) {
-        distance = wl.dist_coef[0] * (wl.position[0] - x.x) +
-                   (wl.dist_coef[1] * (wl.position[1] - x.y) + wl.dist_coef[2] * (wl.position[2] - x.z));
-        if (distance < r + interaction_distance) {
-            n = mk(wl.axis[0], wl.axis[1], wl.axis[2]);
-            return true;
-        }
-        return false;
-    }
-    const double R = sqrt(x.x * x.x + x.y * x.y);
-    distance = wl.radius - R;
-    if (distance < r) {
-        const V3 nr = mk(x.x, x.y, 0.0);
-        n = vdiv(nr, sqrt(vlen2(nr)));
-        return true;
-    }
-    return false;
 }
 
 // The interaction geometry of a wall contact (BaseWall::getInteractionWith,
@@ -1255,9 +1232,9 @@ __global__ void __launch_bounds__(kBlock) FORCE_WAVES_ATTR k_force(Dev<NS> d, Pa
     if (p.nwalls > 0) {
         const uint32_t wm_in = d.wmask[cur][i];
         uint32_t wm_out = 0;
-        const size_t wbase = (size_t)i * p.nwalls;
+        const size_t wbase = (size_t)i * p.wstride;
         for (int w = 0; w < p.nwalls; ++w) {
-            const dem_wall& wl = d.walls[w];
+            const WallDev& wl = d.walls[w];
             bool contact = false;
             double distance = 0.0;
             V3 n = zero3();
@@ -1545,7 +1522,7 @@ __global__ void __launch_bounds__(kBlock) k_cg_contacts(Dev<NS> d, Params p, CgP
             const uint32_t wm = d.wmask[cur][i];
             for (int w = 0; w < p.nwalls; ++w) {
                 if (!((wm >> w) & 1u)) continue;
-                const double4 f4 = d.fw[(size_t)i * p.nwalls + w];
+                const double4 f4 = d.fw[(size_t)i * p.wstride + w];
                 if (f4.w != c.stamp) continue;
                 double distance = 0.0;
                 V3 wn = zero3();
@@ -1613,7 +1590,7 @@ __global__ void __launch_bounds__(kBlock) k_check_particles(Dev<NS> d, Params p,
     V3 img[kMaxImages];
     const int m = candidate_images(p, mk(cr.x, cr.y, cr.z), max_distance[c], img);
     if (blockIdx.x == 0 && (int)threadIdx.x < p.nwalls) {
-        const dem_wall& wl = d.walls[threadIdx.x];
+        const WallDev& wl = d.walls[threadIdx.x];
         const double idist = d.sp[sc * p.ns + wl.species].interaction_distance;
         for (int k = 0; k < m; ++k) {
             double distance;
@@ -1719,7 +1696,7 @@ __global__ void __launch_bounds__(kBlock) k_collect_contacts(Dev<NS> d, Params p
             wm &= wm - 1;
             const int o = atomicAdd(counter, 1);
             if (o >= cap) continue;
-            const size_t s = (size_t)i * p.nwalls + w;
+            const size_t s = (size_t)i * p.wstride + w;
             pid[o] = idi;
             iid[o] = DEM_WALL_KEY | (uint32_t)w;
             for (int a = 0; a < 9; ++a) spring[9 * (size_t)o + a] = a < NS ? d.wrec[cur][s].s[a] : 0.0;
@@ -1784,7 +1761,7 @@ __global__ void __launch_bounds__(kBlock) k_collect_interactions(Dev<NS> d, Para
             wm &= wm - 1;
             const int o = atomicAdd(counter, 1);
             if (o >= cap) continue;
-            const size_t s = (size_t)i * p.nwalls + w;
+            const size_t s = (size_t)i * p.wstride + w;
             double distance = 0.0;
             V3 wn = zero3();
             (void)wall_contact(d.walls[w], xp, pr.w, d.sp[si * p.ns + d.walls[w].species].interaction_distance, distance,
@@ -1883,6 +1860,17 @@ __global__ void k_zero_masks(Dev<NS> d, Params p)
     d.disp[i] = make_double4(0.0, 0.0, 0.0, 0.0);
 }
 
+// WallHandler::removeObject during a run: the interactions with walls whose
+// index is not in `keep` are deleted.
+template <int NS>
+__global__ void k_keep_walls(Dev<NS> d, Params p, uint32_t keep)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= p.n) return;
+    d.wmask[0][i] &= keep;
+    d.wmask[1][i] &= keep;
+}
+
 inline unsigned grid_for(uint64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 
 uint32_t next_pow2(uint64_t x)
@@ -1929,7 +1917,8 @@ struct dem_ctx {
     Dev<3> d3{};
     Dev<9> d9{};
     dem_pair_params* d_sp = nullptr;
-    dem_wall* d_walls = nullptr;
+    WallDev* d_walls = nullptr;
+    int wall_slots = 1;        // particle-wall slots per particle of the current upload
     int* d_flags = nullptr;
     uint32_t nbuckets = 0;
     int64_t last_rebuilds = 0;
@@ -1989,6 +1978,22 @@ void free_all(dem_ctx* ctx)
     ctx->uploaded = false;
 }
 
+// The walls with what setPointsAndLines derives, copied into d_walls (which
+// holds ctx->wall_slots entries).
+int copy_walls(dem_ctx* ctx)
+{
+    std::vector<WallDev> wd(std::max<size_t>(ctx->walls.size(), 1));
+    for (size_t w = 0; w < ctx->walls.size(); ++w) {
+        WallDev& x = wd[w];
+        std::memset(&x, 0, sizeof x);
+        static_cast<dem_wall&>(x) = ctx->walls[w];
+        if (x.type == DEM_WALL_AXISYMMETRIC) derive_wall(x);
+    }
+    if (!ctx->walls.empty())
+        HIPCHK(hipMemcpy(ctx->d_walls, wd.data(), sizeof(WallDev) * ctx->walls.size(), hipMemcpyHostToDevice));
+    return DEM_OK;
+}
+
 template <int NS>
 Dev<NS>& dev(dem_ctx* c);
 template <>
@@ -2013,7 +2018,7 @@ int alloc_particles(dem_ctx* ctx, int64_t n)
     const size_t wslots_packed = ((size_t)n + 63) / 64 * 64 * ctx->maxn_cfg;
     A(d.nbr, slots) A(d.ncount, n) A(d.rev, slots) A(d.nbrc, wslots_packed) A(d.revc, wslots_packed);
     A(d.rec[0], slots) A(d.rec[1], slots) A(d.mask[0], n) A(d.mask[1], n) A(d.fc, slots) A(d.ftc, slots);
-    const size_t wslots = (size_t)n * std::max<size_t>(ctx->walls.size(), 1);
+    const size_t wslots = (size_t)n * ctx->wall_slots;
     A(d.wrec[0], wslots) A(d.wrec[1], wslots) A(d.wmask[0], n) A(d.wmask[1], n) A(d.fw, wslots) A(d.ftw, wslots);
     A(d.key, n) A(d.perm, n);
     A(d.count, ctx->nbuckets) A(d.cursor, ctx->nbuckets) A(d.start, (size_t)ctx->nbuckets + 1);
@@ -2290,11 +2295,12 @@ int upload_empty(dem_ctx* ctx)
     int r;
     if ((r = dalloc(ctx, &ctx->d_flags, F_NUM)) != 0) return r;
     if ((r = dalloc(ctx, &ctx->d_sp, ctx->species.size())) != 0) return r;
-    if ((r = dalloc(ctx, &ctx->d_walls, std::max<size_t>(ctx->walls.size(), 1))) != 0) return r;
+    ctx->wall_slots = (int)std::max<size_t>(ctx->walls.size(), 1);
+    p.wstride = ctx->wall_slots;
+    if ((r = dalloc(ctx, &ctx->d_walls, (size_t)ctx->wall_slots)) != 0) return r;
     HIPCHK(hipMemcpy(ctx->d_sp, ctx->species.data(), sizeof(dem_pair_params) * ctx->species.size(),
                      hipMemcpyHostToDevice));
-    if (!ctx->walls.empty())
-        HIPCHK(hipMemcpy(ctx->d_walls, ctx->walls.data(), sizeof(dem_wall) * ctx->walls.size(), hipMemcpyHostToDevice));
+    if ((r = copy_walls(ctx)) != 0) return r;
     HIPCHK(hipMemset(ctx->d_flags, 0, sizeof(int) * F_NUM));
     ctx->d3 = Dev<3>{};
     ctx->d9 = Dev<9>{};
@@ -2491,11 +2497,53 @@ int dem_set_species(dem_ctx* ctx, int32_t n, const dem_pair_params* table)
 int dem_set_walls(dem_ctx* ctx, int32_t n, const dem_wall* walls)
 {
     if (!ctx || n < 0 || n > kMaxWalls || (n > 0 && !walls)) return set_err(ctx, DEM_ERR_ARG, "set_walls: bad arguments (at most 32 walls)");
-    if (ctx->uploaded) return set_err(ctx, DEM_ERR_STATE, "set_walls must precede upload_particles");
-    for (int k = 0; k < n; ++k)
-        if (walls[k].type != DEM_WALL_INFINITE && walls[k].type != DEM_WALL_CYLINDER)
+    for (int k = 0; k < n; ++k) {
+        const dem_wall& w = walls[k];
+        if (w.type != DEM_WALL_INFINITE && w.type != DEM_WALL_CYLINDER && w.type != DEM_WALL_AXISYMMETRIC)
             return set_err(ctx, DEM_ERR_ARG, "unknown wall type");
+        if (w.type == DEM_WALL_AXISYMMETRIC) {
+            if (w.n_sub < 1 || w.n_sub > DEM_AXI_MAX_SUB)
+                return set_err(ctx, DEM_ERR_ARG, "axisymmetric wall: 1 to 4 sub-walls");
+            bool ok = std::isfinite(w.axis[0]) && std::isfinite(w.axis[1]) && std::isfinite(w.axis[2]) &&
+                      (w.axis[0] != 0.0 || w.axis[1] != 0.0 || w.axis[2] != 0.0);
+            for (int a = 0; a < 3; ++a) ok = ok && std::isfinite(w.position[a]);
+            for (int j = 0; j < w.n_sub; ++j)
+                for (int a = 0; a < 3; ++a)
+                    ok = ok && std::isfinite(w.sub_position[j][a]) && std::isfinite(w.sub_coef[j][a]) &&
+                         std::isfinite(w.sub_axis[j][a]);
+            if (!ok) return set_err(ctx, DEM_ERR_ARG, "axisymmetric wall: values must be finite, the axis non-zero");
+        }
+        if (ctx->uploaded && (w.species < 0 || w.species >= ctx->ns))
+            return set_err(ctx, DEM_ERR_ARG, "wall species out of range");
+    }
+    if (!ctx->uploaded) {
+        ctx->walls.assign(walls, walls + n);
+        return DEM_OK;
+    }
+    // during a run: replace the walls between steps; a wall that keeps its
+    // index and id keeps its interactions
+    if (ctx->in_step) return set_err(ctx, DEM_ERR_STATE, "set_walls inside dem_step_begin/dem_step_end");
+    if (n > ctx->wall_slots)
+        return set_err(ctx, DEM_ERR_CAPACITY, "set_walls: more walls than at the upload of the particles");
+    uint32_t keep = 0;
+    for (int w = 0; w < n && w < (int)ctx->walls.size(); ++w)
+        if (ctx->walls[w].id == walls[w].id && ctx->walls[w].type == walls[w].type) keep |= 1u << w;
     ctx->walls.assign(walls, walls + n);
+    ctx->prm.nwalls = n;
+    HIPCHK(hipSetDevice(ctx->device));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    int r = copy_walls(ctx);
+    if (r) return r;
+    if (ctx->n > 0) {
+        if (ctx->ns_springs == 9)
+            hipLaunchKernelGGL(k_keep_walls<9>, dim3(grid_for(ctx->n)), dim3(kBlock), 0, ctx->stream, ctx->d9, ctx->prm,
+                               keep);
+        else
+            hipLaunchKernelGGL(k_keep_walls<3>, dim3(grid_for(ctx->n)), dim3(kBlock), 0, ctx->stream, ctx->d3, ctx->prm,
+                               keep);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+    }
     return DEM_OK;
 }
 
@@ -2710,11 +2758,12 @@ int dem_upload_particles(dem_ctx* ctx, int64_t n, const uint32_t* id, const int3
     int r;
     if ((r = dalloc(ctx, &ctx->d_flags, F_NUM)) != 0) return r;
     if ((r = dalloc(ctx, &ctx->d_sp, ctx->species.size())) != 0) return r;
-    if ((r = dalloc(ctx, &ctx->d_walls, std::max<size_t>(ctx->walls.size(), 1))) != 0) return r;
+    ctx->wall_slots = (int)std::max<size_t>(ctx->walls.size(), 1);
+    p.wstride = ctx->wall_slots;
+    if ((r = dalloc(ctx, &ctx->d_walls, (size_t)ctx->wall_slots)) != 0) return r;
     HIPCHK(hipMemcpy(ctx->d_sp, ctx->species.data(), sizeof(dem_pair_params) * ctx->species.size(),
                      hipMemcpyHostToDevice));
-    if (!ctx->walls.empty())
-        HIPCHK(hipMemcpy(ctx->d_walls, ctx->walls.data(), sizeof(dem_wall) * ctx->walls.size(), hipMemcpyHostToDevice));
+    if ((r = copy_walls(ctx)) != 0) return r;
     for (const auto& wl : ctx->walls)
         if (wl.species < 0 || wl.species >= ctx->ns) return set_err(ctx, DEM_ERR_ARG, "wall species out of range");
     ctx->n = n;

@@ -31,6 +31,8 @@ FRICTION_FULL = 2
 FRICTION_MINDLIN = 3
 WALL_INFINITE = 0
 WALL_CYLINDER = 1
+WALL_AXISYMMETRIC = 2
+AXI_MAX_SUB = 4
 
 ERRORS = {-1: "DEM_ERR_ARG", -2: "DEM_ERR_STATE", -3: "DEM_ERR_DEVICE", -4: "DEM_ERR_CAPACITY",
           -5: "DEM_ERR_UNSUPPORTED"}
@@ -57,9 +59,12 @@ class PairParams(C.Structure):
 
 
 class Wall(C.Structure):
+    """dem_wall (include/mercurydpm_dem.h)."""
     _fields_ = [("type", C.c_int32), ("species", C.c_int32), ("id", C.c_uint32),
-                ("reserved", C.c_int32), ("position", C.c_double * 3),
-                ("dist_coef", C.c_double * 3), ("axis", C.c_double * 3), ("radius", C.c_double)]
+                ("n_sub", C.c_int32), ("position", C.c_double * 3),
+                ("dist_coef", C.c_double * 3), ("axis", C.c_double * 3), ("radius", C.c_double),
+                ("sub_position", (C.c_double * 3) * 4), ("sub_coef", (C.c_double * 3) * 4),
+                ("sub_axis", (C.c_double * 3) * 4)]
 
 
 class Periodic(C.Structure):

@@ -365,6 +365,53 @@ dem_wall InfiniteWall::toEngine() const
     return w;
 }
 
+void AxisymmetricIntersectionOfWalls::setAxis(const Vec3D& a)
+{
+    InfiniteWall w;
+    w.set(a, Vec3D(0, 0, 0));
+    for (int k = 0; k < 4; ++k) q_[k] = w.getOrientation()[k];
+}
+
+void AxisymmetricIntersectionOfWalls::addObject(Vec3D normal, const Vec3D& point)
+{
+    if (sub_.size() >= DEM_AXI_MAX_SUB)
+        throw std::invalid_argument("AxisymmetricIntersectionOfWalls: at most 4 sub-walls");
+    // Vec3D::normalise (Vector.cc:124-132)
+    const double l = std::sqrt(normal.getLengthSquared());
+    normal = Vec3D(normal.X / l, normal.Y / l, normal.Z / l);
+    InfiniteWall w;
+    w.set(normal, point);
+    sub_.push_back(w);
+}
+
+dem_wall AxisymmetricIntersectionOfWalls::toEngine() const
+{
+    dem_wall w{};
+    w.type = DEM_WALL_AXISYMMETRIC;
+    w.species = species_ ? (int32_t)species_->getIndex() : 0;
+    w.id = id_;
+    w.n_sub = (int32_t)sub_.size();
+    w.position[0] = position_.X;
+    w.position[1] = position_.Y;
+    w.position[2] = position_.Z;
+    // Quaternion::getAxis of the orientation (Quaternion.cc:501-512)
+    const double q0 = q_[0], q1 = q_[1], q2 = q_[2], q3 = q_[3];
+    const double q00 = q0 * q0, q11 = q1 * q1, q22 = q2 * q2, q33 = q3 * q3;
+    const double a12 = q1 * q2, a30 = q3 * q0, a13 = q1 * q3, a20 = q2 * q0;
+    w.axis[0] = q00 - q33 + q11 - q22;
+    w.axis[1] = 2.0 * (a12 + a30);
+    w.axis[2] = 2.0 * (a13 - a20);
+    for (size_t k = 0; k < sub_.size(); ++k) {
+        const dem_wall s = sub_[k].toEngine();
+        for (int a = 0; a < 3; ++a) {
+            w.sub_position[k][a] = s.position[a];
+            w.sub_coef[k][a] = s.dist_coef[a];
+            w.sub_axis[k][a] = s.axis[a];
+        }
+    }
+    return w;
+}
+
 dem_wall CylindricalWall::toEngine() const
 {
     dem_wall w{};
@@ -478,6 +525,7 @@ void DPMBase::initialiseSolve()
     std::vector<dem_wall> walls;
     for (auto& w : wallHandler) walls.push_back(w->toEngine());
     check(dem_set_walls(ctx_, (int32_t)walls.size(), walls.data()), "dem_set_walls");
+    wallsVersion_ = wallHandler.getVersion();
     std::vector<dem_periodic> per;
     for (auto& b : boundaryHandler)
         if (auto* pb = dynamic_cast<PeriodicBoundary*>(b.get())) per.push_back(pb->toEngine());
@@ -522,8 +570,21 @@ void DPMBase::initialiseSolve()
 // DPMBase::computeOneTimeStep (DPMBase.cc:4315-4389): output files first,
 // actionsBeforeTimeStep between the drift and the force pass,
 // actionsAfterTimeStep before the clock advances.
+// Walls added or removed by a hook since the last step go to the device
+// (a removed wall's interactions are deleted there).
+void DPMBase::syncWalls()
+{
+    if (!ctx_ || wallHandler.getVersion() == wallsVersion_) return;
+    std::vector<dem_wall> walls;
+    for (auto& w : wallHandler) walls.push_back(w->toEngine());
+    check(dem_set_walls(ctx_, (int32_t)walls.size(), walls.data()), "dem_set_walls");
+    wallsVersion_ = wallHandler.getVersion();
+    interactionsSynced_ = false;
+}
+
 void DPMBase::computeOneTimeStep()
 {
+    syncWalls();
     writeOutputFiles();
     check(dem_step_begin(ctx_), "dem_step_begin");
     particlesSynced_ = interactionsSynced_ = false;

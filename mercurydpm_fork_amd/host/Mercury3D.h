@@ -405,6 +405,32 @@ public:
     std::unique_ptr<BaseWall> clone() const override { return std::make_unique<InfiniteWall>(*this); }
 };
 
+// AxisymmetricIntersectionOfWalls (AxisymmetricIntersectionOfWalls.cc): the
+// body of revolution about an axis of an IntersectionOfWalls drawn in the
+// (radial, 0, axial) plane, e.g. the pedestal and the casing of
+// Drivers/Calibration/CalibrationHeap.cpp:86-97.
+class AxisymmetricIntersectionOfWalls : public BaseWall {
+public:
+    // setAxis (AxisymmetricIntersectionOfWalls.cc:148-151): the orientation
+    // via the axis as normal
+    void setAxis(const Vec3D& a);
+    void setPosition(const Vec3D& p) { position_ = p; }
+    // IntersectionOfWalls::addObject(normal, point) (IntersectionOfWalls.cc:
+    // 138-148): the normal is normalised, then an InfiniteWall is set in the
+    // (radial, 0, axial) plane; at most DEM_AXI_MAX_SUB sub-walls
+    void addObject(Vec3D normal, const Vec3D& point);
+    unsigned getNumberOfObjects() const { return (unsigned)sub_.size(); }
+    dem_wall toEngine() const override;
+    std::string getName() const override { return "AxisymmetricIntersectionOfWalls"; }
+    std::unique_ptr<BaseWall> clone() const override
+    {
+        return std::make_unique<AxisymmetricIntersectionOfWalls>(*this);
+    }
+
+private:
+    std::vector<InfiniteWall> sub_;
+};
+
 // CylindricalWall (CylindricalWall.cc): axis = z through the origin
 class CylindricalWall : public BaseWall {
 public:
@@ -569,11 +595,32 @@ public:
         W* raw = p.get();
         raw->id_ = nextId_++;
         objects_.push_back(std::move(p));
+        ++version_;
         return raw;
     }
+    // BaseHandler::removeObject (BaseHandler.h:472-505): the last wall takes
+    // the removed index; during a run the interactions of a removed wall go
+    // with it
+    void removeObject(unsigned index)
+    {
+        if (index >= objects_.size()) throw std::out_of_range("WallHandler::removeObject: no such wall");
+        if (index + 1 != objects_.size()) objects_[index] = std::move(objects_.back());
+        objects_.pop_back();
+        ++version_;
+    }
+    // BaseHandler::removeLastObject (BaseHandler.h:510-520)
+    void removeLastObject()
+    {
+        if (objects_.empty()) throw std::out_of_range("WallHandler::removeLastObject: no walls");
+        objects_.pop_back();
+        ++version_;
+    }
+    // changes with every added or removed wall (the device copy follows it)
+    unsigned getVersion() const { return version_; }
 
 private:
     unsigned nextId_ = 0;
+    unsigned version_ = 0;
 };
 
 class BoundaryHandler : public HandlerBase<BaseBoundary> {
@@ -818,6 +865,8 @@ private:
     double skin_ = 0;
     dem_ctx* ctx_ = nullptr;
     bool particlesSynced_ = true, interactionsSynced_ = true;
+    unsigned wallsVersion_ = 0;  // WallHandler::getVersion of the walls on the device
+    void syncWalls();
     // the base actionsBeforeTimeStep marks itself; once it has run, no host
     // round trip is made for the hook again.  The same for the after hook and
     // continueSolve: when none of the three is overridden, solve() runs the

@@ -15,8 +15,8 @@ from typing import List, Optional
 import numpy as np
 
 from . import dem
-from .species import (PI, Species, cylindrical_wall, infinite_wall, periodic_boundary, species_table,
-                      sphere_inv_mass, sphere_inv_mass_array)
+from .species import (PI, Species, axisymmetric_wall, cylindrical_wall, infinite_wall, periodic_boundary,
+                      species_table, sphere_inv_mass, sphere_inv_mass_array)
 
 
 @dataclass
@@ -634,6 +634,18 @@ def mixture(n=400, seed=7, lx=8.0, ly=8.0, lz=8.0, periodic=(True, True), walls=
         wl.append(infinite_wall((1, 0.3, -0.2), (lx, 0, 0), species=0, wall_id=2))
     if "cylinder" in walls:
         wl.append(cylindrical_wall(0.5 * min(lx, ly) + 0.3, species=0, wall_id=3))
+    if "axisymmetric" in walls:
+        # AxisymmetricIntersectionOfWalls about a vertical axis through the
+        # box centre: a pedestal with a chamfered rim (three sub-walls) and a
+        # casing around it, as the calibration heap uses them
+        c = (0.5 * lx, 0.5 * ly, 0.0)
+        rb = 0.3 * min(lx, ly)
+        s3 = 1.0 / math.sqrt(2.0)
+        wl.append(axisymmetric_wall((0, 0, 1), [((-1, 0, 0), (rb, 0, 0)), ((0, 0, -1), (0, 0, 0.3 * lz)),
+                                                ((-s3, 0, -s3), (rb - 0.4, 0, 0.3 * lz))],
+                                    position=c, species=0, wall_id=4))
+        wl.append(axisymmetric_wall((0, 0, 1), [((1, 0, 0), (0.5 * min(lx, ly) - 0.2, 0, 0))], position=c,
+                                    species=min(1, n_species - 1), wall_id=5))
     per = []
     if periodic[0]:
         per.append(periodic_boundary(0, 0.0, lx))

@@ -264,10 +264,11 @@ def orientation_via_normal(normal):
     return tuple(x / l for x in q)
 
 
-def infinite_wall(normal, point, species: int = 0, wall_id: int = 0) -> "dem.Wall":
-    """InfiniteWall::set(normal, point) (InfiniteWall.cc:118-122); the distance
-    coefficients of Quaternion::getDistance (Quaternion.cc:658-669) and the axis
-    of Quaternion::getAxis (Quaternion.cc:501-512)."""
+def _infinite_wall_fields(normal, point):
+    """InfiniteWall::set(normal, point) (InfiniteWall.cc:118-122): the
+    distance coefficients of Quaternion::getDistance (Quaternion.cc:658-669)
+    and the axis of Quaternion::getAxis (Quaternion.cc:501-512) of the
+    orientation via the normal."""
     q0, q1, q2, q3 = orientation_via_normal(normal)
     q00 = q0 * q0
     q02 = 2 * q0 * q2
@@ -283,15 +284,48 @@ def infinite_wall(normal, point, species: int = 0, wall_id: int = 0) -> "dem.Wal
     a13 = q1 * q3
     a20 = q2 * q0
     axis = (q00 - q33 + q11 - q22, 2.0 * (a12 + a30), 2.0 * (a13 - a20))
+    return tuple(float(c) for c in point), coef, axis
+
+
+def infinite_wall(normal, point, species: int = 0, wall_id: int = 0) -> "dem.Wall":
+    """InfiniteWall::set(normal, point) (InfiniteWall.cc:118-122)."""
+    pos, coef, axis = _infinite_wall_fields(normal, point)
     w = dem.Wall()
     w.type = dem.WALL_INFINITE
     w.species = species
     w.id = wall_id
     for k in range(3):
-        w.position[k] = float(point[k])
+        w.position[k] = pos[k]
         w.dist_coef[k] = coef[k]
         w.axis[k] = axis[k]
     w.radius = 0.0
+    return w
+
+
+def axisymmetric_wall(axis, sub_walls, position=(0.0, 0.0, 0.0), species: int = 0, wall_id: int = 0) -> "dem.Wall":
+    """AxisymmetricIntersectionOfWalls (AxisymmetricIntersectionOfWalls.cc):
+    setAxis(axis) (setOrientationViaNormal; the engine uses getOrientation()
+    .getAxis()), setPosition(position), then IntersectionOfWalls::addObject(
+    normal, point) for each (normal, point) of `sub_walls` in the (radial, 0,
+    axial) plane (IntersectionOfWalls.cc:138-148: the normal is normalised,
+    then InfiniteWall::set)."""
+    if not 1 <= len(sub_walls) <= dem.AXI_MAX_SUB:
+        raise ValueError("an axisymmetric wall has 1 to 4 sub-walls")
+    _, _, ax = _infinite_wall_fields(axis, (0.0, 0.0, 0.0))
+    w = dem.Wall()
+    w.type = dem.WALL_AXISYMMETRIC
+    w.species = species
+    w.id = wall_id
+    w.n_sub = len(sub_walls)
+    for k in range(3):
+        w.position[k] = float(position[k])
+        w.axis[k] = ax[k]
+    for j, (normal, point) in enumerate(sub_walls):
+        pos, coef, a = _infinite_wall_fields(_normalise(normal), point)
+        for k in range(3):
+            w.sub_position[j][k] = pos[k]
+            w.sub_coef[j][k] = coef[k]
+            w.sub_axis[j][k] = a[k]
     return w
 
 

@@ -103,6 +103,7 @@ struct oracle_ctx {
     dem_pair_params* species;
     int nWalls;
     dem_wall* walls;
+    struct AxiGeo* wallGeo; /* per wall: what IntersectionOfWalls::setPointsAndLines derives */
     int nPeriodic;
     dem_periodic periodic[3];
     V3 gravity;
@@ -567,26 +568,203 @@ static void computeInternalForce(oracle_ctx* c, int64_t a, int64_t b)
     }
 }
 
+/* IntersectionOfWalls::setPointsAndLines (IntersectionOfWalls.cc:351-443),
+ * called once per added sub-wall n: for m < n the line where sub-walls m and
+ * n meet, AB_[n(n-1)/2+m] = N_m x N_n normalised and A_ a point on it (the
+ * solution of N_m.x = N_m.p_m, N_n.x = N_n.p_n, AB.x = 0); for l < m < n
+ * the corner C_[(n-2)(n-1)n/6+(m-1)m/2+l] where the three planes meet. */
+struct AxiGeo {
+    V3 A[DEM_AXI_MAX_SUB * (DEM_AXI_MAX_SUB - 1) / 2];
+    V3 AB[DEM_AXI_MAX_SUB * (DEM_AXI_MAX_SUB - 1) / 2];
+    V3 C[DEM_AXI_MAX_SUB * (DEM_AXI_MAX_SUB - 1) * (DEM_AXI_MAX_SUB - 2) / 6];
+};
+
+static V3 subN(const dem_wall* w, int k) { return v3(w->sub_axis[k][0], w->sub_axis[k][1], w->sub_axis[k][2]); }
+static V3 subP(const dem_wall* w, int k)
+{
+    return v3(w->sub_position[k][0], w->sub_position[k][1], w->sub_position[k][2]);
+}
+
+/* InfiniteWall::getDistance of a sub-wall (Quaternion::getDistance) */
+static double subDistance(const dem_wall* w, int k, V3 x)
+{
+    return w->sub_coef[k][0] * (w->sub_position[k][0] - x.X) +
+           (w->sub_coef[k][1] * (w->sub_position[k][1] - x.Y) + w->sub_coef[k][2] * (w->sub_position[k][2] - x.Z));
+}
+
+static void setPointsAndLines(const dem_wall* w, struct AxiGeo* g)
+{
+    for (int n = 0; n < w->n_sub; ++n) {
+        V3 nN = subN(w, n);
+        for (int m = 0; m < n; ++m) {
+            int id = (n - 1) * n / 2 + m;
+            V3 nM = subN(w, m);
+            V3 ab = vcross(nM, nN);
+            ab = vdiv(ab, sqrt(vlen2(ab)));
+            g->AB[id] = ab;
+            double invdet = 1.0 / (+nN.X * (nM.Y * ab.Z - ab.Y * nM.Z) - nN.Y * (nM.X * ab.Z - nM.Z * ab.X) +
+                                   nN.Z * (nM.X * ab.Y - nM.Y * ab.X));
+            double pn = vdot(subP(w, n), nN), pm = vdot(subP(w, m), nM);
+            V3 a = v3(+(nM.Y * ab.Z - ab.Y * nM.Z) * pn - (nN.Y * ab.Z - nN.Z * ab.Y) * pm +
+                          (nN.Y * nM.Z - nN.Z * nM.Y) * 0.0,
+                      -(nM.X * ab.Z - nM.Z * ab.X) * pn + (nN.X * ab.Z - nN.Z * ab.X) * pm -
+                          (nN.X * nM.Z - nM.X * nN.Z) * 0.0,
+                      +(nM.X * ab.Y - ab.X * nM.Y) * pn - (nN.X * ab.Y - ab.X * nN.Y) * pm +
+                          (nN.X * nM.Y - nM.X * nN.Y) * 0.0);
+            g->A[id] = vmul(a, invdet);
+        }
+        for (int m = 0; m < n; ++m)
+            for (int l = 0; l < m; ++l) {
+                int id = (n - 2) * (n - 1) * n / 6 + (m - 1) * m / 2 + l;
+                V3 nM = subN(w, m), nL = subN(w, l);
+                double invdet = 1.0 / (+nN.X * (nM.Y * nL.Z - nL.Y * nM.Z) - nN.Y * (nM.X * nL.Z - nM.Z * nL.X) +
+                                       nN.Z * (nM.X * nL.Y - nM.Y * nL.X));
+                double pn = vdot(subP(w, n), nN), pm = vdot(subP(w, m), nM), pl = vdot(subP(w, l), nL);
+                V3 cc = v3(+(nM.Y * nL.Z - nL.Y * nM.Z) * pn - (nN.Y * nL.Z - nN.Z * nL.Y) * pm +
+                               (nN.Y * nM.Z - nN.Z * nM.Y) * pl,
+                           -(nM.X * nL.Z - nM.Z * nL.X) * pn + (nN.X * nL.Z - nN.Z * nL.X) * pm -
+                               (nN.X * nM.Z - nM.X * nN.Z) * pl,
+                           +(nM.X * nL.Y - nL.X * nM.Y) * pn - (nN.X * nL.Y - nL.X * nN.Y) * pm +
+                               (nN.X * nM.Y - nM.X * nN.Y) * pl);
+                g->C[id] = vmul(cc, invdet);
+            }
+    }
+}
+
+/* IntersectionOfWalls::getDistanceAndNormal(const Vec3D&, Mdouble, ...)
+ * (IntersectionOfWalls.cc:567-700): contact only if the point is within
+ * wallInteractionRadius of every sub-wall; the contact is with the sub-wall
+ * of largest distance, or with the edge / corner where it meets the next
+ * ones. */
+static int intersectionDistanceAndNormal(const dem_wall* w, const struct AxiGeo* g, V3 position, double wir,
+                                         double* distOut, V3* normal)
+{
+    if (w->n_sub <= 0) return 0;
+    double distance = -1e20, distance2 = -1e20, distance3 = -1e20;
+    unsigned id = 0, id2 = 0, id3 = 0;
+    for (unsigned i = 0; i < (unsigned)w->n_sub; i++) {
+        double cur = subDistance(w, (int)i, position);
+        if (cur >= wir) {
+            *distOut = distance;
+            return 0;
+        }
+        if (cur > distance) {
+            if (distance > -wir) {
+                if (distance2 > -wir) {
+                    distance3 = distance2;
+                    id3 = id2;
+                }
+                distance2 = distance;
+                id2 = id;
+            }
+            distance = cur;
+            id = i;
+        } else if (cur < -wir) {
+            continue;
+        } else if (cur > distance2) {
+            if (distance2 > -wir) {
+                distance3 = distance2;
+                id3 = id2;
+            }
+            distance2 = cur;
+            id2 = i;
+        } else if (cur > distance3) {
+            distance3 = cur;
+            id3 = i;
+        }
+    }
+    *distOut = distance;
+    if (distance2 > -wir) {
+        V3 D = vadd(position, vmul(subN(w, (int)id), distance));
+        int withId2 = subDistance(w, (int)id2, D) > 0.0;
+        if (distance3 > -wir && subDistance(w, (int)id3, D) > 0.0) {
+            V3 E = vadd(position, vmul(subN(w, (int)id2), distance2));
+            int withId3 = subDistance(w, (int)id3, E) > 0.0;
+            if (withId2) {
+                if (withId3) {
+                    if (id2 > id3) {
+                        unsigned t = id2;
+                        id2 = id3;
+                        id3 = t;
+                    }
+                    unsigned index = (id < id2) ? ((id3 - 2) * (id3 - 1) * id3 / 6 + (id2 - 1) * id2 / 2 + id)
+                                     : (id < id3) ? ((id3 - 2) * (id3 - 1) * id3 / 6 + (id - 1) * id / 2 + id2)
+                                                  : ((id - 2) * (id - 1) * id / 6 + (id3 - 1) * id3 / 2 + id2);
+                    V3 nr = vsub(position, g->C[index]);
+                    double d = sqrt(vlen2(nr));
+                    *distOut = d;
+                    if (d <= wir) {
+                        *normal = vdiv(nr, -d);
+                        return 1;
+                    }
+                    if (d == d) return 0;
+                }
+            } else {
+                withId2 = 1;
+                distance2 = distance3;
+                id2 = id3;
+            }
+        }
+        if (withId2) {
+            unsigned index = (id > id2) ? ((id - 1) * id / 2 + id2) : ((id2 - 1) * id2 / 2 + id);
+            V3 ac = vsub(position, g->A[index]);
+            V3 nr = vsub(ac, vmul(g->AB[index], vdot(ac, g->AB[index])));
+            double d = sqrt(vlen2(nr));
+            *distOut = d;
+            if (d <= wir) {
+                *normal = vdiv(nr, -d);
+                return 1;
+            }
+            if (d == d) return 0;
+        }
+    }
+    *normal = subN(w, (int)id);
+    return 1;
+}
+
 /* InfiniteWall::getDistanceAndNormal (InfiniteWall.cc:151-175, distance via
  * Quaternion::getDistance, Quaternion.cc:658-669); CylindricalWall::
- * getDistanceAndNormal (CylindricalWall.cc:72-83). */
-static int wallDistanceAndNormal(const dem_wall* w, const Particle* p, double wallRadius, double* dist, V3* n)
+ * getDistanceAndNormal (CylindricalWall.cc:72-83);
+ * AxisymmetricIntersectionOfWalls::getDistanceAndNormal
+ * (AxisymmetricIntersectionOfWalls.cc:93-128): the particle in (radial, 0,
+ * axial) coordinates about the wall's origin and axis, the normal turned
+ * back. */
+static int wallDistanceAndNormalAt(const dem_wall* w, const struct AxiGeo* g, V3 pos, double radius,
+                                   double wallRadius, double* dist, V3* n)
 {
     if (w->type == DEM_WALL_INFINITE) {
-        double d = w->dist_coef[0] * (w->position[0] - p->pos.X) +
-                   (w->dist_coef[1] * (w->position[1] - p->pos.Y) + w->dist_coef[2] * (w->position[2] - p->pos.Z));
+        double d = w->dist_coef[0] * (w->position[0] - pos.X) +
+                   (w->dist_coef[1] * (w->position[1] - pos.Y) + w->dist_coef[2] * (w->position[2] - pos.Z));
         *dist = d;
         if (d >= wallRadius) return 0;
         *n = v3(w->axis[0], w->axis[1], w->axis[2]);
         return 1;
     }
-    double R = sqrt(p->pos.X * p->pos.X + p->pos.Y * p->pos.Y);
-    double d = w->radius - R;
-    *dist = d;
-    if (d >= p->radius) return 0;
-    V3 nr = v3(p->pos.X, p->pos.Y, 0.0);
-    *n = vdiv(nr, vlen(nr));
+    if (w->type == DEM_WALL_CYLINDER) {
+        double R = sqrt(pos.X * pos.X + pos.Y * pos.Y);
+        double d = w->radius - R;
+        *dist = d;
+        if (d >= radius) return 0;
+        V3 nr = v3(pos.X, pos.Y, 0.0);
+        *n = vdiv(nr, vlen(nr));
+        return 1;
+    }
+    V3 pOrigin = vsub(pos, v3(w->position[0], w->position[1], w->position[2]));
+    V3 axis = v3(w->axis[0], w->axis[1], w->axis[2]);
+    double a = vdot(pOrigin, axis);
+    V3 radial = vsub(pOrigin, vmul(axis, a));
+    double r = vlen(radial);
+    V3 nn;
+    if (!intersectionDistanceAndNormal(w, g, v3(r, 0, a), wallRadius, dist, &nn)) return 0;
+    if (r != 0) radial = vdiv(radial, r);
+    *n = vadd(vmul(axis, nn.Z), vmul(radial, nn.X));
     return 1;
+}
+
+static int wallDistanceAndNormal(const oracle_ctx* c, int wi, const Particle* p, double wallRadius, double* dist,
+                                 V3* n)
+{
+    return wallDistanceAndNormalAt(&c->walls[wi], &c->wallGeo[wi], p->pos, p->radius, wallRadius, dist, n);
 }
 
 /* DPMBase::computeForcesDueToWalls (DPMBase.cc:3188-3215) with
@@ -599,7 +777,7 @@ static void computeForcesDueToWall(oracle_ctx* c, int64_t pi, int32_t wi)
     double wallRadius = p->radius + mixed(c, p->species, w->species)->interaction_distance;
     double distance;
     V3 normal;
-    if (!wallDistanceAndNormal(w, p, wallRadius, &distance, &normal)) return;
+    if (!wallDistanceAndNormal(c, wi, p, wallRadius, &distance, &normal)) return;
     double overlap = p->radius - distance;
     int32_t k = getInteraction(c, (int32_t)pi, wi, 1, (uint32_t)(c->nt + 1));
     p = &c->p[pi];
@@ -755,7 +933,7 @@ static void evalWall(oracle_ctx* c, int64_t pi, int32_t wi, NewList* nl)
     double wallRadius = p->radius + mixed(c, p->species, w->species)->interaction_distance;
     double distance;
     V3 normal;
-    if (!wallDistanceAndNormal(w, p, wallRadius, &distance, &normal)) return;
+    if (!wallDistanceAndNormal(c, wi, p, wallRadius, &distance, &normal)) return;
     int32_t k = getExisting(c, (int32_t)pi, wi, 1);
     Inter tmp;
     Inter* x;
@@ -1094,6 +1272,7 @@ int oracle_dem_destroy(oracle_ctx* c)
     free(c->freeList);
     free(c->species);
     free(c->walls);
+    free(c->wallGeo);
     free(c->head);
     free(c->next);
     free(c->cell);
@@ -1118,12 +1297,30 @@ int oracle_dem_set_species(oracle_ctx* c, int32_t n, const dem_pair_params* t)
 int oracle_dem_set_walls(oracle_ctx* c, int32_t n, const dem_wall* w)
 {
     if (!c || n < 0 || (n > 0 && !w)) return fail(c, DEM_ERR_ARG, "set_walls: bad arguments");
+    for (int32_t k = 0; k < n; ++k)
+        if (w[k].type == DEM_WALL_AXISYMMETRIC && (w[k].n_sub < 1 || w[k].n_sub > DEM_AXI_MAX_SUB))
+            return fail(c, DEM_ERR_ARG, "axisymmetric wall: 1 to 4 sub-walls");
+    /* during a run (WallHandler::removeObject): the interactions of every
+     * wall that does not keep its index and id go with it */
+    for (int64_t k = 0; k < c->nIn; ++k) {
+        const Inter* x = &c->in[k];
+        if (!x->alive || !x->IisWall) continue;
+        if (x->I >= n || w[x->I].id != c->walls[x->I].id || w[x->I].type != c->walls[x->I].type)
+            interRemove(c, (int32_t)k);
+    }
     free(c->walls);
+    free(c->wallGeo);
     c->walls = NULL;
+    c->wallGeo = NULL;
     if (n > 0) {
         c->walls = (dem_wall*)malloc(sizeof(dem_wall) * (size_t)n);
         memcpy(c->walls, w, sizeof(dem_wall) * (size_t)n);
+        c->wallGeo = (struct AxiGeo*)calloc((size_t)n, sizeof(struct AxiGeo));
+        for (int32_t k = 0; k < n; ++k)
+            if (w[k].type == DEM_WALL_AXISYMMETRIC) setPointsAndLines(&w[k], &c->wallGeo[k]);
     }
+    for (int64_t k = 0; k < c->nIn; ++k)
+        if (c->in[k].alive && c->in[k].IisWall) c->in[k].sp = mixed(c, c->p[c->in[k].P].species, c->walls[c->in[k].I].species);
     c->nWalls = n;
     return DEM_OK;
 }
@@ -1427,7 +1624,7 @@ static int insertableLocal(const oracle_ctx* c, V3 pos, double radius, int32_t s
         double distance;
         V3 normal;
         const double wallRadius = radius + mixed(c, species, c->walls[w].species)->interaction_distance;
-        if (wallDistanceAndNormal(&c->walls[w], &cand, wallRadius, &distance, &normal)) return 0;
+        if (wallDistanceAndNormal(c, w, &cand, wallRadius, &distance, &normal)) return 0;
     }
     for (int64_t i = 0; i < c->nReal; ++i) {
         const Particle* q = &c->p[i];

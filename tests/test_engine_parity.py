@@ -67,6 +67,8 @@ CASES = [
     dict(laws="hvm", periodic=(True, False), walls=("bottom", "cylinder")),
     # radius ratio above 2.5: three grid levels, pairs across all of them
     dict(laws="hvm", periodic=(True, True), walls=("bottom",), radii=(0.15, 0.5)),
+    # AxisymmetricIntersectionOfWalls: a chamfered pedestal and a casing
+    dict(laws="full", periodic=(False, False), walls=("axisymmetric",)),
 ]
 
 
@@ -79,7 +81,7 @@ def reseeded(sc, st):
 
 
 @pytest.mark.parametrize("case", CASES, ids=lambda c: f"{c['laws']}-{'P' if c['periodic'][0] else 'np'}-{len(c['walls'])}w"
-                         + ("-3lv" if "radii" in c else ""))
+                         + ("-3lv" if "radii" in c else "") + ("-axi" if "axisymmetric" in c["walls"] else ""))
 def test_first_pass_and_one_step(case):
     """The force pass of initialiseSolve, then one computeOneTimeStep from
     the oracle's state after that pass (so both start from identical forces)."""
@@ -581,3 +583,35 @@ def test_find_in_region_against_oracle():
     assert any(len(g.find_in_region(0, np.array([0.0, 0.0, -1.0, -3.5]))) > 0 for _ in range(1))
     for e in (g, o):
         e.step_end()
+
+
+def test_engine_axisymmetric_wall_closed_forms(monkeypatch):
+    """The oracle's closed-form checks of AxisymmetricIntersectionOfWalls
+    (pedestal face and rim edge, side face, casing) on the engine."""
+    import test_oracle_golden as tg
+    monkeypatch.setattr(tg.oracle_lib, "engine", lambda: dem.Engine())
+    tg.test_axisymmetric_wall_closed_forms()
+
+
+def test_engine_walls_removed_mid_run():
+    """WallHandler::removeLastObject between steps (the calibration heap
+    lifts its casing, CalibrationHeap.cpp:137-143): the casing's interactions
+    are deleted, the pedestal's kept; the run continues as on the oracle."""
+    sc = S.mixture(n=400, seed=41, lx=7.0, ly=7.0, lz=7.0, inflate=1.12, laws="full", periodic=(False, False),
+                   walls=("axisymmetric",))
+    g, o = make_pair(sc)
+    for e in (g, o):
+        e.compute_forces()
+        e.step(30)
+    casing = [int(i) for i in o.contacts().i_id if int(i) == dem.DEM_WALL_KEY | 1]
+    assert casing, "the casing touches spheres before it is lifted"
+    for e in (g, o):
+        e.set_walls(sc.walls[:1])
+    for e in (g, o):
+        assert not any(int(i) == dem.DEM_WALL_KEY | 1 for i in e.contacts().i_id)
+    assert_contacts_equal(g, o)
+    for e in (g, o):
+        e.step(30)
+    assert_contacts_close(g, o)
+    a, b = g.download(), o.download()
+    np.testing.assert_allclose(a["x"], b["x"], rtol=0, atol=1e-9)
