@@ -18,10 +18,14 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HOST = os.path.join(ROOT, "mercurydpm_fork_amd", "host")
 LIB = os.path.join(ROOT, "mercurydpm_fork_amd", "lib")
 BIN = os.path.join(ROOT, "tests", "drivers", "bin")
-# drivers that check the reference's answers themselves and print "ok"
-DRIVERS = ["TangentialSpringUnitTest", "InclinedPlaneUnitTest", "HertzContactRestitutionUnitTest",
-           "PeriodicWallsUnitTest"]
+# drivers that check the reference's answers themselves and print "ok":
+# name -> (executable, arguments)
 GOLDEN = os.path.join(ROOT, "tests", "golden")
+SETUPS = os.path.join(ROOT, "tests", "drivers", "setups")
+DRIVERS = {"TangentialSpringUnitTest": ("TangentialSpringUnitTest", [GOLDEN]),
+           "InclinedPlaneUnitTest": ("SetupDriver", [os.path.join(SETUPS, "InclinedPlaneUnitTest.setup")]),
+           "HertzContactRestitutionUnitTest": ("HertzContactRestitutionUnitTest", [GOLDEN]),
+           "PeriodicWallsUnitTest": ("PeriodicWallsUnitTest", [GOLDEN])}
 
 SETTERS_SRC = r'''
 #include <cstdio>
@@ -72,12 +76,14 @@ def test_driver_sources_compile(tmp_path):
 
 @pytest.fixture(scope="module")
 def oracle_drivers(tmp_path_factory):
-    return oc.build_on_oracle(DRIVERS, str(tmp_path_factory.mktemp("oracle_drivers")))
+    return oc.build_on_oracle(sorted({exe for exe, _ in DRIVERS.values()}),
+                              str(tmp_path_factory.mktemp("oracle_drivers")))
 
 
-@pytest.mark.parametrize("driver", DRIVERS)
+@pytest.mark.parametrize("driver", sorted(DRIVERS))
 def test_driver_reproduces_reference_answer_on_oracle(driver, oracle_drivers, tmp_path):
-    r = subprocess.run([oracle_drivers[driver], GOLDEN], cwd=str(tmp_path), capture_output=True, text=True,
+    exe, args = DRIVERS[driver]
+    r = subprocess.run([oracle_drivers[exe], *args], cwd=str(tmp_path), capture_output=True, text=True,
                        timeout=300)
     print(r.stdout, r.stderr, file=sys.stderr)
     assert r.returncode == 0, r.stdout + r.stderr
@@ -85,11 +91,12 @@ def test_driver_reproduces_reference_answer_on_oracle(driver, oracle_drivers, tm
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("driver", DRIVERS)
+@pytest.mark.parametrize("driver", sorted(DRIVERS))
 def test_driver_reproduces_reference_answer(driver, tmp_path):
-    exe = os.path.join(BIN, driver)
+    name, args = DRIVERS[driver]
+    exe = os.path.join(BIN, name)
     assert os.path.exists(exe), "run __graft_entry__.build() first"
-    r = subprocess.run([exe, GOLDEN], cwd=str(tmp_path), capture_output=True, text=True, timeout=150)
+    r = subprocess.run([exe, *args], cwd=str(tmp_path), capture_output=True, text=True, timeout=150)
     print(r.stdout, r.stderr, file=sys.stderr)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "ok" in r.stdout

@@ -13,6 +13,13 @@ sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
 import output_check as oc  # noqa: E402
 
 G = oc.GOLDEN
+SETUPS = os.path.join(oc.DRIVERS, "setups")
+
+
+def setup(name):
+    """the generic driver with one of the set-up files"""
+    return ("SetupDriver", [os.path.join(SETUPS, name + ".setup")])
+
 # case -> (driver, arguments, [(file written, gold file, comparison)])
 CASES = {
     "ObliqueImpactSelfTest": ("ObliqueImpactSelfTest", [],
@@ -44,7 +51,7 @@ CASES = {
     "ExtremeOverlapUnitTest": ("ExtremeOverlapUnitTest", [],
                                [("ExtremeOverlapUnitTest.data", "extreme_overlap_unit_test.data", "data")]),
     # ten collisions whose parameters come from the random number generator
-    "ForceSelfTest": ("ForceSelfTest", [],
+    "ForceSelfTest": (*setup("ForceSelfTest"),
                       [(f"ForceSelfTest{k}.ene", f"force_self_test_{k}.ene", "ene") for k in range(10)] +
                       [(f"ForceSelfTest{k}.data", f"force_self_test_{k}.data", "data") for k in (2, 7)]),
     # three insertion boundaries fill an empty box for 201 steps: the RNG
@@ -54,16 +61,16 @@ CASES = {
         (f"InsertionBoundarySelfTest.{e}", f"insertion_boundary_self_test.{e}", k)
         for e, k in (("data", "data"), ("ene", "ene"), ("fstat", "fstat"), ("restart", "restart:1e-14"))]),
     # a falling sphere deleted by a plane and by a box
-    "DeletionBoundarySelfTest": ("DeletionBoundarySelfTest", [],
+    "DeletionBoundarySelfTest": (*setup("DeletionBoundarySelfTest"),
                                  [("DeletionBoundarySelfTest.data", "deletion_boundary_self_test.data", "data")]),
-    "CubeDeletionBoundarySelfTest": ("CubeDeletionBoundarySelfTest", [], [
+    "CubeDeletionBoundarySelfTest": (*setup("CubeDeletionBoundarySelfTest"), [
         ("CubeDeletionBoundarySelfTest.data", "cube_deletion_boundary_self_test.data", "data")]),
     # a chute on a MULTILAYER bottom: ChuteBottom fills a box at random and
     # lets it settle for 2000 steps before a layer is cut from the top; the
     # same 13 base and 2 inflow particles, the inflow ones to 15 digits, the
     # base positions to 1e-3 relative (the settling run amplifies the
     # round-off of force sums taken in another order to about 2e-5)
-    "roughBottomSelfTest": ("roughBottomSelfTest", [], [
+    "roughBottomSelfTest": (*setup("roughBottomSelfTest"), [
         ("roughBottomMultiLayer.restart", "rough_bottom_multi_layer.restart", "restart:1e-3")]),
     # stop at t = 1.05, restart from the .restart file (also with numbered
     # .data files), compare with the uninterrupted run (self-checking)
