@@ -4,12 +4,17 @@
 Workload: the polydisperse periodic chute -- 2,000,000 flowing spheres with
 d ~ U[0.5, 1.5] (three grid levels) over a rough base of fixed spheres,
 tilted 24 degrees, periodic in x and y, HertzianViscoelasticMindlin with
-e = 0.8 and mu = 0.5, dt = t_c / 50, FP64.  The spheres start as a loose,
-non-overlapping seeded packing (a rock-salt lattice) and the configured run
-follows: the chute gravity (|g| = 1) from the first step, 50,000 steps,
-timed chunk by chunk (run_average, and steady_window over the run's last
-chunks, rebuilds included).  Then, after W warm-up steps, K steps are timed:
-`value`.
+e = 0.8 and mu = 0.5, dt = t_c / 50, FP64.
+
+Protocol:
+  1. preparation (untimed): the spheres start on a seeded, non-overlapping
+     rock-salt lattice and fall onto the rough base under |g| = 1 straight
+     down until the bed is at rest -- a random loose packing 30 diameters
+     deep (settle_steps);
+  2. the configured run: 50,000 steps under the chute gravity (|g| = 1, 24
+     degrees), timed chunk by chunk (run_average, and steady_window over the
+     run's last chunks, rebuilds included);
+  3. after W warm-up steps, K steps are timed: `value`.
 
 Prints one JSON line: particle-steps/s (flowing spheres x steps / s), the
 roofline of the dominant kernel (the force kernel, which also integrates the
@@ -55,8 +60,10 @@ def parse():
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--n", type=int, default=2_000_000, help="flowing spheres")
+    ap.add_argument("--settle-steps", type=int, default=80000,
+                    help="preparation: steps of the lattice falling onto the base under |g| straight down")
     ap.add_argument("--run-steps", type=int, default=50000,
-                    help="the configured run before the timed window: steps from the loose packing under the "
+                    help="the configured run before the timed window: steps from the settled packing under the "
                          "chute gravity, timed chunk by chunk (run_average, steady_window)")
     ap.add_argument("--chunk", type=int, default=500, help="steps per timed chunk of the configured run")
     ap.add_argument("--steady-steps", type=int, default=2000,
@@ -130,6 +137,28 @@ def run_ensemble(a):
         "speedup_vs_sequential": t_seq / t_packed}))
 
 
+_roctx = None
+
+
+def profile_region(on):
+    """Open (on) or close the region rocprofv3 --selected-regions records
+    (roctxProfilerResume / roctxProfilerPause of the ROCTx library), so that a
+    kernel trace or counter run of this command covers only the timed window.
+    Without rocprofv3 the calls do nothing."""
+    global _roctx
+    if _roctx is None:
+        import ctypes
+        _roctx = False
+        for name in ("librocprofiler-sdk-roctx.so.1", "librocprofiler-sdk-roctx.so"):
+            try:
+                _roctx = ctypes.CDLL(name)
+                break
+            except OSError:
+                continue
+    if _roctx:
+        (_roctx.roctxProfilerResume if on else _roctx.roctxProfilerPause)(0)
+
+
 def b_alg(cbar, mixed=False):
     """Algorithmic HBM bytes per particle-step of the step (plain FP64,
     sliding-spring laws): state read x, v, omega (72) + radius, inverse mass
@@ -201,8 +230,12 @@ def main():
     eng.compute_forces()
     log(f"generated {n_total} spheres ({n_flow} flowing, {sc.info['n_base']} base, footprint "
         f"{sc.info['footprint']:.1f}) in {time.time() - t_gen:.1f}s")
+    t_settle = time.time()
+    sst = S.settle_chute(eng, sc, a.settle_steps, log=log)
+    settled_cn = sst.n_contacts / n_total
+    log(f"settled in {time.time() - t_settle:.1f}s: C/N {settled_cn:.3f}")
 
-    # the configuration's run: the loose packing under the chute gravity
+    # the configuration's run: the settled packing under the chute gravity
     # (|g| = 1, 24 degrees) for run_steps steps, timed chunk by chunk
     chunks = []
     done = 0
@@ -254,10 +287,12 @@ def main():
     rb0 = eng.stats().n_rebuilds
     torch.cuda.synchronize()
     eng.set_profiling(True)
+    profile_region(True)
     t0 = time.perf_counter()
     eng.step(a.steps)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
+    profile_region(False)
     st = eng.stats()
     c1 = st.n_contacts
     elapsed = t1 - t0
@@ -346,12 +381,14 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64" if a.precision == "fp64" else "f32 contacts / f64 state",
-        "data": "synthetic (seeded loose rock-salt lattice packing, then the configured run under the chute "
-                "gravity |g| = 1)",
+        "data": "synthetic (seeded rock-salt lattice settled onto the rough base under |g| = 1, then the "
+                "configured run under the chute gravity)",
         "config": {"workload": "polydisperse periodic chute, HertzianViscoelasticMindlin e=0.8 mu=0.5, 24 deg",
                    "flowing_spheres": n_flow, "base_spheres": sc.info["n_base"],
                    "diameter": "U[0.5,1.5]", "footprint": round(sc.info["footprint"], 3),
-                   "dt": sc.dt, "steps_before_window": a.run_steps + a.warmup,
+                   "dt": sc.dt, "settle_steps": a.settle_steps,
+                   "contacts_per_particle_settled": round(settled_cn, 4),
+                   "steps_before_window": a.run_steps + a.warmup,
                    "contacts_per_particle": round(cbar, 4), "contacts_per_particle_survey": "2-3 (dense bed)",
                    "list_entries_per_particle": round(lbar, 3),
                    "rebuilds_in_window": int(st.n_rebuilds - rb0),

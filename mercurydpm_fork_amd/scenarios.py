@@ -583,6 +583,26 @@ def chute_hvm(n_flow=2_000_000, seed=3, seed_base=4, angle_deg=24.0, l=None, dep
                     info=dict(n_flowing=n_flow, n_base=int(bx.size), footprint=l))
 
 
+def settle_chute(engine: dem.Engine, sc: Scenario, steps: int = 80_000, chunk: int = 10_000, log=None):
+    """Turn the loose lattice of chute_hvm into a random loose packing: the
+    spheres fall onto the rough base under |g| straight down (no tilt, no
+    drag) for `steps` steps, then the chute gravity is restored.  From the
+    lattice 30 diameters deep the bed is at rest after about 70,000 steps.
+    Returns the engine's stats at the end."""
+    gmag = math.sqrt(sum(c * c for c in sc.gravity))
+    engine.set_gravity((0.0, 0.0, -gmag))
+    done = 0
+    while done < steps:
+        k = min(chunk, steps - done)
+        engine.step(k)
+        done += k
+        if log is not None:
+            st = engine.stats()
+            log(f"settle {done}/{steps} steps (|g| {gmag:g} straight down): C/N {st.n_contacts / sc.state.n:.3f}")
+    engine.set_gravity(sc.gravity)
+    return engine.stats()
+
+
 def mixture(n=400, seed=7, lx=8.0, ly=8.0, lz=8.0, periodic=(True, True), walls=("bottom",), laws="lvf",
             fixed_frac=0.05, vmax=0.5, radii=(0.2, 0.45), omega_max=1.0, n_species=2,
             inflate=1.06) -> Scenario:

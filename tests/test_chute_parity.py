@@ -2,8 +2,9 @@
 (d ~ U[0.5, 1.5] over a rough base of fixed d = 1 spheres, 24 degrees,
 periodic in x and y; three grid levels) -- checked two ways on the GPU:
 
-* a reduced chute (20,000 flowing spheres, every level populated, pairs
-  across all levels) against the oracle on identical input: the initial
+* a reduced chute (20,000 flowing spheres 30 diameters deep, settled as
+  bench.py settles the full one; every level populated, pairs across all
+  levels) against the oracle on identical input: the initial
   force pass and one step bit-identical (forces within the force bar), then
   300 steps with frequent list rebuilds;
 * the full 2,000,000-sphere chute through size-independent properties: the
@@ -26,20 +27,15 @@ from test_engine_parity import (assert_contacts_close, assert_contacts_equal, as
 pytestmark = pytest.mark.gpu
 
 
-def settle(sc, steps_strong=3000, steps_ramp=1000, steps_chute=300, skin=0.0):
-    """A dense, flowing bed from the loose lattice (untimed preparation on
-    the engine): strong gravity straight down, then |g| = 1 with drag, then
-    the chute gravity.  Returns the state and the stored interactions."""
+def settle(sc, steps_settle=80_000, steps_chute=2000, skin=0.0):
+    """A dense bed from the loose lattice (untimed preparation on the
+    engine, as bench.py prepares the full chute): the spheres fall onto the
+    rough base under |g| = 1 straight down, then the chute gravity acts for
+    steps_chute steps.  Returns the state and the stored interactions."""
     e = dem.Engine(skin=skin)
     S.apply(e, sc)
     e.compute_forces()
-    e.set_gravity((0.0, 0.0, -100.0))
-    e.step(steps_strong)
-    e.set_gravity((0.0, 0.0, -1.0))
-    e.set_background_drag(5.0)
-    e.step(steps_ramp)
-    e.set_background_drag(0.0)
-    e.set_gravity(sc.gravity)
+    S.settle_chute(e, sc, steps_settle)
     e.step(steps_chute)
     st, ct, stats = e.download(), e.contacts(), e.stats()
     e.close()
@@ -67,15 +63,19 @@ def cross_level_pairs(sc, ct, levels=3):
 
 
 def test_reduced_chute_three_levels_vs_oracle():
-    sc = S.chute_hvm(n_flow=20_000, seed=13, seed_base=14, depth=8.0)
+    # 20,000 spheres 30 diameters deep on a 28 x 28 footprint
+    sc = S.chute_hvm(n_flow=20_000, seed=13, seed_base=14)
     st, ct, stats = settle(sc)
     assert stats.grid_levels == 3
-    # every level pair meets in a contact
+    # every level pair meets in a contact (the small spheres of level 0 start
+    # on one sublattice of the rock-salt packing, next to large ones only, so
+    # few level-0 pairs have formed yet; a few tens suffice to exercise that
+    # sweep)
     pairs = cross_level_pairs(sc, ct)
     for l0 in range(3):
         for l1 in range(l0, 3):
-            assert pairs.get((l0, l1), 0) > 50, pairs
-    assert ct.n > 1.5 * sc.state.n  # a dense bed: C/N well above one
+            assert pairs.get((l0, l1), 0) >= (10 if (l0, l1) == (0, 0) else 100), pairs
+    assert ct.n > sc.state.n  # a dense bed: more than one stored contact per sphere
     rs = reseeded(sc, st)
     # the initial force pass (with the uploaded history, stamps as stored)
     g, o = make_pair(rs, ct)

@@ -609,7 +609,9 @@ def test_engine_walls_removed_mid_run():
         e.set_walls(sc.walls[:1])
     for e in (g, o):
         assert not any(int(i) == dem.DEM_WALL_KEY | 1 for i in e.contacts().i_id)
-    assert_contacts_equal(g, o)
+    # 30 steps in: the summation order has perturbed the springs at round-off
+    # level, so the sets and stamps are equal and the springs close
+    assert_contacts_close(g, o)
     for e in (g, o):
         e.step(30)
     assert_contacts_close(g, o)
