@@ -342,6 +342,17 @@ int dem_find_in_region(dem_ctx* ctx, int32_t kind, const double* params, int64_t
 int dem_check_particles(dem_ctx* ctx, int64_t k, const double* x, const double* radius, const int32_t* species,
                         const double* max_distance, int32_t* free_out);
 
+/* dem_check_particles without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * The candidates are staged in a static array of the library, one per GPU,
+ * sized for DEM_CHECK_STATIC_MAX candidates (DEM_ERR_CAPACITY above that).
+ * Every call on a GPU shares that GPU's array: calls that overlap on one GPU
+ * race.  Same arguments, checks and results as dem_check_particles. */
+#define DEM_CHECK_STATIC_MAX 4096
+int dem_check_particles_static(dem_ctx* ctx, int64_t k, const double* x, const double* radius,
+                               const int32_t* species, const double* max_distance, int32_t* free_out);
+
 /* Download state in handler order (any pointer may be NULL). */
 int dem_download_particles(dem_ctx* ctx, double* x, double* v, double* w, double* q,
                            double* f, double* t);

@@ -28,6 +28,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1573,42 +1574,70 @@ __device__ inline int candidate_images(const Params& p, V3 x, double max_distanc
     return m;
 }
 
-// busy[c] = 1 when candidate c (cand[c] = x, y, z, radius; species csp[c]) or
-// one of its images touches a wall (BaseWall::getDistanceAndNormal,
-// checkParticleForInteractionLocal, MercuryBase.cc:622-650) or a particle:
-// Vec3D::getDistanceSquared(candidate, particle) below the square of the
-// sum of the radii (BaseParticle::isInContactWith, BaseParticle.cc:863-871).
-// blockIdx.y is the candidate; blocks in x stride over the particles, and
-// block 0 also tests the walls.
+// Candidates of dem_check_particles staged on the device: positions (3 per
+// candidate), radii, image ranges, species and the result flags.
+struct CandBuf {
+    const double* x;
+    const double* r;
+    const double* max_distance;
+    const int* sp;
+    int* free_flag;
+};
+
+// Staging bytes per candidate (doubles first, so every array stays aligned).
+constexpr size_t kCandBytes = 5 * sizeof(double) + 2 * sizeof(int);
+
+inline CandBuf cand_layout(void* buf, int64_t k)
+{
+    double* x = (double*)buf;
+    double* r = x + 3 * k;
+    double* md = r + k;
+    int* sp = (int*)(md + k);
+    return CandBuf{x, r, md, sp, sp + k};
+}
+
+__global__ void __launch_bounds__(kBlock) k_fill_int(int* a, int n, int v)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) a[t] = v;
+}
+
+// free_flag[c] = 0 when candidate c or one of its images touches a wall
+// (BaseWall::getDistanceAndNormal, checkParticleForInteractionLocal,
+// MercuryBase.cc:622-650) or a particle: Vec3D::getDistanceSquared(candidate,
+// particle) below the square of the sum of the radii
+// (BaseParticle::isInContactWith, BaseParticle.cc:863-871).  The flags start
+// at 1.  blockIdx.y is the candidate; blocks in x stride over the particles,
+// and block 0 also tests the walls.
 template <int NS>
-__global__ void __launch_bounds__(kBlock) k_check_particles(Dev<NS> d, Params p, const double4* cand, const int* csp,
-                                                            const double* max_distance, int* busy)
+__global__ void __launch_bounds__(kBlock) k_check_particles(Dev<NS> d, Params p, CandBuf cb)
 {
     const int c = blockIdx.y;
-    const double4 cr = cand[c];
-    const int sc = csp[c];
+    const V3 xc = mk(cb.x[3 * c], cb.x[3 * c + 1], cb.x[3 * c + 2]);
+    const double rc = cb.r[c];
+    const int sc = cb.sp[c];
     V3 img[kMaxImages];
-    const int m = candidate_images(p, mk(cr.x, cr.y, cr.z), max_distance[c], img);
+    const int m = candidate_images(p, xc, cb.max_distance[c], img);
     if (blockIdx.x == 0 && (int)threadIdx.x < p.nwalls) {
         const WallDev& wl = d.walls[threadIdx.x];
         const double idist = d.sp[sc * p.ns + wl.species].interaction_distance;
         for (int k = 0; k < m; ++k) {
             double distance;
             V3 n;
-            if (wall_contact(wl, img[k], cr.w, idist, distance, n)) busy[c] = 1;
+            if (wall_contact(wl, img[k], rc, idist, distance, n)) cb.free_flag[c] = 0;
         }
     }
     GRID_STRIDE(i, p.n)
     {
         const double4 q = d.posr[i];
         const int sq = (int)(d.spc[i] & 0xFFFFu);
-        const double reach = cr.w + q.w + d.sp[sc * p.ns + sq].interaction_distance;
+        const double reach = rc + q.w + d.sp[sc * p.ns + sq].interaction_distance;
         const double reach2 = reach * reach;
         for (int k = 0; k < m; ++k) {
             const double X = img[k].x - q.x;
             const double Y = img[k].y - q.y;
             const double Z = img[k].z - q.z;
-            if (X * X + Y * Y + Z * Z < reach2) busy[c] = 1;
+            if (X * X + Y * Y + Z * Z < reach2) cb.free_flag[c] = 0;
         }
     }
 }
@@ -1878,6 +1907,43 @@ uint32_t next_pow2(uint64_t x)
     uint64_t p = 1;
     while (p < x) p <<= 1;
     return (uint32_t)p;
+}
+
+}  // namespace
+
+// Staging memory of dem_check_particles_static: one array per GPU, in the
+// code object, so that entry point allocates nothing.  Every call on a GPU
+// uses that GPU's copy (calls that overlap on one GPU race).  Host code never
+// passes the symbol itself to a kernel: it uses the device address that
+// scratch_address() resolved for the current device.
+static __device__ __align__(256) unsigned char scratch[(size_t)DEM_CHECK_STATIC_MAX * (5 * sizeof(double) +
+                                                                                      2 * sizeof(int))];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on `device`, resolved once per device with that
+// device current (dem_create calls this).  nullptr on failure.
+void* resolve_scratch(int device)
+{
+    if (device < 0 || device >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void* ptr = nullptr;
+        if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess) return nullptr;
+        g_scratch_ptr[device] = ptr;
+    }
+    return g_scratch_ptr[device];
+}
+
+void* scratch_address(int device)
+{
+    if (device < 0 || device >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    return g_scratch_ptr[device];
 }
 
 }  // namespace
@@ -2447,6 +2513,8 @@ int dem_create(dem_ctx** out, const dem_config* cfg)
     if (ctx->device < 0 || ctx->device >= ndev) return set_err(ctx, DEM_ERR_ARG, "no such HIP device");
     HIPCHK(hipSetDevice(ctx->device));
     HIPCHK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    if (!resolve_scratch(ctx->device))
+        return set_err(ctx, DEM_ERR_DEVICE, "hipGetSymbolAddress of the static candidate staging array failed");
     return DEM_OK;
 }
 
@@ -3176,8 +3244,9 @@ int dem_find_in_region(dem_ctx* ctx, int32_t kind, const double* params, int64_t
     return DEM_OK;
 }
 
-int dem_check_particles(dem_ctx* ctx, int64_t k, const double* x, const double* radius, const int32_t* species,
-                        const double* max_distance, int32_t* free_out)
+// Argument checks shared by both candidate entry points.
+int check_candidates_args(dem_ctx* ctx, int64_t k, const double* x, const double* radius, const int32_t* species,
+                          const double* max_distance, const int32_t* free_out)
 {
     if (!ctx) return DEM_ERR_ARG;
     if (!ctx->uploaded) return set_err(ctx, DEM_ERR_STATE, "no particles uploaded");
@@ -3192,9 +3261,44 @@ int dem_check_particles(dem_ctx* ctx, int64_t k, const double* x, const double* 
             !(radius[j] > 0) || !std::isfinite(radius[j]) || std::isnan(max_distance[j]))
             return set_err(ctx, DEM_ERR_ARG, "check_particles: candidates must be finite with a positive radius");
     }
+    return DEM_OK;
+}
+
+// The candidate test with the candidates staged in buf (at least
+// k * kCandBytes bytes of device memory): copies in, flag fill, test, copy
+// of the flags straight into free_out.  Allocates nothing.
+int check_candidates(dem_ctx* ctx, int64_t k, const double* x, const double* radius, const int32_t* species,
+                     const double* max_distance, int32_t* free_out, void* buf)
+{
+    const CandBuf cb = cand_layout(buf, k);
+    HIPCHK(hipMemcpyAsync((void*)cb.x, x, sizeof(double) * 3 * k, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync((void*)cb.r, radius, sizeof(double) * k, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync((void*)cb.max_distance, max_distance, sizeof(double) * k, hipMemcpyHostToDevice,
+                          ctx->stream));
+    HIPCHK(hipMemcpyAsync((void*)cb.sp, species, sizeof(int) * k, hipMemcpyHostToDevice, ctx->stream));
+    hipLaunchKernelGGL(k_fill_int, dim3(grid_for(k)), dim3(kBlock), 0, ctx->stream, cb.free_flag, (int)k, 1);
+    HIPCHK(hipGetLastError());
+    refresh_params(ctx);
+    const unsigned gx = (unsigned)std::max<int64_t>(1, std::min<int64_t>(grid_for(ctx->n), 1024));
+    const dim3 grid(gx, (unsigned)k);
+    if (ctx->ns_springs == 9)
+        hipLaunchKernelGGL(k_check_particles<9>, grid, dim3(kBlock), 0, ctx->stream, ctx->d9, ctx->prm, cb);
+    else
+        hipLaunchKernelGGL(k_check_particles<3>, grid, dim3(kBlock), 0, ctx->stream, ctx->d3, ctx->prm, cb);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(free_out, cb.free_flag, sizeof(int) * k, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return DEM_OK;
+}
+
+int dem_check_particles(dem_ctx* ctx, int64_t k, const double* x, const double* radius, const int32_t* species,
+                        const double* max_distance, int32_t* free_out)
+{
+    int r = check_candidates_args(ctx, k, x, radius, species, max_distance, free_out);
+    if (r != DEM_OK || k == 0) return r;
     HIPCHK(hipSetDevice(ctx->device));
-    // candidate buffer: k double4, then k species, k distances and k flags
-    const size_t bytes = (size_t)k * (sizeof(double4) + sizeof(double) + 2 * sizeof(int));
+    // staging buffer of the context: grows on demand, outlives re-uploads
+    const size_t bytes = (size_t)k * kCandBytes;
     if ((int64_t)bytes > ctx->check_cap) {
         HIPCHK(hipStreamSynchronize(ctx->stream));
         if (ctx->d_check) (void)hipFree(ctx->d_check);
@@ -3203,31 +3307,21 @@ int dem_check_particles(dem_ctx* ctx, int64_t k, const double* x, const double* 
         HIPCHK(hipMalloc(&ctx->d_check, bytes));
         ctx->check_cap = (int64_t)bytes;
     }
-    double4* dc = (double4*)ctx->d_check;
-    double* dmax = (double*)(dc + k);
-    int* dsp = (int*)(dmax + k);
-    int* dbusy = dsp + k;
-    std::vector<double4> hc(k);
-    for (int64_t j = 0; j < k; ++j) hc[j] = make_double4(x[3 * j], x[3 * j + 1], x[3 * j + 2], radius[j]);
-    std::vector<int> hbusy(k);
-    HIPCHK(hipMemcpyAsync(dc, hc.data(), sizeof(double4) * k, hipMemcpyHostToDevice, ctx->stream));
-    HIPCHK(hipMemcpyAsync(dmax, max_distance, sizeof(double) * k, hipMemcpyHostToDevice, ctx->stream));
-    HIPCHK(hipMemcpyAsync(dsp, species, sizeof(int) * k, hipMemcpyHostToDevice, ctx->stream));
-    HIPCHK(hipMemsetAsync(dbusy, 0, sizeof(int) * k, ctx->stream));
-    refresh_params(ctx);
-    const unsigned gx = (unsigned)std::max<int64_t>(1, std::min<int64_t>(grid_for(ctx->n), 1024));
-    const dim3 grid(gx, (unsigned)k);
-    if (ctx->ns_springs == 9)
-        hipLaunchKernelGGL(k_check_particles<9>, grid, dim3(kBlock), 0, ctx->stream, ctx->d9, ctx->prm,
-                           (const double4*)dc, (const int*)dsp, (const double*)dmax, dbusy);
-    else
-        hipLaunchKernelGGL(k_check_particles<3>, grid, dim3(kBlock), 0, ctx->stream, ctx->d3, ctx->prm,
-                           (const double4*)dc, (const int*)dsp, (const double*)dmax, dbusy);
-    HIPCHK(hipGetLastError());
-    HIPCHK(hipMemcpyAsync(hbusy.data(), dbusy, sizeof(int) * k, hipMemcpyDeviceToHost, ctx->stream));
-    HIPCHK(hipStreamSynchronize(ctx->stream));
-    for (int64_t j = 0; j < k; ++j) free_out[j] = hbusy[j] ? 0 : 1;
-    return DEM_OK;
+    return check_candidates(ctx, k, x, radius, species, max_distance, free_out, ctx->d_check);
+}
+
+int dem_check_particles_static(dem_ctx* ctx, int64_t k, const double* x, const double* radius,
+                               const int32_t* species, const double* max_distance, int32_t* free_out)
+{
+    int r = check_candidates_args(ctx, k, x, radius, species, max_distance, free_out);
+    if (r != DEM_OK || k == 0) return r;
+    if (k > DEM_CHECK_STATIC_MAX)
+        return set_err(ctx, DEM_ERR_CAPACITY, "check_particles_static: at most " +
+                                                  std::to_string(DEM_CHECK_STATIC_MAX) + " candidates per call");
+    HIPCHK(hipSetDevice(ctx->device));
+    void* buf = scratch_address(ctx->device);
+    if (!buf) return set_err(ctx, DEM_ERR_DEVICE, "check_particles_static: scratch array not resolved on this device");
+    return check_candidates(ctx, k, x, radius, species, max_distance, free_out, buf);
 }
 
 int dem_download_particles(dem_ctx* ctx, double* x, double* v, double* w, double* q, double* f, double* t)

@@ -77,6 +77,9 @@ class Config(C.Structure):
                 ("grid_levels", C.c_int32), ("precision", C.c_int32)]
 
 
+# candidates per call of dem_check_particles_static (DEM_CHECK_STATIC_MAX)
+CHECK_STATIC_MAX = 4096
+
 PRECISION_FP64 = 0
 PRECISION_MIXED = 1
 
@@ -144,6 +147,8 @@ def _declare(lib, prefix: str, has_config: bool):
     f("set_kinematics", _I32, _PTR, _I64, _PTR, _PTR, _PTR, _PTR)
     f("add_particles", _I32, _PTR, _I64, *([_PTR] * 9))
     f("check_particles", _I32, _PTR, _I64, *([_PTR] * 5))
+    if hasattr(lib, prefix + "check_particles_static"):  # the HIP library only
+        f("check_particles_static", _I32, _PTR, _I64, *([_PTR] * 5))
     f("remove_particles", _I32, _PTR, _I64, _PTR)
     f("find_in_region", _I32, _PTR, _I32, _PTR, _I64, C.POINTER(_I64), _PTR)
     f("cg_z", _I32, _PTR, C.POINTER(CGConfig), _PTR)
@@ -370,18 +375,22 @@ class Engine:
         self._check(self._fn("find_in_region")(self.h, int(kind), _ptr(prm), n.value, C.byref(n), _ptr(out)))
         return out[:n.value]
 
-    def check_particles(self, x, radius, species, max_distance) -> np.ndarray:
+    def check_particles(self, x, radius, species, max_distance, static: bool = False) -> np.ndarray:
         """DPMBase::checkParticleForInteraction for candidate spheres against
         the resident particles and the walls: True where a candidate (and its
         periodic images within max_distance of a boundary; none where
-        max_distance < 0) touches nothing."""
+        max_distance < 0) touches nothing.  static: through
+        dem_check_particles_static, which stages the candidates in the
+        library's static per-GPU array (at most CHECK_STATIC_MAX) and
+        allocates nothing."""
         x = np.ascontiguousarray(np.asarray(x, np.float64).reshape(-1, 3))
         k = len(x)
         r = np.ascontiguousarray(np.broadcast_to(np.asarray(radius, np.float64), (k,)))
         sp = np.ascontiguousarray(np.broadcast_to(np.asarray(species, np.int32), (k,)))
         md = np.ascontiguousarray(np.broadcast_to(np.asarray(max_distance, np.float64), (k,)))
         out = np.zeros(k, np.int32)
-        self._check(self._fn("check_particles")(self.h, k, _ptr(x), _ptr(r), _ptr(sp), _ptr(md), _ptr(out)))
+        fn = self._fn("check_particles_static" if static else "check_particles")
+        self._check(fn(self.h, k, _ptr(x), _ptr(r), _ptr(sp), _ptr(md), _ptr(out)))
         return out.astype(bool)
 
     # ------------------------------------------------------------- download
