@@ -79,61 +79,91 @@ def parse():
     ap.add_argument("--ensemble", type=int, default=0,
                     help="instead of the chute: this many calibration heaps (sizes log-uniform in [500, 20000]) "
                          "stepped as one packed ensemble, against the same heaps run one at a time")
-    ap.add_argument("--ensemble-steps", type=int, default=20000, help="steps per simulation for sims/hour")
+    ap.add_argument("--ensemble-steps", type=int, default=30000, help="steps per simulation for sims/hour")
+    ap.add_argument("--save-state", default="",
+                    help="after the configured run, write the particle state and stored interactions to this .npz")
+    ap.add_argument("--load-state", default="",
+                    help="start from a state written by --save-state (no preparation, no configured run): lets a "
+                         "profiler cover the timed window's state only")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only copy this many bytes device-to-device (calibrates HBM counters) and exit")
     return ap.parse_args()
 
 
 def run_ensemble(a):
-    """Batched ensemble: a.ensemble heaps (log-uniform sizes in [500, 20000],
-    rolling friction) packed side by side in one context and stepped
-    together, against the same kind of heaps stepped one at a time.  The
-    sequential cost is measured on 16 sizes spread over the range and
-    interpolated (log-log) to every member.  One JSON line: simulations per
-    hour at a.ensemble_steps steps per simulation, both ways."""
+    """Batched ensemble: a.ensemble calibration heaps (Drivers/Calibration/
+    CalibrationHeap.cpp), particle counts log-uniform in [500, 20000], d ~
+    U[0.8, 1.2], each with its own sliding friction U[0.1, 0.8], rolling
+    friction U[0, 0.1] and restitution U[0.5, 0.95], its own base and casing
+    (per-species wall groups), run together in one context for
+    a.ensemble_steps steps of the heap protocol (fill, casing lifted once
+    settled, spheres below the base deleted, angle of repose), with the layout
+    and skin picked by the autotuner.  Against: the same protocol run one heap
+    at a time, timed on 16 heap sizes spread over the range and interpolated
+    (log-log) to every heap.  One JSON line: simulations per hour both ways."""
     import numpy as np
-    import torch
 
     from mercurydpm_fork_amd import dem
     from mercurydpm_fork_amd import ensemble as E
     from mercurydpm_fork_amd import scenarios as S
 
-    rng = np.random.default_rng(a.seed)
-    sizes = np.exp(rng.uniform(np.log(500), np.log(20000), a.ensemble)).astype(int)
-    sp = E.calibration_species()
+    def log(msg):
+        print("# " + msg, file=sys.stderr, flush=True)
 
-    def time_steps(sc, warm, steps):
-        e = dem.Engine(device=0)
-        S.apply(e, sc)
-        e.compute_forces()
-        e.step(warm)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        e.step(steps)
-        e.stats()  # waits for the device
-        dt = (time.perf_counter() - t0) / steps
-        e.close()
-        return dt
+    params = E.draw_heap_params(a.ensemble, a.seed)
+    members = [E.calibration_heap(p) for p in params]
+    sizes = np.array([p.n for p in params])
+    t_tune = time.time()
+    schedule, skin, tune = E.autotune(members, log=log)
+    t_tune = time.time() - t_tune
+    ens = E.pack_heaps(members, schedule=schedule)
+    e = dem.Engine(device=0, skin=skin)
+    S.apply(e, ens.scenario)
+    e.compute_forces()
+    e.stats()
+    t0 = time.perf_counter()
+    run = E.run_heap_protocol(e, ens, a.ensemble_steps, log=log)
+    t_packed = time.perf_counter() - t0
+    e.close()
+    log(f"packed: {a.ensemble} heaps, {int(sizes.sum())} spheres, {a.ensemble_steps} steps in {t_packed:.1f}s")
 
-    members = [E.heap(int(n), 1000 + k, sp) for k, n in enumerate(sizes)]
-    packed = E.pack(members)
-    t_packed = time_steps(packed.scenario, 200, a.steps)
     sample = np.unique(np.geomspace(500, 20000, 16).astype(int))
-    t_sample = np.array([time_steps(E.heap(int(n), 7 + k, sp), 200, a.steps) for k, n in enumerate(sample)])
+    t_sample = []
+    for k, n in enumerate(sample):
+        hp = E.HeapParams(int(n), 0.45, 0.05, 0.725, 900 + k)
+        one = E.pack_heaps([E.calibration_heap(hp)])
+        g = dem.Engine(device=0)
+        S.apply(g, one.scenario)
+        g.compute_forces()
+        g.stats()
+        t1 = time.perf_counter()
+        E.run_heap_protocol(g, one, a.ensemble_steps)
+        t_sample.append(time.perf_counter() - t1)
+        g.close()
+        log(f"sequential: heap of {n} spheres, {a.ensemble_steps} steps in {t_sample[-1]:.2f}s")
+    t_sample = np.array(t_sample)
     t_seq = float(np.sum(np.exp(np.interp(np.log(sizes), np.log(sample), np.log(t_sample)))))
-    per_hour = lambda t: 3600.0 / (t * a.ensemble_steps)  # noqa: E731
+    per_hour = lambda t: 3600.0 * a.ensemble / t  # noqa: E731
+    ok = np.isfinite(run.angle_of_repose)
     print(json.dumps({
         "metric": "calibration heaps per hour (batched ensemble)",
-        "value": per_hour(t_packed) * a.ensemble, "unit": "simulations/hour", "n_gpus": 1,
-        "steps": a.steps, "ms_per_step": t_packed * 1e3, "higher_is_better": True, "dtype": "f64",
-        "data": "synthetic (jittered lattice columns above a floor)",
-        "config": {"workload": "batched calibration heaps, LinearViscoelasticFriction with rolling friction",
-                   "simulations": int(a.ensemble), "particles": int(sizes.sum()),
-                   "size_range": [500, 20000], "steps_per_simulation": a.ensemble_steps},
-        "sequential": {"value": per_hour(t_seq) * a.ensemble, "unit": "simulations/hour",
-                       "ms_per_step_all_members": t_seq * 1e3,
-                       "sample": {int(n): round(t * 1e3, 4) for n, t in zip(sample, t_sample)}},
+        "value": per_hour(t_packed), "unit": "simulations/hour", "n_gpus": 1,
+        "steps": a.ensemble_steps, "ms_per_step": 1e3 * t_packed / a.ensemble_steps, "higher_is_better": True,
+        "dtype": "f64",
+        "data": "synthetic (seeded loose lattice fills of each heap's casing)",
+        "config": {"workload": "CalibrationHeap protocol, LinearViscoelasticFriction with rolling friction, "
+                               "per-heap sliding friction, rolling friction and restitution",
+                   "simulations": int(a.ensemble), "particles": int(sizes.sum()), "size_range": [500, 20000],
+                   "steps_per_simulation": a.ensemble_steps, "layout": schedule, "skin": skin,
+                   "autotune": tune, "autotune_seconds": round(t_tune, 1)},
+        "protocol": {"casings_lifted": int(np.isfinite(run.fill_time).sum()), "angles_measured": int(ok.sum()),
+                     "angle_of_repose_mean": float(np.mean(run.angle_of_repose[ok])) if ok.any() else None,
+                     "angle_at_end_mean": float(np.nanmean(run.angle_at_end)),
+                     "angle_vs_sliding_friction_correlation": float(np.corrcoef(
+                         [p.sliding_mu for p in params], np.nan_to_num(run.angle_at_end))[0, 1]),
+                     "spheres_deleted": int(run.removed)},
+        "sequential": {"value": per_hour(t_seq), "unit": "simulations/hour", "seconds_all": round(t_seq, 1),
+                       "sample_seconds": {int(n): round(float(t), 3) for n, t in zip(sample, t_sample)}},
         "speedup_vs_sequential": t_seq / t_packed}))
 
 
@@ -191,6 +221,15 @@ def cpu_threads():
     return max(1, n)
 
 
+def with_state(S, dem, sc, d):
+    """The scenario sc with the dynamic state d (a download) in place of its
+    initial one."""
+    return S.Scenario(sc.name, sc.species, sc.walls, sc.periodic, sc.gravity, sc.dt,
+                      dem.State(id=sc.state.id, species=sc.state.species, radius=sc.state.radius,
+                                inv_mass=sc.state.inv_mass, inv_inertia=sc.state.inv_inertia,
+                                x=d["x"], v=d["v"], w=d["w"], q=d["q"], f=d["f"], t=d["t"]))
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -226,21 +265,31 @@ def main():
     n_flow = sc.info["n_flowing"]
     prec = dem.PRECISION_MIXED if a.precision == "mixed" else dem.PRECISION_FP64
     eng = dem.Engine(device=0, skin=a.skin, precision=prec)
-    S.apply(eng, sc)
-    eng.compute_forces()
-    log(f"generated {n_total} spheres ({n_flow} flowing, {sc.info['n_base']} base, footprint "
-        f"{sc.info['footprint']:.1f}) in {time.time() - t_gen:.1f}s")
-    t_settle = time.time()
-    sst = S.settle_chute(eng, sc, a.settle_steps, log=log)
-    settled_cn = sst.n_contacts / n_total
-    log(f"settled in {time.time() - t_settle:.1f}s: C/N {settled_cn:.3f}")
+    settled_cn = None
+    if a.load_state:
+        z = np.load(a.load_state)
+        S.apply(eng, with_state(S, dem, sc, z), dem.Contacts(z["p_id"], z["i_id"], z["spring"], z["ts"]))
+        eng.set_time(float(z["time"]), int(z["n_steps"]))
+        eng.compute_forces()
+        a.run_steps = 0
+        log(f"loaded the state of step {int(z['n_steps'])} from {a.load_state}")
+    else:
+        S.apply(eng, sc)
+        eng.compute_forces()
+        log(f"generated {n_total} spheres ({n_flow} flowing, {sc.info['n_base']} base, footprint "
+            f"{sc.info['footprint']:.1f}) in {time.time() - t_gen:.1f}s")
+        t_settle = time.time()
+        sst = S.settle_chute(eng, sc, a.settle_steps, log=log)
+        settled_cn = sst.n_contacts / n_total
+        log(f"settled in {time.time() - t_settle:.1f}s: C/N {settled_cn:.3f}")
+    rb_start = eng.stats().n_rebuilds
 
     # the configuration's run: the settled packing under the chute gravity
     # (|g| = 1, 24 degrees) for run_steps steps, timed chunk by chunk
     chunks = []
     done = 0
     t_run = 0.0
-    rb_prev = 0
+    rb_prev = rb_start
     while done < a.run_steps:
         k = min(a.chunk, a.run_steps - done)
         eng.set_profiling(True)
@@ -263,7 +312,14 @@ def main():
         run_avg = {"value": n_flow * a.run_steps / t_run, "unit": "particle-steps/s", "steps": a.run_steps,
                    "seconds": round(t_run, 3), "ms_per_step": round(1e3 * t_run / a.run_steps, 4),
                    "rebuilds": int(sum(c["rebuilds"] for c in chunks)),
-                   "note": "the whole configured run from the loose packing, rebuilds included"}
+                   "note": "the whole configured run from the settled packing, rebuilds included"}
+    if a.save_state:
+        d = eng.download()
+        ct = eng.contacts()
+        st = eng.stats()
+        np.savez(a.save_state, x=d["x"], v=d["v"], w=d["w"], q=d["q"], f=d["f"], t=d["t"], p_id=ct.p_id,
+                 i_id=ct.i_id, spring=ct.spring, ts=ct.ts, time=st.time, n_steps=st.n_steps)
+        log(f"state of step {st.n_steps} written to {a.save_state}")
     # the late window: the last chunks of the run covering >= steady_steps
     steady = None
     if chunks:
@@ -316,10 +372,7 @@ def main():
     if a.mixed_steps > 0 and a.precision == "fp64":
         d = eng.download()
         ct = eng.contacts()
-        s3 = S.Scenario(sc.name, sc.species, sc.walls, sc.periodic, sc.gravity, sc.dt,
-                        dem.State(id=sc.state.id, species=sc.state.species, radius=sc.state.radius,
-                                  inv_mass=sc.state.inv_mass, inv_inertia=sc.state.inv_inertia,
-                                  x=d["x"], v=d["v"], w=d["w"], q=d["q"], f=d["f"], t=d["t"]))
+        s3 = with_state(S, dem, sc, d)
         em = dem.Engine(device=0, skin=a.skin, precision=dem.PRECISION_MIXED)
         S.apply(em, s3, ct)
         em.set_time(st.time, int(st.n_steps))
@@ -348,10 +401,7 @@ def main():
         import oracle_lib
         d = eng.download()
         ct = eng.contacts()
-        s2 = S.Scenario(sc.name, sc.species, sc.walls, sc.periodic, sc.gravity, sc.dt,
-                        dem.State(id=sc.state.id, species=sc.state.species, radius=sc.state.radius,
-                                  inv_mass=sc.state.inv_mass, inv_inertia=sc.state.inv_inertia,
-                                  x=d["x"], v=d["v"], w=d["w"], q=d["q"], f=d["f"], t=d["t"]))
+        s2 = with_state(S, dem, sc, d)
         threads = cpu_threads()
         o, got = oracle_lib.engine_omp(threads)
         S.apply(o, s2, ct)
@@ -387,7 +437,7 @@ def main():
                    "flowing_spheres": n_flow, "base_spheres": sc.info["n_base"],
                    "diameter": "U[0.5,1.5]", "footprint": round(sc.info["footprint"], 3),
                    "dt": sc.dt, "settle_steps": a.settle_steps,
-                   "contacts_per_particle_settled": round(settled_cn, 4),
+                   "contacts_per_particle_settled": None if settled_cn is None else round(settled_cn, 4),
                    "steps_before_window": a.run_steps + a.warmup,
                    "contacts_per_particle": round(cbar, 4), "contacts_per_particle_survey": "2-3 (dense bed)",
                    "list_entries_per_particle": round(lbar, 3),

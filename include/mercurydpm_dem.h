@@ -48,7 +48,7 @@
 extern "C" {
 #endif
 
-#define DEM_ABI_VERSION 7
+#define DEM_ABI_VERSION 8
 
 /* error codes */
 #define DEM_OK 0
@@ -168,7 +168,15 @@ typedef struct dem_config {
     double skin;             /* Verlet skin distance; <= 0 = 0.2 * min diameter */
     int32_t grid_levels;     /* cell-grid levels; 0 = automatic (<= 3) */
     int32_t precision;       /* DEM_PRECISION_FP64 (0) or DEM_PRECISION_MIXED */
+    int32_t list_sweep;      /* Verlet-list build: DEM_SWEEP_PER_PARTICLE (0) or DEM_SWEEP_WAVEFRONT */
+    int32_t reserved;        /* 0 */
 } dem_config;
+
+/* How the Verlet lists are built from the cell grid (the same lists either
+ * way): every particle sweeps its own candidate cells, or the 64 particles of
+ * a wavefront sweep the union of their cells together, each row of cells
+ * loaded once into LDS (coalesced) and shared. */
+enum dem_sweep { DEM_SWEEP_PER_PARTICLE = 0, DEM_SWEEP_WAVEFRONT = 1 };
 
 /* Arithmetic of the contact forces.  FP64 follows the reference bit for bit
  * where the summation order allows; MIXED computes the contact geometry
@@ -221,6 +229,18 @@ int dem_set_species(dem_ctx* ctx, int32_t n_species, const dem_pair_params* tabl
  * stored interactions, the interactions of every other index are deleted
  * (BaseInteractable::~BaseInteractable, BaseInteractable.cc:97-106). */
 int dem_set_walls(dem_ctx* ctx, int32_t n_walls, const dem_wall* walls);
+
+/* Per-species wall groups (no counterpart in the reference, where every wall
+ * meets every particle; it lets many independent simulations -- each with its
+ * own species and its own walls -- share one context, as a batched ensemble
+ * of calibration heaps does): particles of species s meet only the walls
+ * first[s] .. first[s] + count[s] - 1 (at most 32), in that order.  With
+ * groups set there may be up to 65536 walls in all.  Before
+ * dem_upload_particles it sets the groups (one per species of
+ * dem_set_species); afterwards a group may only drop its last walls, whose
+ * interactions are deleted, as WallHandler::removeLastObject deletes a
+ * wall's interactions (BaseHandler.h:472-505, BaseInteractable.cc:97-106). */
+int dem_set_wall_groups(dem_ctx* ctx, int32_t n_species, const int32_t* first, const int32_t* count);
 
 /* BoundaryHandler periodic boundaries.  Replaces checkInteractionWithBoundaries
  * (DPMBase.cc:3272), checkAndDuplicatePeriodicParticles (DPMBase.cc:5043) and
@@ -416,6 +436,17 @@ typedef struct dem_cg_config {
 
 /* out: n x DEM_CG_FIELDS doubles. */
 int dem_cg_z(dem_ctx* ctx, const dem_cg_config* cfg, double* out);
+
+/* Per particle, in handler order, 3 doubles: the kinetic energy 1/2 m v^2
+ * (zero for fixed particles; DPMBase::getKineticEnergy, DPMBase.cc:
+ * 1542-1553), the elastic energy of the stored interactions the particle
+ * holds as P -- the pairs where it has the lower id, and its wall contacts --
+ * (BaseInteraction::getElasticEnergy of each; DPMBase::getElasticEnergy,
+ * DPMBase.cc:1528-1537, is their sum over all particles), and its height z,
+ * at the state of the last force pass.  Lets a driver evaluate the
+ * reference's kinetic/elastic-energy criteria (CalibrationHeap.cpp:108-140)
+ * per sub-simulation without downloading the interactions. */
+int dem_particle_energies(dem_ctx* ctx, double* out);
 
 /* Per-phase HIP-event timing of dem_step (no counterpart in the reference,
  * whose only timer is the wall clock of Time::toc, MercuryTime.h:70-73).

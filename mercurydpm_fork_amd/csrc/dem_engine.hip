@@ -42,10 +42,12 @@ using namespace dem;
 
 namespace {
 
-constexpr int kMaxWalls = 32;
+constexpr int kMaxWalls = 32;            // walls one particle may meet
+constexpr int kMaxWallsGrouped = 1 << 16; // walls in all, with wall groups
 constexpr int kMaxLevels = 4;
 constexpr int kMaxSlabs = 64;
 constexpr int kBlock = 256;
+constexpr int kWaves = kBlock / 64; // wavefronts per block
 constexpr uint64_t kEmptyKey = ~0ull;
 
 // device flag words
@@ -81,7 +83,9 @@ struct Rec {
 
 struct Params {
     int n, maxn, nwalls, ns, levels, rotation, nper;
-    int wstride; // particle-wall slots per particle (the wall count at upload, at least 1)
+    int wstride; // particle-wall slots per particle (the walls a particle may meet at upload, at least 1)
+    int wgroups; // 1: a particle meets only its species' group of walls (wgrp), 0: every wall
+    int sweep;   // Verlet-list build: DEM_SWEEP_PER_PARTICLE or DEM_SWEEP_WAVEFRONT
     int erase;
     int paxis[3];
     int axis_pb[3]; // periodic boundary index acting on each axis, -1 if none
@@ -153,8 +157,17 @@ struct Dev {
     // tables
     const dem_pair_params* sp;
     const WallDev* walls;
+    const int2* wgrp; // per species: first wall, number of walls (when Params::wgroups)
     int* flags;
 };
+
+// The walls a particle of this species meets: walls first .. first + count - 1,
+// in particle-wall slots 0 .. count - 1.
+template <int NS>
+__device__ inline int2 wall_group(const Dev<NS>& d, const Params& p, int species)
+{
+    return p.wgroups ? d.wgrp[species] : make_int2(0, p.nwalls);
+}
 
 __device__ inline uint64_t mix64(uint64_t k)
 {
@@ -468,11 +481,12 @@ __global__ void __launch_bounds__(kBlock) k_rb_begin(Dev<NS> d, Params p, int cu
             hash_insert(d, key, d.rec[cur][lslot(k, p.maxn, (int)i)]);
         }
         uint32_t wm = d.wmask[cur][i];
+        const int wfirst = wall_group(d, p, (int)(d.spc[i] & 0xFFFF)).x;
         while (wm) {
-            const int w = __ffs(wm) - 1;
+            const int l = __ffs(wm) - 1;
             wm &= wm - 1;
-            const uint64_t key = ((uint64_t)idi << 32) | (DEM_WALL_KEY | (uint32_t)w);
-            hash_insert(d, key, d.wrec[cur][(size_t)i * p.wstride + w]);
+            const uint64_t key = ((uint64_t)idi << 32) | (DEM_WALL_KEY | (uint32_t)(wfirst + l));
+            hash_insert(d, key, d.wrec[cur][(size_t)i * p.wstride + l]);
         }
     }
     // orphans still alive at this step (time stamp >= alive_min) go back in
@@ -627,85 +641,239 @@ __device__ inline V3 min_image_delta(const Params& p, V3 xi, V3 xj)
     return dlt;
 }
 
+// Cell range of the candidate search of a particle at xi with radius r on
+// level m, per axis: cells lo..hi (cell coordinates unwrapped; a periodic
+// axis whose range covers the whole period is given as 0..nc-1 and flagged
+// in *full).  nc: cells per period along a periodic axis, 0 along an open one.
+__device__ inline void level_box(const Params& p, int m, V3 xi, double r, int lo[3], int hi[3], int nc[3], bool* full)
+{
+    const double R = r + p.rmaxl[m] + p.skin;
+    *full = false;
+    for (int a = 0; a < 3; ++a) {
+        const double xa = comp(xi, a);
+        const int pb = p.axis_pb[a];
+        if (pb >= 0) {
+            nc[a] = p.ncell[m][a];
+            lo[a] = floor_to_cell((xa - R - p.pleft[pb]) / p.hc[m][a]);
+            hi[a] = floor_to_cell((xa + R - p.pleft[pb]) / p.hc[m][a]);
+            if (hi[a] - lo[a] + 1 >= nc[a]) {
+                lo[a] = 0;
+                hi[a] = nc[a] - 1;
+                *full = true;
+            }
+        } else {
+            nc[a] = 0;
+            lo[a] = floor_to_cell((xa - R) / p.h[m]);
+            hi[a] = floor_to_cell((xa + R) / p.h[m]);
+        }
+    }
+}
+
+// One list entry candidate q of particle i (cell test already passed): the
+// Verlet-list predicate (radius sum + skin at the minimum image) and the
+// entry, with the writer bit when i has the lower id.
+template <int NS>
+__device__ inline void try_entry(Dev<NS>& d, const Params& p, int i, V3 xi, double ri, uint32_t idi, uint32_t q,
+                                 double4 pq, uint32_t idq, int& cnt)
+{
+    const V3 dl = min_image_delta(p, xi, xyz(pq));
+    const double cut = ri + pq.w + p.skin;
+    if (vlen2(dl) < cut * cut) {
+        if (cnt < p.maxn) d.nbr[lslot(cnt, p.maxn, i)] = q | (idi < idq ? kWriterBit : 0u);
+        ++cnt;
+    }
+}
+
+// Wave-wide minimum / maximum of an int (lanes that take no part pass a
+// neutral value).
+__device__ inline int wave_min(int v)
+{
+    for (int s = 32; s > 0; s >>= 1) v = min(v, __shfl_xor(v, s, 64));
+    return v;
+}
+__device__ inline int wave_max(int v)
+{
+    for (int s = 32; s > 0; s >>= 1) v = max(v, __shfl_xor(v, s, 64));
+    return v;
+}
+
+// orders the LDS accesses of a wavefront's lanes
+__device__ inline void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// widest union of the wavefront's cell ranges along the two slower bucket
+// axes swept through LDS; wider unions take the per-lane sweep
+constexpr int kUnionSpan = 32;
+
+// Verlet lists from the sorted multilevel grid, one particle per lane.  On
+// every level the cells of a particle's candidate box form rows along the
+// fastest bucket axis, and the cells of a row lie in consecutive buckets, so
+// a row segment is one range of the sorted arrays.  The 64 particles of a
+// wavefront are neighbours in the sorted order, so their boxes overlap: the
+// wavefront sweeps the union of its boxes row by row, loading each row
+// segment once, 64 candidates at a time, into LDS (coalesced), and every lane
+// tests the candidates of its own cells of the segment (a sub-range of the
+// loaded one, found from the segment's bucket starts, staged in LDS too).  A lane meets its candidates in
+// the same order as a sweep of its own box alone (level, slow row, middle
+// row, fast-axis cell, sorted index), so the lists do not depend on the
+// grouping.  Wavefronts whose boxes wrap a whole period, or spread over more
+// than kUnionSpan rows along the two slower axes, take that per-lane sweep.
+// bucket starts of a row segment staged in LDS (longer segments read them
+// from global memory)
+constexpr int kSegMax = 256;
+struct ListLds {
+    double4 pos[64];
+    int4 cell[64];
+    uint32_t id[64];
+    uint32_t start[kSegMax + 1];
+};
+
 template <int NS>
 __global__ void __launch_bounds__(kBlock) k_build_lists(Dev<NS> d, Params p)
 {
     if (!rebuild_pending(d.flags)) return;
-    GRID_STRIDE_XCD(ii, p.n)
-    {
-        const int i = (int)ii;
-        const double4 pr = d.t_posr[i];
+    __shared__ ListLds lds[kWaves];
+    const int lane = threadIdx.x & 63;
+    ListLds& L = lds[threadIdx.x >> 6];
+    const int as = p.bax0, am = p.bax1, af = p.bax2;
+    // block-uniform stride loop: the lanes of a wavefront sweep together
+    for (uint64_t i0 = (uint64_t)xcd_block() * blockDim.x; i0 < (uint64_t)p.n;
+         i0 += (uint64_t)gridDim.x * blockDim.x) {
+        const int i = (int)(i0 + threadIdx.x);
+        const bool valid = i < p.n;
+        if (!__any(valid)) continue;
+        const int ic = valid ? i : (int)p.n - 1;
+        const double4 pr = d.t_posr[ic];
         const V3 xi = xyz(pr);
-        const bool fixed_i = (d.t_cellc[i].x & 0x100) != 0;
-        const uint32_t idi = d.t_id[i];
+        const bool fixed_i = (d.t_cellc[ic].x & 0x100) != 0;
+        const uint32_t idi = d.t_id[ic];
         int cnt = 0;
         for (int m = 0; m < p.levels; ++m) {
-            const double R = pr.w + p.rmaxl[m] + p.skin;
             int lo[3], hi[3], nc[3];
-            for (int a = 0; a < 3; ++a) {
-                const double xa = comp(xi, a);
-                const int pb = p.axis_pb[a];
-                if (pb >= 0) {
-                    nc[a] = p.ncell[m][a];
-                    lo[a] = floor_to_cell((xa - R - p.pleft[pb]) / p.hc[m][a]);
-                    hi[a] = floor_to_cell((xa + R - p.pleft[pb]) / p.hc[m][a]);
-                    if (hi[a] - lo[a] + 1 >= nc[a]) {
-                        lo[a] = 0;
-                        hi[a] = nc[a] - 1;
-                    }
-                } else {
-                    nc[a] = 0;
-                    lo[a] = floor_to_cell((xa - R) / p.h[m]);
-                    hi[a] = floor_to_cell((xa + R) / p.h[m]);
-                }
-            }
-            // Buckets of one level are numbered with the fastest axis
-            // varying fastest, so the cells of a row along that axis lie in
-            // consecutive buckets and their particles in one range of the
-            // sorted arrays: walk each row as a single range (split where the
-            // bucket box wraps).  Cell coordinates are stored wrapped along
-            // periodic axes and as they are along open ones.
-            const int as = p.bax0, am = p.bax1, af = p.bax2;
+            bool full;
+            level_box(p, m, xi, pr.w, lo, hi, nc, &full);
             const int nbf = p.nbd[m][af];
-            for (int ts = lo[as]; ts <= hi[as]; ++ts) {
-                const int cs = nc[as] ? wrap_mod(ts, nc[as]) : ts;
-                for (int tm = lo[am]; tm <= hi[am]; ++tm) {
-                    const int cm = nc[am] ? wrap_mod(tm, nc[am]) : tm;
-                    for (int t = lo[af]; t <= hi[af];) {
-                        const int bw = wrap_mod(t, nbf);
-                        const int len = min(hi[af] - t + 1, nbf - bw);
-                        const int s0 = nc[af] ? bw : t;
-                        const int s1 = s0 + len - 1;
-                        int cc[3];
-                        cc[as] = cs;
-                        cc[am] = cm;
-                        cc[af] = s0;
-                        const uint32_t b = cell_key(p, m, cc[0], cc[1], cc[2]);
-                        const uint32_t qs = d.start[b];
-                        const uint32_t qe = d.start[b + (uint32_t)len];
-                        for (uint32_t q = qs; q < qe; ++q) {
-                            const int4 cq = d.t_cellc[q];
-                            if ((cq.x & 0xFF) != m || cell_coord(cq, as) != cs || cell_coord(cq, am) != cm) continue;
-                            const int cf = cell_coord(cq, af);
-                            if (cf < s0 || cf > s1) continue;
-                            if (q == (uint32_t)i) continue;
-                            if (fixed_i && (cq.x & 0x100)) continue;
-                            const double4 pq = d.t_posr[q];
-                            const V3 dl = min_image_delta(p, xi, xyz(pq));
-                            const double cut = pr.w + pq.w + p.skin;
-                            if (vlen2(dl) < cut * cut) {
-                                if (cnt < p.maxn) d.nbr[lslot(cnt, p.maxn, i)] = q | (idi < d.t_id[q] ? kWriterBit : 0u);
-                                ++cnt;
+            int ulo[3], uhi[3];
+            bool wide = p.sweep != DEM_SWEEP_WAVEFRONT || __any(valid && full);
+            for (int a = 0; a < 3; ++a) {
+                ulo[a] = wave_min(valid ? lo[a] : 0x7FFFFFFF);
+                uhi[a] = wave_max(valid ? hi[a] : -0x7FFFFFFF);
+                if (a != af && uhi[a] - ulo[a] + 1 > kUnionSpan) wide = true;
+            }
+            if (!wide) {
+                for (int ts = ulo[as]; ts <= uhi[as]; ++ts) {
+                    const int cs = nc[as] ? wrap_mod(ts, nc[as]) : ts;
+                    for (int tm = ulo[am]; tm <= uhi[am]; ++tm) {
+                        const int cm = nc[am] ? wrap_mod(tm, nc[am]) : tm;
+                        const bool row = valid && ts >= lo[as] && ts <= hi[as] && tm >= lo[am] && tm <= hi[am];
+                        if (!__any(row)) continue;
+                        for (int t = ulo[af]; t <= uhi[af];) {
+                            const int bw = wrap_mod(t, nbf);
+                            const int len = min(uhi[af] - t + 1, nbf - bw);
+                            const int s0 = nc[af] ? bw : t;
+                            int cc[3];
+                            cc[as] = cs;
+                            cc[am] = cm;
+                            cc[af] = s0;
+                            const uint32_t bk = cell_key(p, m, cc[0], cc[1], cc[2]);
+                            // this lane's own cells of the segment, t + o0 .. t + o1,
+                            // and their range ms .. me of the sorted arrays: the
+                            // segment's bucket starts are staged in LDS
+                            const int o0 = max(lo[af], t) - t;
+                            const int o1 = min(hi[af], t + len - 1) - t;
+                            const bool mine = row && o0 <= o1;
+                            uint32_t qs, qe, ms = 0, me = 0;
+                            if (len < kSegMax) {
+                                wave_sync();
+                                for (int c = lane; c <= len; c += 64) L.start[c] = d.start[bk + (uint32_t)c];
+                                wave_sync();
+                                qs = L.start[0];
+                                qe = L.start[len];
+                                if (mine) {
+                                    ms = L.start[o0];
+                                    me = L.start[o1 + 1];
+                                }
+                            } else {
+                                qs = d.start[bk];
+                                qe = d.start[bk + (uint32_t)len];
+                                if (mine) {
+                                    ms = d.start[bk + (uint32_t)o0];
+                                    me = d.start[bk + (uint32_t)o1 + 1u];
+                                }
                             }
+                            const int s0o = s0 + o0, s1o = s0 + o1;
+                            if (__any(ms < me)) {
+                                for (uint32_t c0 = qs; c0 < qe; c0 += 64) {
+                                    const uint32_t nchunk = min(64u, qe - c0);
+                                    wave_sync();
+                                    if ((uint32_t)lane < nchunk) {
+                                        L.pos[lane] = d.t_posr[c0 + lane];
+                                        L.cell[lane] = d.t_cellc[c0 + lane];
+                                        L.id[lane] = d.t_id[c0 + lane];
+                                    }
+                                    wave_sync();
+                                    const uint32_t k0 = ms > c0 ? ms - c0 : 0u;
+                                    const uint32_t k1 = me < c0 + nchunk ? (me > c0 ? me - c0 : 0u) : nchunk;
+                                    for (uint32_t k = k0; k < k1; ++k) {
+                                        const int4 cq = L.cell[k];
+                                        if ((cq.x & 0xFF) != m || cell_coord(cq, as) != cs || cell_coord(cq, am) != cm)
+                                            continue;
+                                        const int cf = cell_coord(cq, af);
+                                        if (cf < s0o || cf > s1o) continue;
+                                        const uint32_t q = c0 + k;
+                                        if (q == (uint32_t)i) continue;
+                                        if (fixed_i && (cq.x & 0x100)) continue;
+                                        try_entry(d, p, i, xi, pr.w, idi, q, L.pos[k], L.id[k], cnt);
+                                    }
+                                }
+                            }
+                            t += len;
                         }
-                        t += len;
+                    }
+                }
+            } else if (valid) {
+                // per-lane sweep of the particle's own box
+                for (int ts = lo[as]; ts <= hi[as]; ++ts) {
+                    const int cs = nc[as] ? wrap_mod(ts, nc[as]) : ts;
+                    for (int tm = lo[am]; tm <= hi[am]; ++tm) {
+                        const int cm = nc[am] ? wrap_mod(tm, nc[am]) : tm;
+                        for (int t = lo[af]; t <= hi[af];) {
+                            const int bw = wrap_mod(t, nbf);
+                            const int len = min(hi[af] - t + 1, nbf - bw);
+                            const int s0 = nc[af] ? bw : t;
+                            const int s1 = s0 + len - 1;
+                            int cc[3];
+                            cc[as] = cs;
+                            cc[am] = cm;
+                            cc[af] = s0;
+                            const uint32_t bk = cell_key(p, m, cc[0], cc[1], cc[2]);
+                            const uint32_t qs = d.start[bk];
+                            const uint32_t qe = d.start[bk + (uint32_t)len];
+                            for (uint32_t q = qs; q < qe; ++q) {
+                                const int4 cq = d.t_cellc[q];
+                                if ((cq.x & 0xFF) != m || cell_coord(cq, as) != cs || cell_coord(cq, am) != cm)
+                                    continue;
+                                const int cf = cell_coord(cq, af);
+                                if (cf < s0 || cf > s1) continue;
+                                if (q == (uint32_t)i) continue;
+                                if (fixed_i && (cq.x & 0x100)) continue;
+                                try_entry(d, p, i, xi, pr.w, idi, q, d.t_posr[q], d.t_id[q], cnt);
+                            }
+                            t += len;
+                        }
                     }
                 }
             }
         }
-        if (cnt > p.maxn) atomicOr(&d.flags[F_LIST_OVERFLOW], 1);
-        atomicMax(&d.flags[F_MAX_LIST], cnt);
-        d.ncount[i] = cnt < p.maxn ? cnt : p.maxn;
+        if (valid) {
+            if (cnt > p.maxn) atomicOr(&d.flags[F_LIST_OVERFLOW], 1);
+            atomicMax(&d.flags[F_MAX_LIST], cnt);
+            d.ncount[i] = cnt < p.maxn ? cnt : p.maxn;
+        }
     }
 }
 
@@ -759,11 +927,12 @@ __global__ void __launch_bounds__(kBlock) k_rematch(Dev<NS> d, Params p, int cur
         }
         d.mask[cur][i] = m;
         uint32_t wm = 0;
-        for (int w = 0; w < p.nwalls; ++w) {
+        const int2 wg = wall_group(d, p, (int)(d.t_spc[i] & 0xFFFF));
+        for (int l = 0; l < wg.y; ++l) {
             Rec<NS> r;
-            if (hash_find(d, ((uint64_t)idi << 32) | (DEM_WALL_KEY | (uint32_t)w), r)) {
-                d.wrec[cur][(size_t)i * p.wstride + w] = r;
-                wm |= 1u << w;
+            if (hash_find(d, ((uint64_t)idi << 32) | (DEM_WALL_KEY | (uint32_t)(wg.x + l)), r)) {
+                d.wrec[cur][(size_t)i * p.wstride + l] = r;
+                wm |= 1u << l;
             }
         }
         d.wmask[cur][i] = wm;
@@ -864,14 +1033,6 @@ __device__ inline void wall_geometry(V3 x, double r, double distance, V3 n, Cont
 //            queue order, i.e. in list order: the sum is deterministic.
 // Both particles of a pair evaluate it (no atomics); the lower id writes the
 // history record.
-constexpr int kWaves = kBlock / 64;
-
-__device__ inline void wave_sync()
-{
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 
 __device__ inline uint32_t lane_prefix(unsigned long long ballot)
 {
@@ -1234,8 +1395,9 @@ __global__ void __launch_bounds__(kBlock) FORCE_WAVES_ATTR k_force(Dev<NS> d, Pa
         const uint32_t wm_in = d.wmask[cur][i];
         uint32_t wm_out = 0;
         const size_t wbase = (size_t)i * p.wstride;
-        for (int w = 0; w < p.nwalls; ++w) {
-            const WallDev& wl = d.walls[w];
+        const int2 wg = wall_group(d, p, si);
+        for (int w = 0; w < wg.y; ++w) {
+            const WallDev& wl = d.walls[wg.x + w];
             bool contact = false;
             double distance = 0.0;
             V3 n = zero3();
@@ -1339,6 +1501,58 @@ __device__ inline void pair_geometry(const Params& p, V3 xp, double rp, V3 xj, d
     g.overlap = rp + rj - g.distance;
     g.contact = vsub(xp, vmul(g.normal, rp - 0.5 * g.overlap));
     g.i_is_wall = false;
+}
+
+// Per particle (handler order): the kinetic energy (DPMBase::getKineticEnergy,
+// DPMBase.cc:1542-1553: 1/2 m v^2, zero for fixed particles), the elastic
+// energy of the stored interactions it holds as P (the pairs it writes and its
+// wall contacts; DPMBase::getElasticEnergy, DPMBase.cc:1528-1537, sums them
+// over the interaction handler), at the positions of the last force pass, and
+// its height z.
+template <int NS>
+__global__ void __launch_bounds__(kBlock) k_particle_energies(Dev<NS> d, Params p, int cur, double* out)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= p.n) return;
+    const double4 pr = d.posr[i];
+    const double4 vl = d.vel[i];
+    const V3 xp = xyz(pr);
+    const int si = (int)(d.spc[i] & 0xFFFF);
+    const double ke = vl.w != 0.0 ? 0.5 * (1.0 / vl.w) * (vl.x * vl.x + vl.y * vl.y + vl.z * vl.z) : 0.0;
+    double ee = 0.0;
+    unsigned long long m = d.mask[cur][i];
+    while (m) {
+        const int k = __ffsll((long long)m) - 1;
+        m &= m - 1;
+        const size_t s = lslot(k, p.maxn, i);
+        const uint32_t j = d.nbr[s] & ~kWriterBit;
+        const double4 pq = d.posr[j];
+        ContactGeom g;
+        pair_geometry(p, xp, pr.w, xyz(pq), pq.w, g);
+        const double eff_r = 1.0 / (1.0 / pr.w + 1.0 / pq.w);
+        const double eff_m = 1.0 / (vl.w + d.vel[j].w);
+        const int sj = (int)(d.spc[j] & 0xFFFF);
+        ee += elastic_energy<NS>(d.sp[si * p.ns + sj], g.overlap, eff_r, eff_m, d.rec[cur][s].s);
+    }
+    if (p.nwalls > 0) {
+        uint32_t wm = d.wmask[cur][i];
+        const int wfirst = wall_group(d, p, si).x;
+        while (wm) {
+            const int l = __ffs(wm) - 1;
+            wm &= wm - 1;
+            const WallDev& wl = d.walls[wfirst + l];
+            const dem_pair_params& sp = d.sp[si * p.ns + wl.species];
+            double distance = 0.0;
+            V3 wn = zero3();
+            (void)wall_contact(wl, xp, pr.w, sp.interaction_distance, distance, wn);
+            const double eff_m = vl.w != 0.0 ? 1.0 / vl.w : 0.0;
+            ee += elastic_energy<NS>(sp, pr.w - distance, pr.w, eff_m, d.wrec[cur][(size_t)i * p.wstride + l].s);
+        }
+    }
+    const uint32_t o = d.orig[i];
+    out[3 * (size_t)o] = ke;
+    out[3 * (size_t)o + 1] = ee;
+    out[3 * (size_t)o + 2] = pr.z;
 }
 
 // ------------------------------------------------------------------------
@@ -1521,15 +1735,16 @@ __global__ void __launch_bounds__(kBlock) k_cg_contacts(Dev<NS> d, Params p, CgP
         // walls (BaseWall::isFixed is true): CONTACT_TO_P unless P is fixed
         if (p.nwalls > 0 && !p_fixed) {
             const uint32_t wm = d.wmask[cur][i];
-            for (int w = 0; w < p.nwalls; ++w) {
-                if (!((wm >> w) & 1u)) continue;
-                const double4 f4 = d.fw[(size_t)i * p.wstride + w];
+            const int si = (int)(d.spc[i] & 0xFFFF);
+            const int2 wg = wall_group(d, p, si);
+            for (int l = 0; l < wg.y; ++l) {
+                if (!((wm >> l) & 1u)) continue;
+                const double4 f4 = d.fw[(size_t)i * p.wstride + l];
                 if (f4.w != c.stamp) continue;
                 double distance = 0.0;
                 V3 wn = zero3();
-                const int si = (int)(d.spc[i] & 0xFFFF);
-                if (!wall_contact(d.walls[w], xp, pr.w, d.sp[si * p.ns + d.walls[w].species].interaction_distance,
-                                  distance, wn))
+                const WallDev& wl = d.walls[wg.x + l];
+                if (!wall_contact(wl, xp, pr.w, d.sp[si * p.ns + wl.species].interaction_distance, distance, wn))
                     continue;
                 ContactGeom g;
                 wall_geometry(xp, pr.w, distance, wn, g);
@@ -1618,8 +1833,9 @@ __global__ void __launch_bounds__(kBlock) k_check_particles(Dev<NS> d, Params p,
     const int sc = cb.sp[c];
     V3 img[kMaxImages];
     const int m = candidate_images(p, xc, cb.max_distance[c], img);
-    if (blockIdx.x == 0 && (int)threadIdx.x < p.nwalls) {
-        const WallDev& wl = d.walls[threadIdx.x];
+    const int2 wg = wall_group(d, p, sc);
+    if (blockIdx.x == 0 && (int)threadIdx.x < wg.y) {
+        const WallDev& wl = d.walls[wg.x + (int)threadIdx.x];
         const double idist = d.sp[sc * p.ns + wl.species].interaction_distance;
         for (int k = 0; k < m; ++k) {
             double distance;
@@ -1720,14 +1936,15 @@ __global__ void __launch_bounds__(kBlock) k_collect_contacts(Dev<NS> d, Params p
     }
     if (p.nwalls > 0) {
         uint32_t wm = d.wmask[cur][i];
+        const int wfirst = wall_group(d, p, (int)(d.spc[i] & 0xFFFF)).x;
         while (wm) {
-            const int w = __ffs(wm) - 1;
+            const int l = __ffs(wm) - 1;
             wm &= wm - 1;
             const int o = atomicAdd(counter, 1);
             if (o >= cap) continue;
-            const size_t s = (size_t)i * p.wstride + w;
+            const size_t s = (size_t)i * p.wstride + l;
             pid[o] = idi;
-            iid[o] = DEM_WALL_KEY | (uint32_t)w;
+            iid[o] = DEM_WALL_KEY | (uint32_t)(wfirst + l);
             for (int a = 0; a < 9; ++a) spring[9 * (size_t)o + a] = a < NS ? d.wrec[cur][s].s[a] : 0.0;
             ts[o] = d.wrec[cur][s].ts;
         }
@@ -1785,12 +2002,14 @@ __global__ void __launch_bounds__(kBlock) k_collect_interactions(Dev<NS> d, Para
     if (p.nwalls > 0) {
         uint32_t wm = d.wmask[cur][i];
         const int si = (int)(d.spc[i] & 0xFFFF);
+        const int wfirst = wall_group(d, p, si).x;
         while (wm) {
-            const int w = __ffs(wm) - 1;
+            const int l = __ffs(wm) - 1;
             wm &= wm - 1;
+            const int w = wfirst + l;
             const int o = atomicAdd(counter, 1);
             if (o >= cap) continue;
-            const size_t s = (size_t)i * p.wstride + w;
+            const size_t s = (size_t)i * p.wstride + l;
             double distance = 0.0;
             V3 wn = zero3();
             (void)wall_contact(d.walls[w], xp, pr.w, d.sp[si * p.ns + d.walls[w].species].interaction_distance, distance,
@@ -1900,13 +2119,27 @@ __global__ void k_keep_walls(Dev<NS> d, Params p, uint32_t keep)
     d.wmask[1][i] &= keep;
 }
 
+// A species' wall group shrinks during a run (a batched heap lifts its
+// casing, the last wall of its group): the interactions with the walls beyond
+// the new count are deleted.
+template <int NS>
+__global__ void k_shrink_groups(Dev<NS> d, Params p)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= p.n) return;
+    const int c = d.wgrp[d.spc[i] & 0xFFFF].y;
+    const uint32_t keep = c >= 32 ? ~0u : ((1u << c) - 1u);
+    d.wmask[0][i] &= keep;
+    d.wmask[1][i] &= keep;
+}
+
 inline unsigned grid_for(uint64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 
-uint32_t next_pow2(uint64_t x)
+uint64_t next_pow2(uint64_t x)
 {
     uint64_t p = 1;
     while (p < x) p <<= 1;
-    return (uint32_t)p;
+    return p;
 }
 
 }  // namespace
@@ -1958,6 +2191,7 @@ struct dem_ctx {
     int maxn_cfg = 64;
     double skin_cfg = 0.0;
     int levels_cfg = 0;
+    int sweep_cfg = DEM_SWEEP_PER_PARTICLE;
     // host-side settings
     std::vector<dem_pair_params> species;
     int ns = 0;
@@ -1985,6 +2219,9 @@ struct dem_ctx {
     dem_pair_params* d_sp = nullptr;
     WallDev* d_walls = nullptr;
     int wall_slots = 1;        // particle-wall slots per particle of the current upload
+    int wall_cap = 1;          // walls d_walls holds
+    std::vector<int2> wgrp;    // per species: first wall, number of walls (empty: every wall for every particle)
+    int2* d_wgrp = nullptr;
     int* d_flags = nullptr;
     uint32_t nbuckets = 0;
     int64_t last_rebuilds = 0;
@@ -2040,12 +2277,44 @@ void free_all(dem_ctx* ctx)
     ctx->allocs.clear();
     ctx->d_sp = nullptr;
     ctx->d_walls = nullptr;
+    ctx->d_wgrp = nullptr;
     ctx->d_flags = nullptr;
     ctx->uploaded = false;
 }
 
+// The wall tables of an upload: every wall in d_walls; per species the wall
+// group when groups are set (then a particle meets at most 32 walls, and
+// there may be more walls in all); particle-wall slots per particle.
+int setup_walls(dem_ctx* ctx, Params& p)
+{
+    const int nw = (int)ctx->walls.size();
+    int slots = nw;
+    if (!ctx->wgrp.empty()) {
+        if ((int)ctx->wgrp.size() != ctx->ns)
+            return set_err(ctx, DEM_ERR_ARG, "wall groups: one group per species (set them after set_species)");
+        slots = 0;
+        for (const int2& g : ctx->wgrp) {
+            if (g.x + g.y > nw) return set_err(ctx, DEM_ERR_ARG, "wall groups: a group reaches beyond the walls");
+            slots = std::max(slots, g.y);
+        }
+    } else if (nw > kMaxWalls) {
+        return set_err(ctx, DEM_ERR_ARG, "more than 32 walls need wall groups (dem_set_wall_groups)");
+    }
+    ctx->wall_cap = std::max(nw, 1);
+    ctx->wall_slots = std::max(slots, 1);
+    p.wstride = ctx->wall_slots;
+    p.wgroups = ctx->wgrp.empty() ? 0 : 1;
+    int r;
+    if ((r = dalloc(ctx, &ctx->d_walls, (size_t)ctx->wall_cap)) != 0) return r;
+    if (!ctx->wgrp.empty()) {
+        if ((r = dalloc(ctx, &ctx->d_wgrp, ctx->wgrp.size())) != 0) return r;
+        HIPCHK(hipMemcpy(ctx->d_wgrp, ctx->wgrp.data(), sizeof(int2) * ctx->wgrp.size(), hipMemcpyHostToDevice));
+    }
+    return DEM_OK;
+}
+
 // The walls with what setPointsAndLines derives, copied into d_walls (which
-// holds ctx->wall_slots entries).
+// holds ctx->wall_cap entries).
 int copy_walls(dem_ctx* ctx)
 {
     std::vector<WallDev> wd(std::max<size_t>(ctx->walls.size(), 1));
@@ -2090,7 +2359,9 @@ int alloc_particles(dem_ctx* ctx, int64_t n)
     A(d.count, ctx->nbuckets) A(d.cursor, ctx->nbuckets) A(d.start, (size_t)ctx->nbuckets + 1);
     const size_t ntiles = ((size_t)ctx->nbuckets + kScanTile - 1) / kScanTile;
     A(d.tsum, ntiles) A(d.toff, ntiles);
-    d.hcap = next_pow2(std::max<int64_t>(4096, 8 * n + 4 * n * (int64_t)ctx->walls.size()));
+    // room for every stored interaction: pairs (a few per particle) and the
+    // particle-wall slots
+    d.hcap = next_pow2(std::max<int64_t>(4096, 8 * n + 4 * n * (int64_t)ctx->wall_slots));
     A(d.hkey, d.hcap) A(d.hval, d.hcap) A(d.hmatched, d.hcap);
     d.ocap = (int)std::min<int64_t>(std::max<int64_t>(1024, n), 1 << 22);
     A(d.okey[0], d.ocap) A(d.okey[1], d.ocap) A(d.orec[0], d.ocap) A(d.orec[1], d.ocap);
@@ -2101,6 +2372,7 @@ int alloc_particles(dem_ctx* ctx, int64_t n)
     HIPCHK(hipMemset(d.tsum, 0, sizeof(uint32_t) * ntiles));
     d.sp = ctx->d_sp;
     d.walls = ctx->d_walls;
+    d.wgrp = ctx->d_wgrp;
     d.flags = ctx->d_flags;
     return DEM_OK;
 }
@@ -2125,7 +2397,11 @@ void launch_rebuild(dem_ctx* ctx)
     hipStream_t s = ctx->stream;
     ctx->hash_pending = false;
     const Params& p = ctx->prm;
-    const dim3 g(kRebuildBlocks), b(kBlock);
+    // grid-stride kernels: as many blocks as particles need, at most
+    // kRebuildBlocks (a launch with no rebuild pending costs about the time
+    // to dispatch its blocks)
+    const dim3 g(std::min<unsigned>(kRebuildBlocks, xcd_grid(grid_for((uint64_t)std::max<int64_t>(ctx->n, 1))))),
+        b(kBlock);
     hipLaunchKernelGGL(k_rb_begin<NS>, g, b, 0, s, d, p, ctx->cur);
     launch_scan<NS>(ctx);
     hipLaunchKernelGGL(k_scatter<NS>, g, b, 0, s, d, p);
@@ -2361,9 +2637,7 @@ int upload_empty(dem_ctx* ctx)
     int r;
     if ((r = dalloc(ctx, &ctx->d_flags, F_NUM)) != 0) return r;
     if ((r = dalloc(ctx, &ctx->d_sp, ctx->species.size())) != 0) return r;
-    ctx->wall_slots = (int)std::max<size_t>(ctx->walls.size(), 1);
-    p.wstride = ctx->wall_slots;
-    if ((r = dalloc(ctx, &ctx->d_walls, (size_t)ctx->wall_slots)) != 0) return r;
+    if ((r = setup_walls(ctx, p)) != 0) return r;
     HIPCHK(hipMemcpy(ctx->d_sp, ctx->species.data(), sizeof(dem_pair_params) * ctx->species.size(),
                      hipMemcpyHostToDevice));
     if ((r = copy_walls(ctx)) != 0) return r;
@@ -2372,6 +2646,7 @@ int upload_empty(dem_ctx* ctx)
     ctx->d9 = Dev<9>{};
     ctx->d3.sp = ctx->d9.sp = ctx->d_sp;
     ctx->d3.walls = ctx->d9.walls = ctx->d_walls;
+    ctx->d3.wgrp = ctx->d9.wgrp = ctx->d_wgrp;
     ctx->d3.flags = ctx->d9.flags = ctx->d_flags;
     ctx->n = 0;
     ctx->nbuckets = 0;
@@ -2503,9 +2778,12 @@ int dem_create(dem_ctx** out, const dem_config* cfg)
         ctx->skin_cfg = cfg->skin;
         ctx->levels_cfg = cfg->grid_levels;
         ctx->precision = cfg->precision;
+        ctx->sweep_cfg = cfg->list_sweep;
     }
     if (ctx->precision != DEM_PRECISION_FP64 && ctx->precision != DEM_PRECISION_MIXED)
         return set_err(ctx, DEM_ERR_ARG, "unknown precision mode");
+    if (ctx->sweep_cfg != DEM_SWEEP_PER_PARTICLE && ctx->sweep_cfg != DEM_SWEEP_WAVEFRONT)
+        return set_err(ctx, DEM_ERR_ARG, "unknown list sweep");
     if (ctx->maxn_cfg > 64) return set_err(ctx, DEM_ERR_ARG, "max_neighbors must be <= 64");
     if (ctx->levels_cfg > kMaxLevels) return set_err(ctx, DEM_ERR_ARG, "grid_levels must be <= 4");
     int ndev = 0;
@@ -2564,7 +2842,8 @@ int dem_set_species(dem_ctx* ctx, int32_t n, const dem_pair_params* table)
 
 int dem_set_walls(dem_ctx* ctx, int32_t n, const dem_wall* walls)
 {
-    if (!ctx || n < 0 || n > kMaxWalls || (n > 0 && !walls)) return set_err(ctx, DEM_ERR_ARG, "set_walls: bad arguments (at most 32 walls)");
+    if (!ctx || n < 0 || n > kMaxWallsGrouped || (n > 0 && !walls))
+        return set_err(ctx, DEM_ERR_ARG, "set_walls: bad arguments");
     for (int k = 0; k < n; ++k) {
         const dem_wall& w = walls[k];
         if (w.type != DEM_WALL_INFINITE && w.type != DEM_WALL_CYLINDER && w.type != DEM_WALL_AXISYMMETRIC)
@@ -2591,6 +2870,8 @@ int dem_set_walls(dem_ctx* ctx, int32_t n, const dem_wall* walls)
     // during a run: replace the walls between steps; a wall that keeps its
     // index and id keeps its interactions
     if (ctx->in_step) return set_err(ctx, DEM_ERR_STATE, "set_walls inside dem_step_begin/dem_step_end");
+    if (!ctx->wgrp.empty())
+        return set_err(ctx, DEM_ERR_STATE, "set_walls during a run with wall groups: drop walls with dem_set_wall_groups");
     if (n > ctx->wall_slots)
         return set_err(ctx, DEM_ERR_CAPACITY, "set_walls: more walls than at the upload of the particles");
     uint32_t keep = 0;
@@ -2609,6 +2890,44 @@ int dem_set_walls(dem_ctx* ctx, int32_t n, const dem_wall* walls)
         else
             hipLaunchKernelGGL(k_keep_walls<3>, dim3(grid_for(ctx->n)), dim3(kBlock), 0, ctx->stream, ctx->d3, ctx->prm,
                                keep);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+    }
+    return DEM_OK;
+}
+
+int dem_set_wall_groups(dem_ctx* ctx, int32_t n, const int32_t* first, const int32_t* count)
+{
+    if (!ctx || n <= 0 || !first || !count) return set_err(ctx, DEM_ERR_ARG, "set_wall_groups: bad arguments");
+    if (n != ctx->ns) return set_err(ctx, DEM_ERR_ARG, "set_wall_groups: one group per species");
+    for (int s = 0; s < n; ++s)
+        if (first[s] < 0 || count[s] < 0 || count[s] > kMaxWalls || first[s] + count[s] > (int)ctx->walls.size())
+            return set_err(ctx, DEM_ERR_ARG, "set_wall_groups: a group must lie within the walls and hold at most 32");
+    std::vector<int2> g(n);
+    for (int s = 0; s < n; ++s) g[s] = make_int2(first[s], count[s]);
+    if (!ctx->uploaded) {
+        ctx->wgrp = g;
+        return DEM_OK;
+    }
+    // during a run: a group may only drop its last walls (their interactions
+    // are deleted)
+    if (ctx->in_step) return set_err(ctx, DEM_ERR_STATE, "set_wall_groups inside dem_step_begin/dem_step_end");
+    if (ctx->wgrp.empty())
+        return set_err(ctx, DEM_ERR_STATE, "wall groups must be set before the particles are uploaded");
+    for (int s = 0; s < n; ++s)
+        if (g[s].x != ctx->wgrp[s].x || g[s].y > ctx->wgrp[s].y)
+            return set_err(ctx, DEM_ERR_STATE, "set_wall_groups during a run: a group may only drop its last walls");
+    ctx->wgrp = g;
+    HIPCHK(hipSetDevice(ctx->device));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    HIPCHK(hipMemcpy(ctx->d_wgrp, g.data(), sizeof(int2) * g.size(), hipMemcpyHostToDevice));
+    if (ctx->n > 0) {
+        if (ctx->ns_springs == 9)
+            hipLaunchKernelGGL(k_shrink_groups<9>, dim3(grid_for(ctx->n)), dim3(kBlock), 0, ctx->stream, ctx->d9,
+                               ctx->prm);
+        else
+            hipLaunchKernelGGL(k_shrink_groups<3>, dim3(grid_for(ctx->n)), dim3(kBlock), 0, ctx->stream, ctx->d3,
+                               ctx->prm);
         HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(ctx->stream));
     }
@@ -2736,6 +3055,7 @@ int dem_upload_particles(dem_ctx* ctx, int64_t n, const uint32_t* id, const int3
     p.nper = (int)ctx->periodic.size();
     p.rmax = rmax;
     p.skin = skin;
+    p.sweep = ctx->sweep_cfg;
     p.thr2 = (0.5 * skin) * (0.5 * skin) * 0.999;
     for (int a = 0; a < 3; ++a) p.axis_pb[a] = -1;
     for (int b = 0; b < p.nper; ++b) {
@@ -2826,9 +3146,7 @@ int dem_upload_particles(dem_ctx* ctx, int64_t n, const uint32_t* id, const int3
     int r;
     if ((r = dalloc(ctx, &ctx->d_flags, F_NUM)) != 0) return r;
     if ((r = dalloc(ctx, &ctx->d_sp, ctx->species.size())) != 0) return r;
-    ctx->wall_slots = (int)std::max<size_t>(ctx->walls.size(), 1);
-    p.wstride = ctx->wall_slots;
-    if ((r = dalloc(ctx, &ctx->d_walls, (size_t)ctx->wall_slots)) != 0) return r;
+    if ((r = setup_walls(ctx, p)) != 0) return r;
     HIPCHK(hipMemcpy(ctx->d_sp, ctx->species.data(), sizeof(dem_pair_params) * ctx->species.size(),
                      hipMemcpyHostToDevice));
     if ((r = copy_walls(ctx)) != 0) return r;
@@ -3023,6 +3341,31 @@ int dem_cg_z(dem_ctx* ctx, const dem_cg_config* cfg, double* out)
     if (ctx->fc_pass == 0) return set_err(ctx, DEM_ERR_STATE, "dem_cg_z before the first force pass");
     HIPCHK(hipSetDevice(ctx->device));
     return ctx->ns_springs == 9 ? run_cg_z<9>(ctx, cfg, out) : run_cg_z<3>(ctx, cfg, out);
+}
+
+int dem_particle_energies(dem_ctx* ctx, double* out)
+{
+    int r = ready(ctx);
+    if (r) return r;
+    if (!out) return set_err(ctx, DEM_ERR_ARG, "particle_energies: null output");
+    if (ctx->in_step) return set_err(ctx, DEM_ERR_STATE, "particle_energies inside dem_step_begin/dem_step_end");
+    if (ctx->n == 0) return DEM_OK;
+    if (ctx->fc_pass == 0) return set_err(ctx, DEM_ERR_STATE, "particle_energies before the first force pass");
+    HIPCHK(hipSetDevice(ctx->device));
+    const size_t bytes = sizeof(double) * 3 * (size_t)ctx->n;
+    // the gather buffer t_frc holds 4 doubles per particle and nothing
+    // between steps: it stages the 3 values per particle
+    double* stage = (double*)(ctx->ns_springs == 9 ? ctx->d9.t_frc : ctx->d3.t_frc);
+    if (ctx->ns_springs == 9)
+        hipLaunchKernelGGL(k_particle_energies<9>, dim3(grid_for(ctx->n)), dim3(kBlock), 0, ctx->stream, ctx->d9,
+                           ctx->prm, ctx->cur, stage);
+    else
+        hipLaunchKernelGGL(k_particle_energies<3>, dim3(grid_for(ctx->n)), dim3(kBlock), 0, ctx->stream, ctx->d3,
+                           ctx->prm, ctx->cur, stage);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(out, stage, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return DEM_OK;
 }
 
 int dem_set_kinematics(dem_ctx* ctx, int64_t n, const double* x, const double* v, const double* w, const double* q)

@@ -188,4 +188,43 @@ __device__ inline void contact_force(const dem_pair_params& s, const Body& P, co
     }
 }
 
+// BaseInteraction::getElasticEnergy of one interaction at this overlap:
+// the normal law's (LinearViscoelasticInteraction.cc:140-148,
+// HertzianViscoelasticInteraction.cc:148-159) plus the friction law's spring
+// energies (SlidingFrictionInteraction.cc:241-248, FrictionInteraction.cc:
+// 237-249, MindlinInteraction.cc:357-363).  eff_r / eff_m: effective radius
+// and mass of the pair (for a wall: the particle's).
+template <int NS>
+__device__ inline double elastic_energy(const dem_pair_params& s, double overlap, double eff_r, double eff_m,
+                                        const double* spring)
+{
+    double e = 0.0;
+    if (overlap > 0) {
+        if (s.normal_law == DEM_NORMAL_LINEAR_VISCOELASTIC) {
+            e = 0.5 * (s.stiffness * (overlap * overlap));
+            if (s.constant_restitution) e *= 2.0 * eff_m;
+        } else {
+            e = 8. / 15. * s.elastic_modulus * sqrt(eff_r * overlap) * (overlap * overlap);
+        }
+    }
+    const double xi2 = spring[0] * spring[0] + spring[1] * spring[1] + spring[2] * spring[2];
+    if (s.friction_law == DEM_FRICTION_MINDLIN) {
+        const double kt = 8.0 * s.shear_modulus * sqrt(eff_r * overlap);
+        e += overlap > 0 ? 0.5 * kt * xi2 : 0.0;
+    } else if (s.friction_law == DEM_FRICTION_SLIDING || s.friction_law == DEM_FRICTION_FULL) {
+        e += s.constant_restitution ? eff_m * s.sliding_stiffness * xi2 : 0.5 * s.sliding_stiffness * xi2;
+        if constexpr (NS == 9) {
+            if (s.friction_law == DEM_FRICTION_FULL) {
+                const double r2 = spring[3] * spring[3] + spring[4] * spring[4] + spring[5] * spring[5];
+                const double t2 = spring[6] * spring[6] + spring[7] * spring[7] + spring[8] * spring[8];
+                if (s.constant_restitution)
+                    e += eff_m * s.rolling_stiffness * r2 + eff_m * s.torsion_stiffness * t2;
+                else
+                    e += 0.5 * s.rolling_stiffness * r2 + 0.5 * s.torsion_stiffness * t2;
+            }
+        }
+    }
+    return e;
+}
+
 }  // namespace dem

@@ -74,7 +74,12 @@ class Periodic(C.Structure):
 
 class Config(C.Structure):
     _fields_ = [("device", C.c_int32), ("max_neighbors", C.c_int32), ("skin", C.c_double),
-                ("grid_levels", C.c_int32), ("precision", C.c_int32)]
+                ("grid_levels", C.c_int32), ("precision", C.c_int32), ("list_sweep", C.c_int32),
+                ("reserved", C.c_int32)]
+
+
+SWEEP_PER_PARTICLE = 0
+SWEEP_WAVEFRONT = 1
 
 
 # candidates per call of dem_check_particles_static (DEM_CHECK_STATIC_MAX)
@@ -132,6 +137,7 @@ def _declare(lib, prefix: str, has_config: bool):
     f("last_error", C.c_char_p, _PTR)
     f("set_species", _I32, _PTR, _I32, _PTR)
     f("set_walls", _I32, _PTR, _I32, _PTR)
+    f("set_wall_groups", _I32, _PTR, _I32, _PTR, _PTR)
     f("set_periodic", _I32, _PTR, _I32, _PTR)
     f("set_gravity", _I32, _PTR, _PTR)
     f("set_background_drag", _I32, _PTR, C.c_double)
@@ -156,6 +162,8 @@ def _declare(lib, prefix: str, has_config: bool):
     f("contact_count", _I32, _PTR, C.POINTER(_I64))
     f("download_contacts", _I32, _PTR, _I64, C.POINTER(_I64), _PTR, _PTR, _PTR, _PTR)
     f("download_interactions", _I32, _PTR, _I64, C.POINTER(_I64), _PTR)
+    if hasattr(lib, prefix + "particle_energies"):  # the HIP library only
+        f("particle_energies", _I32, _PTR, _PTR)
 
 
 _hip_lib = None
@@ -228,13 +236,13 @@ class Engine:
     """One simulation context behind the C ABI."""
 
     def __init__(self, lib=None, prefix: str = "dem_", device: int = 0, max_neighbors: int = 0,
-                 skin: float = 0.0, grid_levels: int = 0, precision: int = 0):
+                 skin: float = 0.0, grid_levels: int = 0, precision: int = 0, list_sweep: int = 0):
         self.lib = lib if lib is not None else load_hip_library()
         self.prefix = prefix
         self._has_stats = hasattr(self.lib, prefix + "get_stats") and prefix == "dem_"
         h = _PTR()
         if prefix == "dem_":
-            cfg = Config(device, max_neighbors, skin, grid_levels, precision)
+            cfg = Config(device, max_neighbors, skin, grid_levels, precision, list_sweep, 0)
             rc = self._fn("create")(C.byref(h), C.byref(cfg))
         else:
             rc = self._fn("create")(C.byref(h))
@@ -270,12 +278,19 @@ class Engine:
 
     # ----------------------------------------------------------------- setup
     def set_species(self, table: Sequence[PairParams], n_species: int):
-        arr = (PairParams * len(table))(*table)
+        arr = table if isinstance(table, C.Array) else (PairParams * len(table))(*table)
         self._check(self._fn("set_species")(self.h, n_species, C.cast(arr, C.c_void_p)))
 
     def set_walls(self, walls: Sequence[Wall]):
         arr = (Wall * max(1, len(walls)))(*walls)
         self._check(self._fn("set_walls")(self.h, len(walls), C.cast(arr, C.c_void_p)))
+
+    def set_wall_groups(self, first, count):
+        """Per-species wall groups: particles of species s meet walls
+        first[s] .. first[s] + count[s] - 1 only (dem_set_wall_groups)."""
+        f = np.ascontiguousarray(first, np.int32)
+        c = np.ascontiguousarray(count, np.int32)
+        self._check(self._fn("set_wall_groups")(self.h, len(f), _ptr(f), _ptr(c)))
 
     def set_periodic(self, bnds: Sequence[Periodic]):
         arr = (Periodic * max(1, len(bnds)))(*bnds)
@@ -401,6 +416,13 @@ class Engine:
         self._check(self._fn("download_particles")(self.h, _ptr(x), _ptr(v), _ptr(w), _ptr(q),
                                                    _ptr(f), _ptr(t)))
         return dict(x=x, v=v, w=w, q=q, f=f, t=t)
+
+    def particle_energies(self) -> np.ndarray:
+        """(n, 3) in handler order: kinetic energy, elastic energy of the
+        interactions held as P, height z (dem_particle_energies)."""
+        out = np.empty((self.n, 3))
+        self._check(self._fn("particle_energies")(self.h, _ptr(out)))
+        return out
 
     def contact_count(self) -> int:
         n = _I64()

@@ -33,12 +33,19 @@ class Scenario:
     rotation: Optional[bool] = None
     mixed_overrides: Optional[dict] = None
     info: dict = field(default_factory=dict)
+    # per-species wall groups (first wall, number of walls), or None: every
+    # particle meets every wall
+    wall_groups: Optional[tuple] = None
+    # a ready species table (dem.PairParams array) in place of table()
+    pair_table: Optional[object] = None
 
     @property
     def n_species(self) -> int:
         return len(self.species)
 
     def table(self):
+        if self.pair_table is not None:
+            return self.pair_table
         return species_table(self.species, self.mixed_overrides)
 
     def uses_rotation(self) -> bool:
@@ -60,6 +67,8 @@ class Scenario:
 def apply(engine: dem.Engine, sc: Scenario, contacts: Optional[dem.Contacts] = None):
     engine.set_species(sc.table(), sc.n_species)
     engine.set_walls(sc.walls)
+    if sc.wall_groups is not None:
+        engine.set_wall_groups(*sc.wall_groups)
     engine.set_periodic(sc.periodic)
     engine.set_gravity(sc.gravity)
     engine.set_background_drag(sc.drag)

@@ -16,6 +16,10 @@ int dem_destroy(dem_ctx* ctx) { return oracle_dem_destroy(O(ctx)); }
 const char* dem_last_error(const dem_ctx* ctx) { return ctx ? oracle_dem_last_error((const oracle_ctx*)ctx) : ""; }
 int dem_set_species(dem_ctx* ctx, int32_t n, const dem_pair_params* t) { return oracle_dem_set_species(O(ctx), n, t); }
 int dem_set_walls(dem_ctx* ctx, int32_t n, const dem_wall* w) { return oracle_dem_set_walls(O(ctx), n, w); }
+int dem_set_wall_groups(dem_ctx* ctx, int32_t n, const int32_t* first, const int32_t* count)
+{
+    return oracle_dem_set_wall_groups(O(ctx), n, first, count);
+}
 int dem_set_periodic(dem_ctx* ctx, int32_t n, const dem_periodic* b) { return oracle_dem_set_periodic(O(ctx), n, b); }
 int dem_set_gravity(dem_ctx* ctx, const double g[3]) { return oracle_dem_set_gravity(O(ctx), g); }
 int dem_set_background_drag(dem_ctx* ctx, double d) { return oracle_dem_set_background_drag(O(ctx), d); }

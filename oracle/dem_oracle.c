@@ -103,6 +103,9 @@ struct oracle_ctx {
     dem_pair_params* species;
     int nWalls;
     dem_wall* walls;
+    /* per species: the walls its particles meet, wgFirst .. wgFirst + wgCount - 1
+     * (NULL: every wall; the engine's dem_set_wall_groups) */
+    int32_t *wgFirst, *wgCount;
     struct AxiGeo* wallGeo; /* per wall: what IntersectionOfWalls::setPointsAndLines derives */
     int nPeriodic;
     dem_periodic periodic[3];
@@ -128,6 +131,14 @@ struct oracle_ctx {
     int64_t* cell; /* 3 ints per particle, packed */
     int64_t nBuckets;
 };
+
+/* whether particles of this species meet wall w (wall groups) */
+static int wallMeets(const oracle_ctx* c, int32_t species, int32_t w)
+{
+    if (!c->wgFirst) return 1;
+    return w >= c->wgFirst[species] && w < c->wgFirst[species] + c->wgCount[species];
+}
+
 
 static int fail(oracle_ctx* c, int code, const char* msg)
 {
@@ -1017,7 +1028,7 @@ static void computeAllForcesOmp(oracle_ctx* c)
         for (int32_t w = 0; w < c->nWalls; ++w) {
 #pragma omp for schedule(dynamic, 1024)
             for (int64_t i = 0; i < nP; ++i)
-                if (!isFixed(&c->p[i])) evalWall(c, i, w, nl);
+                if (!isFixed(&c->p[i]) && wallMeets(c, c->p[i].species, w)) evalWall(c, i, w, nl);
         }
     }
     /* InteractionHandler::addNewObjectsOMP: the new interactions join the
@@ -1181,7 +1192,8 @@ static void computeAllForces(oracle_ctx* c)
     pairLoop(c);
     for (int32_t w = 0; w < c->nWalls; ++w)
         for (int64_t i = 0; i < c->nP; ++i)
-            if (!isFixed(&c->p[i]) && c->p[i].from < 0) computeForcesDueToWall(c, i, w);
+            if (!isFixed(&c->p[i]) && c->p[i].from < 0 && wallMeets(c, c->p[i].species, w))
+                computeForcesDueToWall(c, i, w);
 #endif
 }
 
@@ -1273,6 +1285,8 @@ int oracle_dem_destroy(oracle_ctx* c)
     free(c->species);
     free(c->walls);
     free(c->wallGeo);
+    free(c->wgFirst);
+    free(c->wgCount);
     free(c->head);
     free(c->next);
     free(c->cell);
@@ -1322,6 +1336,28 @@ int oracle_dem_set_walls(oracle_ctx* c, int32_t n, const dem_wall* w)
     for (int64_t k = 0; k < c->nIn; ++k)
         if (c->in[k].alive && c->in[k].IisWall) c->in[k].sp = mixed(c, c->p[c->in[k].P].species, c->walls[c->in[k].I].species);
     c->nWalls = n;
+    return DEM_OK;
+}
+
+/* The engine's per-species wall groups (dem_set_wall_groups): particles of
+ * species s meet walls first[s] .. first[s] + count[s] - 1; the interactions
+ * with walls a particle no longer meets are deleted. */
+int oracle_dem_set_wall_groups(oracle_ctx* c, int32_t n, const int32_t* first, const int32_t* count)
+{
+    if (!c || n <= 0 || !first || !count || n != c->nSpecies) return fail(c, DEM_ERR_ARG, "set_wall_groups: bad arguments");
+    for (int32_t s = 0; s < n; ++s)
+        if (first[s] < 0 || count[s] < 0 || count[s] > 32 || first[s] + count[s] > c->nWalls)
+            return fail(c, DEM_ERR_ARG, "set_wall_groups: a group must lie within the walls and hold at most 32");
+    free(c->wgFirst);
+    free(c->wgCount);
+    c->wgFirst = (int32_t*)malloc(sizeof(int32_t) * (size_t)n);
+    c->wgCount = (int32_t*)malloc(sizeof(int32_t) * (size_t)n);
+    memcpy(c->wgFirst, first, sizeof(int32_t) * (size_t)n);
+    memcpy(c->wgCount, count, sizeof(int32_t) * (size_t)n);
+    for (int64_t k = 0; k < c->nIn; ++k) {
+        const Inter* x = &c->in[k];
+        if (x->alive && x->IisWall && !wallMeets(c, c->p[x->P].species, x->I)) interRemove(c, (int32_t)k);
+    }
     return DEM_OK;
 }
 
@@ -1621,6 +1657,7 @@ static int insertableLocal(const oracle_ctx* c, V3 pos, double radius, int32_t s
     cand.radius = radius;
     cand.species = species;
     for (int32_t w = 0; w < c->nWalls; ++w) {
+        if (!wallMeets(c, species, w)) continue;
         double distance;
         V3 normal;
         const double wallRadius = radius + mixed(c, species, c->walls[w].species)->interaction_distance;

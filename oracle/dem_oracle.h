@@ -19,6 +19,7 @@ int oracle_dem_destroy(oracle_ctx* c);
 const char* oracle_dem_last_error(const oracle_ctx* c);
 int oracle_dem_set_species(oracle_ctx* c, int32_t n, const dem_pair_params* table);
 int oracle_dem_set_walls(oracle_ctx* c, int32_t n, const dem_wall* walls);
+int oracle_dem_set_wall_groups(oracle_ctx* c, int32_t n, const int32_t* first, const int32_t* count);
 int oracle_dem_set_periodic(oracle_ctx* c, int32_t n, const dem_periodic* b);
 int oracle_dem_set_gravity(oracle_ctx* c, const double g[3]);
 int oracle_dem_set_background_drag(oracle_ctx* c, double drag);

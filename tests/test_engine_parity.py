@@ -147,6 +147,33 @@ def test_many_steps_with_frequent_rebuilds():
     assert g.stats().n_rebuilds > 100
 
 
+@pytest.mark.parametrize("case", CASES, ids=lambda c: f"{c['laws']}-{'P' if c['periodic'][0] else 'np'}-{len(c['walls'])}w"
+                         + ("-3lv" if "radii" in c else ""))
+def test_wavefront_list_sweep_equals_per_particle(case):
+    """The Verlet lists built by the wavefronts' shared LDS sweep
+    (DEM_SWEEP_WAVEFRONT) are the per-particle sweep's, entry for entry: 200
+    steps with a small skin (a rebuild every few steps) give bit-identical
+    states and springs either way; the first pass's contacts equal the
+    oracle's."""
+    sc = S.mixture(n=600, seed=23, lx=8.0, ly=8.0, lz=8.0, inflate=1.12, vmax=1.0, **case)
+    outs = []
+    for sweep in (dem.SWEEP_PER_PARTICLE, dem.SWEEP_WAVEFRONT):
+        g = dem.Engine(skin=0.01, list_sweep=sweep)
+        S.apply(g, sc)
+        g.compute_forces()
+        if sweep == dem.SWEEP_WAVEFRONT:
+            o = oracle_lib.engine()
+            S.apply(o, sc)
+            o.compute_forces()
+            assert_contacts_equal(g, o)
+        g.step(200)
+        assert g.stats().n_rebuilds > 10
+        outs.append((g.download(), g.contacts()))
+    for k in ("x", "v", "w", "f", "t"):
+        np.testing.assert_array_equal(outs[0][0][k], outs[1][0][k])
+    np.testing.assert_array_equal(outs[0][1].spring, outs[1][1].spring)
+
+
 def test_deterministic():
     sc = S.mixture(n=400, seed=9, laws="hvm", periodic=(True, True), walls=("bottom",))
     outs = []

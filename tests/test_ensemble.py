@@ -76,3 +76,138 @@ def test_ensemble_equals_members_on_engine():
         own = {(int(a) - lo, int(b) if b & dem.DEM_WALL_KEY else int(b) - lo)
                for a, b in zip(cp.p_id, cp.i_id) if lo <= a < hi}
         assert own == e.contacts().keys()
+
+
+# ---------------------------------------------------------------------------
+# Calibration heaps with per-simulation parameters and walls
+# (Drivers/Calibration/CalibrationHeap.cpp)
+# ---------------------------------------------------------------------------
+
+def heap_members(sizes, seed=5):
+    rng = np.random.default_rng(seed)
+    ps = [E.HeapParams(int(n), float(rng.uniform(0.1, 0.8)), float(rng.uniform(0.0, 0.1)),
+                       float(rng.uniform(0.5, 0.95)), int(seed * 1000 + k)) for k, n in enumerate(sizes)]
+    return [E.calibration_heap(p) for p in ps]
+
+
+def test_heap_packing_and_wall_groups():
+    """Each heap: spheres inside its casing, above its base, not overlapping;
+    the packed scenario gives heap k species 2k (casing 2k + 1), walls 2k and
+    2k + 1 at its centre, and the wall group (2k, 2) to its particles."""
+    ms = heap_members([120, 300, 80])
+    for sc in ms:
+        x, r, R = sc.state.x, sc.state.radius, sc.info["base_radius"]
+        d = np.sqrt(((x[:, None] - x[None]) ** 2).sum(-1))
+        np.fill_diagonal(d, np.inf)
+        assert np.all(d >= r[:, None] + r[None] - 1e-12)
+        assert np.all(np.hypot(x[:, 0], x[:, 1]) + r <= R + 1e-12) and np.all(x[:, 2] >= r - 1e-12)
+    for schedule in ("grid", "shelf"):
+        ens = E.pack_heaps(ms, schedule=schedule)
+        sc = ens.scenario
+        first, count = sc.wall_groups
+        assert list(first) == [0, 0, 2, 2, 4, 4] and list(count) == [2, 0, 2, 0, 2, 0]
+        for k in range(3):
+            lo, hi = ens.id_first[k], ens.id_first[k + 1]
+            assert np.all(sc.state.species[lo:hi] == 2 * k)
+            for w in (sc.walls[2 * k], sc.walls[2 * k + 1]):
+                assert (w.position[0], w.position[1]) == (ens.offsets[k, 0], ens.offsets[k, 1])
+            assert sc.walls[2 * k].species == 2 * k and sc.walls[2 * k + 1].species == 2 * k + 1
+        # bases at least the gap apart
+        for a in range(3):
+            for b in range(a + 1, 3):
+                gap = np.hypot(*(ens.offsets[a, :2] - ens.offsets[b, :2])) - ens.base_radius[a] - ens.base_radius[b]
+                assert gap >= 8.0 * 0.6 - 1e-9
+        # the table: each heap's own 2 x 2 block
+        ns = 6
+        for k in range(3):
+            own = S.species_table(ms[k].species)
+            for a in range(2):
+                for b in range(2):
+                    assert bytes(sc.table()[(2 * k + a) * ns + 2 * k + b]) == bytes(own[2 * a + b])
+
+
+def _heap_run(sc, steps, engine, lift_at=None, first_count=None):
+    """steps steps; at lift_at, the casing goes (alone: removeLastObject;
+    packed: the given wall groups)."""
+    e = engine()
+    S.apply(e, sc)
+    e.compute_forces()
+    if lift_at is None:
+        e.step(steps)
+        return e
+    e.step(lift_at)
+    if first_count is None:
+        e.set_walls(sc.walls[:1])
+    else:
+        e.set_wall_groups(*first_count)
+    e.step(steps - lift_at)
+    return e
+
+
+def _lifted(ens, which):
+    first, count = (np.array(a) for a in ens.scenario.wall_groups)
+    count[2 * np.asarray(which)] = 1
+    return first, count
+
+
+def test_heap_members_on_oracle():
+    """On the oracle: three heaps with their own parameters packed (wall
+    groups), 2500 steps (the fill falls about 3 diameters), the first heap's
+    casing lifted at step 1500: every heap's positions equal its own run's
+    (alone, WallHandler::removeLastObject at the same step) to round-off
+    (the tile offset rounds every position, and 2500 steps of collisions
+    amplify it to a few 1e-10)."""
+    ms = heap_members([150, 260, 200], seed=7)
+    ens = E.pack_heaps(ms)
+    e = _heap_run(ens.scenario, 2500, oracle_lib.engine, 1500, _lifted(ens, [0]))
+    x = e.download()["x"]
+    assert any(b & dem.DEM_WALL_KEY for b in e.contacts().i_id)
+    for k, m in enumerate(ms):
+        xk = _heap_run(m, 2500, oracle_lib.engine, 1500 if k == 0 else None).download()["x"]
+        lo, hi = ens.id_first[k], ens.id_first[k + 1]
+        np.testing.assert_allclose(x[lo:hi] - ens.offsets[k], xk, rtol=0, atol=1e-8)
+
+
+@pytest.mark.gpu
+def test_heap_members_engine_vs_oracle():
+    """On the engine: four heaps of 300 to 1500 spheres with their own
+    sliding/rolling friction and restitution packed with wall groups, 2000
+    steps, casings of heaps 1 and 3 lifted at step 1000: each heap's contacts
+    (ids mapped back) equal those of its own run on the oracle, positions to
+    round-off of differently ordered force sums and of the tile offset,
+    amplified by 2000 steps of dense collisions (measured up to 1e-8)."""
+    ms = heap_members([300, 1500, 700, 450], seed=9)
+    ens = E.pack_heaps(ms)
+    g = _heap_run(ens.scenario, 2000, dem.Engine, 1000, _lifted(ens, [1, 3]))
+    x = g.download()["x"]
+    cp = g.contacts()
+    for k, m in enumerate(ms):
+        o = _heap_run(m, 2000, oracle_lib.engine, 1000 if k in (1, 3) else None)
+        lo, hi = ens.id_first[k], ens.id_first[k + 1]
+        np.testing.assert_allclose(x[lo:hi] - ens.offsets[k], o.download()["x"], rtol=0, atol=1e-7)
+        own = {(int(a) - lo, int(b) - 2 * k if b & dem.DEM_WALL_KEY else int(b) - lo)
+               for a, b in zip(cp.p_id, cp.i_id) if lo <= a < hi}
+        want = o.contacts().keys()
+        assert own == want, (k, len(own ^ want))
+        assert any(b & dem.DEM_WALL_KEY for _, b in want)
+
+
+@pytest.mark.gpu
+def test_heap_protocol_runs_to_angles_of_repose():
+    """The CalibrationHeap protocol on six packed heaps (500 to 2000 spheres)
+    for 30,000 steps: every casing is lifted once its heap has settled,
+    spheres that fall off the base are deleted, and the angles of repose of
+    the heaps (after twice the fill time once settled, else at the end) lie
+    between 5 and 60 degrees."""
+    ps = E.draw_heap_params(6, 3, n_range=(500, 2000))
+    ens = E.pack_heaps([E.calibration_heap(p) for p in ps])
+    g = dem.Engine()
+    S.apply(g, ens.scenario)
+    g.compute_forces()
+    run = E.run_heap_protocol(g, ens, 30000)
+    assert np.all(np.isfinite(run.fill_time)), run.fill_time
+    assert run.removed > 0
+    assert np.all((run.angle_at_end > 5) & (run.angle_at_end < 60)), run.angle_at_end
+    ok = np.isfinite(run.angle_of_repose)
+    assert ok.sum() >= 1, run.angle_of_repose
+    assert np.all((run.angle_of_repose[ok] > 5) & (run.angle_of_repose[ok] < 60)), run.angle_of_repose

@@ -1,0 +1,92 @@
+"""Mixed precision (FP32 contact geometry and pair forces, FP64 positions,
+velocities, springs and sums) on its own workload: the mixed-precision
+variant of the 5000-sphere box (LinearViscoelasticFriction e = 0.88, mu = 0.5,
+periodic in x and y, floor, gravity -z).  GPU.
+
+* forces: at ten states of the 20,000-step FP64 run, both precisions compute
+  the forces on identical inputs (the FP64 state and stored interactions);
+  the per-particle error relative to the sum of the magnitudes of the
+  particle's contact forces (mean and max) and the error of the whole force
+  field relative to its norm are stated and bounded;
+* energy: FP64 and mixed runs of 50,000 steps from the same seeded start;
+  their kinetic (translational + rotational) energies every 5000 steps agree
+  within 1% of the run's largest kinetic energy, and their potential energies
+  within 1e-3 of the FP64 potential energy (the trajectories part at round-off
+  level and then diverge, as two FP64 runs with different summation orders
+  would, and the settled beds differ in detail; the macroscopic state must
+  not).  Measured: 2.2e-3 and 1.3e-4.
+"""
+import numpy as np
+import pytest
+
+from mercurydpm_fork_amd import dem
+from mercurydpm_fork_amd import scenarios as S
+from test_engine_parity import _contact_force_scale, reseeded
+
+pytestmark = pytest.mark.gpu
+
+
+def energies(e, sc):
+    st = e.download()
+    m = 1.0 / sc.state.inv_mass
+    inertia = 1.0 / sc.state.inv_inertia[:, 0]
+    ekin = 0.5 * np.sum(m * np.sum(st["v"] ** 2, axis=1)) + 0.5 * np.sum(inertia * np.sum(st["w"] ** 2, axis=1))
+    epot = -np.sum(m * (st["x"] @ np.asarray(sc.gravity)))
+    return ekin, epot
+
+
+def test_mixed_precision_force_error_on_the_box():
+    sc = S.box_lvf()
+    ref = dem.Engine()
+    S.apply(ref, sc)
+    ref.compute_forces()
+    worst_mean, worst_max, worst_rms = 0.0, 0.0, 0.0
+    for k in range(10):
+        ref.step(2000)
+        st, ct, stats = ref.download(), ref.contacts(), ref.stats()
+        runs = {}
+        for prec in (dem.PRECISION_FP64, dem.PRECISION_MIXED):
+            e = dem.Engine(precision=prec)
+            S.apply(e, reseeded(sc, st), ct)
+            e.set_time(stats.time, int(stats.n_steps))
+            e.compute_forces()
+            runs[prec] = e
+        a, b = runs[dem.PRECISION_FP64], runs[dem.PRECISION_MIXED]
+        assert a.contacts().keys() == b.contacts().keys()
+        fa, fb = a.download()["f"], b.download()["f"]
+        scale = _contact_force_scale(a, sc)
+        m = scale > 0
+        err = np.linalg.norm(fb - fa, axis=1)[m] / scale[m]
+        rms = np.linalg.norm(fb - fa) / np.linalg.norm(fa)
+        worst_mean = max(worst_mean, float(err.mean()))
+        worst_max = max(worst_max, float(err.max()))
+        worst_rms = max(worst_rms, float(rms))
+        for e in (a, b):
+            e.close()
+    print(f"mixed precision on the box: per-particle error mean {worst_mean:.3g}, max {worst_max:.3g}; "
+          f"force field {worst_rms:.3g}")
+    assert worst_rms <= 1e-5, worst_rms
+    assert worst_mean <= 1e-6, worst_mean
+    assert worst_max <= 2e-4, worst_max
+
+
+def test_mixed_precision_energy_over_50000_steps():
+    sc = S.box_lvf()
+    runs = []
+    for prec in (dem.PRECISION_FP64, dem.PRECISION_MIXED):
+        e = dem.Engine(precision=prec)
+        S.apply(e, sc)
+        e.compute_forces()
+        trace = [energies(e, sc)]
+        for _ in range(10):
+            e.step(5000)
+            trace.append(energies(e, sc))
+        runs.append(np.array(trace))
+        e.close()
+    a, b = runs
+    kmax = np.max(a[:, 0])
+    dk = np.max(np.abs(a[:, 0] - b[:, 0])) / kmax
+    dp = np.max(np.abs(a[:, 1] - b[:, 1]) / np.abs(a[:, 1]))
+    print(f"mixed precision, 50,000 steps: kinetic energy within {dk:.3g} of its peak, potential within {dp:.3g}")
+    assert dk <= 0.01, dk
+    assert dp <= 1e-3, dp
