@@ -37,7 +37,7 @@ def _run(cmd):
 
 def build_engine():
     os.makedirs(LIB, exist_ok=True)
-    srcs = [os.path.join(HERE, "csrc", f) for f in ("dem_engine.hip", "closure_mlp.hip")]
+    srcs = [os.path.join(HERE, "csrc", f) for f in ("dem_engine.hip", "closure_mlp.hip", "closure_mfma.hip")]
     out = os.path.join(LIB, "libmercurydpm_dem.so")
     tmp = out + ".tmp"
     _run([HIPCC, *HIP_FLAGS, "-shared", "-o", tmp, *srcs])

@@ -117,3 +117,95 @@ def mu_of_I(I, mu1=np.tan(np.radians(20.9)), mu2=np.tan(np.radians(32.8)), I0=0.
 def phi_of_I(I, phi_max=0.6, phi_min=0.5):
     """phi(I) = phi_max - (phi_max - phi_min) I (GDR MiDi 2004), for I < 1."""
     return phi_max - (phi_max - phi_min) * I
+
+
+# ---------------------------------------------------------------------------
+# The configured network: 4 -> 3 x 128 GELU -> 2, BF16 weights, FP32
+# accumulation, matrix-core kernels (csrc/closure_mfma.hip).
+# ---------------------------------------------------------------------------
+N_PARAMS128 = 33922
+_SHAPES128 = [(128, 4), (128,), (128, 128), (128,), (128, 128), (128,), (2, 128), (2,)]
+
+
+def _lib128():
+    lib = dem.load_hip_library()
+    if not getattr(lib, "_closure128_declared", False):
+        for name, args in (("dem_closure_create", [C.POINTER(_P), C.c_int32]), ("dem_closure_destroy", [_P]),
+                           ("dem_closure_set_params", [_P, _P]), ("dem_closure_get_params", [_P, _P]),
+                           ("dem_closure_forward", [_P, C.c_int64, _P, _P]),
+                           ("dem_closure_gradient", [_P, C.c_int64, _P, _P, _P, _P]),
+                           ("dem_closure_train", [_P, C.c_int64, _P, _P, C.c_int32, C.c_float, _P])):
+            fn = getattr(lib, name)
+            fn.restype = C.c_int32
+            fn.argtypes = args
+        lib._closure128_declared = True
+    return lib
+
+
+def glorot128(seed: int = 0) -> np.ndarray:
+    """Glorot-uniform weights and zero biases, float32, in the kernels' layout."""
+    rng = np.random.default_rng(seed)
+    parts = []
+    for shape in _SHAPES128:
+        if len(shape) == 2:
+            lim = np.sqrt(6.0 / (shape[0] + shape[1]))
+            parts.append(rng.uniform(-lim, lim, shape).ravel())
+        else:
+            parts.append(np.zeros(shape[0]))
+    return np.concatenate(parts).astype(np.float32)
+
+
+class Closure:
+    """The configured closure network on one GPU (FP32 arrays in and out)."""
+
+    def __init__(self, device: int = 0, seed: int = 0):
+        self.lib = _lib128()
+        self.h = _P()
+        self._check(self.lib.dem_closure_create(C.byref(self.h), device))
+        self.set_params(glorot128(seed))
+
+    def _check(self, rc):
+        if rc != 0:
+            raise RuntimeError(f"closure network call failed ({rc})")
+
+    def close(self):
+        if self.h:
+            self.lib.dem_closure_destroy(self.h)
+            self.h = _P()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        assert p.shape == (N_PARAMS128,)
+        self._check(self.lib.dem_closure_set_params(self.h, _ptr(p)))
+
+    def params(self):
+        p = np.empty(N_PARAMS128, np.float32)
+        self._check(self.lib.dem_closure_get_params(self.h, _ptr(p)))
+        return p
+
+    def __call__(self, x):
+        x = np.ascontiguousarray(x, np.float32).reshape(-1, 4)
+        y = np.empty((len(x), 2), np.float32)
+        self._check(self.lib.dem_closure_forward(self.h, len(x), _ptr(x), _ptr(y)))
+        return y
+
+    def gradient(self, x, t):
+        x = np.ascontiguousarray(x, np.float32).reshape(-1, 4)
+        t = np.ascontiguousarray(t, np.float32).reshape(-1, 2)
+        g = np.empty(N_PARAMS128, np.float32)
+        loss = np.zeros(1, np.float32)
+        self._check(self.lib.dem_closure_gradient(self.h, len(x), _ptr(x), _ptr(t), _ptr(g), _ptr(loss)))
+        return float(loss[0]), g
+
+    def train(self, x, t, epochs: int, lr: float = 1e-3) -> float:
+        x = np.ascontiguousarray(x, np.float32).reshape(-1, 4)
+        t = np.ascontiguousarray(t, np.float32).reshape(-1, 2)
+        loss = np.zeros(1, np.float32)
+        self._check(self.lib.dem_closure_train(self.h, len(x), _ptr(x), _ptr(t), int(epochs), float(lr), _ptr(loss)))
+        return float(loss[0])

@@ -491,6 +491,7 @@ def hertz_mindlin_species(restitution=0.8, mu=0.5, tc=0.005, density=6. / PI, po
     s.shear_modulus = E / (2.0 * (1.0 + poisson))
     s.sliding_dissipation = s.dissipation
     s.set_sliding_friction(mu)
+    s.info_tc = tc
     return s
 
 
@@ -550,24 +551,25 @@ def rocksalt_packing(d, lx, ly, z0, rng, jitter=0.004):
 
 
 def chute_hvm(n_flow=2_000_000, seed=3, seed_base=4, angle_deg=24.0, l=None, depth=30.0,
-              base_d=1.0) -> Scenario:
-    """Polydisperse periodic chute: n_flow spheres with d ~ U[0.5, 1.5],
-    tilted by angle_deg (gravity (sin, 0, -cos), |g| = 1, the SilbertPeriodic
-    convention of Chute::setChuteAngleAndMagnitudeOfGravity, Chute.cc:789-805),
-    periodic in x and y, a rough base of fixed spheres (d = base_d, square
-    lattice with random heights) over an InfiniteWall.  HertzianViscoelastic-
-    Mindlin with e = 0.8 and mu = 0.5, dt = t_c / 50.  The flowing spheres
-    start on a rock-salt lattice (loose, non-overlapping); the footprint is
-    chosen so that the bed would be `depth` diameters deep at packing
-    fraction 0.55."""
+              base_d=1.0, d_range=(0.5, 1.5), species=None) -> Scenario:
+    """Polydisperse periodic chute: n_flow spheres with d ~ U[d_range]
+    (default [0.5, 1.5]), tilted by angle_deg (gravity (sin, 0, -cos), |g| = 1,
+    the SilbertPeriodic convention of Chute::setChuteAngleAndMagnitudeOfGravity,
+    Chute.cc:789-805), periodic in x and y, a rough base of fixed spheres
+    (d = base_d, square lattice with random heights) over an InfiniteWall.
+    HertzianViscoelasticMindlin with e = 0.8 and mu = 0.5 (or the given
+    species, a hertz_mindlin_species, with its collision time), dt = t_c / 50.
+    The flowing spheres start on a rock-salt lattice (loose, non-overlapping);
+    the footprint is chosen so that the bed would be `depth` diameters deep at
+    packing fraction 0.55."""
     rng = np.random.default_rng(seed)
-    d = rng.uniform(0.5, 1.5, n_flow)
+    d = rng.uniform(d_range[0], d_range[1], n_flow)
     r_flow = 0.5 * d
     if l is None:
         vol = float(np.sum(PI / 6 * d ** 3))
         l = math.sqrt(vol / 0.55 / depth)
-    s = hertz_mindlin_species()
-    tc = 0.005
+    s = hertz_mindlin_species() if species is None else species
+    tc = s.info_tc if hasattr(s, "info_tc") else 0.005
     # rough base
     rngb = np.random.default_rng(seed_base)
     nb = int(math.floor(l / base_d))

@@ -110,3 +110,76 @@ def test_closure_trained_on_dem_profiles_predicts_held_out_angles():
     err_const = np.mean((ttr[:, 0].mean() - tte[:, 0]) ** 2)
     print(f"held-out mu error: network {err_net:.4g}, constant {err_const:.4g}")
     assert err_net < 0.8 * err_const
+
+
+# ---------------------------------------------------------------------------
+# The configured network: 4 -> 3 x 128 GELU -> 2, BF16 weights, matrix cores
+# ---------------------------------------------------------------------------
+
+def test_oracle128_gradient_against_finite_differences():
+    """The back-propagation of the 128-wide network (float64, no rounding)
+    against central finite differences, and its parameter count and
+    initialisation against the binding's."""
+    rng = np.random.default_rng(0)
+    x = rng.standard_normal((40, 4))
+    t = rng.standard_normal((40, 2))
+    p = O.init128(3).astype(np.float64)
+    _, g = O.loss_and_gradient128(p, x, t, quantize=False)
+    for k in rng.choice(O.N_PARAMS128, 60, replace=False):
+        e = np.zeros(O.N_PARAMS128)
+        e[k] = 1e-6
+        fd = (O.loss_and_gradient128(p + e, x, t, False)[0] - O.loss_and_gradient128(p - e, x, t, False)[0]) / 2e-6
+        assert abs(fd - g[k]) <= 1e-6 * max(1.0, abs(g[k])), (k, fd, g[k])
+    assert O.N_PARAMS128 == closure.N_PARAMS128
+    np.testing.assert_array_equal(O.init128(3), closure.glorot128(3))
+
+
+def test_oracle_bf16_rounding():
+    """Round to nearest even: exact values, ties to the even mantissa."""
+    v = np.array([1.0, 1.0 + 2 ** -8, 1.0 + 3 * 2 ** -9, 1.0 + 2 ** -7 + 2 ** -8, -2.5e-3, 3.0e38], np.float32)
+    # bit patterns: 1.0, tie down to even, up, tie up to even, a negative
+    # value, and 3e38 = 1.7627 x 2^127 -> 226/128 x 2^127
+    want = np.array([0x3F800000, 0x3F800000, 0x3F810000, 0x3F820000, 0xBB240000, 0x7F620000], np.uint32)
+    np.testing.assert_array_equal(O.bf16(v).view(np.uint32), want)
+
+
+@pytest.mark.gpu
+def test_closure128_kernels_against_oracle():
+    """Forward pass and gradient of the matrix-core kernels against the
+    float32 oracle with the same BF16 roundings.  The FP32 sums run in
+    another order, so now and then an activation near a BF16 rounding
+    boundary rounds the other way and moves by one BF16 step (2^-8
+    relative): 99% of the outputs agree within 1e-5 of their scale and all
+    within 1e-3 (measured: 0.6% of them beyond 1e-5, at most 5e-4); the
+    gradient within 1e-3 of its largest entry, the loss within 1e-4."""
+    x, t = samples(1000, 2)
+    net = closure.Closure(seed=5)
+    p = net.params()
+    y = net(x)
+    yo = O.forward128(p, x)[0]
+    scale = np.abs(yo).max()
+    err = np.abs(y - yo)
+    assert np.mean(err <= 1e-5 * scale) >= 0.99, np.mean(err <= 1e-5 * scale)
+    assert err.max() <= 1e-3 * scale, err.max() / scale
+    loss, g = net.gradient(x, t)
+    lo, go = O.loss_and_gradient128(p, x, t)
+    assert abs(loss - lo) <= 1e-4 * lo
+    np.testing.assert_allclose(g, go, rtol=0, atol=1e-3 * np.abs(go).max())
+    # and a sample count that is no multiple of the 16-row tiles
+    y7 = net(x[:7])
+    np.testing.assert_array_equal(y7, y[:7])
+
+
+@pytest.mark.gpu
+def test_closure128_trained_beats_constant():
+    xtr, ttr = samples(4000, 3)
+    xte, tte = samples(1000, 4)
+    net = closure.Closure(seed=7)
+    first = net.gradient(xtr, ttr)[0]
+    net.train(xtr, ttr, epochs=600, lr=1e-3)
+    last = net.gradient(xtr, ttr)[0]
+    assert last < 0.05 * first
+    err_net = np.mean((net(xte) - tte) ** 2, axis=0)
+    err_const = np.mean((ttr.mean(0) - tte) ** 2, axis=0)
+    print(f"128-wide closure, synthetic rheology: error {err_net} against constant {err_const}")
+    assert np.all(err_net < 0.05 * err_const), (err_net, err_const)
