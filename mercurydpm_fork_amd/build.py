@@ -50,7 +50,7 @@ HOST_FLAGS = ["-std=c++17", "-O2", "-fPIC", "-Wall", "-Wextra", "-ffp-contract=o
 DRIVER_DIR = os.path.join(ROOT, "tests", "drivers")
 
 
-HOST_SOURCES = [os.path.join(HERE, "host", f) for f in ("Mercury3D.cc", "DPMBaseOutput.cc", "DPMBaseRestart.cc",
+HOST_SOURCES = [os.path.join(HERE, "host", f) for f in ("Mercury3D.cc", "DPMBaseOutput.cc", "DPMBaseRestart.cc", "Logger.cc",
                                                                 "InsertionBoundary.cc", "Chute.cc")]
 
 

@@ -63,13 +63,17 @@ __global__ void __launch_bounds__(kBlock) k_mlp_forward(const double* __restrict
 
 // Gradient of L = (1/n) sum_s sum_k (y_k - t_k)^2 with respect to every
 // parameter.  Each thread back-propagates its sample; the block sums the
-// per-sample gradients parameter by parameter through LDS and adds them to
-// the global gradient with one atomic per parameter and block.
+// per-sample gradients parameter by parameter through LDS (a fixed tree) and
+// writes its partial sums, kP gradient entries and then the loss, to row
+// blockIdx.x of `part`; k_mlp_sum_blocks adds the rows in block order, so the
+// gradient is the same bits on every run.
+constexpr int kRow = kP + 1;
+
 __global__ void __launch_bounds__(kBlock) k_mlp_gradient(const double* __restrict__ p, int n,
                                                           const double* __restrict__ x,
-                                                          const double* __restrict__ t, double* __restrict__ grad,
-                                                          double* __restrict__ loss)
+                                                          const double* __restrict__ t, double* __restrict__ part)
 {
+    double* row = part + (size_t)blockIdx.x * kRow;
     __shared__ double red[kBlock];
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     const bool live = s < n;
@@ -108,7 +112,7 @@ __global__ void __launch_bounds__(kBlock) k_mlp_gradient(const double* __restric
             if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
             __syncthreads();
         }
-        if (threadIdx.x == 0 && red[0] != 0.0) atomicAdd(&grad[index], red[0]);
+        if (threadIdx.x == 0) row[index] = red[0];
         __syncthreads();
     };
     for (int j = 0; j < kH; ++j) {
@@ -129,7 +133,22 @@ __global__ void __launch_bounds__(kBlock) k_mlp_gradient(const double* __restric
         if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
         __syncthreads();
     }
-    if (threadIdx.x == 0) atomicAdd(loss, red[0]);
+    if (threadIdx.x == 0) row[kP] = red[0];
+}
+
+// grad[i] (i < kP) and *loss (i = kP): the sum of column i of the nb rows of
+// `part`, in row order.
+__global__ void k_mlp_sum_blocks(const double* __restrict__ part, int nb, double* __restrict__ grad,
+                                 double* __restrict__ loss)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > kP) return;
+    double s = 0.0;
+    for (int b = 0; b < nb; ++b) s += part[(size_t)b * kRow + i];
+    if (i < kP)
+        grad[i] = s;
+    else
+        *loss = s;
 }
 
 // Adam (Kingma & Ba 2015) with bias correction; `step` counts from 1.
@@ -159,6 +178,7 @@ struct dem_mlp {
     double* p = nullptr;  // parameters, gradient, Adam moments, loss: one allocation
     double *g = nullptr, *m = nullptr, *v = nullptr, *loss = nullptr;
     double *x = nullptr, *t = nullptr, *y = nullptr;  // sample buffers
+    double* part = nullptr;                            // per-block partial gradients, kRow per block
     int64_t cap = 0;
     int adam_step = 0;
 };
@@ -171,14 +191,26 @@ int ensure(dem_mlp* c, int64_t n)
     if (c->x) (void)hipFree(c->x);
     c->x = c->t = c->y = nullptr;
     c->cap = 0;
-    if (hipMalloc(&c->x, sizeof(double) * (size_t)n * (kIn + 2 * kOut)) != hipSuccess) return DEM_ERR_DEVICE;
+    const size_t nb = (size_t)((n + kBlock - 1) / kBlock);
+    if (hipMalloc(&c->x, sizeof(double) * ((size_t)n * (kIn + 2 * kOut) + nb * kRow)) != hipSuccess)
+        return DEM_ERR_DEVICE;
     c->t = c->x + (size_t)n * kIn;
     c->y = c->t + (size_t)n * kOut;
+    c->part = c->y + (size_t)n * kOut;
     c->cap = n;
     return DEM_OK;
 }
 
 unsigned blocks(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+
+// the loss and gradient of the samples on the device, into c->loss and c->g
+void launch_gradient(dem_mlp* c, int64_t n)
+{
+    hipLaunchKernelGGL(k_mlp_gradient, dim3(blocks(n)), dim3(kBlock), 0, c->stream, (const double*)c->p, (int)n,
+                       (const double*)c->x, (const double*)c->t, c->part);
+    hipLaunchKernelGGL(k_mlp_sum_blocks, dim3(blocks(kRow)), dim3(kBlock), 0, c->stream, (const double*)c->part,
+                       (int)blocks(n), c->g, c->loss);
+}
 
 int gradient(dem_mlp* c, int64_t n, const double* x, const double* t, double* loss_out)
 {
@@ -186,10 +218,7 @@ int gradient(dem_mlp* c, int64_t n, const double* x, const double* t, double* lo
     if (hipMemcpyAsync(c->x, x, sizeof(double) * n * kIn, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
         hipMemcpyAsync(c->t, t, sizeof(double) * n * kOut, hipMemcpyHostToDevice, c->stream) != hipSuccess)
         return DEM_ERR_DEVICE;
-    hipLaunchKernelGGL(k_zero, dim3(blocks(kP + 1)), dim3(kBlock), 0, c->stream, c->g, kP);
-    hipLaunchKernelGGL(k_zero, dim3(1), dim3(kBlock), 0, c->stream, c->loss, 1);
-    hipLaunchKernelGGL(k_mlp_gradient, dim3(blocks(n)), dim3(kBlock), 0, c->stream, (const double*)c->p, (int)n,
-                       (const double*)c->x, (const double*)c->t, c->g, c->loss);
+    launch_gradient(c, n);
     if (hipGetLastError() != hipSuccess) return DEM_ERR_DEVICE;
     if (loss_out && (hipMemcpyAsync(loss_out, c->loss, sizeof(double), hipMemcpyDeviceToHost, c->stream) !=
                          hipSuccess ||
@@ -205,11 +234,14 @@ extern "C" {
 int dem_mlp_create(dem_mlp** out, int32_t device)
 {
     if (!out) return DEM_ERR_ARG;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) return DEM_ERR_ARG;
     auto* c = new dem_mlp;
     c->device = device;
     if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess ||
         hipMalloc(&c->p, sizeof(double) * (4 * kP + 1)) != hipSuccess) {
-        delete c;
+        dem_mlp_destroy(c);
         return DEM_ERR_DEVICE;
     }
     c->g = c->p + kP;
@@ -217,7 +249,10 @@ int dem_mlp_create(dem_mlp** out, int32_t device)
     c->v = c->m + kP;
     c->loss = c->v + kP;
     hipLaunchKernelGGL(k_zero, dim3(blocks(4 * kP + 1)), dim3(kBlock), 0, c->stream, c->p, 4 * kP + 1);
-    if (hipStreamSynchronize(c->stream) != hipSuccess) return DEM_ERR_DEVICE;
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(c->stream) != hipSuccess) {
+        dem_mlp_destroy(c);
+        return DEM_ERR_DEVICE;
+    }
     *out = c;
     return DEM_OK;
 }
@@ -291,10 +326,7 @@ int dem_mlp_train(dem_mlp* c, int64_t n, const double* x, const double* target, 
         hipMemcpyAsync(c->t, target, sizeof(double) * n * kOut, hipMemcpyHostToDevice, c->stream) != hipSuccess)
         return DEM_ERR_DEVICE;
     for (int e = 0; e < epochs; ++e) {
-        hipLaunchKernelGGL(k_zero, dim3(blocks(kP + 1)), dim3(kBlock), 0, c->stream, c->g, kP);
-        hipLaunchKernelGGL(k_zero, dim3(1), dim3(kBlock), 0, c->stream, c->loss, 1);
-        hipLaunchKernelGGL(k_mlp_gradient, dim3(blocks(n)), dim3(kBlock), 0, c->stream, (const double*)c->p, (int)n,
-                           (const double*)c->x, (const double*)c->t, c->g, c->loss);
+        launch_gradient(c, n);
         hipLaunchKernelGGL(k_mlp_adam, dim3(blocks(kP)), dim3(kBlock), 0, c->stream, c->p, (const double*)c->g,
                            c->m, c->v, lr, 0.9, 0.999, 1e-8, ++c->adam_step);
     }

@@ -23,7 +23,7 @@ void Chute::setupSideWalls()
 void Chute::setupInitialConditions()
 {
     if (speciesHandler.getNumberOfObjects() == 0)
-        throw std::runtime_error("Chute::setupInitialConditions: no species defined");
+        logger(ERROR, "Chute::setupInitialConditions: no species defined");
     setupSideWalls();
     SphericalParticle particleToInsert(speciesHandler.getObject(0));
     PSD psd;
@@ -183,7 +183,7 @@ void ChuteBottom::makeRoughBottom(Chute& chute)
     // restitution 0.343008, time step 2% of 9.41823 collision times
     ParticleSpecies* s0 = speciesHandler.getObject(0);
     if (s0->params.normal_law != DEM_NORMAL_LINEAR_VISCOELASTIC)
-        throw std::runtime_error("ChuteBottom: species 0 must have a linear viscoelastic normal force");
+        logger(ERROR, "ChuteBottom: species 0 must have a linear viscoelastic normal force");
     LinearViscoelasticSpecies lin;
     lin.params = s0->params;
     lin.setDensity(s0->getDensity());

@@ -31,7 +31,8 @@ FileType fileTypeFromName(const std::string& s)
     if (s == "ONE_FILE" || s == "1") return FileType::ONE_FILE;
     if (s == "MULTIPLE_FILES" || s == "2") return FileType::MULTIPLE_FILES;
     if (s == "MULTIPLE_FILES_PADDED" || s == "3") return FileType::MULTIPLE_FILES_PADDED;
-    throw std::runtime_error("restart: unknown file type " + s);
+    logger(ERROR, "restart: unknown file type %", s);
+    return FileType::NO_FILE;  // not reached: logger(ERROR, ...) does not return
 }
 
 // File::write (File.cc:442-453)
@@ -151,7 +152,7 @@ Fields parseLine(const std::string& line)
         std::vector<double> vals(arity(key));
         for (double& x : vals) {
             std::string tok;
-            if (!(is >> tok)) throw std::runtime_error("restart: truncated line: " + line);
+            if (!(is >> tok)) logger(ERROR, "%", "restart: truncated line: " + line);
             x = std::stod(tok);
         }
         f.v[key] = vals;
@@ -170,12 +171,12 @@ bool nextLine(std::istream& is, std::string& line)
 unsigned countLine(std::istream& is, const char* keyword)
 {
     std::string line;
-    if (!nextLine(is, line)) throw std::runtime_error(std::string("restart: missing ") + keyword);
+    if (!nextLine(is, line)) logger(ERROR, "%", std::string("restart: missing ") + keyword);
     std::istringstream ls(line);
     std::string k;
     unsigned n = 0;
     ls >> k >> n;
-    if (k != keyword) throw std::runtime_error(std::string("restart: expected ") + keyword + ", read " + line);
+    if (k != keyword) logger(ERROR, "%", std::string("restart: expected ") + keyword + ", read " + line);
     return n;
 }
 
@@ -377,13 +378,13 @@ bool DPMBase::readRestartFile(const std::string& fileName)
 void DPMBase::read(std::istream& is)
 {
     std::string line;
-    if (!nextLine(is, line)) throw std::runtime_error("restart: empty file");
+    if (!nextLine(is, line)) logger(ERROR, "restart: empty file");
     {
         std::istringstream ls(line);
         std::string word, version;
         ls >> word >> version;
         if (word != "MercuryDPM" && word != "restart_version")
-            throw std::runtime_error("restart: not a MercuryDPM restart file");
+            logger(ERROR, "restart: not a MercuryDPM restart file");
         std::string key;
         while (ls >> key)
             if (key == "name") {
@@ -408,13 +409,13 @@ void DPMBase::read(std::istream& is)
     Fields dom = parseLine("domain " + line);
     min_ = Vec3D(dom.get("xMin"), dom.get("yMin"), dom.get("zMin"));
     max_ = Vec3D(dom.get("xMax"), dom.get("yMax"), dom.get("zMax"));
-    if (!nextLine(is, line)) throw std::runtime_error("restart: missing time line");
+    if (!nextLine(is, line)) logger(ERROR, "restart: missing time line");
     {
         std::istringstream ls(line);
         std::string k;
         ls >> k >> dt_ >> k >> time_ >> k >> nt_ >> k >> tMax_;
     }
-    if (!nextLine(is, line)) throw std::runtime_error("restart: missing dimensions line");
+    if (!nextLine(is, line)) logger(ERROR, "restart: missing dimensions line");
     {
         std::istringstream ls(line);
         std::string k;
@@ -435,7 +436,7 @@ void DPMBase::read(std::istream& is)
     std::vector<Fields> mixed;
     unsigned seen = 0;
     while (seen < ns || mixed.size() < (size_t)ns * (ns - 1) / 2) {
-        if (!nextLine(is, line)) throw std::runtime_error("restart: species list ends early");
+        if (!nextLine(is, line)) logger(ERROR, "restart: species list ends early");
         Fields f = parseLine(line);
         if (f.name.find("MixedSpecies") != std::string::npos) {
             mixed.push_back(f);
@@ -453,7 +454,7 @@ void DPMBase::read(std::istream& is)
         else if (f.name == "HertzianViscoelasticMindlinSpecies")
             sp = speciesHandler.copyAndAddObject(HertzianViscoelasticMindlinSpecies());
         else
-            throw std::runtime_error("restart: species " + f.name + " is not supported");
+            logger(ERROR, "%", "restart: species " + f.name + " is not supported");
         sp->params = prm;
         sp->setDensity(f.get("density", 1.0));
         ++seen;
@@ -467,7 +468,7 @@ void DPMBase::read(std::istream& is)
     std::map<unsigned, unsigned> wallIndex;
     const unsigned nw = countLine(is, "Walls");
     for (unsigned k = 0; k < nw; ++k) {
-        if (!nextLine(is, line)) throw std::runtime_error("restart: wall list ends early");
+        if (!nextLine(is, line)) logger(ERROR, "restart: wall list ends early");
         const Fields f = parseLine(line);
         const ParticleSpecies* sp = speciesHandler.getObject((unsigned)f.get("indSpecies"));
         const std::vector<double> q = f.has("orientation") ? f.v.at("orientation") : std::vector<double>{1, 0, 0, 0};
@@ -480,13 +481,13 @@ void DPMBase::read(std::istream& is)
         } else if (f.name == "CylindricalWall") {
             const Vec3D pos = f.vec("position");
             if (pos.getLengthSquared() != 0 || q[0] != 1 || q[1] != 0 || q[2] != 0 || q[3] != 0)
-                throw std::runtime_error("restart: only a CylindricalWall about the z axis is supported");
+                logger(ERROR, "restart: only a CylindricalWall about the z axis is supported");
             CylindricalWall cw;
             cw.set(f.get("radius", 1.0));
             cw.setSpecies(sp);
             w = wallHandler.copyAndAddObject(cw);
         } else {
-            throw std::runtime_error("restart: wall " + f.name + " is not supported");
+            logger(ERROR, "%", "restart: wall " + f.name + " is not supported");
         }
         w->id_ = (unsigned)f.get("id");
         wallIndex[w->id_] = k;
@@ -495,9 +496,9 @@ void DPMBase::read(std::istream& is)
     boundaryHandler.clear();
     const unsigned nb = countLine(is, "Boundaries");
     for (unsigned k = 0; k < nb; ++k) {
-        if (!nextLine(is, line)) throw std::runtime_error("restart: boundary list ends early");
+        if (!nextLine(is, line)) logger(ERROR, "restart: boundary list ends early");
         const Fields f = parseLine(line);
-        if (f.name != "PeriodicBoundary") throw std::runtime_error("restart: boundary " + f.name + " is not supported");
+        if (f.name != "PeriodicBoundary") logger(ERROR, "%", "restart: boundary " + f.name + " is not supported");
         PeriodicBoundary b;
         b.setFromRestart(f.vec("normal"), f.get("scaleFactor", 1.0), f.get("distanceLeft"), f.get("distanceRight"),
                          f.vec("shift"));
@@ -510,10 +511,10 @@ void DPMBase::read(std::istream& is)
     const unsigned np = countLine(is, "Particles");
     unsigned maxId = 0;
     for (unsigned k = 0; k < np; ++k) {
-        if (!nextLine(is, line)) throw std::runtime_error("restart: particle list ends early");
+        if (!nextLine(is, line)) logger(ERROR, "restart: particle list ends early");
         const Fields f = parseLine(line);
         if (f.name != "BaseParticle" && f.name != "SphericalParticle")
-            throw std::runtime_error("restart: particle " + f.name + " is not supported");
+            logger(ERROR, "%", "restart: particle " + f.name + " is not supported");
         SphericalParticle p;
         p.setSpecies(speciesHandler.getObject((unsigned)f.get("indSpecies")));
         p.setRadius(f.get("radius", 1.0));
@@ -537,7 +538,7 @@ void DPMBase::read(std::istream& is)
     objs.clear();
     const unsigned ni = countLine(is, "Interactions");
     for (unsigned k = 0; k < ni; ++k) {
-        if (!nextLine(is, line)) throw std::runtime_error("restart: interaction list ends early");
+        if (!nextLine(is, line)) logger(ERROR, "restart: interaction list ends early");
         const Fields f = parseLine(line);
         Interaction c;
         c.isWall = f.has("particleWallIds");
@@ -641,7 +642,7 @@ bool DPMBase::readArguments(int argc, char** argv)
             } else {
                 file = argv[i + 1];
             }
-            if (!readRestartFile(file)) throw std::runtime_error("could not read restart file " + file);
+            if (!readRestartFile(file)) logger(ERROR, "%", "could not read restart file " + file);
         } else {
             return false;
         }

@@ -76,7 +76,7 @@ BaseParticle InsertionBoundary::generateParticle(RNG& random)
         }
         check -= probability_[k];
     }
-    if (particleToCopy_.empty()) throw std::runtime_error(getName() + ": particleToCopy not set");
+    if (particleToCopy_.empty()) logger(ERROR, "%", getName() + ": particleToCopy not set");
     BaseParticle P = particleToCopy_.at(chosenSpecies_);
     PSD& psd = psd_.at(chosenSpecies_);
     if (psd.getParticleSizeDistribution().empty()) psd.setDistributionUniform(P.getRadius(), P.getRadius(), 50);

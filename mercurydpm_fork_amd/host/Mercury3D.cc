@@ -485,9 +485,9 @@ bool DPMBase::getRotation() const { return rotationSet_ ? rotation_ : speciesHan
 void DPMBase::check(int rc, const char* what)
 {
     if (rc != DEM_OK) {
-        // the reference's logger(ERROR, ...) ends the program
-        // (Logger.cc:194); here the error is thrown so a driver may catch it
-        throw std::runtime_error(std::string(what) + ": " + dem_last_error(ctx_));
+        // an engine error ends the program through logger(ERROR, ...), as the
+        // reference's errors do, unless the driver installed handlers of its own
+        logger(ERROR, "%: %", what, dem_last_error(ctx_));
     }
 }
 
@@ -512,7 +512,7 @@ void DPMBase::initialiseSolve()
         interactionHandler.objects().clear();
         setupInitialConditions();
     }
-    if (dt_ <= 0) throw std::runtime_error("DPMBase::solve: time step not set");
+    if (dt_ <= 0) logger(ERROR, "DPMBase::solve: time step not set");
     if (ctx_) dem_destroy(ctx_);
     ctx_ = nullptr;
     dem_config cfg{};

@@ -30,6 +30,7 @@
 #include <vector>
 
 #include "../../include/mercurydpm_dem.h"
+#include "Logger.h"
 
 namespace constants {
 const double pi = 3.1415926535897932384626433832795;        // ExtendedMath.h:44

@@ -4,8 +4,11 @@
 // of a 10 x 5 chute tilted by 24 degrees, over a rough bottom of fixed
 // spheres (argument: ordered | disordered | triangular | flat), with side
 // walls, for 0.5 time units; particles leaving the end are removed by
-// cleanChute.  Writes ChuteInflowTest.ene and .data (every 500 steps) and a
-// summary line: bottom particles, flow particles, inserted, and the largest x.
+// cleanChute.  A second argument "outflow" raises the inflow velocity to 10
+// and the run to 1.5 time units, so that the first spheres leave the chute's
+// end and cleanChute removes them.  Writes ChuteInflowTest.ene and .data
+// (every 500 steps) and a summary line: bottom particles, flow particles,
+// inserted, and the largest x.
 #include <cstdio>
 #include <cstring>
 
@@ -13,7 +16,7 @@
 
 class ChuteInflowTest : public Chute {
 public:
-    explicit ChuteInflowTest(RoughBottomType bottom)
+    ChuteInflowTest(RoughBottomType bottom, bool outflow)
     {
         setName("ChuteInflowTest");
         LinearViscoelasticFrictionSpecies species;
@@ -32,10 +35,10 @@ public:
         setFixedParticleRadius(0.5);
         setFixedParticleSpacing(0.1);
         setRoughBottomType(bottom);
-        setInflowVelocity(1.0);
+        setInflowVelocity(outflow ? 10.0 : 1.0);
         setInflowVelocityVariance(0.1);
         setTimeStep(1e-4);
-        setTimeMax(0.5);
+        setTimeMax(outflow ? 1.5 : 0.5);
         setSaveCount(500);
         fStatFile.setFileType(FileType::NO_FILE);
         restartFile.setFileType(FileType::NO_FILE);
@@ -48,7 +51,7 @@ int main(int argc, char* argv[])
     if (argc > 1 && !std::strcmp(argv[1], "ordered")) bottom = MONOLAYER_ORDERED;
     if (argc > 1 && !std::strcmp(argv[1], "triangular")) bottom = MONOLAYER_TRIANGULAR;
     if (argc > 1 && !std::strcmp(argv[1], "flat")) bottom = FLAT;
-    ChuteInflowTest problem(bottom);
+    ChuteInflowTest problem(bottom, argc > 2 && !std::strcmp(argv[2], "outflow"));
     problem.solve();
     unsigned fixed = 0, flow = 0;
     double xmax = -1e300;

@@ -13,7 +13,7 @@ HOST = os.path.join(ROOT, "mercurydpm_fork_amd", "host")
 ORACLE = os.path.join(ROOT, "oracle")
 DRIVERS = os.path.join(ROOT, "tests", "drivers")
 GOLDEN = os.path.join(ROOT, "tests", "golden")
-HOST_SOURCES = ["Mercury3D.cc", "DPMBaseOutput.cc", "DPMBaseRestart.cc", "InsertionBoundary.cc", "Chute.cc"]
+HOST_SOURCES = ["Mercury3D.cc", "DPMBaseOutput.cc", "DPMBaseRestart.cc", "Logger.cc", "InsertionBoundary.cc", "Chute.cc"]
 FLAGS = ["-std=c++17", "-O2", "-ffp-contract=off", "-I" + HOST]
 
 

@@ -31,8 +31,8 @@ def first_frame(path):
     return rows[:, 0:3], rows[:, 6]
 
 
-def run(exe, bottom, cwd):
-    r = subprocess.run([exe, bottom], cwd=cwd, capture_output=True, text=True, timeout=900)
+def run(exe, bottom, cwd, *extra):
+    r = subprocess.run([exe, bottom, *extra], cwd=cwd, capture_output=True, text=True, timeout=900)
     assert r.returncode == 0, r.stdout + r.stderr
     summary = dict(zip(*[iter(r.stdout.split()[-8:])] * 2))
     return {k: float(v) for k, v in summary.items()}
@@ -83,19 +83,31 @@ def test_chute_on_oracle(bottom, oracle_driver, tmp_path):
     assert s["inserted"] > 10 and s["flow"] <= s["inserted"]
 
 
+def test_chute_outflow_removed_on_oracle(oracle_driver, tmp_path):
+    """Spheres entering at speed 10 reach the end of the 10-long chute within
+    the run: cleanChute (every 101st step) removes those beyond x = 10, so
+    fewer remain than were inserted, and none is left further out than one
+    cleaning interval of travel."""
+    s = run(oracle_driver, "ordered", str(tmp_path), "outflow")
+    assert s["flow"] < s["inserted"], s
+    assert s["xmax"] < L + 101 * 1e-4 * 12.0, s
+
+
 @pytest.mark.gpu
-@pytest.mark.parametrize("bottom", ["ordered", "disordered"])
-def test_chute_engine_vs_oracle(bottom, oracle_driver, tmp_path):
+@pytest.mark.parametrize("bottom,extra", [("ordered", ()), ("disordered", ()), ("ordered", ("outflow",))])
+def test_chute_engine_vs_oracle(bottom, extra, oracle_driver, tmp_path):
     """The same chute on the engine and on the oracle: the same bottom, the
-    same number of inserted particles, and energies and centre of mass
-    within 1e-6 (force sums differ in order, so the trajectories part at
-    round-off level)."""
+    same number of inserted (and, with outflow, removed) particles, and
+    energies and centre of mass within 1e-6 (force sums differ in order, so
+    the trajectories part at round-off level)."""
     exe = os.path.join(oc.DRIVERS, "bin", "ChuteInflowTest")
     assert os.path.exists(exe), "run __graft_entry__.build() first"
     (tmp_path / "g").mkdir()
     (tmp_path / "o").mkdir()
-    sg = run(exe, bottom, str(tmp_path / "g"))
-    so = run(oracle_driver, bottom, str(tmp_path / "o"))
+    sg = run(exe, bottom, str(tmp_path / "g"), *extra)
+    so = run(oracle_driver, bottom, str(tmp_path / "o"), *extra)
+    if extra:
+        assert so["flow"] < so["inserted"]
     assert sg["bottom"] == so["bottom"] and sg["inserted"] == so["inserted"] and sg["flow"] == so["flow"]
     a = np.array(oc.numeric_rows(str(tmp_path / "g" / "ChuteInflowTest.ene")))
     b = np.array(oc.numeric_rows(str(tmp_path / "o" / "ChuteInflowTest.ene")))

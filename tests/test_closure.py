@@ -58,6 +58,10 @@ def test_kernels_against_oracle():
     lo, go = O.loss_and_gradient(p, x, t)
     assert abs(loss - lo) <= 1e-12 * lo
     np.testing.assert_allclose(g, go, rtol=1e-10, atol=1e-13 * np.abs(go).max())
+    # per-block partial sums added in block order: the same bits every call
+    loss2, g2 = net.gradient(x, t)
+    assert loss2 == loss
+    np.testing.assert_array_equal(g2, g)
 
 
 @pytest.mark.gpu
@@ -168,6 +172,10 @@ def test_closure128_kernels_against_oracle():
     # and a sample count that is no multiple of the 16-row tiles
     y7 = net(x[:7])
     np.testing.assert_array_equal(y7, y[:7])
+    # chunked weight gradients summed in chunk order: the same bits every call
+    loss2, g2 = net.gradient(x, t)
+    assert loss2 == loss
+    np.testing.assert_array_equal(g2, g)
 
 
 @pytest.mark.gpu

@@ -139,3 +139,72 @@ def test_rng_matches_reference_state(tmp_path):
     want = [float(r[4]) for r in frame]
     got = [float(x) for x in radii.split()]
     assert [float(f"{g:.10g}") for g in got] == want
+
+
+LOGGER_SRC = r'''
+#include <cstdio>
+#include <cstring>
+#include "Mercury3D.h"
+int main(int argc, char** argv) {
+    const char* mode = argc > 1 ? argv[1] : "";
+    if (!std::strcmp(mode, "info")) {
+        logger(INFO, "a % b % c \\% d", 1, 2.5);
+        logger(VERBOSE, "dropped at the default level");
+        return 0;
+    }
+    if (!std::strcmp(mode, "solve")) {  // no time step: the default ERROR handler ends the program
+        Mercury3D p;
+        p.solve();
+        std::puts("not reached");
+        return 0;
+    }
+    if (!std::strcmp(mode, "throwing")) {  // a driver's own handler that throws
+        LoggerOutput mine = loggerOutputDefault;
+        mine.onError = [](std::string, std::string msg, Flusher) { throw std::invalid_argument(msg); };
+        loggerOutput = &mine;
+        Mercury3D p;
+        try {
+            p.solve();
+        } catch (const std::invalid_argument& e) {
+            std::printf("caught: %s\n", e.what());
+        }
+        loggerOutput = &loggerOutputDefault;
+        return 0;
+    }
+    if (!std::strcmp(mode, "returning")) {  // a handler that returns: the operation still stops
+        LoggerOutput mine = loggerOutputDefault;
+        mine.onError = [](std::string, std::string, Flusher) {};
+        loggerOutput = &mine;
+        try {
+            logger(ERROR, "value % out of range", 7);
+        } catch (const std::runtime_error& e) {
+            std::printf("stopped: %s\n", e.what());
+        }
+        return 0;
+    }
+    return 1;
+}
+'''
+
+
+def test_logger_levels_and_error_handlers(tmp_path):
+    """The host's logger: `%` placeholders, level filtering, ERROR ending the
+    program with exit status (signed char)-15 by default (the reference's
+    Logger.cc), a driver's own handler taking over, and a handler that
+    returns still stopping the failing operation."""
+    src = tmp_path / "logger.cpp"
+    src.write_text(LOGGER_SRC)
+    exe = tmp_path / "logger"
+    subprocess.run(["g++", "-std=c++17", "-O2", "-I" + HOST, "-o", str(exe), str(src),
+                    *[os.path.join(HOST, f) for f in oc.HOST_SOURCES], "-L" + LIB,
+                    "-lmercurydpm_dem", "-Wl,-rpath," + LIB], check=True)
+    r = subprocess.run([str(exe), "info"], capture_output=True, text=True)
+    assert r.returncode == 0 and r.stdout == "a 1 b 2.5 c % d\n", r
+    r = subprocess.run([str(exe), "solve"], capture_output=True, text=True)
+    assert r.returncode == (-15) % 256, r
+    assert "ERROR in MercuryKernel: DPMBase::solve: time step not set" in r.stderr
+    assert "not reached" not in r.stdout
+    r = subprocess.run([str(exe), "throwing"], capture_output=True, text=True)
+    assert r.returncode == 0 and r.stdout == "caught: DPMBase::solve: time step not set\n", r
+    r = subprocess.run([str(exe), "returning"], capture_output=True, text=True)
+    assert r.returncode == 0 and r.stdout == "stopped: MercuryKernel: value 7 out of range\n", r
