@@ -125,9 +125,9 @@ struct Dev {
     uint8_t* rev;
     // the same lists packed back to back per wavefront of 64 particles: the
     // lists of particles 64w .. 64w+63 fill, in order, the region starting at
-    // 64w * stride -- what the force kernel walks
-    uint32_t* nbrc;
-    uint8_t* revc;
+    // 64w * stride -- what the force kernel walks.  Each entry holds the list
+    // entry and the reverse slot (0xFF: none) together, one load per entry.
+    uint2* nbrc;
     // contact records
     Rec<NS>* rec[2];
     unsigned long long* mask[2];
@@ -902,7 +902,6 @@ __global__ void __launch_bounds__(kBlock) k_rematch(Dev<NS> d, Params p, int cur
         for (int k = 0; k < cnt; ++k) {
             const size_t sl = lslot(k, p.maxn, i);
             const uint32_t e = d.nbr[sl];
-            d.nbrc[off + k] = e;
             const uint32_t j = e & ~kWriterBit;
             if (e & kWriterBit) {
                 Rec<NS> r;
@@ -911,7 +910,7 @@ __global__ void __launch_bounds__(kBlock) k_rematch(Dev<NS> d, Params p, int cur
                     m |= 1ull << k;
                 }
                 d.rev[sl] = 0xFF;
-                d.revc[off + k] = 0xFF;
+                d.nbrc[off + k] = make_uint2(e, 0xFFu);
             } else {
                 const int cj = d.ncount[j];
                 int kk = -1;
@@ -922,7 +921,7 @@ __global__ void __launch_bounds__(kBlock) k_rematch(Dev<NS> d, Params p, int cur
                     }
                 if (kk < 0) atomicOr(&d.flags[F_LIST_ASYM], 1);
                 d.rev[sl] = (uint8_t)(kk < 0 ? 0xFF : kk);
-                d.revc[off + k] = (uint8_t)(kk < 0 ? 0xFF : kk);
+                d.nbrc[off + k] = make_uint2(e, kk < 0 ? 0xFFu : (uint32_t)kk);
             }
         }
         d.mask[cur][i] = m;
@@ -1056,8 +1055,14 @@ constexpr int kScanU = DEM_SCAN_UNROLL;
 #define FORCE_WAVES_ATTR
 #endif
 constexpr int kQueue = 64 + 64 * kScanU;
+// The owners' state the contact phase reads for every contact it evaluates
+// lives in LDS: position and radius, velocity and inverse mass, angular
+// velocity and inverse inertia (each contact would otherwise re-read the
+// owner's velocities through the vector memory path).
 struct OwnerLds {
     double px[64], py[64], pz[64], r[64];
+    double vx[64], vy[64], vz[64], im[64];
+    double wx[64], wy[64], wz[64], ii[64];
     unsigned long long m_in[64];
     int32_t sp[64];
     int32_t pre[64];    // exclusive prefix of the owners' list lengths
@@ -1068,9 +1073,35 @@ struct OwnerLds {
 
 __device__ inline int q_owner(unsigned long long e) { return (int)((e >> 38) & 63); }
 
+// The parameters the Hertz-Mindlin law reads, per species pair, staged in LDS
+// by k_force when the table has at most kLdsPairs pairs: elastic modulus,
+// dissipation, shear modulus, sliding dissipation, sliding friction.
+constexpr int kLdsPairs = 16;
+struct LawLds {
+    double v[kLdsPairs][5];
+};
+
+template <int LAW>
+__device__ inline dem_pair_params pair_params(const dem_pair_params* __restrict__ table, const LawLds& T, bool staged,
+                                              int pair)
+{
+    if constexpr (LAW == LAW_HERTZ_MINDLIN) {
+        if (staged) {
+            dem_pair_params s{};
+            s.elastic_modulus = T.v[pair][0];
+            s.dissipation = T.v[pair][1];
+            s.shear_modulus = T.v[pair][2];
+            s.sliding_dissipation = T.v[pair][3];
+            s.sliding_mu = T.v[pair][4];
+            return s;
+        }
+    }
+    return table[pair];
+}
+
 template <int NS, int LAW, int PREC>
-__device__ inline void process_batch(Dev<NS>& d, const Params& p, int cur, OwnerLds& L, int wbase, int base, int nb,
-                                     int lane)
+__device__ inline void process_batch(Dev<NS>& d, const Params& p, int cur, OwnerLds& L, const LawLds& T,
+                                     int wbase, int base, int nb, int lane)
 {
     if (lane < nb) {
         const unsigned long long e = L.q[base + lane];
@@ -1088,12 +1119,12 @@ __device__ inline void process_batch(Dev<NS>& d, const Params& p, int cur, Owner
         // species and both candidate history records
         const size_t slot = lslot(k, p.maxn, i);
         const Rec<NS>* rin = d.rec[cur];
-        const double4 vi = d.vel[i];
-        const double4 wi = d.omg[i];
         const double4 pq = d.posr[j];
         const double4 vq = d.vel[j];
         const double4 oq = d.omg[j];
-        const int sj = (int)(d.spc[j] & 0xFFFF);
+        // one species: every pair is pair 0, and the partner's species need
+        // not be read
+        const int sj = p.ns == 1 ? 0 : (int)(d.spc[j] & 0xFFFF);
         const bool had = writer && ((L.m_in[o] >> k) & 1ull);
         Rec<NS> hist;
         bool has_hist = false;
@@ -1107,10 +1138,10 @@ __device__ inline void process_batch(Dev<NS>& d, const Params& p, int cur, Owner
         }
         Body bi;
         bi.pos = mk(L.px[o], L.py[o], L.pz[o]);
-        bi.vel = xyz(vi);
-        bi.omega = xyz(wi);
+        bi.vel = mk(L.vx[o], L.vy[o], L.vz[o]);
+        bi.omega = mk(L.wx[o], L.wy[o], L.wz[o]);
         bi.radius = L.r[o];
-        bi.inv_mass = vi.w;
+        bi.inv_mass = L.im[o];
         V3 xj = xyz(pq);
         for (int b = 0; b < p.nper; ++b) {
             const int c = (codes >> (2 * b)) & 3;
@@ -1145,7 +1176,7 @@ __device__ inline void process_batch(Dev<NS>& d, const Params& p, int cur, Owner
                 spring[8] = -spring[8];
             }
         }
-        const dem_pair_params& sp = d.sp[L.sp[o] * p.ns + sj];
+        const dem_pair_params sp = pair_params<LAW>(d.sp, T, p.ns * p.ns <= kLdsPairs, L.sp[o] * p.ns + sj);
         V3 cF, cT = zero3(), ft;
         if constexpr (PREC == DEM_PRECISION_MIXED) {
             ContactM gm;
@@ -1218,6 +1249,20 @@ __global__ void __launch_bounds__(kBlock) FORCE_WAVES_ATTR k_force(Dev<NS> d, Pa
     // force pass: its lists were not rebuilt
     if (__atomic_load_n(&d.flags[F_NONFINITE], __ATOMIC_RELAXED)) return;
     __shared__ OwnerLds lds[kWaves];
+    __shared__ LawLds law;
+    if constexpr (LAW == LAW_HERTZ_MINDLIN) {
+        const int npairs = p.ns * p.ns;
+        if (npairs <= kLdsPairs && (int)threadIdx.x < 5 * npairs) {
+            const dem_pair_params& s = d.sp[threadIdx.x / 5];
+            const int f = threadIdx.x % 5;
+            law.v[threadIdx.x / 5][f] = f == 0   ? s.elastic_modulus
+                                        : f == 1 ? s.dissipation
+                                        : f == 2 ? s.shear_modulus
+                                        : f == 3 ? s.sliding_dissipation
+                                                 : s.sliding_mu;
+        }
+        __syncthreads();
+    }
     const int lane = threadIdx.x & 63;
     OwnerLds& L = lds[threadIdx.x >> 6];
     const int i = xcd_block() * blockDim.x + threadIdx.x;
@@ -1226,14 +1271,23 @@ __global__ void __launch_bounds__(kBlock) FORCE_WAVES_ATTR k_force(Dev<NS> d, Pa
     const int ii = valid ? i : 0;
     const int cnt = valid ? d.ncount[ii] : 0;
     {
-        // the owners' positions go to LDS for the contact phase (velocities
-        // are read where needed); the epilogue reads them back, so that they
-        // occupy no registers meanwhile
+        // the owners' state goes to LDS for the contact phase; the epilogue
+        // reads it back, so that it occupies no registers meanwhile
         const double4 pr = d.posr[ii];
+        const double4 vl = d.vel[ii];
+        const double4 om = d.omg[ii];
         L.px[lane] = pr.x;
         L.py[lane] = pr.y;
         L.pz[lane] = pr.z;
         L.r[lane] = pr.w;
+        L.vx[lane] = vl.x;
+        L.vy[lane] = vl.y;
+        L.vz[lane] = vl.z;
+        L.im[lane] = vl.w;
+        L.wx[lane] = om.x;
+        L.wy[lane] = om.y;
+        L.wz[lane] = om.z;
+        L.ii[lane] = om.w;
         L.m_in[lane] = valid ? d.mask[cur][ii] : 0ull;
         L.sp[lane] = (int)(d.spc[ii] & 0xFFFF);
     }
@@ -1262,7 +1316,7 @@ __global__ void __launch_bounds__(kBlock) FORCE_WAVES_ATTR k_force(Dev<NS> d, Pa
         while (qn - head >= 64 || (all && qn > head)) {
             const int nb = qn - head < 64 ? qn - head : 64;
             wave_sync();
-            process_batch<NS, LAW, PREC>(d, p, cur, L, wbase, head, nb, lane);
+            process_batch<NS, LAW, PREC>(d, p, cur, L, law, wbase, head, nb, lane);
             L.first[lane] = -1;
             wave_sync();
             if (lane < nb) {
@@ -1304,8 +1358,9 @@ __global__ void __launch_bounds__(kBlock) FORCE_WAVES_ATTR k_force(Dev<NS> d, Pa
             ej[u] = 0u;
             kr[u] = 0xFFu;
             if (e < total) {
-                ej[u] = d.nbrc[lbase + (size_t)e];
-                kr[u] = d.revc[lbase + (size_t)e];
+                const uint2 nr = d.nbrc[lbase + (size_t)e];
+                ej[u] = nr.x;
+                kr[u] = nr.y;
             }
         }
 #pragma unroll
@@ -1363,8 +1418,8 @@ __global__ void __launch_bounds__(kBlock) FORCE_WAVES_ATTR k_force(Dev<NS> d, Pa
     if (!valid) return;
     wave_sync();
     const double4 pr = make_double4(L.px[lane], L.py[lane], L.pz[lane], L.r[lane]);
-    const double4 vl = d.vel[i];
-    const double4 om = d.omg[i];
+    const double4 vl = make_double4(L.vx[lane], L.vy[lane], L.vz[lane], L.im[lane]);
+    const double4 om = make_double4(L.wx[lane], L.wy[lane], L.wz[lane], L.ii[lane]);
     const int si = L.sp[lane];
     const unsigned long long m_in = L.m_in[lane];
     const bool fixed_i = vl.w == 0.0;
@@ -2351,7 +2406,7 @@ int alloc_particles(dem_ctx* ctx, int64_t n)
     A(d.t_id, n) A(d.t_orig, n) A(d.t_spc, n) A(d.t_cellc, n);
     const size_t slots = (size_t)n * ctx->maxn_cfg;
     const size_t wslots_packed = ((size_t)n + 63) / 64 * 64 * ctx->maxn_cfg;
-    A(d.nbr, slots) A(d.ncount, n) A(d.rev, slots) A(d.nbrc, wslots_packed) A(d.revc, wslots_packed);
+    A(d.nbr, slots) A(d.ncount, n) A(d.rev, slots) A(d.nbrc, wslots_packed);
     A(d.rec[0], slots) A(d.rec[1], slots) A(d.mask[0], n) A(d.mask[1], n) A(d.fc, slots) A(d.ftc, slots);
     const size_t wslots = (size_t)n * ctx->wall_slots;
     A(d.wrec[0], wslots) A(d.wrec[1], wslots) A(d.wmask[0], n) A(d.wmask[1], n) A(d.fw, wslots) A(d.ftw, wslots);

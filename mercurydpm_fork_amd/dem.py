@@ -36,6 +36,7 @@ AXI_MAX_SUB = 4
 
 ERRORS = {-1: "DEM_ERR_ARG", -2: "DEM_ERR_STATE", -3: "DEM_ERR_DEVICE", -4: "DEM_ERR_CAPACITY",
           -5: "DEM_ERR_UNSUPPORTED"}
+ERR_ARG, ERR_STATE, ERR_DEVICE, ERR_CAPACITY = -1, -2, -3, -4
 
 
 class DemError(RuntimeError):

@@ -16,6 +16,8 @@ along z, such as the floor); no periodic boundaries.  Members must stay
 within their tile plus half the gap for the duration of the run (the packer
 reports the gap; `check_separated` verifies it).
 """
+import dataclasses
+import time
 from dataclasses import dataclass
 from typing import List, Sequence
 
@@ -344,6 +346,7 @@ class HeapRun:
     angle_at_end: np.ndarray    # degrees, atan(4 z_com / R) of the last state (nan: casing still on)
     removed: int                # particles deleted below the base
     steps: int
+    seconds: dict = None        # wall time in stepping, the energy checks and the deletions
 
 
 def deletion_sequence(n: int, doomed) -> tuple:
@@ -395,9 +398,15 @@ def run_heap_protocol(e: dem.Engine, ens: HeapEnsemble, steps: int, check_every:
     dt = ens.scenario.dt
     st0 = e.stats()
     t0 = st0.time
+    secs = {"step": 0.0, "check": 0.0, "delete": 0.0}
+    clock = time.perf_counter
     while done < steps:
         k = min(check_every, steps - done)
+        c0 = clock()
         e.step(k)
+        e.stats()  # waits for the steps
+        c1 = clock()
+        secs["step"] += c1 - c0
         done += k
         t = t0 + done * dt
         en = e.particle_energies()
@@ -422,6 +431,8 @@ def run_heap_protocol(e: dem.Engine, ens: HeapEnsemble, steps: int, check_every:
         aor[measure] = np.degrees(np.arctan(4.0 * com[measure] / ens.base_radius[measure]))
         at_end = np.where(fill, np.nan, np.degrees(np.arctan(4.0 * com / ens.base_radius)))
         checks += 1
+        c2 = clock()
+        secs["check"] += c2 - c1
         if not fill.all() and (checks % delete_every == 0 or done >= steps):
             idx = e.find_in_region(0, (0.0, 0.0, -1.0, 0.0))  # DEM_REGION_HALF_SPACE
             if len(idx):
@@ -429,37 +440,60 @@ def run_heap_protocol(e: dem.Engine, ens: HeapEnsemble, steps: int, check_every:
                 e.remove_particles(seq)
                 ids, inv_mass = ids[order], inv_mass[order]
                 removed += len(idx)
+            secs["delete"] += clock() - c2
         if log is not None:
             log(f"heaps: {done}/{steps} steps, {int((~fill).sum())}/{m} casings lifted, "
                 f"{int(np.isfinite(aor).sum())} angles, {removed} removed")
-    return HeapRun(fill_time, aor, at_end, removed, done)
+    return HeapRun(fill_time, aor, at_end, removed, done, {k: round(v, 3) for k, v in secs.items()})
 
 
-def autotune(members: Sequence[S.Scenario], skins=(0.2, 0.4, 0.8), schedules=("shelf", "grid"), steps: int = 300,
-             device: int = 0, log=None):
+def autotune(members: Sequence[S.Scenario], skins=(0.2, 0.4, 0.8, 1.6, 2.4), schedules=("shelf", "grid"),
+             steps: int = 300, warm_steps: int = 3000, device: int = 0, log=None):
     """The layout schedule (heap_layout) and Verlet-list skin (a multiple of
     the smallest radius: the cell size of every grid level is twice the
-    level's largest radius plus the skin) that step this ensemble fastest:
-    each candidate is packed, uploaded and timed over `steps` steps from the
-    start (after 50 untimed ones).  Returns (schedule, skin, table of
+    level's largest radius plus the skin) that step this ensemble fastest.
+    Per schedule the packed ensemble is first run warm_steps steps (the fill:
+    spheres falling into the casings, the phase whose displacements trigger
+    most list rebuilds); every skin is then timed over `steps` steps from that
+    same state (state and interactions uploaded again, after 20 untimed
+    steps).  A skin whose lists overflow the neighbour capacity
+    (DEM_ERR_CAPACITY) is left out.  Returns (schedule, skin, table of
     candidates with their ms per step)."""
-    import time
     rmin = min(float(m.state.radius.min()) for m in members)
     table = []
     best = None
     for sch in schedules:
         ens = pack_heaps(members, schedule=sch)
+        w = dem.Engine(device=device)
+        S.apply(w, ens.scenario)
+        w.compute_forces()
+        w.step(warm_steps)
+        st, ct, stats = w.download(), w.contacts(), w.stats()
+        w.close()
+        s0 = ens.scenario.state
+        warm = dataclasses.replace(ens.scenario, state=dem.State(
+            id=s0.id, species=s0.species, radius=s0.radius, inv_mass=s0.inv_mass, inv_inertia=s0.inv_inertia,
+            x=st["x"], v=st["v"], w=st["w"], q=st["q"], f=st["f"], t=st["t"]))
         for f in skins:
             e = dem.Engine(device=device, skin=f * rmin)
-            S.apply(e, ens.scenario)
-            e.compute_forces()
-            e.step(50)
-            e.stats()
-            t0 = time.perf_counter()
-            e.step(steps)
-            e.stats()
-            ms = 1e3 * (time.perf_counter() - t0) / steps
-            e.close()
+            try:
+                S.apply(e, warm, ct)
+                e.set_time(stats.time, int(stats.n_steps))
+                e.compute_forces()
+                e.step(20)
+                e.stats()
+                t0 = time.perf_counter()
+                e.step(steps)
+                e.stats()
+                ms = 1e3 * (time.perf_counter() - t0) / steps
+            except dem.DemError as err:
+                if err.code != dem.ERR_CAPACITY:
+                    raise
+                if log is not None:
+                    log(f"autotune: {sch} layout, skin {f * rmin:.3g}: lists overflow, left out")
+                continue
+            finally:
+                e.close()
             table.append({"schedule": sch, "skin": f * rmin, "ms_per_step": round(ms, 4)})
             if log is not None:
                 log(f"autotune: {sch} layout, skin {f * rmin:.3g}: {ms:.3f} ms per step")

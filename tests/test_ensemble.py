@@ -211,3 +211,24 @@ def test_heap_protocol_runs_to_angles_of_repose():
     ok = np.isfinite(run.angle_of_repose)
     assert ok.sum() >= 1, run.angle_of_repose
     assert np.all((run.angle_of_repose[ok] > 5) & (run.angle_of_repose[ok] < 60)), run.angle_of_repose
+
+
+@pytest.mark.gpu
+def test_autotune_picks_the_fastest_candidate():
+    """The autotuner times every (layout, skin) candidate from the same
+    warmed-up state and returns the fastest; a skin whose lists would
+    overflow is left out rather than failing the search."""
+    params = E.draw_heap_params(6, 11, n_range=(500, 1500))
+    members = [E.calibration_heap(p) for p in params]
+    sch, skin, table = E.autotune(members, skins=(0.2, 0.8, 2.4), steps=50, warm_steps=500)
+    assert {row["schedule"] for row in table} == {"shelf", "grid"}
+    best = min(table, key=lambda row: row["ms_per_step"])
+    assert (sch, skin) == (best["schedule"], best["skin"])
+    # the picked set-up runs the protocol
+    ens = E.pack_heaps(members, schedule=sch)
+    e = dem.Engine(skin=skin)
+    S.apply(e, ens.scenario)
+    e.compute_forces()
+    run = E.run_heap_protocol(e, ens, 2000)
+    e.close()
+    assert run.steps == 2000 and set(run.seconds) == {"step", "check", "delete"}

@@ -7,7 +7,8 @@ periodic in x and y, floor, gravity -z).  GPU.
   the forces on identical inputs (the FP64 state and stored interactions);
   the per-particle error relative to the sum of the magnitudes of the
   particle's contact forces (mean and max) and the error of the whole force
-  field relative to its norm are stated and bounded;
+  field relative to its norm are stated and bounded: the worst particle of
+  all ten states within 1e-5 (measured: mean 2.0e-7, max 9.5e-7);
 * energy: FP64 and mixed runs of 50,000 steps from the same seeded start;
   their kinetic (translational + rotational) energies every 5000 steps agree
   within 1% of the run's largest kinetic energy, and their potential energies
@@ -67,7 +68,9 @@ def test_mixed_precision_force_error_on_the_box():
           f"force field {worst_rms:.3g}")
     assert worst_rms <= 1e-5, worst_rms
     assert worst_mean <= 1e-6, worst_mean
-    assert worst_max <= 2e-4, worst_max
+    # the 1e-5 relative force error the mixed-precision variant is held to,
+    # per particle at its worst (measured: 9.5e-7)
+    assert worst_max <= 1e-5, worst_max
 
 
 def test_mixed_precision_energy_over_50000_steps():
