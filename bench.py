@@ -89,6 +89,12 @@ def parse():
                          "profiler cover the timed window's state only")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only copy this many bytes device-to-device (calibrates HBM counters) and exit")
+    ap.add_argument("--list-sweep", choices=("particle", "wavefront"), default="particle",
+                    help="Verlet-list build: one sweep of the neighbour cells per particle, or the wavefront-shared "
+                         "sweep staged in LDS (dem_config.list_sweep)")
+    ap.add_argument("--engine-lib", default="",
+                    help="path of another build of libmercurydpm_dem.so to measure (an A/B comparison of builds); "
+                         "default: the in-tree library")
     return ap.parse_args()
 
 
@@ -263,6 +269,9 @@ def main():
     from mercurydpm_fork_amd import dem
     from mercurydpm_fork_amd import scenarios as S
 
+    if a.engine_lib:
+        dem.HIP_LIB_PATH = os.path.abspath(a.engine_lib)
+    sweep = dem.SWEEP_WAVEFRONT if a.list_sweep == "wavefront" else dem.SWEEP_PER_PARTICLE
     if a.ensemble > 0:
         return run_ensemble(a)
 
@@ -274,7 +283,7 @@ def main():
     n_total = sc.state.n
     n_flow = sc.info["n_flowing"]
     prec = dem.PRECISION_MIXED if a.precision == "mixed" else dem.PRECISION_FP64
-    eng = dem.Engine(device=0, skin=a.skin, precision=prec)
+    eng = dem.Engine(device=0, skin=a.skin, precision=prec, list_sweep=sweep)
     settled_cn = None
     if a.load_state:
         z = np.load(a.load_state)
@@ -383,7 +392,7 @@ def main():
         d = eng.download()
         ct = eng.contacts()
         s3 = with_state(S, dem, sc, d)
-        em = dem.Engine(device=0, skin=a.skin, precision=dem.PRECISION_MIXED)
+        em = dem.Engine(device=0, skin=a.skin, precision=dem.PRECISION_MIXED, list_sweep=sweep)
         S.apply(em, s3, ct)
         em.set_time(st.time, int(st.n_steps))
         em.compute_forces()
@@ -446,7 +455,7 @@ def main():
         "config": {"workload": "polydisperse periodic chute, HertzianViscoelasticMindlin e=0.8 mu=0.5, 24 deg",
                    "flowing_spheres": n_flow, "base_spheres": sc.info["n_base"],
                    "diameter": "U[0.5,1.5]", "footprint": round(sc.info["footprint"], 3),
-                   "dt": sc.dt, "settle_steps": a.settle_steps, "verlet_skin": a.skin,
+                   "dt": sc.dt, "settle_steps": a.settle_steps, "verlet_skin": a.skin, "list_sweep": a.list_sweep,
                    "contacts_per_particle_settled": None if settled_cn is None else round(settled_cn, 4),
                    "steps_before_window": a.run_steps + a.warmup,
                    "contacts_per_particle": round(cbar, 4), "contacts_per_particle_survey": "2-3 (dense bed)",
