@@ -25,6 +25,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -66,7 +67,8 @@ enum Flag {
     F_ORPH_OVERFLOW = 11,
     F_DONE_BLOCKS = 12, // blocks of the last rebuild kernel that have finished
     F_NONFINITE = 13,   // a particle position is not finite: the run diverged
-    F_NUM = 14
+    F_STEP = 14,        // the step counter of steps replayed from a captured graph
+    F_NUM = 15
 };
 
 // Integration fused into the force kernel: none (a force pass only), the
@@ -97,6 +99,9 @@ struct Params {
     int kick;           // integration fused into the force pass (KICK_*)
     int store_fc;       // keep the contact forces of this pass (for coarse graining)
     double fc_stamp;    // stamp of this pass in fc / fw
+    // in steps replayed from a captured graph: the step counter on the device
+    // (nt and alive_min both; nullptr: nt and alive_min above)
+    const uint32_t* step_dev;
     // grid
     double h[kMaxLevels];       // cell size of a non-periodic axis
     double rmaxl[kMaxLevels];   // largest radius on each level
@@ -289,6 +294,12 @@ __device__ inline bool rebuild_pending(const int* flags)
     return __atomic_load_n(&flags[F_REBUILD], __ATOMIC_RELAXED) != 0 &&
            __atomic_load_n(&flags[F_NONFINITE], __ATOMIC_RELAXED) == 0;
 }
+
+// The step counter of this launch and the oldest time stamp still alive: the
+// host's values, or in a step replayed from a captured graph the device's
+// counter (there alive_min equals the counter).
+__device__ inline uint32_t step_nt(const Params& p) { return p.step_dev ? *p.step_dev : p.nt; }
+__device__ inline uint32_t step_alive_min(const Params& p) { return p.step_dev ? *p.step_dev : p.alive_min; }
 
 // floor(v) as a cell coordinate, clamped to +-2^29 (NaN: 0), so that cell
 // ranges stay small and loops over them end for any position.
@@ -495,7 +506,7 @@ __global__ void __launch_bounds__(kBlock) k_rb_begin(Dev<NS> d, Params p, int cu
     GRID_STRIDE(k, no)
     {
         const Rec<NS> r = d.orec[par][k];
-        if (r.ts >= p.alive_min) hash_insert(d, d.okey[par][k], r);
+        if (r.ts >= step_alive_min(p)) hash_insert(d, d.okey[par][k], r);
     }
 }
 
@@ -621,6 +632,10 @@ __global__ void __launch_bounds__(kBlock) k_gather(Dev<NS> d, Params p)
         d.t_cellc[i] = d.cellc[o];
     }
 }
+
+// the device step counter of captured steps
+__global__ void k_step_set(int* flags, uint32_t nt) { flags[F_STEP] = (int)nt; }
+__global__ void k_step_advance(int* flags) { flags[F_STEP] += 1; }
 
 // ------------------------------------------------------------------------
 // Rebuild: Verlet lists from the sorted multilevel grid.
@@ -970,7 +985,7 @@ __global__ void __launch_bounds__(kBlock) k_orphans(Dev<NS> d, Params p)
         const bool gone_wall = (key & DEM_WALL_KEY) && (int)(key & ~(unsigned long long)DEM_WALL_KEY & 0xFFFFFFFFull) >= p.nwalls;
         if (!d.hmatched[h] && !gone_wall) {
             const Rec<NS> r = d.hval[h];
-            if (!p.erase || r.ts > p.nt) {
+            if (!p.erase || r.ts > step_nt(p)) {
                 const int o = atomicAdd(&d.flags[F_ORPH_COUNT0 + par], 1);
                 if (o < d.ocap) {
                     d.okey[par][o] = key;
@@ -1221,7 +1236,7 @@ __device__ inline void process_batch(Dev<NS>& d, const Params& p, int cur, Owner
         if (writer) {
             Rec<NS> r;
             for (int a = 0; a < NS; ++a) r.s[a] = spring[a];
-            r.ts = p.nt + 1;
+            r.ts = step_nt(p) + 1;
             r.pad = 0;
             // An image contact is stored through the ghost's copy of the
             // interaction, which replaces the real-real record only when its
@@ -1431,14 +1446,15 @@ __global__ void __launch_bounds__(kBlock) FORCE_WAVES_ATTR k_force(Dev<NS> d, Pa
     bi.inv_mass = vl.w;
     const Rec<NS>* rin = d.rec[cur];
     Rec<NS>* rout = d.rec[cur ^ 1];
-    const uint32_t ts_new = p.nt + 1;
+    const uint32_t nt = step_nt(p);
+    const uint32_t ts_new = nt + 1;
     unsigned long long m_out = touched;
     unsigned long long carry = m_in & ~touched;
     while (carry) {
         const int k = __ffsll((long long)carry) - 1;
         carry &= carry - 1;
         const Rec<NS> r = rin[lslot(k, p.maxn, i)];
-        if (!p.erase || r.ts > p.nt) {
+        if (!p.erase || r.ts > nt) {
             rout[lslot(k, p.maxn, i)] = r;
             m_out |= 1ull << k;
         }
@@ -1501,7 +1517,7 @@ __global__ void __launch_bounds__(kBlock) FORCE_WAVES_ATTR k_force(Dev<NS> d, Pa
                 wm_out |= 1u << w;
             } else if ((wm_in >> w) & 1u) {
                 const Rec<NS> r = d.wrec[cur][wbase + w];
-                if (!p.erase || r.ts > p.nt) {
+                if (!p.erase || r.ts > nt) {
                     d.wrec[cur ^ 1][wbase + w] = r;
                     wm_out |= 1u << w;
                 }
@@ -2247,6 +2263,20 @@ struct dem_ctx {
     double skin_cfg = 0.0;
     int levels_cfg = 0;
     int sweep_cfg = DEM_SWEEP_PER_PARTICLE;
+    // steady steps of dem_step replayed from captured graphs of two steps;
+    // a graph is reused only while its key (every device pointer, the static
+    // parameters, the particle and bucket counts and the buffer parity) is
+    // unchanged, otherwise captured again
+    struct StepGraph {
+        hipGraphExec_t exec = nullptr;
+        std::vector<unsigned char> key;
+    };
+    StepGraph graphs[2];
+    int graph_next = 0;
+    // -1: not read yet; 0: graphs in use (DEM_STEP_GRAPH=1 in the
+    // environment); 1: off (the default: on this ROCm a replayed step is
+    // slower than its launches), or capture failed
+    int graphs_off = -1;
     // host-side settings
     std::vector<dem_pair_params> species;
     int ns = 0;
@@ -2326,8 +2356,18 @@ int dalloc(dem_ctx* ctx, T** p, size_t count)
     return DEM_OK;
 }
 
+void drop_graphs(dem_ctx* ctx)
+{
+    for (auto& g : ctx->graphs) {
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+        g.exec = nullptr;
+        g.key.clear();
+    }
+}
+
 void free_all(dem_ctx* ctx)
 {
+    drop_graphs(ctx);
     for (void* v : ctx->allocs) (void)hipFree(v);
     ctx->allocs.clear();
     ctx->d_sp = nullptr;
@@ -2544,10 +2584,121 @@ int ensure_events(dem_ctx* ctx, size_t n)
 }
 
 template <int NS>
+void run_one_step(dem_ctx* ctx, int64_t s, int64_t nsteps)
+{
+    // the first half-step and drift use the forces stored by the previous
+    // force pass; later ones are fused into the force kernel
+    if (s == 0) launch_kick<NS>(ctx, 0, 1);
+    ctx->prm.erase = 1;
+    ctx->prm.nt = (uint32_t)ctx->nt;
+    ctx->prm.alive_min = ctx->alive_min;
+    launch_rebuild<NS>(ctx);
+    launch_force<NS>(ctx, 1, s + 1 < nsteps ? KICK_BOTH : KICK_AFTER);
+}
+
+inline void advance_clock(dem_ctx* ctx)
+{
+    ctx->time += ctx->dt;
+    ctx->nt += 1;
+    ctx->alive_min = (uint32_t)ctx->nt;
+    ctx->prm.alive_min = ctx->alive_min;
+}
+
+// The key of a captured two-step graph: what its launches bake in.
+template <int NS>
+std::vector<unsigned char> graph_key(dem_ctx* ctx)
+{
+    Params q = ctx->prm;
+    q.nt = 0;
+    q.alive_min = 0;
+    q.erase = 1;
+    q.kick = KICK_BOTH;
+    q.store_fc = 0;
+    q.fc_stamp = 0.0;
+    q.step_dev = nullptr;
+    const Dev<NS>& d = dev<NS>(ctx);
+    const int64_t extra[4] = {ctx->n, (int64_t)ctx->nbuckets, ctx->cur, NS};
+    std::vector<unsigned char> k(sizeof q + sizeof d + sizeof extra);
+    std::memcpy(k.data(), &q, sizeof q);
+    std::memcpy(k.data() + sizeof q, &d, sizeof d);
+    std::memcpy(k.data() + sizeof q + sizeof d, extra, sizeof extra);
+    return k;
+}
+
+// A graph of two steady steps (rebuild launches, force pass with the fused
+// integration, counter advance; twice, so that the double buffers return to
+// their state at the start), captured on first use.  nullptr if capture is
+// not possible (then every step is launched on its own).
+template <int NS>
+hipGraphExec_t step_graph(dem_ctx* ctx)
+{
+    if (ctx->graphs_off < 0) {
+        const char* env = std::getenv("DEM_STEP_GRAPH");
+        ctx->graphs_off = !(env && std::strcmp(env, "1") == 0);
+    }
+    if (ctx->graphs_off) return nullptr;
+    std::vector<unsigned char> key = graph_key<NS>(ctx);
+    for (auto& g : ctx->graphs)
+        if (g.exec && g.key == key) return g.exec;
+    auto& slot = ctx->graphs[ctx->graph_next];
+    ctx->graph_next ^= 1;
+    if (slot.exec) (void)hipGraphExecDestroy(slot.exec);
+    slot.exec = nullptr;
+    slot.key.clear();
+    hipGraph_t graph = nullptr;
+    if (hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+        (void)hipGetLastError();
+        ctx->graphs_off = 1;
+        return nullptr;
+    }
+    ctx->prm.step_dev = (const uint32_t*)(ctx->d_flags + F_STEP);
+    for (int k = 0; k < 2; ++k) {
+        ctx->prm.erase = 1;
+        launch_rebuild<NS>(ctx);
+        launch_force<NS>(ctx, 1, KICK_BOTH);
+        hipLaunchKernelGGL(k_step_advance, dim3(1), dim3(1), 0, ctx->stream, ctx->d_flags);
+    }
+    ctx->prm.step_dev = nullptr;
+    const hipError_t ec = hipStreamEndCapture(ctx->stream, &graph);
+    hipGraphExec_t exec = nullptr;
+    if (ec != hipSuccess || !graph || hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) {
+        (void)hipGetLastError();
+        if (graph) (void)hipGraphDestroy(graph);
+        ctx->graphs_off = 1;
+        return nullptr;
+    }
+    (void)hipGraphDestroy(graph);
+    slot.exec = exec;
+    slot.key = std::move(key);
+    return exec;
+}
+
+template <int NS>
 int run_steps(dem_ctx* ctx, int64_t nsteps)
 {
     // profiling: events e[4s..4s+3] bracket kick | rebuild | force of step s
     const bool prof = ctx->profiling;
+    if (!prof && nsteps >= 4) {
+        // the first step, pairs of steady steps from the graph, the rest
+        run_one_step<NS>(ctx, 0, nsteps);
+        advance_clock(ctx);
+        int64_t s = 1;
+        hipGraphExec_t g = step_graph<NS>(ctx);
+        if (g) {
+            hipLaunchKernelGGL(k_step_set, dim3(1), dim3(1), 0, ctx->stream, ctx->d_flags, (uint32_t)ctx->nt);
+            for (; s + 2 < nsteps; s += 2) {
+                HIPCHK(hipGraphLaunch(g, ctx->stream));
+                advance_clock(ctx);
+                advance_clock(ctx);
+            }
+        }
+        for (; s < nsteps; ++s) {
+            run_one_step<NS>(ctx, s, nsteps);
+            advance_clock(ctx);
+        }
+        HIPCHK(hipGetLastError());
+        return check_flags(ctx);
+    }
     constexpr int64_t kChunk = 256;
     for (int64_t s0 = 0; s0 < nsteps; s0 += kChunk) {
         const int64_t n = std::min(kChunk, nsteps - s0);
@@ -2558,8 +2709,6 @@ int run_steps(dem_ctx* ctx, int64_t nsteps)
         for (int64_t k = 0; k < n; ++k) {
             const int64_t s = s0 + k;
             if (prof) HIPCHK(hipEventRecord(ctx->events[4 * k], ctx->stream));
-            // the first half-step and drift use the forces stored by the
-            // previous force pass; later ones are fused into the force kernel
             if (s == 0) launch_kick<NS>(ctx, 0, 1);
             if (prof) HIPCHK(hipEventRecord(ctx->events[4 * k + 1], ctx->stream));
             ctx->prm.erase = 1;
@@ -2569,10 +2718,7 @@ int run_steps(dem_ctx* ctx, int64_t nsteps)
             if (prof) HIPCHK(hipEventRecord(ctx->events[4 * k + 2], ctx->stream));
             launch_force<NS>(ctx, 1, s + 1 < nsteps ? KICK_BOTH : KICK_AFTER);
             if (prof) HIPCHK(hipEventRecord(ctx->events[4 * k + 3], ctx->stream));
-            ctx->time += ctx->dt;
-            ctx->nt += 1;
-            ctx->alive_min = (uint32_t)ctx->nt;
-            ctx->prm.alive_min = ctx->alive_min;
+            advance_clock(ctx);
         }
         if (prof) {
             HIPCHK(hipStreamSynchronize(ctx->stream));

@@ -644,3 +644,51 @@ def test_engine_walls_removed_mid_run():
     assert_contacts_close(g, o)
     a, b = g.download(), o.download()
     np.testing.assert_allclose(a["x"], b["x"], rtol=0, atol=1e-9)
+
+
+@pytest.mark.parametrize("laws", ["full", "hvm"])
+def test_step_graph_replays_equal_launches(monkeypatch, laws):
+    """Steady steps of dem_step replayed from captured two-step HIP graphs
+    (DEM_STEP_GRAPH=1) against every step launched on its own (the
+    default): bit-identical states, springs, time stamps and
+    rebuild counts after calls of odd and even lengths, with a rebuild
+    almost every step, downloads and coarse graining between the calls, and
+    a re-upload of the state (new buffers: the graph is captured again)."""
+    sc = S.mixture(n=500, seed=31, lx=8.0, ly=8.0, lz=8.0, inflate=1.12, vmax=2.0, laws=laws,
+                   periodic=(True, True), walls=("bottom", "top"), radii=(0.15, 0.5))
+    outs = []
+    for graphs in ("1", ""):
+        if graphs:
+            monkeypatch.setenv("DEM_STEP_GRAPH", graphs)
+        else:
+            monkeypatch.delenv("DEM_STEP_GRAPH", raising=False)
+        g = dem.Engine(skin=0.004)
+        S.apply(g, sc)
+        g.compute_forces()
+        trace = []
+        for k, n in enumerate((7, 40, 5, 64, 3, 101)):
+            g.step(n)
+            st = g.download()
+            trace.append(st["x"].copy())
+            if k == 2:
+                g.cg_z(16, 0.5, lo=(0.0, 0.0, 0.0), hi=(8.0, 8.0, 8.0))
+            if k == 3:
+                ct = g.contacts()
+                stats = g.stats()
+                g.upload(reseeded(sc, st).state)
+                g.upload_contacts(ct)
+                g.set_time(stats.time, int(stats.n_steps))
+                g.compute_forces()
+        outs.append((trace, g.download(), g.contacts(), g.stats()))
+        g.close()
+    (ta, a, ca, sa), (tb, b, cb, sb) = outs
+    for xa, xb in zip(ta, tb):
+        np.testing.assert_array_equal(xa, xb)
+    for k in ("x", "v", "w", "q", "f", "t"):
+        np.testing.assert_array_equal(a[k], b[k])
+    assert ca.keys() == cb.keys()
+    oa, ob = np.lexsort((ca.i_id, ca.p_id)), np.lexsort((cb.i_id, cb.p_id))
+    np.testing.assert_array_equal(ca.spring[oa], cb.spring[ob])
+    np.testing.assert_array_equal(ca.ts[oa], cb.ts[ob])
+    assert sa.n_rebuilds == sb.n_rebuilds and sa.n_rebuilds > 100
+    assert sa.n_steps == sb.n_steps and sa.time == sb.time
