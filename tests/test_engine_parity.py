@@ -692,3 +692,28 @@ def test_step_graph_replays_equal_launches(monkeypatch, laws):
     np.testing.assert_array_equal(ca.ts[oa], cb.ts[ob])
     assert sa.n_rebuilds == sb.n_rebuilds and sa.n_rebuilds > 100
     assert sa.n_steps == sb.n_steps and sa.time == sb.time
+
+
+@pytest.mark.parametrize("offset", [(4096.3, -2048.7, 1024.1), (-3.0e5, 1.7e5, 2.5e4)])
+def test_contacts_far_from_the_origin(offset):
+    """A mixture translated far from the origin (cell coordinates in the
+    thousands and in the hundred-thousands) keeps the oracle's contact set,
+    springs and forces bit for bit through the first pass and one step, and
+    its contact set over 20 more."""
+    sc = S.mixture(n=400, seed=37, lx=7.0, ly=7.0, lz=7.0, inflate=1.12, laws="full", periodic=(False, False),
+                   walls=())
+    sc.state.x[:] += np.asarray(offset)
+    g, o = make_pair(sc)
+    g.compute_forces()
+    o.compute_forces()
+    assert assert_contacts_equal(g, o) > 100
+    assert_forces_close(g, o)
+    g, o = make_pair(reseeded(sc, o.download()), o.contacts())
+    g.step(1)
+    o.step(1)
+    assert_contacts_equal(g, o)
+    a, b = assert_forces_close(g, o)
+    np.testing.assert_array_equal(a["x"], b["x"])
+    g.step(20)
+    o.step(20)
+    assert_contacts_close(g, o)
