@@ -2608,7 +2608,11 @@ inline void advance_clock(dem_ctx* ctx)
 template <int NS>
 std::vector<unsigned char> graph_key(dem_ctx* ctx)
 {
-    Params q = ctx->prm;
+    // a byte copy (a member-wise copy may leave the padding indeterminate);
+    // differing padding could only cost a needless capture, never the reuse
+    // of a graph whose launches differ
+    Params q;
+    std::memcpy(&q, &ctx->prm, sizeof q);
     q.nt = 0;
     q.alive_min = 0;
     q.erase = 1;
