@@ -191,3 +191,58 @@ def test_closure128_trained_beats_constant():
     err_const = np.mean((ttr.mean(0) - tte) ** 2, axis=0)
     print(f"128-wide closure, synthetic rheology: error {err_net} against constant {err_const}")
     assert np.all(err_net < 0.05 * err_const), (err_net, err_const)
+
+
+def friction_data():
+    """{(friction, angle): states} of tests/golden/closure_chute_friction.npz."""
+    d = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "closure_chute_friction.npz"))
+    out = {}
+    for k in d.files:
+        mu, angle = k[2:].split("_angle")
+        out[(float(mu), float(angle))] = d[k].astype(np.float64)
+    return out
+
+
+def test_friction_data_follow_the_rheology():
+    """The chute states over friction and angle (the engine's own flows):
+    steady flows carry mu_eff = tan(theta), so the median mu_eff grows with
+    the angle at every friction; at one angle the flow of less frictional
+    spheres is faster (a larger inertial number)."""
+    data = friction_data()
+    for fr in (0.3, 0.6):
+        angles = sorted(a for f, a in data if f == fr)
+        med = np.array([np.median(data[(fr, a)][:, 4]) for a in angles])
+        assert np.all(np.diff(med) > 0), (fr, med)
+    for a in (22.0, 26.0):
+        I = [np.median(data[(fr, a)][:, 0]) for fr in (0.3, 0.45, 0.6)]
+        assert I[0] > I[1] > I[2], (a, I)
+
+
+@pytest.mark.gpu
+def test_closure128_on_held_out_angle_and_friction():
+    """The configured network trained on the chute states of frictions 0.3
+    and 0.6 at 20-28 degrees except 24.  Held-out angle: at 24 degrees (both
+    frictions) the steady flow carries mu_eff = tan(24) at every depth (force
+    balance), and the network's mean prediction over the run's states must
+    give it within 0.01.  Held-out friction (0.45, at 22 and 26 degrees;
+    friction is no input of the network, so it must be read from the state):
+    per state, a fraction of the squared mu_eff error of the constant closure
+    (the mean training mu_eff)."""
+    from mercurydpm_fork_amd import closure_data as CD
+    data = friction_data()
+    train = [k for k in data if k[0] in (0.3, 0.6) and k[1] != 24.0]
+    xtr, ttr = CD.standardise(np.concatenate([data[k] for k in train]))
+    net = closure.Closure(seed=3)
+    net.train(xtr, ttr, epochs=2000, lr=1e-3)
+    for key in ((0.3, 24.0), (0.6, 24.0)):
+        x, t = CD.standardise(data[key])
+        mu_net = float(np.mean(0.5 + 0.1 * net(x)[:, 0]))
+        print(f"held-out angle 24, friction {key[0]}: mean predicted mu_eff {mu_net:.4f}, "
+              f"DEM median {np.median(data[key][:, 4]):.4f}, tan(24) {np.tan(np.radians(24.0)):.4f}")
+        assert abs(mu_net - np.tan(np.radians(24.0))) < 0.01, (key, mu_net)
+    xte, tte = CD.standardise(np.concatenate([data[(0.45, 22.0)], data[(0.45, 26.0)]]))
+    err_net = np.mean((net(xte)[:, 0] - tte[:, 0]) ** 2)
+    err_const = np.mean((ttr[:, 0].mean() - tte[:, 0]) ** 2)
+    print(f"held-out friction 0.45: mu_eff error per state {err_net:.4g} against the constant closure's "
+          f"{err_const:.4g}")
+    assert err_net < 0.5 * err_const, (err_net, err_const)
