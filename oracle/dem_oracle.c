@@ -30,7 +30,13 @@
  * the three MindlinSelfTest .fstat files, the .ene files of
  * TwoParticleElasticCollisionSelfTest and FreeFallSelfTest and the .data file
  * of ExtremeOverlapUnitTest.  oracle_dem_download_interactions: its forces sum
- * to the particle forces and its geometry follows from the radii.
+ * to the particle forces and its geometry follows from the radii.  The
+ * rolling and torsion springs of FrictionInteraction: the .torque gold files
+ * of RollingFrictionSelfTest and TorsionFrictionSelfTest, and
+ * PeriodicWallsWithSlidingFrictionUnitTest.  oracle_dem_cg_z: the .stat gold
+ * of CGBasicSelfTest and the closed forms of the density and virial stress
+ * integrals.  Wall groups (oracle_dem_set_wall_groups): every member of a
+ * packed ensemble evolves as its own run (tests/test_ensemble.py).
  *
  * Built with -fopenmp (oracle/liboracle_dem_omp.so, the multi-core CPU
  * baseline of bench.py; never a parity build) the pair, wall and integration
