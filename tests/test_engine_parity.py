@@ -717,3 +717,58 @@ def test_contacts_far_from_the_origin(offset):
     g.step(20)
     o.step(20)
     assert_contacts_close(g, o)
+
+
+def test_removal_reorients_periodic_image_interactions_only():
+    """The one deviation after removals: the reference orients an interaction
+    found through a periodic image by handler index (the lower index is P,
+    InteractionHandler.cc:327-368), the engine by id.  The highest-index
+    sphere with an image contact is made the last one and then moves to index
+    0 (removeObject's swap with the last), so its image interactions change
+    orientation in the reference and not in the engine.  After the step every
+    stored interaction of the engine is the oracle's, either as stored or
+    reversed -- P and I exchanged, sliding and torsion springs negated,
+    rolling not (reverseHistory), to round-off -- with at least one
+    reversed; forces and positions agree."""
+    sc = S.mixture(n=400, seed=53, lx=7.0, ly=7.0, lz=7.0, inflate=1.12, laws="full", periodic=(True, True),
+                   walls=("bottom",))
+    ref = oracle_lib.engine()
+    S.apply(ref, sc)
+    ref.compute_forces()
+    ref.step(20)
+    st, ct = ref.download(), ref.contacts()
+    x = st["x"]
+    index = {int(v): k for k, v in enumerate(sc.state.id)}
+    pair = (ct.i_id & dem.DEM_WALL_KEY) == 0
+    pid, iid = ct.p_id[pair], ct.i_id[pair]
+    dx = np.abs(x[[index[int(p)] for p in pid]] - x[[index[int(i)] for i in iid]])
+    image = (dx[:, 0] > 3.5) | (dx[:, 1] > 3.5)
+    involved = {index[int(p)] for p in pid[image]} | {index[int(i)] for i in iid[image]}
+    h = max(involved)
+    assert 0 < h < 399
+    gone = list(range(399, h, -1)) + [0]  # pop everything above h, then h (now last) takes index 0
+    g, o = make_pair(reseeded(sc, st), ct)
+    for e in (g, o):
+        e.set_time(20 * sc.dt, 20)
+        e.step_begin()
+        e.remove_particles(gone)
+        e.step_end()
+    assert g.n == o.n == 400 - len(gone)
+    cg, co = g.contacts(), o.contacts()
+    oracle = {(int(p), int(i)): s for p, i, s in zip(co.p_id, co.i_id, co.spring)}
+    flip = np.array([-1, -1, -1, 1, 1, 1, -1, -1, -1], dtype=float)
+    scale = np.abs(co.spring).max()
+    reversed_pairs = 0
+    assert cg.n == co.n
+    for p, i, s in zip(cg.p_id, cg.i_id, cg.spring):
+        key = (int(p), int(i))
+        if key in oracle:
+            np.testing.assert_array_equal(s, oracle[key])
+        else:
+            # evaluated from the other side: the contact point, hence the
+            # relative velocity, rounds differently
+            np.testing.assert_allclose(s, flip * oracle[(int(i), int(p))], rtol=1e-10, atol=1e-12 * scale)
+            reversed_pairs += 1
+    assert reversed_pairs > 0
+    a, b = assert_forces_close(g, o)
+    np.testing.assert_array_equal(a["x"], b["x"])
