@@ -204,6 +204,9 @@ typedef struct dem_stats {
     double t_force_ms;            /* pair/wall/external force kernel */
     double t_kick_ms;             /* velocity-Verlet kick/drift kernels */
     double t_rebuild_ms;          /* grid sort + Verlet-list rebuild sequence */
+    /* particles the device arrays hold: dem_add_particles appends in place
+     * up to this count (beyond it, it re-uploads into larger arrays) */
+    int64_t capacity;
 } dem_stats;
 
 /* Create/destroy a context bound to one GPU.  No counterpart object in the
@@ -323,7 +326,12 @@ int dem_set_kinematics(dem_ctx* ctx, int64_t n, const double* x, const double* v
  * dem_step_begin and dem_step_end, where the reference's boundaries insert
  * (DPMBase.cc:4348-4351): the new particles take part in that step's force
  * pass and closing half-kick.  An empty system (dem_upload_particles with
- * n = 0) is allowed: its steps only advance the clock. */
+ * n = 0) is allowed: its steps only advance the clock.
+ * Cost: when the particles fit the resident arrays (n + k <= capacity of
+ * dem_get_stats) and each radius is within the largest radius of its grid
+ * level, only the k new particles are copied; otherwise the whole state and
+ * its interactions go through the host once, into arrays with a quarter more
+ * room for the appends that follow. */
 int dem_add_particles(dem_ctx* ctx, int64_t k, const uint32_t* id, const int32_t* species, const double* radius,
                       const double* inv_mass, const double* inv_inertia, const double* x, const double* v,
                       const double* w, const double* q);

@@ -2179,6 +2179,22 @@ __global__ void k_zero_masks(Dev<NS> d, Params p)
     d.disp[i] = make_double4(0.0, 0.0, 0.0, 0.0);
 }
 
+// Particles appended at [n0, n0 + k) by dem_add_particles: no list, no
+// stored interactions, no displacement since the (forced) next rebuild.
+template <int NS>
+__global__ void k_clear_appended(Dev<NS> d, int n0, int k)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= k) return;
+    const int i = n0 + j;
+    d.mask[0][i] = 0;
+    d.mask[1][i] = 0;
+    d.wmask[0][i] = 0;
+    d.wmask[1][i] = 0;
+    d.ncount[i] = 0;
+    d.disp[i] = make_double4(0.0, 0.0, 0.0, 0.0);
+}
+
 // WallHandler::removeObject during a run: the interactions with walls whose
 // index is not in `keep` are deleted.
 template <int NS>
@@ -2312,7 +2328,9 @@ struct dem_ctx {
     int64_t last_rebuilds = 0;
     uint32_t alive_min = 0;
     uint32_t max_id = 0;          // largest resident particle id
-    bool hash_pending = false;    // uploaded interactions wait in the hash for the next rebuild
+    int64_t cap = 0;              // particles the per-particle arrays hold (>= n)
+    double hl[kMaxLevels] = {};   // grid levels: a particle of diameter <= hl[l] goes to the first such l
+    bool hash_pending = false;   // uploaded interactions wait in the hash for the next rebuild
     int64_t rebuilds_before = 0;  // rebuilds counted before the last re-upload by dem_add_particles
     // candidate buffer of dem_check_particles (not in allocs: it outlives re-uploads)
     void* d_check = nullptr;
@@ -2431,6 +2449,8 @@ Dev<3>& dev<3>(dem_ctx* c) { return c->d3; }
 template <>
 Dev<9>& dev<9>(dem_ctx* c) { return c->d9; }
 
+// Per-particle arrays for n particles: the resident count, or more, which
+// leaves room for dem_add_particles to append in place.
 template <int NS>
 int alloc_particles(dem_ctx* ctx, int64_t n)
 {
@@ -2814,6 +2834,84 @@ void refresh_params(dem_ctx* ctx)
     p.rotation = ctx->rotation;
 }
 
+// The checks of one particle i of an upload or append.
+int check_particle(dem_ctx* ctx, int64_t i, const uint32_t* id, const int32_t* species, const double* radius,
+                   const double* inv_mass, const double* inv_inertia, const double* x, const double* v, const double* w)
+{
+    for (int a = 0; a < 3; ++a)
+        if (!std::isfinite(x[3 * i + a]) || !std::isfinite(v[3 * i + a]) || (w && !std::isfinite(w[3 * i + a])))
+            return set_err(ctx, DEM_ERR_ARG, "positions and velocities must be finite");
+    if (id[i] >= DEM_WALL_KEY) return set_err(ctx, DEM_ERR_ARG, "particle ids must be < 2^31");
+    if (species[i] < 0 || species[i] >= ctx->ns) return set_err(ctx, DEM_ERR_ARG, "species index out of range");
+    if (!(radius[i] > 0) || !std::isfinite(radius[i]))
+        return set_err(ctx, DEM_ERR_ARG, "radius must be positive and finite");
+    const double* I = inv_inertia + 6 * i;
+    if (I[1] != 0.0 || I[2] != 0.0 || I[4] != 0.0 || I[0] != I[3] || I[0] != I[5])
+        return set_err(ctx, DEM_ERR_UNSUPPORTED, "inverse inertia must be isotropic (spheres)");
+    if ((inv_mass[i] == 0.0) != (I[0] == 0.0))
+        return set_err(ctx, DEM_ERR_ARG, "fixed particles need zero inverse mass and inertia");
+    return DEM_OK;
+}
+
+// The grid level of a particle of radius r (the first level whose diameter
+// threshold it does not exceed; the coarsest takes the rest).
+int level_of(const dem_ctx* ctx, double r)
+{
+    int l = 0;
+    while (l < ctx->prm.levels - 1 && 2.0 * r > ctx->hl[l]) ++l;
+    return l;
+}
+
+// dem_add_particles when the appended particles fit the resident grid: they
+// are written at [n, n + k) of the (sorted) particle arrays with no
+// interactions, and the rebuild forced for the next force pass sorts them
+// into their buckets.  Only the k new particles cross PCIe.
+template <int NS>
+int append_in_place(dem_ctx* ctx, int64_t k, const uint32_t* id, const int32_t* species, const double* radius,
+                    const double* inv_mass, const double* inv_inertia, const double* x, const double* v,
+                    const double* w, const double* q)
+{
+    Dev<NS>& d = dev<NS>(ctx);
+    hipStream_t s = ctx->stream;
+    const int64_t n0 = ctx->n, m = n0 + k;
+    std::vector<double> h4(24 * k);
+    std::vector<uint32_t> u(3 * k);
+    double* hp = h4.data();
+    for (int64_t j = 0; j < k; ++j) {
+        double* o = hp + 4 * j;
+        o[0] = x[3 * j]; o[1] = x[3 * j + 1]; o[2] = x[3 * j + 2]; o[3] = radius[j];
+        o = hp + 4 * (k + j);
+        o[0] = v[3 * j]; o[1] = v[3 * j + 1]; o[2] = v[3 * j + 2]; o[3] = inv_mass[j];
+        o = hp + 4 * (2 * k + j);
+        for (int a = 0; a < 3; ++a) o[a] = w ? w[3 * j + a] : 0.0;
+        o[3] = inv_inertia[6 * j];
+        o = hp + 4 * (3 * k + j);
+        for (int a = 0; a < 4; ++a) o[a] = q ? q[4 * j + a] : (a == 0 ? 1.0 : 0.0);
+        u[j] = id[j];
+        u[k + j] = (uint32_t)(n0 + j);  // handler index: appended at the end
+        u[2 * k + j] = (uint32_t)species[j] | ((uint32_t)level_of(ctx, radius[j]) << 16);
+    }
+    // force and torque (the last 2k entries) start at zero
+    double4* dst[6] = {d.posr, d.vel, d.omg, d.quat, d.frc, d.trq};
+    HIPCHK(hipStreamSynchronize(s));
+    for (int a = 0; a < 6; ++a)
+        HIPCHK(hipMemcpyAsync(dst[a] + n0, hp + 4 * a * k, sizeof(double4) * k, hipMemcpyHostToDevice, s));
+    uint32_t* udst[3] = {d.id, d.orig, d.spc};
+    for (int a = 0; a < 3; ++a)
+        HIPCHK(hipMemcpyAsync(udst[a] + n0, u.data() + a * k, sizeof(uint32_t) * k, hipMemcpyHostToDevice, s));
+    const uint32_t end = (uint32_t)m;
+    const int one = 1;
+    HIPCHK(hipMemcpyAsync(d.start + ctx->nbuckets, &end, sizeof end, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(ctx->d_flags + F_REBUILD, &one, sizeof one, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_clear_appended<NS>, dim3(grid_for(k)), dim3(kBlock), 0, s, d, (int)n0, (int)k);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(s));
+    ctx->n = m;
+    ctx->prm.n = (int)m;
+    ctx->max_id = id[k - 1];
+    return DEM_OK;
+}
+
 // An empty ParticleHandler (a run that starts before its insertion
 // boundaries have added anything): no particle arrays, only the tables that
 // dem_check_particles reads.  Steps then only advance the clock.
@@ -2859,6 +2957,7 @@ int upload_empty(dem_ctx* ctx)
     ctx->cur = 0;
     ctx->alive_min = 0;
     ctx->rebuilds_before = 0;
+    ctx->cap = 0;
     ctx->uploaded = true;
     ctx->in_step = false;
     return DEM_OK;
@@ -3191,35 +3290,28 @@ int dem_set_time(dem_ctx* ctx, double time, int64_t nt)
     return DEM_OK;
 }
 
-int dem_upload_particles(dem_ctx* ctx, int64_t n, const uint32_t* id, const int32_t* species, const double* radius,
-                         const double* inv_mass, const double* inv_inertia, const double* x, const double* v,
-                         const double* w, const double* q, const double* f, const double* t)
+// The upload of dem_upload_particles into arrays for cap >= n particles.
+static int upload_particles(dem_ctx* ctx, int64_t n, int64_t cap, const uint32_t* id, const int32_t* species,
+                     const double* radius, const double* inv_mass, const double* inv_inertia, const double* x,
+                     const double* v, const double* w, const double* q, const double* f, const double* t)
 {
-    if (!ctx || n < 0) return set_err(ctx, DEM_ERR_ARG, "upload_particles: bad arguments");
+    if (!ctx || n < 0 || cap < n) return set_err(ctx, DEM_ERR_ARG, "upload_particles: bad arguments");
     if (ctx->ns == 0) return set_err(ctx, DEM_ERR_STATE, "set_species before upload_particles");
     if (n == 0) return upload_empty(ctx);
     if (!id || !species || !radius || !inv_mass || !inv_inertia || !x || !v)
         return set_err(ctx, DEM_ERR_ARG, "upload_particles: bad arguments");
-    if (n > (int64_t)0x7FFFFFFF) return set_err(ctx, DEM_ERR_ARG, "too many particles");
+    if (cap > (int64_t)0x7FFFFFFF) return set_err(ctx, DEM_ERR_ARG, "too many particles");
     HIPCHK(hipSetDevice(ctx->device));
     // --- validate and derive the grid on the host
     double rmin = 1e300, rmax = 0.0;
     double xmin[3] = {1e300, 1e300, 1e300}, xmax[3] = {-1e300, -1e300, -1e300};
     for (int64_t i = 0; i < n; ++i) {
+        const int e = check_particle(ctx, i, id, species, radius, inv_mass, inv_inertia, x, v, w);
+        if (e) return e;
         for (int a = 0; a < 3; ++a) {
-            if (!std::isfinite(x[3 * i + a]) || !std::isfinite(v[3 * i + a]) || (w && !std::isfinite(w[3 * i + a])))
-                return set_err(ctx, DEM_ERR_ARG, "positions and velocities must be finite");
             xmin[a] = std::min(xmin[a], x[3 * i + a]);
             xmax[a] = std::max(xmax[a], x[3 * i + a]);
         }
-        if (id[i] >= DEM_WALL_KEY) return set_err(ctx, DEM_ERR_ARG, "particle ids must be < 2^31");
-        if (species[i] < 0 || species[i] >= ctx->ns) return set_err(ctx, DEM_ERR_ARG, "species index out of range");
-        if (!(radius[i] > 0) || !std::isfinite(radius[i])) return set_err(ctx, DEM_ERR_ARG, "radius must be positive and finite");
-        const double* I = inv_inertia + 6 * i;
-        if (I[1] != 0.0 || I[2] != 0.0 || I[4] != 0.0 || I[0] != I[3] || I[0] != I[5])
-            return set_err(ctx, DEM_ERR_UNSUPPORTED, "inverse inertia must be isotropic (spheres)");
-        if ((inv_mass[i] == 0.0) != (I[0] == 0.0))
-            return set_err(ctx, DEM_ERR_ARG, "fixed particles need zero inverse mass and inertia");
         rmin = std::min(rmin, radius[i]);
         rmax = std::max(rmax, radius[i]);
     }
@@ -3358,8 +3450,10 @@ int dem_upload_particles(dem_ctx* ctx, int64_t n, const uint32_t* id, const int3
     for (const auto& wl : ctx->walls)
         if (wl.species < 0 || wl.species >= ctx->ns) return set_err(ctx, DEM_ERR_ARG, "wall species out of range");
     ctx->n = n;
-    r = ctx->ns_springs == 9 ? alloc_particles<9>(ctx, n) : alloc_particles<3>(ctx, n);
+    r = ctx->ns_springs == 9 ? alloc_particles<9>(ctx, cap) : alloc_particles<3>(ctx, cap);
     if (r) return r;
+    ctx->cap = cap;
+    for (int l = 0; l < kMaxLevels; ++l) ctx->hl[l] = l < L ? hl[l] : 0.0;
 
     // --- state
     std::vector<double> h4(4 * n);
@@ -3419,6 +3513,13 @@ int dem_upload_particles(dem_ctx* ctx, int64_t n, const uint32_t* id, const int3
     ctx->in_step = false;
     ctx->rebuilds_before = 0;
     return DEM_OK;
+}
+
+int dem_upload_particles(dem_ctx* ctx, int64_t n, const uint32_t* id, const int32_t* species, const double* radius,
+                         const double* inv_mass, const double* inv_inertia, const double* x, const double* v,
+                         const double* w, const double* q, const double* f, const double* t)
+{
+    return upload_particles(ctx, n, n, id, species, radius, inv_mass, inv_inertia, x, v, w, q, f, t);
 }
 
 int dem_upload_contacts(dem_ctx* ctx, int64_t n, const uint32_t* p_id, const uint32_t* i_id, const double* spring,
@@ -3590,8 +3691,12 @@ int dem_set_kinematics(dem_ctx* ctx, int64_t n, const double* x, const double* v
     return ctx->ns_springs == 9 ? run_set_kinematics<9>(ctx, x, v, w, q) : run_set_kinematics<3>(ctx, x, v, w, q);
 }
 
-// The appended particles join the resident ones through a re-upload of the
-// whole state and history: the grid (levels, cell sizes, bucket box) is
+// Appended particles that fit the resident arrays and grid (room for them,
+// each radius within the largest of its grid level) are written in place at
+// the end of the particle arrays; the rebuild forced for the next force pass
+// sorts them in.  Otherwise they join through a re-upload of the whole state
+// and history into arrays with a quarter more room (at least 1024 particles)
+// for the appends that follow: the grid (levels, cell sizes, bucket box) is
 // derived from the full particle set again, and the stored interactions are
 // matched to the new sorted order by their id keys, as after any re-sort.
 int dem_add_particles(dem_ctx* ctx, int64_t k, const uint32_t* id, const int32_t* species, const double* radius,
@@ -3610,6 +3715,18 @@ int dem_add_particles(dem_ctx* ctx, int64_t k, const uint32_t* id, const int32_t
     if (ctx->n + k > (int64_t)0x7FFFFFFF) return set_err(ctx, DEM_ERR_ARG, "too many particles");
     HIPCHK(hipSetDevice(ctx->device));
     const int64_t n = ctx->n, m = n + k;
+    if (n > 0 && m <= ctx->cap) {
+        bool fits = true;
+        for (int64_t j = 0; j < k; ++j) {
+            if ((r = check_particle(ctx, j, id, species, radius, inv_mass, inv_inertia, x, v, w)) != 0) return r;
+            fits = fits && radius[j] <= ctx->prm.rmaxl[level_of(ctx, radius[j])];
+        }
+        if (fits) {
+            if ((r = check_flags(ctx)) != 0) return r;
+            return ctx->ns_springs == 9 ? append_in_place<9>(ctx, k, id, species, radius, inv_mass, inv_inertia, x, v, w, q)
+                                        : append_in_place<3>(ctx, k, id, species, radius, inv_mass, inv_inertia, x, v, w, q);
+        }
+    }
     std::vector<uint32_t> aid;
     std::vector<int32_t> asp;
     std::vector<double> ar, aim, aii;
@@ -3648,8 +3765,9 @@ int dem_add_particles(dem_ctx* ctx, int64_t k, const uint32_t* id, const int32_t
     }
     const bool in_step = ctx->in_step;
     const uint32_t alive_min = ctx->alive_min;
-    if ((r = dem_upload_particles(ctx, m, aid.data(), asp.data(), ar.data(), aim.data(), aii.data(), ax.data(),
-                                  av.data(), aw.data(), aq.data(), af.data(), at.data())) != 0)
+    const int64_t cap = std::min<int64_t>(m + std::max<int64_t>(m / 4, 1024), 0x7FFFFFFF);
+    if ((r = upload_particles(ctx, m, cap, aid.data(), asp.data(), ar.data(), aim.data(), aii.data(), ax.data(),
+                              av.data(), aw.data(), aq.data(), af.data(), at.data())) != 0)
         return r;
     if (nc > 0 && (r = dem_upload_contacts(ctx, nc, cp.data(), ci.data(), cs.data(), cts.data())) != 0) return r;
     ctx->in_step = in_step;
@@ -4038,6 +4156,7 @@ int dem_get_stats(dem_ctx* ctx, dem_stats* s)
     s->t_force_ms = ctx->t_force;
     s->t_kick_ms = ctx->t_kick;
     s->t_rebuild_ms = ctx->t_rebuild;
+    s->capacity = ctx->cap;
     if (!ctx->uploaded || ctx->n == 0) return DEM_OK;
     int r = check_flags(ctx);
     if (r) return r;

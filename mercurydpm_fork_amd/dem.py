@@ -96,7 +96,7 @@ class Stats(C.Structure):
                 ("n_list_entries", C.c_int64), ("max_list_length", C.c_int32),
                 ("grid_levels", C.c_int32), ("time", C.c_double), ("skin", C.c_double),
                 ("cell_size", C.c_double * 4), ("n_timed_steps", C.c_int64), ("t_force_ms", C.c_double),
-                ("t_kick_ms", C.c_double), ("t_rebuild_ms", C.c_double)]
+                ("t_kick_ms", C.c_double), ("t_rebuild_ms", C.c_double), ("capacity", C.c_int64)]
 
 
 _PTR = C.c_void_p

@@ -531,6 +531,61 @@ def test_add_particles_mid_step(case):
     np.testing.assert_allclose(a["x"], b["x"], rtol=0, atol=1e-9)
 
 
+@pytest.mark.parametrize("case", [CASES[0], CASES[4]], ids=["lvf-P", "hvm-Px"])
+def test_add_particles_in_place(case):
+    """Appends that fit the resident arrays and grid are written in place:
+    the capacity dem_get_stats reports stays, and only the new particles are
+    copied.  Three batches: the first goes through a re-upload that leaves
+    room, the next two are appended in place, one between step_begin and
+    step_end and one between steps.  After the step that takes the first
+    in-place batch, contacts and positions equal the oracle's and forces
+    agree to round-off; 20 steps after the last batch the states still agree
+    to round-off."""
+    sc = S.mixture(n=400, seed=47, lx=7.0, ly=7.0, lz=7.0, inflate=1.12, **case)
+    # the largest particles first, so that each later one is within the
+    # largest radius of its grid level; ids in insertion order
+    order = np.argsort(-sc.state.radius, kind="stable")
+    sc = _subset(sc, order)
+    sc.state.id = np.arange(1, 401, dtype=np.uint32)
+    batches = [np.arange(0, 200), np.arange(200, 250), np.arange(250, 330), np.arange(330, 400)]
+    base = _subset(sc, batches[0])
+    ref = oracle_lib.engine()
+    S.apply(ref, base)
+    ref.compute_forces()
+    g, o = make_pair(reseeded(base, ref.download()), ref.contacts())
+    assert g.stats().capacity == 200
+    for e in (g, o):
+        e.add_particles(_subset(sc, batches[1]).state)
+    cap = g.stats().capacity
+    assert cap == 250 + 1024
+    for e in (g, o):
+        e.step_begin()
+        e.add_particles(_subset(sc, batches[2]).state)
+        e.step_end()
+    assert g.n == o.n == 330
+    assert g.stats().capacity == cap  # in place
+    assert_contacts_equal(g, o)
+    a, b = assert_forces_close(g, o)
+    np.testing.assert_array_equal(a["x"], b["x"])
+    g.step(5)
+    o.step(5)
+    # ids above those the periodic images took meanwhile on the oracle
+    # (each image of PeriodicBoundary::createPeriodicParticle takes the
+    # handler's next id)
+    last = _subset(sc, batches[3]).state
+    last.id = last.id + np.uint32(100000)
+    for e in (g, o):
+        e.add_particles(last)
+    assert g.n == o.n == 400
+    assert g.stats().capacity == cap
+    g.step(20)
+    o.step(20)
+    assert_contacts_close(g, o)
+    a, b = g.download(), o.download()
+    np.testing.assert_allclose(a["x"], b["x"], rtol=0, atol=1e-9)
+    np.testing.assert_allclose(a["v"], b["v"], rtol=0, atol=1e-8)
+
+
 def test_empty_system_then_insertion():
     """A run that starts empty (its insertion boundaries fill it): steps of
     an empty system only advance the clock; particles added later run as on
