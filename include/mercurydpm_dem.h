@@ -48,7 +48,7 @@
 extern "C" {
 #endif
 
-#define DEM_ABI_VERSION 8
+#define DEM_ABI_VERSION 9
 
 /* error codes */
 #define DEM_OK 0
@@ -343,7 +343,10 @@ int dem_add_particles(dem_ctx* ctx, int64_t k, const uint32_t* id, const int32_t
  * refers to the handler as the previous removals left it.  Allowed between
  * steps and between dem_step_begin and dem_step_end, where deletion
  * boundaries act (DPMBase::checkInteractionWithBoundaries, DPMBase.cc:
- * 3272-3300). */
+ * 3272-3300).  Cost: the state stays on the device (only the k indices and
+ * the k deleted ids cross PCIe), plus one grid and list rebuild over the
+ * kept particles.  The ids of deleted particles are not given out again:
+ * later appends still need ids above every id the context has held. */
 int dem_remove_particles(dem_ctx* ctx, int64_t k, const int64_t* index);
 
 /* The particles a deletion boundary removes, found on the device: the

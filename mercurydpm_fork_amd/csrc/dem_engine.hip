@@ -31,6 +31,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/mercurydpm_dem.h"
@@ -50,6 +51,9 @@ constexpr int kMaxSlabs = 64;
 constexpr int kBlock = 256;
 constexpr int kWaves = kBlock / 64; // wavefronts per block
 constexpr uint64_t kEmptyKey = ~0ull;
+// marks, in its id, a particle that dem_remove_particles deletes in place
+// (particle ids are below 2^31)
+constexpr uint32_t kRemovedBit = 0x80000000u;
 
 // device flag words
 enum Flag {
@@ -462,6 +466,32 @@ __device__ inline bool hash_find(const Dev<NS>& d, uint64_t key, Rec<NS>& r)
     return false;
 }
 
+// The stored interactions of particle i (pair records in its list slots,
+// wall records in its wall slots) into the hash, keyed by ids.  With
+// REMOVING, pairs whose partner's id carries kRemovedBit are left out: the
+// interactions of a deleted particle go with it.
+template <int NS, bool REMOVING>
+__device__ inline void hash_records(Dev<NS>& d, const Params& p, int cur, int i, uint32_t idi)
+{
+    unsigned long long m = d.mask[cur][i];
+    while (m) {
+        const int k = __ffsll((long long)m) - 1;
+        m &= m - 1;
+        const uint32_t j = d.nbr[lslot(k, p.maxn, i)] & ~kWriterBit;
+        const uint32_t idj = d.id[j];
+        if (REMOVING && (idj & kRemovedBit)) continue;
+        hash_insert(d, ((uint64_t)idi << 32) | idj, d.rec[cur][lslot(k, p.maxn, i)]);
+    }
+    uint32_t wm = d.wmask[cur][i];
+    const int wfirst = wall_group(d, p, (int)(d.spc[i] & 0xFFFF)).x;
+    while (wm) {
+        const int l = __ffs(wm) - 1;
+        wm &= wm - 1;
+        const uint64_t key = ((uint64_t)idi << 32) | (DEM_WALL_KEY | (uint32_t)(wfirst + l));
+        hash_insert(d, key, d.wrec[cur][(size_t)i * p.wstride + l]);
+    }
+}
+
 template <int NS>
 __global__ void __launch_bounds__(kBlock) k_rb_begin(Dev<NS> d, Params p, int cur)
 {
@@ -482,23 +512,7 @@ __global__ void __launch_bounds__(kBlock) k_rb_begin(Dev<NS> d, Params p, int cu
             atomicAdd(&d.count[key], 1u);
             atomicAdd(&d.tsum[key / kScanTile], 1u);
         }
-        const uint32_t idi = d.id[i];
-        unsigned long long m = d.mask[cur][i];
-        while (m) {
-            const int k = __ffsll((long long)m) - 1;
-            m &= m - 1;
-            const uint32_t j = d.nbr[lslot(k, p.maxn, (int)i)] & ~kWriterBit;
-            const uint64_t key = ((uint64_t)idi << 32) | d.id[j];
-            hash_insert(d, key, d.rec[cur][lslot(k, p.maxn, (int)i)]);
-        }
-        uint32_t wm = d.wmask[cur][i];
-        const int wfirst = wall_group(d, p, (int)(d.spc[i] & 0xFFFF)).x;
-        while (wm) {
-            const int l = __ffs(wm) - 1;
-            wm &= wm - 1;
-            const uint64_t key = ((uint64_t)idi << 32) | (DEM_WALL_KEY | (uint32_t)(wfirst + l));
-            hash_insert(d, key, d.wrec[cur][(size_t)i * p.wstride + l]);
-        }
+        hash_records<NS, false>(d, p, cur, (int)i, d.id[i]);
     }
     // orphans still alive at this step (time stamp >= alive_min) go back in
     const int par = d.flags[F_ORPH_PARITY];
@@ -2195,6 +2209,124 @@ __global__ void k_clear_appended(Dev<NS> d, int n0, int k)
     d.disp[i] = make_double4(0.0, 0.0, 0.0, 0.0);
 }
 
+// dem_remove_particles in place.  The host turns the removals
+// (BaseHandler::removeObject: the last particle takes the removed index) into
+// changes of handler index, chg: (old index, new index) sorted by old index,
+// new == kRemovedBit for a deleted particle.
+//
+// 1. Each particle applies its change to orig; a deleted one is marked in its
+//    id.  Listed: the deleted ids (cnt[0]), the holes (deleted particles
+//    below the new count, cnt[1]) and the movers (kept particles at or above
+//    it, cnt[2]); there are as many holes as movers.
+// (Each list holds k entries; a count beyond k, which a consistent orig
+// never gives, stops the writes and is refused by the host.)
+template <int NS>
+__global__ void k_remove_mark(Dev<NS> d, int n, int n_new, const uint2* chg, int nchg, uint32_t* cnt, uint32_t* ids,
+                              uint32_t* holes, uint32_t* movers, uint32_t k)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t o = d.orig[i];
+    int lo = 0, hi = nchg;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (chg[mid].x < o)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    const bool hit = lo < nchg && chg[lo].x == o;
+    const bool gone = hit && chg[lo].y == kRemovedBit;
+    if (gone) {
+        const uint32_t id = d.id[i];
+        const uint32_t t = atomicAdd(&cnt[0], 1u);
+        if (t < k) ids[t] = id;
+        d.id[i] = id | kRemovedBit;
+        if (i < n_new) {
+            const uint32_t u = atomicAdd(&cnt[1], 1u);
+            if (u < k) holes[u] = (uint32_t)i;
+        }
+    } else {
+        if (hit) d.orig[i] = chg[lo].y;
+        if (i >= n_new) {
+            const uint32_t u = atomicAdd(&cnt[2], 1u);
+            if (u < k) movers[u] = (uint32_t)i;
+        }
+    }
+}
+
+__device__ inline bool in_sorted(const uint32_t* a, int n, uint32_t v)
+{
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (a[mid] < v)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo < n && a[lo] == v;
+}
+
+// 2. The stored interactions of the kept particles with kept partners go to
+//    the hash (as at the start of a rebuild), and every list and wall mask is
+//    cleared: the lists are rebuilt over the compacted particles, where the
+//    rebuild matches these records again by their id keys.  The orphaned
+//    interactions still alive whose particles are both kept go to the hash
+//    too (dead: the sorted deleted ids), and the orphan buffer is emptied.
+template <int NS>
+__global__ void k_remove_hash(Dev<NS> d, Params p, int cur, const uint32_t* dead, int ndead)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < p.n) {
+        const uint32_t idi = d.id[i];
+        if (!(idi & kRemovedBit)) hash_records<NS, true>(d, p, cur, i, idi);
+    }
+    const int par = d.flags[F_ORPH_PARITY];
+    const int no = min(d.flags[F_ORPH_COUNT0 + par], d.ocap);
+    for (int k = i; k < no; k += gridDim.x * blockDim.x) {
+        const Rec<NS> r = d.orec[par][k];
+        const uint64_t key = d.okey[par][k];
+        const uint32_t a = (uint32_t)(key >> 32), b = (uint32_t)key;
+        if (r.ts >= step_alive_min(p) && !in_sorted(dead, ndead, a) &&
+            ((b & DEM_WALL_KEY) || !in_sorted(dead, ndead, b)))
+            hash_insert(d, key, r);
+    }
+}
+
+// (after k_remove_hash, whose every thread reads the masks and the orphan count)
+template <int NS>
+__global__ void k_remove_clear(Dev<NS> d, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) d.flags[F_ORPH_COUNT0 + d.flags[F_ORPH_PARITY]] = 0;
+    if (i >= n) return;
+    d.mask[0][i] = 0;
+    d.mask[1][i] = 0;
+    d.wmask[0][i] = 0;
+    d.wmask[1][i] = 0;
+}
+
+// 3. Each mover fills a hole: the particles end at [0, n_new), in an order
+//    the rebuild that follows sorts again.
+template <int NS>
+__global__ void k_remove_compact(Dev<NS> d, const uint32_t* holes, const uint32_t* movers, int h)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= h) return;
+    const uint32_t s = movers[t], o = holes[t];
+    d.posr[o] = d.posr[s];
+    d.vel[o] = d.vel[s];
+    d.omg[o] = d.omg[s];
+    d.quat[o] = d.quat[s];
+    d.frc[o] = d.frc[s];
+    d.trq[o] = d.trq[s];
+    d.disp[o] = d.disp[s];
+    d.id[o] = d.id[s];
+    d.orig[o] = d.orig[s];
+    d.spc[o] = d.spc[s];
+}
+
 // WallHandler::removeObject during a run: the interactions with walls whose
 // index is not in `keep` are deleted.
 template <int NS>
@@ -2338,6 +2470,9 @@ struct dem_ctx {
     // counter and index buffer of dem_find_in_region (grow-only, outlives re-uploads)
     void* d_region = nullptr;
     int64_t region_cap = 0;
+    // index changes and lists of dem_remove_particles in place (grow-only)
+    void* d_remove = nullptr;
+    int64_t remove_cap = 0;
     // profiling
     bool profiling = false;
     std::vector<hipEvent_t> events;
@@ -2912,6 +3047,94 @@ int append_in_place(dem_ctx* ctx, int64_t k, const uint32_t* id, const int32_t* 
     return DEM_OK;
 }
 
+// dem_remove_particles when particles remain: the removals
+// (BaseHandler::removeObject, BaseHandler.h:472-505: the last particle takes
+// each removed index) become a list of handler-index changes, at most 2k
+// long; the device applies it, drops the deleted particles' interactions,
+// compacts the particle arrays and rebuilds the grid and lists over the kept
+// ones.  Nothing of the particle state crosses PCIe.
+template <int NS>
+int remove_in_place(dem_ctx* ctx, int64_t k, const int64_t* index)
+{
+    Dev<NS>& d = dev<NS>(ctx);
+    hipStream_t s = ctx->stream;
+    const int64_t n = ctx->n, n_new = n - k;
+    // H: handler position -> original index of the particle there, where
+    // the two differ
+    std::unordered_map<int64_t, int64_t> H;
+    auto at = [&](int64_t h) {
+        const auto it = H.find(h);
+        return it == H.end() ? h : it->second;
+    };
+    std::vector<uint2> chg;
+    int64_t m = n;
+    for (int64_t j = 0; j < k; ++j) {
+        const int64_t i = index[j], last = m - 1;
+        chg.push_back(make_uint2((uint32_t)at(i), kRemovedBit));
+        const int64_t moved = at(last);
+        if (i != last) H[i] = moved;
+        H.erase(last);
+        --m;
+    }
+    for (const auto& e : H)
+        if (e.second != e.first) chg.push_back(make_uint2((uint32_t)e.second, (uint32_t)e.first));
+    std::sort(chg.begin(), chg.end(), [](uint2 a, uint2 b) { return a.x < b.x; });
+    const size_t nchg = chg.size();
+    const size_t bytes = sizeof(uint2) * nchg + 4 * sizeof(uint32_t) + 4 * sizeof(uint32_t) * (size_t)k;
+    if ((int64_t)bytes > ctx->remove_cap) {
+        HIPCHK(hipStreamSynchronize(s));
+        if (ctx->d_remove) (void)hipFree(ctx->d_remove);
+        ctx->d_remove = nullptr;
+        ctx->remove_cap = 0;
+        HIPCHK(hipMalloc(&ctx->d_remove, bytes));
+        ctx->remove_cap = (int64_t)bytes;
+    }
+    uint2* dchg = (uint2*)ctx->d_remove;
+    uint32_t* cnt = (uint32_t*)(dchg + nchg);
+    uint32_t* ids = cnt + 4;
+    uint32_t* holes = ids + k;
+    uint32_t* movers = holes + k;
+    uint32_t* dead = movers + k;
+    HIPCHK(hipMemcpyAsync(dchg, chg.data(), sizeof(uint2) * nchg, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemsetAsync(cnt, 0, 4 * sizeof(uint32_t), s));
+    hipLaunchKernelGGL(k_remove_mark<NS>, dim3(grid_for(n)), dim3(kBlock), 0, s, d, (int)n, (int)n_new,
+                       (const uint2*)dchg, (int)nchg, cnt, ids, holes, movers, (uint32_t)k);
+    HIPCHK(hipGetLastError());
+    uint32_t hc[4];
+    std::vector<uint32_t> hid(k);
+    HIPCHK(hipMemcpyAsync(hc, cnt, sizeof hc, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(hid.data(), ids, sizeof(uint32_t) * k, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if ((int64_t)hc[0] != k || hc[1] != hc[2] || (int64_t)hc[1] > k)
+        return set_err(ctx, DEM_ERR_STATE, "internal error: in-place removal found " + std::to_string(hc[0]) +
+                                               " of " + std::to_string(k) + " particles");
+    std::sort(hid.begin(), hid.end());
+    HIPCHK(hipMemcpyAsync(dead, hid.data(), sizeof(uint32_t) * k, hipMemcpyHostToDevice, s));
+    // a rebuild outside the step's own, as for uploaded interactions
+    // (match_uploaded): unmatched interactions stay as orphans
+    refresh_params(ctx);
+    Params& p = ctx->prm;
+    p.erase = 0;
+    p.nt = (uint32_t)ctx->nt;
+    p.alive_min = ctx->alive_min;
+    hipLaunchKernelGGL(k_remove_hash<NS>, dim3(grid_for(n)), dim3(kBlock), 0, s, d, p, ctx->cur, (const uint32_t*)dead,
+                       (int)k);
+    hipLaunchKernelGGL(k_remove_clear<NS>, dim3(grid_for(n)), dim3(kBlock), 0, s, d, (int)n);
+    if (hc[1] > 0)
+        hipLaunchKernelGGL(k_remove_compact<NS>, dim3(grid_for(hc[1])), dim3(kBlock), 0, s, d, (const uint32_t*)holes,
+                           (const uint32_t*)movers, (int)hc[1]);
+    HIPCHK(hipGetLastError());
+    const uint32_t end = (uint32_t)n_new;
+    const int one = 1;
+    HIPCHK(hipMemcpyAsync(d.start + ctx->nbuckets, &end, sizeof end, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(ctx->d_flags + F_REBUILD, &one, sizeof one, hipMemcpyHostToDevice, s));
+    ctx->n = n_new;
+    p.n = (int)n_new;
+    launch_rebuild<NS>(ctx);
+    HIPCHK(hipGetLastError());
+    return check_flags(ctx);
+}
+
 // An empty ParticleHandler (a run that starts before its insertion
 // boundaries have added anything): no particle arrays, only the tables that
 // dem_check_particles reads.  Steps then only advance the clock.
@@ -3108,6 +3331,7 @@ int dem_destroy(dem_ctx* ctx)
     free_all(ctx);
     if (ctx->d_check) (void)hipFree(ctx->d_check);
     if (ctx->d_region) (void)hipFree(ctx->d_region);
+    if (ctx->d_remove) (void)hipFree(ctx->d_remove);
     for (hipEvent_t e : ctx->events) (void)hipEventDestroy(e);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -3793,6 +4017,11 @@ int dem_remove_particles(dem_ctx* ctx, int64_t k, const int64_t* index)
     }
     HIPCHK(hipSetDevice(ctx->device));
     if ((r = check_flags(ctx)) != 0) return r;
+    if (k < n) {
+        if ((r = match_uploaded(ctx)) != 0) return r;
+        return ctx->ns_springs == 9 ? remove_in_place<9>(ctx, k, index) : remove_in_place<3>(ctx, k, index);
+    }
+    // every particle goes: an empty system
     const int64_t rebuilds = ctx->last_rebuilds;
     std::vector<uint32_t> aid;
     std::vector<int32_t> asp;

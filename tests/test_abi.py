@@ -36,8 +36,9 @@ def test_library_exports_every_declared_symbol():
 
 def test_abi_version_and_struct_sizes():
     lib = dem.load_hip_library()
-    assert lib.dem_abi_version() == 8
+    assert lib.dem_abi_version() == 9
     assert ctypes.sizeof(dem.PairParams) == 4 * 4 + 17 * 8
+    assert ctypes.sizeof(dem.Stats) == 6 * 8 + 2 * 4 + 6 * 8 + 8 + 3 * 8 + 8
     assert ctypes.sizeof(dem.Wall) == 16 + 10 * 8 + 3 * 4 * 3 * 8
     assert ctypes.sizeof(dem.Periodic) == 8 + 3 * 8
 

@@ -642,6 +642,45 @@ def test_remove_particles(case):
     np.testing.assert_allclose(g.download()["x"], o.download()["x"], rtol=0, atol=1e-9)
 
 
+@pytest.mark.parametrize("case", [CASES[1], CASES[6]], ids=["lvf-np-3w", "full-np-axi"])
+def test_remove_many_particles(case):
+    """Removals of many particles at once (the device compacts the particle
+    arrays and rebuilds over the kept ones): 120 random handler indices
+    between dem_step_begin and dem_step_end, then 150 between steps, then
+    every remaining particle.  Contacts and positions after the first step
+    equal the oracle's; 20 steps after the second removal the states agree
+    to round-off; with every particle gone, steps only advance the clock."""
+    sc = S.mixture(n=400, seed=51, lx=7.0, ly=7.0, lz=7.0, inflate=1.12, **case)
+    ref = oracle_lib.engine()
+    S.apply(ref, sc)
+    ref.compute_forces()
+    g, o = make_pair(reseeded(sc, ref.download()), ref.contacts())
+    rng = np.random.default_rng(11)
+    first = [int(rng.integers(0, 400 - j)) for j in range(120)]
+    for e in (g, o):
+        e.step_begin()
+        e.remove_particles(first)
+        e.step_end()
+    assert g.n == o.n == 280
+    n0 = assert_contacts_equal(g, o)
+    assert n0 > 50
+    a, b = assert_forces_close(g, o)
+    np.testing.assert_array_equal(a["x"], b["x"])
+    np.testing.assert_allclose(a["v"], b["v"], rtol=0, atol=1e-12)
+    second = [int(rng.integers(0, 280 - j)) for j in range(150)]
+    for e in (g, o):
+        e.remove_particles(second)
+        e.step(20)
+    assert g.n == o.n == 130
+    assert_contacts_close(g, o)
+    np.testing.assert_allclose(g.download()["x"], o.download()["x"], rtol=0, atol=1e-9)
+    for e in (g, o):
+        e.remove_particles(np.zeros(130, np.int64))
+        e.step(3)
+    assert g.n == o.n == 0
+    assert g.stats().n_steps == 24
+
+
 def test_find_in_region_against_oracle():
     """dem_find_in_region (the deletion boundaries' test on the device)
     against the oracle: half-spaces of several orientations and open boxes,
