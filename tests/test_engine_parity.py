@@ -531,7 +531,7 @@ def test_add_particles_mid_step(case):
     np.testing.assert_allclose(a["x"], b["x"], rtol=0, atol=1e-9)
 
 
-@pytest.mark.parametrize("case", [CASES[0], CASES[4]], ids=["lvf-P", "hvm-Px"])
+@pytest.mark.parametrize("case", [CASES[0], CASES[4], CASES[5]], ids=["lvf-P", "hvm-Px", "hvm-P-3lv"])
 def test_add_particles_in_place(case):
     """Appends that fit the resident arrays and grid are written in place:
     the capacity dem_get_stats reports stays, and only the new particles are
@@ -679,6 +679,42 @@ def test_remove_many_particles(case):
         e.step(3)
     assert g.n == o.n == 0
     assert g.stats().n_steps == 24
+
+
+@pytest.mark.parametrize("case", [CASES[1], CASES[6]], ids=["lvf-np-3w", "full-np-axi"])
+def test_remove_then_append_in_the_same_step(case):
+    """A deletion boundary and an insertion boundary acting in one step:
+    removals compact the particle arrays on the device, and the appended
+    particles then go in place into the room the removals left (the
+    capacity stays).  After the step contacts and positions equal the
+    oracle's; 20 steps later the states agree to round-off."""
+    sc = S.mixture(n=400, seed=53, lx=7.0, ly=7.0, lz=7.0, inflate=1.12, **case)
+    # the largest particles first (each appended one is within the largest
+    # radius of its grid level); ids in insertion order
+    sc = _subset(sc, np.argsort(-sc.state.radius, kind="stable"))
+    sc.state.id = np.arange(1, 401, dtype=np.uint32)
+    base = _subset(sc, np.arange(350))
+    ref = oracle_lib.engine()
+    S.apply(ref, base)
+    ref.compute_forces()
+    g, o = make_pair(reseeded(base, ref.download()), ref.contacts())
+    rng = np.random.default_rng(13)
+    gone = [int(rng.integers(0, 350 - j)) for j in range(60)]
+    added = _subset(sc, np.arange(350, 400)).state
+    for e in (g, o):
+        e.step_begin()
+        e.remove_particles(gone)
+        e.add_particles(added)
+        e.step_end()
+    assert g.n == o.n == 340
+    assert g.stats().capacity == 350
+    assert_contacts_equal(g, o)
+    a, b = assert_forces_close(g, o)
+    np.testing.assert_array_equal(a["x"], b["x"])
+    g.step(20)
+    o.step(20)
+    assert_contacts_close(g, o)
+    np.testing.assert_allclose(g.download()["x"], o.download()["x"], rtol=0, atol=1e-9)
 
 
 def test_find_in_region_against_oracle():
