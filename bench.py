@@ -82,6 +82,10 @@ def parse():
                     help="instead of the chute: this many calibration heaps (sizes log-uniform in [500, 20000]) "
                          "stepped as one packed ensemble, against the same heaps run one at a time")
     ap.add_argument("--ensemble-steps", type=int, default=30000, help="steps per simulation for sims/hour")
+    ap.add_argument("--churn", type=int, default=0,
+                    help="instead of the timed chute steps: the cost of dem_add_particles and dem_remove_particles "
+                         "during a run of the chute (--n flowing spheres), appending batches of this many spheres "
+                         "and removing 1 to 20,000 of them")
     ap.add_argument("--save-state", default="",
                     help="after the configured run, write the particle state and stored interactions to this .npz")
     ap.add_argument("--load-state", default="",
@@ -182,6 +186,84 @@ def run_ensemble(a):
         "speedup_vs_sequential": t_seq / t_packed}))
 
 
+def run_churn(a):
+    """Insertion and removal during a run (ParticleHandler::copyAndAddObject
+    of the insertion boundaries, removeObject of the deletion boundaries):
+    the chute of a.n flowing spheres is uploaded and stepped; then five
+    batches of a.churn spheres are appended above the bed (the first through
+    the re-upload that leaves room, the others in place; every second one
+    between dem_step_begin and dem_step_end), and 1, 100, 2000 and 20,000
+    spheres at random handler indices are removed, each followed by 20
+    steps.  One JSON line: the milliseconds of each call."""
+    import numpy as np
+
+    from mercurydpm_fork_amd import dem
+    from mercurydpm_fork_amd import scenarios as S
+
+    def log(msg):
+        print("# " + msg, file=sys.stderr, flush=True)
+
+    sc = S.chute_hvm(n_flow=a.n, seed=a.seed)
+    st = sc.state
+    l = sc.info["footprint"]
+    top = float(st.x[:, 2].max())
+    rng = np.random.default_rng(a.seed)
+    nc = int(l / 1.6)
+    cells = rng.permutation(nc * nc)
+    flow = np.flatnonzero(st.inv_mass > 0)
+    e = dem.Engine(device=0, skin=a.skin)
+    S.apply(e, sc)
+    e.compute_forces()
+    e.step(10)
+    n0 = e.n
+    k = a.churn
+    next_id = int(st.id.max()) + 1
+    appends = []
+    for b in range(5):
+        c = cells[(b * k) % cells.size:][:k]
+        src = rng.choice(flow, c.size)
+        pos = np.stack([(c % nc + 0.5) * 1.6, (c // nc + 0.5) * 1.6, np.full(c.size, top + 2.0 + 1.6 * b)], 1)
+        part = dem.State(id=np.arange(next_id, next_id + c.size, dtype=np.uint32), species=st.species[src],
+                         radius=st.radius[src], inv_mass=st.inv_mass[src], inv_inertia=st.inv_inertia[src],
+                         x=pos, v=np.zeros((c.size, 3)), w=np.zeros((c.size, 3)),
+                         q=np.tile([1.0, 0.0, 0.0, 0.0], (c.size, 1)), f=None, t=None)
+        next_id += c.size
+        cap0 = e.stats().capacity
+        mid = b % 2 == 1
+        if mid:
+            e.step_begin()
+        t = time.perf_counter()
+        e.add_particles(part)
+        ms = 1e3 * (time.perf_counter() - t)
+        if mid:
+            e.step_end()
+        cap = e.stats().capacity
+        appends.append({"k": int(c.size), "ms": round(ms, 3), "in_place": bool(cap == cap0), "mid_step": mid,
+                        "n_after": int(e.n), "capacity": int(cap)})
+        log(f"append {c.size}: {ms:.2f} ms ({'in place' if cap == cap0 else 're-upload'})")
+        e.step(20)
+    removals = []
+    for kr in (1, 100, 2000, 20000):
+        n = e.n
+        idx = [int(rng.integers(0, n - j)) for j in range(kr)]
+        t = time.perf_counter()
+        e.remove_particles(idx)
+        ms = 1e3 * (time.perf_counter() - t)
+        removals.append({"k": kr, "ms": round(ms, 3), "n_after": int(e.n)})
+        log(f"remove {kr}: {ms:.2f} ms")
+        e.step(20)
+    x = e.download()["x"]
+    if not np.all(np.isfinite(x)):
+        raise RuntimeError("non-finite positions after the insertions and removals")
+    e.close()
+    print(json.dumps({"metric": "milliseconds per insertion / removal call during a run",
+                      "unit": "ms", "n_gpus": 1, "higher_is_better": False, "dtype": "f64",
+                      "data": "synthetic (the chute's loose lattice; appended spheres copy resident radii)",
+                      "config": {"workload": "polydisperse chute, HertzianViscoelasticMindlin",
+                                 "spheres_at_start": int(n0), "append_batch": k},
+                      "append": appends, "remove": removals}))
+
+
 _roctx = None
 
 
@@ -274,6 +356,8 @@ def main():
     sweep = dem.SWEEP_WAVEFRONT if a.list_sweep == "wavefront" else dem.SWEEP_PER_PARTICLE
     if a.ensemble > 0:
         return run_ensemble(a)
+    if a.churn > 0:
+        return run_churn(a)
 
     def log(msg):
         print("# " + msg, file=sys.stderr, flush=True)
