@@ -586,6 +586,43 @@ def test_add_particles_in_place(case):
     np.testing.assert_allclose(a["v"], b["v"], rtol=0, atol=1e-8)
 
 
+@pytest.mark.parametrize("case", [CASES[3], CASES[5]], ids=["hvm-P", "hvm-P-3lv"])
+def test_add_particles_larger_than_the_grid(case):
+    """Appended particles larger than the largest of their grid level do not
+    fit the resident cells: they go through the re-upload that derives the
+    grid again, although the arrays have room (the capacity grows).  The
+    states after it agree with the oracle's to round-off."""
+    sc = S.mixture(n=400, seed=55, lx=7.0, ly=7.0, lz=7.0, inflate=1.12, **case)
+    # the smallest particles first: every later one is larger than the
+    # resident ones
+    sc = _subset(sc, np.argsort(sc.state.radius, kind="stable"))
+    sc.state.id = np.arange(1, 401, dtype=np.uint32)
+    base = _subset(sc, np.arange(300))
+    ref = oracle_lib.engine()
+    S.apply(ref, base)
+    ref.compute_forces()
+    g, o = make_pair(reseeded(base, ref.download()), ref.contacts())
+    for e in (g, o):
+        e.step_begin()
+        e.add_particles(_subset(sc, np.arange(300, 320)).state)
+        e.step_end()
+    cap = g.stats().capacity
+    assert cap == 320 + 1024
+    assert_contacts_equal(g, o)
+    a, b = assert_forces_close(g, o)
+    np.testing.assert_array_equal(a["x"], b["x"])
+    last = _subset(sc, np.arange(320, 400)).state
+    last.id = last.id + np.uint32(100000)  # above the ids periodic images took on the oracle
+    for e in (g, o):
+        e.add_particles(last)
+    assert g.n == o.n == 400
+    assert g.stats().capacity == 400 + 1024  # re-uploaded, although 400 <= cap
+    g.step(20)
+    o.step(20)
+    assert_contacts_close(g, o)
+    np.testing.assert_allclose(g.download()["x"], o.download()["x"], rtol=0, atol=1e-9)
+
+
 def test_empty_system_then_insertion():
     """A run that starts empty (its insertion boundaries fill it): steps of
     an empty system only advance the clock; particles added later run as on
