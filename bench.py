@@ -542,7 +542,7 @@ def main():
                    "dt": sc.dt, "settle_steps": a.settle_steps, "verlet_skin": a.skin, "list_sweep": a.list_sweep,
                    "contacts_per_particle_settled": None if settled_cn is None else round(settled_cn, 4),
                    "steps_before_window": a.run_steps + a.warmup,
-                   "contacts_per_particle": round(cbar, 4), "contacts_per_particle_survey": "2-3 (dense bed)",
+                   "contacts_per_particle": round(cbar, 4), "contacts_per_particle_dense_bed": "2-3",
                    "list_entries_per_particle": round(lbar, 3),
                    "rebuilds_in_window": int(st.n_rebuilds - rb0),
                    "parallelism": "single GPU"},

@@ -9,7 +9,7 @@ mu = tan(theta) bin by bin gives I(z), hence the shear rate
 gamma_dot = I sqrt(P / rho_p) / d and, integrated up from the base, the
 steady velocity profile u(z).  A DEM run started from the settled bed with
 that profile (instead of at rest) reaches the steady flow in fewer steps;
-DEM itself, run to steady state from rest, judges the profile.
+the profile is measured against DEM itself, run to steady state from rest.
 
 The granular temperature enters the network as an input; in steady flow it
 follows the shear rate, T ~ (gamma_dot d)^2, and its law is fitted to the

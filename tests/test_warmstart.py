@@ -11,7 +11,7 @@
   (tests/golden/closure_chute128.npz, 24 degrees held out) predicts the steady
   flow at 24 degrees; DEM started from the settled bed in that flow reaches
   the steady CG velocity profile (within 5%, the profile of the same DEM run
-  from rest to steady state being the judge) in at least 3x fewer steps than
+  from rest to steady state being the reference) in at least 3x fewer steps than
   DEM started at rest, and stays within 5% of it.
 """
 import os
