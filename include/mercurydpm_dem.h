@@ -345,8 +345,9 @@ int dem_add_particles(dem_ctx* ctx, int64_t k, const uint32_t* id, const int32_t
  * boundaries act (DPMBase::checkInteractionWithBoundaries, DPMBase.cc:
  * 3272-3300).  Cost: the state stays on the device (only the k indices and
  * the k deleted ids cross PCIe), plus one grid and list rebuild over the
- * kept particles.  The ids of deleted particles are not given out again:
- * later appends still need ids above every id the context has held. */
+ * kept particles.  The ids of deleted particles are not given out again
+ * while particles remain: later appends still need ids above every id the
+ * context has held. */
 int dem_remove_particles(dem_ctx* ctx, int64_t k, const int64_t* index);
 
 /* The particles a deletion boundary removes, found on the device: the
