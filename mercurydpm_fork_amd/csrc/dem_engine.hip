@@ -492,6 +492,31 @@ __device__ inline void hash_records(Dev<NS>& d, const Params& p, int cur, int i,
     }
 }
 
+// atomicAdd(&ctr[key], 1) for every active lane of the wavefront, one atomic
+// per run of active lanes with equal keys (the particles come sorted by
+// bucket from the previous rebuild, so runs are long and the counters of
+// busy buckets and tiles are not hammered lane by lane).  Returns what the
+// lane's own atomic would have returned had the lanes added in lane order.
+__device__ inline uint32_t wave_run_add(uint32_t* ctr, uint32_t key)
+{
+    const int lane = threadIdx.x & 63;
+    const unsigned long long act = __ballot(1);
+    const uint32_t prev = __shfl_up(key, 1, 64);
+    const bool prev_act = lane > 0 && ((act >> (lane - 1)) & 1ull);
+    const bool head = !(prev_act && prev == key);
+    const unsigned long long heads = __ballot(head);
+    // a run ends before the next head or inactive lane
+    const unsigned long long brk = heads | ~act;
+    const unsigned long long above = brk & ~((2ull << lane) - 1ull);
+    uint32_t base = 0;
+    if (head) {
+        const int len = above ? __ffsll((long long)above) - 1 - lane : 64 - lane;
+        base = atomicAdd(&ctr[key], (uint32_t)len);
+    }
+    const int hb = 63 - __clzll(heads & ((2ull << lane) - 1ull));
+    return __shfl(base, hb, 64) + (uint32_t)(lane - hb);
+}
+
 template <int NS>
 __global__ void __launch_bounds__(kBlock) k_rb_begin(Dev<NS> d, Params p, int cur)
 {
@@ -509,8 +534,8 @@ __global__ void __launch_bounds__(kBlock) k_rb_begin(Dev<NS> d, Params p, int cu
             d.key[i] = key;
             if (d.vel[i].w == 0.0) c.x |= 0x100; // fixed particle
             d.cellc[i] = c;
-            atomicAdd(&d.count[key], 1u);
-            atomicAdd(&d.tsum[key / kScanTile], 1u);
+            wave_run_add(d.count, key);
+            wave_run_add(d.tsum, key / kScanTile);
         }
         hash_records<NS, false>(d, p, cur, (int)i, d.id[i]);
     }
@@ -598,7 +623,7 @@ __global__ void __launch_bounds__(kBlock) k_scatter(Dev<NS> d, Params p)
     GRID_STRIDE(i, p.n)
     {
         const uint32_t key = d.key[i];
-        const uint32_t pos = d.start[key] + atomicAdd(&d.cursor[key], 1u);
+        const uint32_t pos = d.start[key] + wave_run_add(d.cursor, key);
         d.perm[pos] = (uint32_t)i;
     }
 }
