@@ -4042,11 +4042,13 @@ int dem_remove_particles(dem_ctx* ctx, int64_t k, const int64_t* index)
     }
     HIPCHK(hipSetDevice(ctx->device));
     if ((r = check_flags(ctx)) != 0) return r;
-    if (k < n) {
+    // (the change list holds up to 2k entries, counted in 32-bit ints)
+    if (k < n && k <= ((int64_t)1 << 30)) {
         if ((r = match_uploaded(ctx)) != 0) return r;
         return ctx->ns_springs == 9 ? remove_in_place<9>(ctx, k, index) : remove_in_place<3>(ctx, k, index);
     }
-    // every particle goes: an empty system
+    // every particle goes (an empty system), or more than 2^30 do: through
+    // the host
     const int64_t rebuilds = ctx->last_rebuilds;
     std::vector<uint32_t> aid;
     std::vector<int32_t> asp;
